@@ -1,0 +1,252 @@
+/*
+ * gpugraphs.h -- C ABI of the MI355X (gfx950) semiring sparse kernels.
+ *
+ * Drop-in boundary for the hot path of GPUGraphs.jl: the storage types, the
+ * semiring SpMV / SpMM / SpMSpV kernels and the BFS / Bellman-Ford loops built
+ * on them.  Every entry point takes plain pointers and sizes; device buffers
+ * are HIP device pointers; indices are 1-based Int32 exactly as in the
+ * reference's structs, so a Julia wrapper passes its struct fields unchanged.
+ *
+ * Conventions
+ *  - Every function returns a gg_status.  GG_OK = 0.  The other codes name the
+ *    Julia exception the reference throws for the same condition:
+ *      GG_ERR_DIMENSION  -> DimensionMismatch  (src/spmv/CSR.jl:374-379,400-402)
+ *      GG_ERR_ARGUMENT   -> ArgumentError      (src/spmv/CSR.jl:381-387,
+ *                                                src/storage/CSR.jl:48-63)
+ *      GG_ERR_ASSERTION  -> AssertionError     (src/spmm/CSR.jl:131-133,161)
+ *      GG_ERR_BOUNDS     -> BoundsError        (src/storage/CSR.jl:155-157)
+ *    gg_last_error() returns the message of the last failure on this thread.
+ *  - Kernels are enqueued on the context's stream and return without waiting
+ *    (the reference launches asynchronously and the caller synchronises,
+ *    test/spmv.jl:24-25).  Algorithm entry points (BFS, Bellman-Ford, the
+ *    batched forms) return after their result is complete.
+ *  - Operators are codes, one per GPUGraphs_* function of src/algebra.jl.
+ */
+#ifndef GPUGRAPHS_H
+#define GPUGRAPHS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GG_ABI_VERSION 1
+
+typedef enum gg_status {
+  GG_OK = 0,
+  GG_ERR_DIMENSION = 1,
+  GG_ERR_ARGUMENT = 2,
+  GG_ERR_ASSERTION = 3,
+  GG_ERR_BOUNDS = 4,
+  GG_ERR_UNSUPPORTED = 5,  /* operator / type combination without a kernel */
+  GG_ERR_HIP = 6,          /* a HIP runtime call failed */
+  GG_ERR_NOMEM = 7,
+  GG_ERR_NOT_CONVERGED = 8,/* iteration cap reached (negative cycle) */
+  GG_ERR_BUDGET = 9        /* request exceeds the serving memory budget */
+} gg_status;
+
+/* Element types.  GG_BOOL is one byte holding 0 or 1 (Julia Bool). */
+typedef enum gg_dtype {
+  GG_BOOL = 1,
+  GG_INT32 = 2,
+  GG_INT64 = 3,
+  GG_FLOAT32 = 4,
+  GG_FLOAT64 = 5,
+  GG_UINT64 = 6
+} gg_dtype;
+
+/* Operators of src/algebra.jl, one code per function. */
+typedef enum gg_op {
+  GG_OP_FIRST = 1,     /* GPUGraphs_first     algebra.jl:31 */
+  GG_OP_SECOND = 2,    /* GPUGraphs_second    algebra.jl:32 */
+  GG_OP_ANY = 3,       /* GPUGraphs_any       algebra.jl:35 (max) */
+  GG_OP_PAIR = 4,      /* GPUGraphs_pair      algebra.jl:40 */
+  GG_OP_ADD = 5,       /* GPUGraphs_add       algebra.jl:43 */
+  GG_OP_SAFE_ADD = 6,  /* GPUGraphs_safe_add  algebra.jl:47 */
+  GG_OP_MUL = 7,       /* GPUGraphs_mul       algebra.jl:55 */
+  GG_OP_MIN = 8,       /* GPUGraphs_min       algebra.jl:65 */
+  GG_OP_MAX = 9,       /* GPUGraphs_max       algebra.jl:68 */
+  GG_OP_LOR = 10,      /* GPUGraphs_lor       algebra.jl:77 */
+  GG_OP_LAND = 11,     /* GPUGraphs_land      algebra.jl:80 */
+  GG_OP_BOR = 12,      /* GPUGraphs_bor       algebra.jl:95 */
+  GG_OP_BAND = 13,     /* GPUGraphs_band      algebra.jl:98 */
+  GG_OP_SECONDI = 14   /* GPUGraphs_secondi   algebra.jl:109 (B_i * y) */
+} gg_op;
+
+typedef enum gg_format {
+  GG_CSR = 1,   /* SparseGPUMatrixCSR  src/storage/CSR.jl:14-24 */
+  GG_SELL = 2,  /* SparseGPUMatrixSELL src/storage/SELL.jl:3-18 */
+  GG_CSC = 3    /* SparseGPUMatrixCSC  src/storage/CSC.jl:2-12 */
+} gg_format;
+
+typedef enum gg_layout {
+  GG_COL_MAJOR = 0, /* Julia Matrix layout, the reference's SpMM operands */
+  GG_ROW_MAJOR = 1  /* one contiguous row of k values per vertex */
+} gg_layout;
+
+/*
+ * Non-owning view of a sparse matrix whose arrays live in device memory.
+ * CSR:  ptr = rowptr[m+1], idx = colval[nnz], val = nzval[nnz]
+ * CSC:  ptr = colptr[n+1], idx = rowval[nnz], val = nzval[nnz]
+ * SELL: ptr = slice_ptr[nslices+1], idx = colval[n_stored] (pad -1),
+ *       val = nzval[n_stored] (pad 0).  Entry k of slot r of slice s sits at
+ *       slice_ptr[s] + k*slice_size + r (1-based), src/storage/SELL.jl:128-161.
+ *       perm == NULL and n_heavy == 0 is exactly the reference layout.
+ *       perm[r] (1-based, 0 = empty slot) is the matrix row held by slot r
+ *       when rows were sorted by length inside windows of sigma rows;
+ *       slot_len[r] is that slot's entry count.  Rows longer than the SELL
+ *       bound are kept out of the slices in a CSR "heavy" partition that the
+ *       same launch processes in segments of heavy_seg_len entries.
+ */
+typedef struct gg_matrix {
+  int32_t format;     /* gg_format */
+  int32_t val_dtype;  /* gg_dtype of val / heavy_val */
+  int64_t m, n, nnz;
+  const int32_t* ptr;
+  const int32_t* idx;
+  const void* val;
+  /* SELL-C-sigma only */
+  int64_t slice_size;
+  int64_t nslices;
+  int64_t n_stored;
+  const int32_t* perm;      /* [nslices*slice_size] or NULL */
+  const int32_t* slot_len;  /* [nslices*slice_size] or NULL */
+  int64_t n_heavy;
+  const int32_t* heavy_row; /* [n_heavy] 1-based rows */
+  const int64_t* heavy_ptr; /* [n_heavy+1] 0-based offsets */
+  const int32_t* heavy_seg; /* [n_heavy+1] prefix count of segments */
+  int64_t heavy_seg_len;
+  const int32_t* heavy_col; /* 1-based columns */
+  const void* heavy_val;
+} gg_matrix;
+
+/* Host-side SELL-C-sigma arrays produced by gg_sell_build (owned by the
+ * library until gg_sell_host_free). */
+typedef struct gg_sell_host {
+  int64_t m, n, nnz, slice_size, sigma, nslices, n_stored, n_slots;
+  int32_t val_dtype;
+  int32_t* slice_ptr;  /* [nslices+1] */
+  int32_t* colval;     /* [n_stored] */
+  void* nzval;         /* [n_stored] */
+  int32_t* perm;       /* [n_slots] or NULL (identity) */
+  int32_t* slot_len;   /* [n_slots] or NULL */
+  int64_t n_heavy, heavy_nnz, heavy_seg_len, n_seg;
+  int32_t* heavy_row;  /* [n_heavy] */
+  int64_t* heavy_ptr;  /* [n_heavy+1] */
+  int32_t* heavy_seg;  /* [n_heavy+1] */
+  int32_t* heavy_col;  /* [heavy_nnz] */
+  void* heavy_val;     /* [heavy_nnz] */
+} gg_sell_host;
+
+typedef struct gg_ctx gg_ctx;
+
+/* ---------------------------------------------------------------- runtime */
+int gg_abi_version(void);
+const char* gg_status_string(int status);
+const char* gg_last_error(void);
+/* stream: a hipStream_t, or NULL for the device's null stream. */
+int gg_ctx_create(int device, void* stream, gg_ctx** out);
+int gg_ctx_destroy(gg_ctx* ctx);
+int gg_ctx_synchronize(gg_ctx* ctx);
+int gg_ctx_set_stream(gg_ctx* ctx, void* stream);
+/* Device memory for callers without an allocator of their own (Julia's
+ * KernelAbstractions.allocate counterpart). */
+int gg_malloc(gg_ctx* ctx, size_t bytes, void** out);
+int gg_free(gg_ctx* ctx, void* ptr);
+int gg_memcpy_htod(gg_ctx* ctx, void* dst, const void* src, size_t bytes);
+int gg_memcpy_dtoh(gg_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Milliseconds the most recent kernel launch of the named family took, timed
+ * with HIP events on the context stream when timing is enabled. */
+int gg_ctx_enable_timing(gg_ctx* ctx, int on);
+int gg_ctx_last_kernel_ms(gg_ctx* ctx, double* ms);
+
+/* ---------------------------------------------------------------- storage */
+/* Validation of a CSR / CSC triple held in host memory; same checks and order
+ * as the inner constructors src/storage/CSR.jl:48-63, src/storage/CSC.jl:31-46. */
+int gg_csr_check(int64_t m, int64_t n, const int64_t* rowptr, int64_t rowptr_len,
+                 const int64_t* colval, int64_t colval_len, int64_t nzval_len);
+/* SELL inner-constructor checks, src/storage/SELL.jl:42-52. */
+int gg_sell_check(int64_t n, int64_t nslices, int64_t n_stored, int64_t slice_ptr_len,
+                  const int64_t* colval, int64_t colval_len, int64_t nzval_len);
+/* SELL-C-sigma builder over a host CSR (1-based).  sigma <= 1 and
+ * heavy_threshold <= 0 give the reference layout, src/storage/SELL.jl:99-181;
+ * sigma > 1 sorts rows by length inside windows of sigma rows;
+ * heavy_threshold > 0 moves rows longer than it into the heavy partition. */
+int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* colval,
+                  const void* nzval, int val_dtype, int64_t slice_size, int64_t sigma,
+                  int64_t heavy_threshold, int64_t heavy_seg_len, gg_sell_host** out);
+void gg_sell_host_free(gg_sell_host* h);
+
+/* ---------------------------------------------------------------- kernels */
+/* C = accum(C, A (add.mul) B) on rows selected by mask (mask[r] != 0) and the
+ * 1-based row range [range_start, range_start+range_len).  mask == NULL: all
+ * rows; range_len < 0: no range.  Replaces gpu_spmv! for CSR, SELL and CSC:
+ * src/spmv/CSR.jl:2-29, src/spmv/SELL.jl:2-29, src/spmv/CSC.jl:23-61. */
+int gg_spmv(gg_ctx* ctx, void* C, int c_dtype, int64_t c_len, const gg_matrix* A,
+            const void* B, int b_dtype, int64_t b_len, int add, int mul, int accum,
+            const void* mask, int mask_dtype, int64_t mask_len, int64_t range_start,
+            int64_t range_len);
+
+/* Dense-operand SpMM, C[m x k] = accum(C, A (add.mul) B[n x k]).  Replaces
+ * gpu_spmm!, src/spmm/CSR.jl:111-204 and src/spmm/SELL.jl:84-124.  The
+ * reference passes Julia (column-major) matrices; GG_ROW_MAJOR is the layout
+ * of the batched multi-source kernels. */
+int gg_spmm(gg_ctx* ctx, void* C, int c_dtype, int64_t c_rows, int64_t c_cols,
+            int c_layout, const gg_matrix* A, const void* B, int b_dtype, int64_t b_rows,
+            int64_t b_cols, int b_layout, int add, int mul, int accum, const void* mask,
+            int mask_dtype, int64_t mask_len, int64_t range_start, int64_t range_len);
+
+/* Masked sparse-matrix x sparse-vector product (the kernel src/spmspv.jl:2-54
+ * intends): for every stored b_j, every stored A[i,j] contributes
+ * mul(A[i,j], b_j) to c_i under add; rows i with mask[i] == 0 (or != 0 when
+ * mask_complement) are skipped.  A is CSC (columns expand directly) or CSR of
+ * the transpose.  The output is sparse with sorted indices:
+ * c_nzind/c_nzval must hold c_len entries; *c_nnz receives the count. */
+int gg_spmspv(gg_ctx* ctx, int32_t* c_nzind, void* c_nzval, int64_t* c_nnz, int c_dtype,
+              int64_t c_len, const gg_matrix* A, const int32_t* b_nzind, const void* b_nzval,
+              int64_t b_nnz, int b_dtype, int64_t b_len, int add, int mul,
+              const void* mask, int mask_dtype, int64_t mask_len, int mask_complement);
+
+/* ------------------------------------------------------------- algorithms */
+/* Level-synchronous BFS over A_T (row v = in-neighbours of v); dist[v] = level,
+ * unreached = typemax(Int32); source is 1-based.  src/algorithms/bfs.jl:1-111. */
+int gg_bfs_distances(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, int32_t* dist,
+                     int use_mask);
+/* BFS parents: parents[source] = 0, unreached = typemax(Int32), otherwise the
+ * 1-based frontier in-neighbour chosen by GPUGraphs_any (CSR: first in row
+ * order; SELL: largest).  src/algorithms/bfs.jl:113-183. */
+int gg_bfs_parents(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, int32_t* parents,
+                   int64_t batch_size);
+/* Direction-optimising BFS levels: pull (masked SpMV over A_T) while the
+ * frontier is large, push (SpMSpV over A's out-edges) while it is small.
+ * A_out is CSR of A (row u = out-neighbours of u); for a symmetric graph pass
+ * A_T again.  alpha/beta < 0 pick the tuned switch thresholds. */
+int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_out,
+                        int64_t source, int32_t* dist, double alpha, double beta);
+/* Jacobi Bellman-Ford (min, safe_add) from one source; dist has dist_dtype
+ * (Int32 for Bool matrices, else the value type), src/algorithms/shortest_path.jl:12-83. */
+int gg_shortest_path(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, void* dist,
+                     int dist_dtype, int64_t* iterations);
+/* k-source Bellman-Ford on an m x k column-major dist,
+ * src/algorithms/shortest_path.jl:88-161.  sources is a host array. */
+int gg_shortest_path_multi(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources,
+                           int64_t k, void* dist, int dist_dtype, int64_t* iterations);
+
+/* ------------------------------------------------------------ batched form */
+/* Multi-source BFS as or-and SpMM over 64-bit source bitmasks: k sources
+ * (any k >= 1, ceil(k/64) words per vertex).  dist is m x k column-major
+ * (dist[:, s] equals gg_bfs_distances from sources[s]). */
+int gg_msbfs_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources, int64_t k,
+                       int32_t* dist, int64_t* levels);
+/* Multi-source min-plus SSSP over a row-major m x k working set; dist is
+ * m x k column-major like gg_shortest_path_multi. */
+int gg_mssp_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources, int64_t k,
+                      void* dist, int dist_dtype, int64_t* iterations);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUGRAPHS_H */
