@@ -1,0 +1,273 @@
+#!/usr/bin/env python3
+"""BFS queries/s (batched vs plain) and SELL SpMV GB/s on an R-MAT scale-22 graph.
+
+Workload (one GPU): R-MAT scale 22, edge factor 16, a/b/c/d = .57/.19/.19/.05,
+symmetrised, self-loops and duplicates removed, ids permuted, Int32 indices;
+SELL-C-sigma with C = 64, sigma = 4096 (rows longer than the heavy threshold in
+the segmented partition).  A step is one batched BFS from 64 random
+non-isolated roots (or-and SpMM over 64-bit source masks); the plain
+one-root-at-a-time BFS, the direction-optimising BFS and the fp32 plus-times
+SELL SpMV are measured beside it, with the oracle on the host cores as the
+CPU baseline.  Inputs are resident in HBM before any timed region.
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpugraphs.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs), q))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--roots", type=int, default=64)
+    ap.add_argument("--heavy", type=int, default=256, help="SELL heavy-row threshold")
+    ap.add_argument("--seg", type=int, default=4096, help="heavy segment length")
+    ap.add_argument("--plain-roots", type=int, default=64)
+    ap.add_argument("--spmv-reps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only", default="", help="profile one part: spmv | msbfs | bfs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the per-GPU shard and how it would shard are described in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import gpugraphs as gg
+    from gpugraphs import _lib
+    from gpugraphs import generators as G
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    R0 = gg.ROCBackend(0)
+    seed = 20261203
+
+    t = time.time()
+    Rg = G.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=seed, dtype=np.float32, values=G.UNIFORM01)
+    n, nnz = Rg.m, Rg.nnz
+    deg = Rg.degrees()
+    log(f"rmat{args.scale}: n={n} nnz={nnz} max_deg={deg.max()} gen {time.time() - t:.1f}s")
+
+    t = time.time()
+    A_f = Rg.to_sell(R0, 64, 4096, args.heavy, args.seg)          # fp32 values for SpMV
+    Rb = G.HostCSR(n, n, Rg.rowptr, Rg.colval, np.ones(nnz, np.bool_))
+    A_b = Rb.to_sell(R0, 64, 4096, args.heavy, args.seg)          # Bool pattern for BFS
+    A_csr = Rb.to_csr(R0)                                          # out-edges for push (symmetric)
+    torch.cuda.synchronize()
+    log(f"SELL builds + upload {time.time() - t:.1f}s; stored/nnz {(A_f.n_stored + (int(A_f.heavy_ptr[-1].item()) if A_f.n_heavy else 0)) / nnz:.3f}")
+
+    rng = np.random.default_rng(seed)
+    roots = rng.choice(np.nonzero(deg)[0], args.roots, replace=False).astype(np.int32) + 1
+    ctx = _lib.context(0)
+    res = {}
+
+    # ---------------------------------------------------------------- SpMV
+    if args.only in ("", "spmv"):
+        x = torch.from_numpy(rng.random(n, dtype=np.float32)).to(dev)
+        y = torch.zeros(n, dtype=torch.float32, device=dev)
+        for _ in range(3):
+            gg.gpu_spmv_(y, A_f, x)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.spmv_reps):
+            gg.gpu_spmv_(y, A_f, x)
+        e1.record()
+        torch.cuda.synchronize()
+        spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
+        _lib.enable_timing(ctx, True)
+        _lib.hot_kernel_stats(ctx, reset=True)
+        for _ in range(args.spmv_reps):
+            gg.gpu_spmv_(y, A_f, x)
+        kms, kn = _lib.hot_kernel_stats(ctx, reset=True)
+        _lib.enable_timing(ctx, False)
+        spmv_kernel_ms = kms / max(kn, 1)
+        B_spmv = nnz * 8 + 4 * (n + 1) + 4 * n + 4 * n  # algorithmic bytes per SpMV
+        S64 = Rg.to_scipy().astype(np.float64)
+        ref = S64 @ x.cpu().numpy().astype(np.float64)
+        got = y.cpu().numpy()
+        relerr = float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)))
+        assert relerr <= 1e-5, f"SpMV parity failed: max rel err {relerr}"
+        del S64
+        res["spmv"] = dict(ms=spmv_ms, kernel_ms=spmv_kernel_ms, bytes=B_spmv,
+                           gbs=B_spmv / spmv_ms / 1e6, kernel_gbs=B_spmv / spmv_kernel_ms / 1e6,
+                           max_rel_err_vs_fp64=relerr)
+        log(f"SELL SpMV: {spmv_ms * 1e3:.1f} us/SpMV ({B_spmv / spmv_ms / 1e6:.0f} GB/s), kernel "
+            f"{spmv_kernel_ms * 1e3:.1f} us ({B_spmv / spmv_kernel_ms / 1e6:.0f} GB/s), rel err {relerr:.1e}")
+
+    # ---------------------------------------------------------- batched BFS
+    D = None
+    if args.only in ("", "msbfs"):
+        for _ in range(args.warmup):
+            D = gg.multi_source_bfs(A_b, roots)
+        torch.cuda.synchronize()
+        lat = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            D = gg.multi_source_bfs(A_b, roots)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t0)
+        step_s = float(np.mean(lat))
+        _lib.enable_timing(ctx, True)
+        _lib.hot_kernel_stats(ctx, reset=True)
+        D = gg.multi_source_bfs(A_b, roots)
+        kms, kn = _lib.hot_kernel_stats(ctx, reset=True)
+        _lib.enable_timing(ctx, False)
+        k = len(roots)
+        B_ms = 4 * nnz + 2 * n * (k // 8)  # per level: pattern once + source masks in and out
+        res["msbfs"] = dict(step_ms=step_s * 1e3, qps=k / step_s, p50_ms=pct(lat, 50) * 1e3,
+                            p99_ms=pct(lat, 99) * 1e3, levels=kn, kernel_ms=kms / max(kn, 1),
+                            bytes_per_launch=B_ms)
+        log(f"batched BFS k={k}: {step_s * 1e3:.2f} ms/step, {k / step_s:.0f} q/s, "
+            f"{kn} levels, kernel avg {kms / max(kn, 1) * 1e3:.1f} us")
+
+    # ------------------------------------------------------------ plain BFS
+    if args.only in ("", "bfs"):
+        pr = roots[:args.plain_roots]
+        gg.bfs_distances(A_b, int(pr[0]))
+        gg.bfs_distances_do(A_b, int(pr[0]), A_out=A_csr)
+        torch.cuda.synchronize()
+        lat_plain, lat_do = [], []
+        mismatches = 0
+        for i, s in enumerate(pr):
+            t0 = time.perf_counter()
+            d = gg.bfs_distances(A_b, int(s))
+            torch.cuda.synchronize()
+            lat_plain.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            d2 = gg.bfs_distances_do(A_b, int(s), A_out=A_csr)
+            torch.cuda.synchronize()
+            lat_do.append(time.perf_counter() - t0)
+            if not torch.equal(d, d2):
+                mismatches += 1
+            if D is not None and not torch.equal(d, D[:, i]):
+                mismatches += 1
+        assert mismatches == 0, f"{mismatches} BFS results differ between plain, DO and batched"
+        _lib.enable_timing(ctx, True)
+        _lib.hot_kernel_stats(ctx, reset=True)
+        gg.bfs_distances(A_b, int(pr[0]))
+        kms, kn = _lib.hot_kernel_stats(ctx, reset=True)
+        _lib.enable_timing(ctx, False)
+        d0 = d if len(pr) == 1 else gg.bfs_distances(A_b, int(pr[0]))
+        d0h = d0.cpu().numpy()
+        reach = d0h != np.iinfo(np.int32).max
+        comp_edges = int(deg[reach].sum()) // 2
+        res["bfs"] = dict(plain_qps=len(pr) / sum(lat_plain), plain_p50_ms=pct(lat_plain, 50) * 1e3,
+                          plain_p99_ms=pct(lat_plain, 99) * 1e3, do_qps=len(pr) / sum(lat_do),
+                          do_p50_ms=pct(lat_do, 50) * 1e3, do_p99_ms=pct(lat_do, 99) * 1e3,
+                          levels=kn, level_kernel_ms=kms / max(kn, 1),
+                          gteps_plain=comp_edges / float(np.median(lat_plain)) / 1e9,
+                          gteps_do=comp_edges / float(np.median(lat_do)) / 1e9)
+        log(f"plain BFS: {len(pr) / sum(lat_plain):.0f} q/s p99 {pct(lat_plain, 99) * 1e3:.2f} ms; "
+            f"DO BFS: {len(pr) / sum(lat_do):.0f} q/s p99 {pct(lat_do, 99) * 1e3:.2f} ms")
+
+        # parity at full size: plain levels against the oracle from the first root
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as orc
+            cores = min(16, os.cpu_count() or 1)
+            Ao = orc.csr(n, n, Rg.rowptr, Rg.colval, np.ones(nnz, np.uint8))
+            t0 = time.perf_counter()
+            dref = orc.bfs_distances(Ao, int(pr[0]), nthreads=cores)
+            cpu_bfs_s = time.perf_counter() - t0
+            assert np.array_equal(dref, d0h), "GPU BFS levels differ from the oracle at full size"
+            Af = orc.csr(n, n, Rg.rowptr, Rg.colval, Rg.nzval)
+            xh = np.random.default_rng(1).random(n, dtype=np.float32)
+            yh = np.zeros(n, np.float32)
+            t0 = time.perf_counter()
+            orc.spmv(yh, Af, xh, nthreads=cores)
+            cpu_spmv_s = time.perf_counter() - t0
+            res["cpu"] = dict(bfs_s=cpu_bfs_s, spmv_s=cpu_spmv_s, cores=cores,
+                              qps=1.0 / cpu_bfs_s,
+                              spmv_gbs=(nnz * 8 + 12 * n) / cpu_spmv_s / 1e9)
+            log(f"CPU oracle ({cores} threads): BFS {cpu_bfs_s:.2f} s, SpMV {cpu_spmv_s * 1e3:.0f} ms")
+
+    # ---------------------------------------------------------------- report
+    ms = res.get("msbfs")
+    bfs = res.get("bfs", {})
+    spmv = res.get("spmv", {})
+    if ms:
+        value = ms["qps"]
+        kernel_ms = ms["kernel_ms"]
+        achieved = ms["bytes_per_launch"] / kernel_ms / 1e6
+        roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                        kernel="msbfs_sell_kernel", bytes_per_launch=ms["bytes_per_launch"],
+                        avg_launch_us=round(kernel_ms * 1e3, 2))
+    else:
+        value, roofline = None, None
+    spmv_roof = None
+    if spmv:
+        spmv_roof = dict(bound="hbm", achieved=round(spmv["kernel_gbs"], 1), peak=HBM_PEAK_GBS,
+                         unit="GB/s", frac=round(spmv["kernel_gbs"] / HBM_PEAK_GBS, 4),
+                         kernel="sell_spmv_kernel", bytes_per_launch=spmv["bytes"],
+                         avg_launch_us=round(spmv["kernel_ms"] * 1e3, 2),
+                         spmv_call_gbs=round(spmv["gbs"], 1))
+    cpu = res.get("cpu")
+    out = {
+        "metric": "BFS/SSSP queries/s + p99 latency (batched vs plain); SpMV GB/s vs HBM peak",
+        "value": None if value is None else round(value, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": None if not ms else round(ms["step_ms"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bool (or-and over u64 source masks)",
+        "data": "synthetic (seeded R-MAT generator)",
+        "config": {"workload": f"rmat{args.scale} ef16 a.57 b.19 c.19 symmetrised permuted; "
+                               f"batched BFS from {args.roots} random non-isolated roots "
+                               "(one step = one k=64 or-and SpMM traversal); SELL-64-sigma4096",
+                   "n": n, "nnz": nnz, "roots": args.roots, "slice_size": 64, "sigma": 4096,
+                   "heavy_threshold": args.heavy, "parallelism": "single GPU"},
+        "p50_ms": None if not ms else round(ms["p50_ms"], 3),
+        "p99_ms": None if not ms else round(ms["p99_ms"], 3),
+        "plain_qps": round(bfs["plain_qps"], 1) if bfs else None,
+        "plain_p50_ms": round(bfs["plain_p50_ms"], 3) if bfs else None,
+        "plain_p99_ms": round(bfs["plain_p99_ms"], 3) if bfs else None,
+        "do_qps": round(bfs["do_qps"], 1) if bfs else None,
+        "do_p99_ms": round(bfs["do_p99_ms"], 3) if bfs else None,
+        "batched_vs_plain": round(value / bfs["plain_qps"], 2) if (bfs and value) else None,
+        "gteps_plain": round(bfs["gteps_plain"], 3) if bfs else None,
+        "gteps_do": round(bfs["gteps_do"], 3) if bfs else None,
+        "spmv_gbs": round(spmv["gbs"], 1) if spmv else None,
+        "spmv_us": round(spmv["ms"] * 1e3, 1) if spmv else None,
+        "spmv_roofline": spmv_roof,
+        "roofline": roofline,
+        "cpu_baseline": None if not cpu else {
+            "value": round(cpu["qps"], 3), "unit": "queries/s", "cores": cpu["cores"],
+            "kind": "port",
+            "sample": "one BFS (reference pull loop, levels) from the first root on the full graph; "
+                      f"one fp32 SpMV took {cpu['spmv_s'] * 1e3:.0f} ms "
+                      f"({cpu['spmv_gbs']:.1f} GB/s)"},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,327 @@
+// batched.hip -- multi-source BFS as or-and SpMM over source bitmasks
+// (gg_msbfs_distances).
+//
+// k BFS runs from k sources share one traversal: vertex v holds W = ceil(k/64)
+// 64-bit words whose bit s says "v is in source s's frontier".  One level is
+// the or-and SpMM  next[v] = (OR over in-neighbours u of front[u]) & ~seen[v]
+// (the bitmask form of src/algorithms/bfs.jl:52-67 applied to k columns at
+// once); each newly set bit s records dist[v, s] = level.  A row stops
+// scanning as soon as every source has reached it (the or-monoid's absorbing
+// all-ones), rows reached by every source are skipped, and a 1-bit
+// "in any frontier" summary (n bits, L2-resident) avoids fetching the W-word
+// masks of vertices that are in no frontier.
+//
+// dist[:, s] equals the single-source BFS levels from sources[s]: the
+// bit-parallel traversal is k independent level-synchronous BFS runs.
+#include "frontier.hpp"
+
+namespace gg {
+
+constexpr int kM = 256;
+
+struct MsArgs {
+  int64_t m, k;
+  int layout;                  // dist layout: GG_COL_MAJOR (m x k) or GG_ROW_MAJOR
+  const uint64_t* front;       // [m*W]
+  const uint32_t* front_any;   // [ceil(m/32)]
+  uint64_t* next;              // [m*W]
+  uint32_t* next_any;
+  uint64_t* seen;              // [m*W]
+  int32_t* dist;
+  int32_t level;
+  unsigned int* found;         // rows with a new bit this level
+};
+
+// Bits of word w that belong to one of the k sources.
+template <int W> __device__ __forceinline__ uint64_t last_mask(int64_t k, int w) {
+  const int64_t lo = (int64_t)w * 64;
+  if (lo >= k) return 0ull;
+  if (lo + 64 <= k) return ~0ull;
+  return (1ull << (k - lo)) - 1ull;
+}
+
+template <int W>
+__device__ __forceinline__ bool finish_row(const MsArgs& a, int64_t row, const uint64_t* acc,
+                                           const uint64_t* sn) {
+  bool any = false;
+  uint64_t nw[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    nw[w] = acc[w] & ~sn[w] & last_mask<W>(a.k, w);
+    any |= nw[w] != 0;
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    a.next[row * W + w] = nw[w];
+    if (nw[w]) a.seen[row * W + w] = sn[w] | nw[w];
+  }
+  if (any) {
+    bit_claim(a.next_any, row);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      uint64_t b = nw[w];
+      while (b) {
+        const int s = w * 64 + __ffsll((unsigned long long)b) - 1;
+        b &= b - 1;
+        if (a.layout == GG_ROW_MAJOR) a.dist[row * a.k + s] = a.level;
+        else a.dist[row + (int64_t)s * a.m] = a.level;
+      }
+    }
+  }
+  return any;
+}
+
+template <int W>
+__device__ __forceinline__ bool saturated(const uint64_t* acc, const uint64_t* sn, int64_t k) {
+  bool full = true;
+#pragma unroll
+  for (int w = 0; w < W; ++w) full &= ((acc[w] | sn[w]) & last_mask<W>(k, w)) == last_mask<W>(k, w);
+  return full;
+}
+
+template <class TA, bool UNIT, int W>
+__global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_sell_blocks, MsArgs a) {
+  if ((int64_t)blockIdx.x < n_sell_blocks) {
+    int64_t row, base, len;
+    bool has_len;
+    bool newly = false;
+    const int64_t slot = (int64_t)blockIdx.x * kM + threadIdx.x;
+    if (sell_slot(g, slot, row, base, len, has_len)) {
+      uint64_t sn[W], acc[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) { sn[w] = a.seen[row * W + w]; acc[w] = 0; }
+      if (!saturated<W>(acc, sn, a.k)) {
+        const TA* val = (const TA*)g.val;
+        for (int64_t kk = 0; kk < len; ++kk) {
+          const int64_t i = base + kk * g.C;
+          const int32_t col = g.colval[i];
+          if (!has_len && col == -1) break;
+          if (!bit_get(a.front_any, col - 1) || !(UNIT || edge_on(val, i))) continue;
+#pragma unroll
+          for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
+          if (saturated<W>(acc, sn, a.k)) break;
+        }
+        newly = finish_row<W>(a, row, acc, sn);
+      } else {
+#pragma unroll
+        for (int w = 0; w < W; ++w) a.next[row * W + w] = 0;
+      }
+    }
+    const unsigned long long b = __ballot(newly);
+    if (b && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)b) - 1)
+      atomicAdd(a.found, (unsigned)__popcll(b));
+    return;
+  }
+  // one block per heavy row: OR across the block, early exit once saturated
+  __shared__ uint64_t red[kM / 64][W];
+  __shared__ int s_full;
+  const int64_t h = (int64_t)blockIdx.x - n_sell_blocks;
+  const int64_t row = g.heavy_row[h] - 1;
+  uint64_t sn[W], acc[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) { sn[w] = a.seen[row * W + w]; acc[w] = 0; }
+  if (threadIdx.x == 0) s_full = saturated<W>(acc, sn, a.k);
+  __syncthreads();
+  if (!s_full) {
+    const TA* hv = (const TA*)g.heavy_val;
+    for (int64_t i = g.heavy_ptr[h] + threadIdx.x; i < g.heavy_ptr[h + 1]; i += kM) {
+      if (__atomic_load_n(&s_full, __ATOMIC_RELAXED)) break;
+      const int32_t col = g.heavy_col[i];
+      if (!bit_get(a.front_any, col - 1) || !(UNIT || edge_on(hv, i))) continue;
+#pragma unroll
+      for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
+      if (saturated<W>(acc, sn, a.k)) __atomic_store_n(&s_full, 1, __ATOMIC_RELAXED);
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    for (int o = 32; o > 0; o >>= 1) acc[w] |= shfl_xor(acc[w], o);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int w = 0; w < W; ++w) red[threadIdx.x >> 6][w] = acc[w];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      for (int q = 1; q < kM / 64; ++q) acc[w] |= red[q][w];
+    if (finish_row<W>(a, row, acc, sn)) atomicAdd(a.found, 1u);
+  }
+}
+
+template <class TA, bool UNIT, int W>
+__global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict__ rowptr,
+                                                       const int32_t* __restrict__ colval,
+                                                       const TA* __restrict__ val, MsArgs a) {
+  const int64_t row = (int64_t)blockIdx.x * kM + threadIdx.x;
+  bool newly = false;
+  if (row < a.m) {
+    uint64_t sn[W], acc[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) { sn[w] = a.seen[row * W + w]; acc[w] = 0; }
+    if (!saturated<W>(acc, sn, a.k)) {
+      for (int32_t i = rowptr[row] - 1; i < rowptr[row + 1] - 1; ++i) {
+        const int32_t col = colval[i];
+        if (!bit_get(a.front_any, col - 1) || !(UNIT || edge_on(val, i))) continue;
+#pragma unroll
+        for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
+        if (saturated<W>(acc, sn, a.k)) break;
+      }
+      newly = finish_row<W>(a, row, acc, sn);
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; ++w) a.next[row * W + w] = 0;
+    }
+  }
+  const unsigned long long b = __ballot(newly);
+  if (b && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)b) - 1)
+    atomicAdd(a.found, (unsigned)__popcll(b));
+}
+
+__global__ void msbfs_seed_kernel(const int32_t* sources, int64_t k, int64_t m, int W, int layout,
+                                  uint64_t* front, uint32_t* front_any, uint64_t* seen,
+                                  int32_t* dist) {
+  // one thread: several sources may share a vertex
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int64_t s = 0; s < k; ++s) {
+    const int64_t v = sources[s] - 1;
+    front[v * W + s / 64] |= 1ull << (s & 63);
+    seen[v * W + s / 64] |= 1ull << (s & 63);
+    front_any[v >> 5] |= 1u << (v & 31);
+    if (layout == GG_ROW_MAJOR) dist[v * k + s] = 0;
+    else dist[v + s * m] = 0;
+  }
+}
+
+__global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void ms_zero_kernel(unsigned int* p) { *p = 0u; }
+
+// Runs one batch of k <= 512 sources.
+int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64_t k,
+              int32_t* dist, int layout, int64_t* levels_out) {
+  const int64_t m = A->m;
+  int W = (int)((k + 63) / 64);
+  int Wc = W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : 8;
+  const int64_t words = (m + 31) / 32;
+  void *f, *nx, *sn, *fa, *na, *cnt;
+  GG_TRY(ws_get(ctx, WS_MS0, sizeof(uint64_t) * m * Wc, &f));
+  GG_TRY(ws_get(ctx, WS_MS1, sizeof(uint64_t) * m * Wc, &nx));
+  GG_TRY(ws_get(ctx, WS_MS2, sizeof(uint64_t) * m * Wc, &sn));
+  GG_TRY(ws_get(ctx, WS_BFS1, sizeof(uint32_t) * words, &fa));
+  GG_TRY(ws_get(ctx, WS_BFS2, sizeof(uint32_t) * words, &na));
+  GG_TRY(ws_get(ctx, WS_FLAG, 64, &cnt));
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(m * Wc, kM)), dim3(kM), 0, ctx->stream,
+                     (uint64_t*)f, m * Wc, 0ull);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(m * Wc, kM)), dim3(kM), 0, ctx->stream,
+                     (uint64_t*)sn, m * Wc, 0ull);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, ctx->stream,
+                     (uint32_t*)fa, words, 0u);
+  hipLaunchKernelGGL(fill_i32v_kernel, dim3(blocks_for(m * k, kM)), dim3(kM), 0, ctx->stream,
+                     dist, m * k, (int32_t)2147483647);
+  hipLaunchKernelGGL(msbfs_seed_kernel, dim3(1), dim3(1), 0, ctx->stream, sources_dev, k, m, Wc,
+                     layout, (uint64_t*)f, (uint32_t*)fa, (uint64_t*)sn, dist);
+  GG_HIP(hipGetLastError());
+  MsArgs a{};
+  a.m = m;
+  a.k = k;
+  a.layout = layout;
+  a.seen = (uint64_t*)sn;
+  a.dist = dist;
+  a.found = (unsigned*)cnt;
+  uint64_t* fr = (uint64_t*)f;
+  uint64_t* nxp = (uint64_t*)nx;
+  uint32_t* fany = (uint32_t*)fa;
+  uint32_t* nany = (uint32_t*)na;
+  const bool unit = (A->flags & GG_FLAG_UNIT_PATTERN) != 0;
+  int32_t level = 0;
+  while (true) {
+    ++level;
+    a.front = fr;
+    a.front_any = fany;
+    a.next = nxp;
+    a.next_any = nany;
+    a.level = level;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, ctx->stream,
+                       nany, words, 0u);
+    hipLaunchKernelGGL(ms_zero_kernel, dim3(1), dim3(1), 0, ctx->stream, (unsigned*)cnt);
+    int st;
+    {
+    HotTimer ht(ctx);
+    st = with_type(A->val_dtype, [&](auto ta) -> int {
+      typedef typename decltype(ta)::type TA;
+      auto go = [&](auto ut, auto wt) -> int {
+        constexpr bool U = decltype(ut)::value;
+        constexpr int WW = decltype(wt)::value;
+        if (A->format == GG_CSR) {
+          hipLaunchKernelGGL((msbfs_csr_kernel<TA, U, WW>), dim3(blocks_for(m, kM)), dim3(kM), 0,
+                             ctx->stream, A->ptr, A->idx, (const TA*)A->val, a);
+        } else {
+          SellView g = sell_view(A);
+          const int64_t nsb = (g.n_slots + kM - 1) / kM;
+          hipLaunchKernelGGL((msbfs_sell_kernel<TA, U, WW>), dim3((unsigned)(nsb + g.n_heavy)),
+                             dim3(kM), 0, ctx->stream, g, nsb, a);
+        }
+        GG_HIP(hipGetLastError());
+        return GG_OK;
+      };
+      auto pick_w = [&](auto ut) -> int {
+        switch (Wc) {
+          case 1: return go(ut, std::integral_constant<int, 1>{});
+          case 2: return go(ut, std::integral_constant<int, 2>{});
+          case 4: return go(ut, std::integral_constant<int, 4>{});
+          default: return go(ut, std::integral_constant<int, 8>{});
+        }
+      };
+      if (unit) return pick_w(std::true_type{});
+      if constexpr (std::is_floating_point<TA>::value) {
+        return set_error(GG_ERR_ARGUMENT, "or-and BFS needs Bool or integer matrix values");
+      } else {
+        return pick_w(std::false_type{});
+      }
+    });
+    }
+    GG_TRY(st);
+    GG_HIP(hipMemcpyAsync(ctx->host_pinned, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    GG_HIP(hipStreamSynchronize(ctx->stream));
+    if (*(unsigned*)ctx->host_pinned == 0) break;
+    std::swap(fr, nxp);
+    std::swap(fany, nany);
+  }
+  if (levels_out) *levels_out = level - 1;
+  return GG_OK;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_msbfs_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources,
+                                  int64_t k, int32_t* dist, int64_t* levels) {
+  if (!ctx || !dist || (k > 0 && !sources)) return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  clear_error();
+  GG_TRY(check_matrix(A_T));
+  if (A_T->format == GG_CSC) return set_error(GG_ERR_ARGUMENT, "BFS runs on CSR or SELL A_T");
+  if (A_T->m != A_T->n) return set_error(GG_ERR_DIMENSION, "A_T must be square");
+  if (k < 1) return set_error(GG_ERR_ARGUMENT, "need at least one source");
+  for (int64_t i = 0; i < k; ++i)
+    if (sources[i] < 1 || sources[i] > A_T->m)
+      return set_error(GG_ERR_BOUNDS, "source %lld outside 1:%lld", (long long)sources[i],
+                       (long long)A_T->m);
+  timer_begin(ctx);
+  void* src;
+  GG_TRY(ws_get(ctx, WS_MS3, sizeof(int32_t) * k, &src));
+  GG_HIP(hipMemcpyAsync(src, sources, sizeof(int32_t) * k, hipMemcpyHostToDevice, ctx->stream));
+  // column-major m x k: batches of <= 512 sources write disjoint column blocks
+  int64_t maxlev = 0;
+  for (int64_t b0 = 0; b0 < k; b0 += 512) {
+    const int64_t kb = std::min<int64_t>(512, k - b0);
+    int64_t lv = 0;
+    GG_TRY(msbfs_run(ctx, A_T, (const int32_t*)src + b0, kb, dist + b0 * A_T->m, GG_COL_MAJOR, &lv));
+    maxlev = std::max(maxlev, lv);
+  }
+  if (levels) *levels = maxlev;
+  timer_end(ctx);
+  return GG_OK;
+}

@@ -1,0 +1,535 @@
+// bfs.hip -- BFS levels and parents over A_T (gg_bfs_distances,
+// gg_bfs_parents, gg_bfs_distances_do).
+//
+// One level of src/algorithms/bfs.jl:50-71 is a masked or-and SpMV
+// next = A_T (bor.band) curr over the unvisited rows, followed by four
+// broadcasts (dist, to_explore, curr, next) and an OR-reduce.  Here a level is
+// one kernel: every unvisited row scans its in-neighbours against the frontier
+// bitmap, stops at the first hit (true absorbs under bor, src/spmv/CSR.jl:49-51),
+// and on a hit writes dist = level, sets its bit in the next frontier and in
+// the visited set, and is counted.  The frontier (n bits, 512 KiB at 4M
+// vertices) stays in L2 while the rows stream from HBM.  A BFS level is
+// unique, so the levels equal the reference's whatever the traversal order.
+//
+// Parents follow src/algorithms/bfs.jl:135-183 with GPUGraphs_any resolved as
+// the reference resolves it: CSR takes the first frontier in-neighbour in row
+// order (any_dense_masked_csr_spmv_kernel!, src/spmv/CSR.jl:159-170), SELL the
+// largest (GPUGraphs_any = max, src/algebra.jl:35, over the whole row).
+//
+// gg_bfs_distances_do switches between this pull step and a push step over
+// A's out-edges (a frontier queue expanded with claim-by-atomic-OR) with the
+// direction-optimising rule of Beamer et al. (SC'12): go bottom-up when the
+// frontier's out-edges exceed the unexplored edges / alpha, return top-down
+// when the frontier shrinks below n / beta.
+#include "frontier.hpp"
+
+namespace gg {
+
+constexpr int kB = 256;
+
+__global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void fill_i32v_kernel(int32_t* p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+struct BfsCounters {
+  unsigned int found;        // vertices claimed by the level
+  unsigned int qlen;         // push: length of the next queue
+  unsigned long long scout;  // push: out-degree sum of the claimed vertices
+};
+
+__global__ void bfs_seed_kernel(int32_t* out, int32_t out_val, uint32_t* visited,
+                                uint32_t* frontier, int32_t* queue, int64_t s) {
+  out[s] = out_val;
+  visited[s >> 5] |= 1u << (s & 31);
+  frontier[s >> 5] |= 1u << (s & 31);
+  if (queue) queue[0] = (int32_t)s;
+}
+
+// All 64 lanes must be active.
+__device__ __forceinline__ void count_wave(bool newly, unsigned int* found) {
+  const unsigned long long b = __ballot(newly);
+  if (b && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)b) - 1)
+    atomicAdd(found, (unsigned)__popcll(b));
+}
+
+// ------------------------------------------------------------ pull, SELL
+// Light slots: one lane per row.  Heavy rows: one block per segment for
+// levels (any hit settles the row), one block per row for parents (max over
+// the row).
+template <class TA, bool UNIT, bool PARENTS>
+__global__ __launch_bounds__(kB) void bfs_pull_sell_kernel(
+    SellView g, int64_t n_sell_blocks, const uint32_t* __restrict__ frontier,
+    uint32_t* __restrict__ next, uint32_t* __restrict__ visited, int32_t* __restrict__ out,
+    int32_t level, BfsCounters* cnt) {
+  if ((int64_t)blockIdx.x < n_sell_blocks) {
+    bool newly = false;
+    int64_t row, base, len;
+    bool has_len;
+    const int64_t slot = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (sell_slot(g, slot, row, base, len, has_len) && !bit_get(visited, row)) {
+      const TA* val = (const TA*)g.val;
+      int32_t hit = 0;
+      for (int64_t k = 0; k < len; ++k) {
+        const int64_t i = base + k * g.C;
+        const int32_t col = g.colval[i];
+        if (!has_len && col == -1) break;
+        if (bit_get(frontier, col - 1) && (PARENTS || UNIT || edge_on(val, i))) {
+          if (!PARENTS) { hit = col; break; }
+          hit = max(hit, col);
+        }
+      }
+      if (hit) {
+        out[row] = PARENTS ? hit : level;
+        bit_claim(next, row);
+        newly = bit_claim(visited, row);
+      }
+    }
+    count_wave(newly, &cnt->found);
+    return;
+  }
+  __shared__ int s_skip;
+  __shared__ int s_hit;
+  __shared__ int s_red[kB / 64];
+  const int64_t hb = (int64_t)blockIdx.x - n_sell_blocks;
+  int64_t h, s, e;
+  if (PARENTS) {
+    h = hb;
+    s = g.heavy_ptr[h];
+    e = g.heavy_ptr[h + 1];
+  } else {
+    int64_t lo = 0, hi = g.n_heavy;  // heavy_seg[lo] <= hb < heavy_seg[lo+1]
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (g.heavy_seg[mid] <= hb) lo = mid; else hi = mid;
+    }
+    h = lo;
+    s = g.heavy_ptr[h] + (hb - g.heavy_seg[h]) * g.seg_len;
+    e = min(s + g.seg_len, g.heavy_ptr[h + 1]);
+  }
+  const int64_t row = g.heavy_row[h] - 1;
+  if (threadIdx.x == 0) {
+    s_skip = bit_get(visited, row);
+    s_hit = 0;
+  }
+  __syncthreads();
+  if (s_skip) return;
+  const TA* hv = (const TA*)g.heavy_val;
+  int32_t best = 0;
+  for (int64_t i = s + threadIdx.x; i < e; i += kB) {
+    if (!PARENTS && __atomic_load_n(&s_hit, __ATOMIC_RELAXED)) break;
+    const int32_t col = g.heavy_col[i];
+    if (bit_get(frontier, col - 1) && (PARENTS || UNIT || edge_on(hv, i))) {
+      if (!PARENTS) {
+        __atomic_store_n(&s_hit, 1, __ATOMIC_RELAXED);
+        break;
+      }
+      best = max(best, col);
+    }
+  }
+  if (PARENTS) {
+    for (int o = 32; o > 0; o >>= 1) best = max(best, __shfl_xor(best, o, 64));
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = best;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t hit = 0;
+    if (PARENTS) {
+      for (int w = 0; w < kB / 64; ++w) hit = max(hit, s_red[w]);
+    } else {
+      hit = s_hit;
+    }
+    if (hit) {
+      out[row] = PARENTS ? hit : level;
+      bit_claim(next, row);
+      if (bit_claim(visited, row)) atomicAdd(&cnt->found, 1u);
+    }
+  }
+}
+
+// ------------------------------------------------------------- pull, CSR
+// Groups of L lanes per row, the whole wavefront iterating together so the
+// ballots see every lane.  The group's first hit in row order is the parent.
+template <class TA, bool UNIT, bool PARENTS>
+__global__ __launch_bounds__(kB) void bfs_pull_csr_kernel(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
+    const TA* __restrict__ val, int64_t m, int logL, const uint32_t* __restrict__ frontier,
+    uint32_t* __restrict__ next, uint32_t* __restrict__ visited, int32_t* __restrict__ out,
+    int32_t level, BfsCounters* cnt) {
+  const int L = 1 << logL;
+  const int wl = threadIdx.x & 63;
+  const int lane = wl & (L - 1);
+  const int gbase = wl & ~(L - 1);
+  const unsigned long long gmask = L == 64 ? ~0ull : (((1ull << L) - 1) << gbase);
+  const int64_t groups_per_wave = 64 >> logL;
+  const int64_t wave = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * kB) >> 6;
+  for (int64_t w0 = wave * groups_per_wave; w0 < m; w0 += nwaves * groups_per_wave) {
+    const int64_t row = w0 + (wl >> logL);
+    bool newly = false;
+    const bool live = row < m && !bit_get(visited, row);
+    int32_t s = 0, e = 0;
+    if (live) {
+      s = rowptr[row] - 1;
+      e = rowptr[row + 1] - 1;
+    }
+    int32_t hitpos = -1;
+    for (int32_t b = s; b < e; b += L) {
+      const int32_t i = b + lane;
+      bool h = false;
+      if (i < e) {
+        const int32_t col = colval[i];
+        h = bit_get(frontier, col - 1) && (PARENTS || UNIT || edge_on(val, i));
+      }
+      const unsigned long long bal = __ballot(h) & gmask;
+      if (bal) {
+        hitpos = b + (__ffsll(bal) - 1 - gbase);
+        break;
+      }
+    }
+    if (hitpos >= 0 && lane == 0) {
+      out[row] = PARENTS ? colval[hitpos] : level;
+      bit_claim(next, row);
+      newly = bit_claim(visited, row);
+    }
+    count_wave(newly, &cnt->found);
+  }
+}
+
+// -------------------------------------------------------------- push, CSR
+// Frontier queue over A_out (row u = out-neighbours of u), groups of L lanes
+// per frontier vertex; claimed vertices go to the next queue (one atomic per
+// wavefront) and to the next bitmap.
+__global__ __launch_bounds__(kB) void bfs_push_kernel(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
+    const int32_t* __restrict__ queue, int64_t qlen, int logL, int32_t* __restrict__ next_queue,
+    uint32_t* __restrict__ next, uint32_t* __restrict__ visited, int32_t* __restrict__ dist,
+    int32_t level, BfsCounters* cnt) {
+  const int L = 1 << logL;
+  const int wl = threadIdx.x & 63;
+  const int lane = wl & (L - 1);
+  const int64_t groups_per_wave = 64 >> logL;
+  const int64_t wave = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * kB) >> 6;
+  for (int64_t w0 = wave * groups_per_wave; w0 < qlen; w0 += nwaves * groups_per_wave) {
+    const int64_t q = w0 + (wl >> logL);
+    int32_t s = 0, e = 0;
+    if (q < qlen) {
+      const int32_t u = queue[q];
+      s = rowptr[u] - 1;
+      e = rowptr[u + 1] - 1;
+    }
+    int32_t trips = (e - s + L - 1) >> logL;
+    for (int o = 32; o > 0; o >>= 1) trips = max(trips, __shfl_xor(trips, o, 64));
+    for (int32_t t = 0; t < trips; ++t) {
+      const int32_t i = s + t * L + lane;
+      bool newly = false;
+      int32_t v = 0;
+      if (i < e) {
+        v = colval[i] - 1;
+        if (!bit_get(visited, v) && bit_claim(visited, v)) {
+          newly = true;
+          dist[v] = level;
+          bit_claim(next, v);
+        }
+      }
+      const unsigned long long b = __ballot(newly);
+      if (b) {
+        const int leader = __ffsll(b) - 1;
+        unsigned base = 0;
+        unsigned long long deg = newly ? (unsigned long long)(rowptr[v + 1] - rowptr[v]) : 0ull;
+        for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o, 64);
+        if (wl == leader) {
+          base = atomicAdd(&cnt->qlen, (unsigned)__popcll(b));
+          atomicAdd(&cnt->found, (unsigned)__popcll(b));
+          atomicAdd(&cnt->scout, deg);
+        }
+        base = __shfl(base, leader, 64);
+        if (newly) next_queue[base + __popcll(b & ((1ull << wl) - 1))] = v;
+      }
+    }
+  }
+}
+
+// Bitmap -> queue of set bits (ascending order is not required).
+__global__ __launch_bounds__(kB) void bitmap_to_queue_kernel(const uint32_t* __restrict__ bm,
+                                                             int64_t words, int64_t n,
+                                                             int32_t* __restrict__ queue,
+                                                             BfsCounters* cnt) {
+  const int64_t w = (int64_t)blockIdx.x * kB + threadIdx.x;
+  const int wl = threadIdx.x & 63;
+  uint32_t bits = w < words ? bm[w] : 0u;
+  if (w == words - 1 && (n & 31)) bits &= (1u << (n & 31)) - 1u;
+  int c = __popc(bits);
+  int incl = c;  // inclusive scan across the wavefront
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (wl >= o) incl += y;
+  }
+  const int total = __shfl(incl, 63, 64);
+  unsigned base = 0;
+  if (wl == 63 && total) base = atomicAdd(&cnt->qlen, (unsigned)total);
+  base = __shfl(base, 63, 64);
+  unsigned pos = base + incl - c;
+  while (bits) {
+    const int b = __ffs(bits) - 1;
+    bits &= bits - 1;
+    queue[pos++] = (int32_t)(w * 32 + b);
+  }
+}
+
+__global__ void zero_counters_kernel(BfsCounters* c) {
+  c->found = 0;
+  c->qlen = 0;
+  c->scout = 0;
+}
+
+__global__ void degree_kernel(const int32_t* rowptr, int64_t v, BfsCounters* c) {
+  c->scout = (unsigned long long)(rowptr[v + 1] - rowptr[v]);
+}
+
+// --------------------------------------------------------------- host side
+struct BfsWork {
+  int64_t m = 0, words = 0;
+  uint32_t* visited = nullptr;
+  uint32_t* front = nullptr;
+  uint32_t* next = nullptr;
+  int32_t* q0 = nullptr;
+  int32_t* q1 = nullptr;
+  BfsCounters* cnt = nullptr;
+};
+
+static int bfs_work(gg_ctx* ctx, int64_t m, bool queues, BfsWork& w) {
+  w.m = m;
+  w.words = (m + 31) / 32;
+  void* p;
+  GG_TRY(ws_get(ctx, WS_BFS0, sizeof(uint32_t) * w.words, &p)); w.visited = (uint32_t*)p;
+  GG_TRY(ws_get(ctx, WS_BFS1, sizeof(uint32_t) * w.words, &p)); w.front = (uint32_t*)p;
+  GG_TRY(ws_get(ctx, WS_BFS2, sizeof(uint32_t) * w.words, &p)); w.next = (uint32_t*)p;
+  GG_TRY(ws_get(ctx, WS_FLAG, sizeof(BfsCounters), &p)); w.cnt = (BfsCounters*)p;
+  if (queues) {
+    GG_TRY(ws_get(ctx, WS_BFS3, sizeof(int32_t) * m, &p)); w.q0 = (int32_t*)p;
+    GG_TRY(ws_get(ctx, WS_BFS4, sizeof(int32_t) * m, &p)); w.q1 = (int32_t*)p;
+  }
+  return GG_OK;
+}
+
+static void fill_words(gg_ctx* ctx, uint32_t* p, int64_t n, uint32_t v) {
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(n, kB)), dim3(kB), 0, ctx->stream, p, n, v);
+}
+
+static int read_counters(gg_ctx* ctx, const BfsWork& w, BfsCounters* host) {
+  GG_HIP(hipMemcpyAsync(ctx->host_pinned, w.cnt, sizeof(BfsCounters), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  GG_HIP(hipStreamSynchronize(ctx->stream));
+  memcpy(host, ctx->host_pinned, sizeof(BfsCounters));
+  return GG_OK;
+}
+
+static int csr_lanes(const gg_matrix* A) {
+  if (A->m == 0) return 0;
+  const double avg = (double)A->nnz / (double)A->m;
+  int lg = 0;
+  while (lg < 6 && (double)(1 << lg) < avg) ++lg;
+  return lg;
+}
+
+// One pull level (levels or parents) over A_T.
+static int pull_level(gg_ctx* ctx, const gg_matrix* A, bool parents, BfsWork& w, int32_t* out,
+                      int32_t level) {
+  const bool unit = (A->flags & GG_FLAG_UNIT_PATTERN) != 0;
+  HotTimer ht(ctx);
+  return with_type(A->val_dtype, [&](auto ta) -> int {
+    typedef typename decltype(ta)::type TA;
+    auto go = [&](auto unit_tag, auto par_tag) -> int {
+      constexpr bool U = decltype(unit_tag)::value;
+      constexpr bool P = decltype(par_tag)::value;
+      if (A->format == GG_CSR) {
+        const int lg = csr_lanes(A);
+        unsigned nb = blocks_for(A->m << lg, kB);
+        const unsigned cap = (unsigned)ctx->num_cus * 32;
+        if (nb > cap) nb = cap;
+        hipLaunchKernelGGL((bfs_pull_csr_kernel<TA, U, P>), dim3(nb), dim3(kB), 0, ctx->stream,
+                           A->ptr, A->idx, (const TA*)A->val, A->m, lg, w.front, w.next,
+                           w.visited, out, level, w.cnt);
+      } else {
+        SellView g = sell_view(A);
+        const int64_t nsb = (g.n_slots + kB - 1) / kB;
+        const int64_t nb = nsb + (P ? g.n_heavy : g.n_seg);
+        if (nb > 0)
+          hipLaunchKernelGGL((bfs_pull_sell_kernel<TA, U, P>), dim3((unsigned)nb), dim3(kB), 0,
+                             ctx->stream, g, nsb, w.front, w.next, w.visited, out, level, w.cnt);
+      }
+      GG_HIP(hipGetLastError());
+      return GG_OK;
+    };
+    if (parents) return go(std::true_type{}, std::true_type{});
+    if (unit) return go(std::true_type{}, std::false_type{});
+    if constexpr (std::is_floating_point<TA>::value) {
+      return set_error(GG_ERR_ARGUMENT, "or-and BFS needs Bool or integer matrix values "
+                                        "(GPUGraphs_band is bitwise, src/algebra.jl:98)");
+    } else {
+      return go(std::false_type{}, std::false_type{});
+    }
+  });
+}
+
+static int bfs_begin(gg_ctx* ctx, const gg_matrix* A, int64_t source, int32_t* out,
+                     int32_t src_val, BfsWork& w, bool queues) {
+  GG_TRY(bfs_work(ctx, A->m, queues, w));
+  hipLaunchKernelGGL(fill_i32v_kernel, dim3(blocks_for(A->m, kB)), dim3(kB), 0, ctx->stream, out,
+                     A->m, (int32_t)2147483647);
+  fill_words(ctx, w.visited, w.words, 0u);
+  fill_words(ctx, w.front, w.words, 0u);
+  hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(1), 0, ctx->stream, out, src_val, w.visited,
+                     w.front, queues ? w.q0 : nullptr, source - 1);
+  GG_HIP(hipGetLastError());
+  return GG_OK;
+}
+
+static int check_square_source(const gg_matrix* A, int64_t source) {
+  GG_TRY(check_matrix(A));
+  if (A->format == GG_CSC) return set_error(GG_ERR_ARGUMENT, "BFS runs on CSR or SELL A_T");
+  if (A->m != A->n) return set_error(GG_ERR_DIMENSION, "A_T must be square");
+  if (source < 1 || source > A->m) return set_error(GG_ERR_BOUNDS, "source %lld outside 1:%lld",
+                                                    (long long)source, (long long)A->m);
+  return GG_OK;
+}
+
+int bfs_levels_pull(gg_ctx* ctx, const gg_matrix* A, int64_t source, int32_t* dist,
+                    int64_t* nlevels) {
+  BfsWork w;
+  GG_TRY(bfs_begin(ctx, A, source, dist, 0, w, false));
+  int32_t level = 0;
+  BfsCounters hc{};
+  while (true) {
+    ++level;
+    fill_words(ctx, w.next, w.words, 0u);
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
+    GG_TRY(pull_level(ctx, A, false, w, dist, level));
+    GG_TRY(read_counters(ctx, w, &hc));
+    if (hc.found == 0) break;
+    std::swap(w.front, w.next);
+  }
+  if (nlevels) *nlevels = level - 1;
+  return GG_OK;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" {
+
+int gg_bfs_distances(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, int32_t* dist,
+                     int use_mask) {
+  (void)use_mask;  // the unvisited mask is applied in-kernel either way; levels are equal
+  if (!ctx || !dist) return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  clear_error();
+  GG_TRY(check_square_source(A_T, source));
+  timer_begin(ctx);
+  int st = bfs_levels_pull(ctx, A_T, source, dist, nullptr);
+  timer_end(ctx);
+  return st;
+}
+
+int gg_bfs_parents(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, int32_t* parents,
+                   int64_t batch_size) {
+  if (!ctx || !parents) return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  clear_error();
+  GG_TRY(check_square_source(A_T, source));
+  if (batch_size < 1) return set_error(GG_ERR_ARGUMENT, "batch_size must be >= 1");
+  timer_begin(ctx);
+  BfsWork w;
+  GG_TRY(bfs_begin(ctx, A_T, source, parents, 0, w, false));
+  BfsCounters hc{};
+  int32_t level = 0;
+  while (true) {
+    ++level;
+    fill_words(ctx, w.next, w.words, 0u);
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
+    GG_TRY(pull_level(ctx, A_T, true, w, parents, level));
+    GG_TRY(read_counters(ctx, w, &hc));
+    if (hc.found == 0) break;
+    std::swap(w.front, w.next);
+  }
+  timer_end(ctx);
+  return GG_OK;
+}
+
+int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_out,
+                        int64_t source, int32_t* dist, double alpha, double beta) {
+  if (!ctx || !dist) return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  clear_error();
+  GG_TRY(check_square_source(A_T, source));
+  GG_TRY(check_matrix(A_out));
+  if (A_out->format != GG_CSR || A_out->m != A_T->m || A_out->n != A_T->n)
+    return set_error(GG_ERR_ARGUMENT, "A_out must be the CSR of A (out-neighbour rows), same size");
+  if (!(A_T->flags & GG_FLAG_UNIT_PATTERN) || !(A_out->flags & GG_FLAG_UNIT_PATTERN))
+    return set_error(GG_ERR_ARGUMENT, "direction-optimising BFS needs unit-pattern matrices");
+  if (alpha <= 0) alpha = 15.0;
+  if (beta <= 0) beta = 18.0;
+  timer_begin(ctx);
+  BfsWork w;
+  GG_TRY(bfs_begin(ctx, A_T, source, dist, 0, w, true));
+  const int64_t n = A_T->m;
+  BfsCounters hc{};
+  hipLaunchKernelGGL(degree_kernel, dim3(1), dim3(1), 0, ctx->stream, A_out->ptr, source - 1, w.cnt);
+  GG_TRY(read_counters(ctx, w, &hc));
+  double edges_to_check = (double)A_out->nnz;
+  double scout = (double)hc.scout;
+  int64_t qlen = 1;
+  int32_t level = 0;
+  const int lg_push = 4;
+  while (qlen > 0) {
+    if (scout > edges_to_check / alpha) {
+      int64_t awake = qlen, old;
+      do {
+        old = awake;
+        ++level;
+        fill_words(ctx, w.next, w.words, 0u);
+        hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
+        GG_TRY(pull_level(ctx, A_T, false, w, dist, level));
+        GG_TRY(read_counters(ctx, w, &hc));
+        awake = hc.found;
+        std::swap(w.front, w.next);
+      } while (awake > 0 && (awake >= old || (double)awake > (double)n / beta));
+      if (awake == 0) break;
+      hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
+      hipLaunchKernelGGL(bitmap_to_queue_kernel, dim3(blocks_for(w.words, kB)), dim3(kB), 0,
+                         ctx->stream, w.front, w.words, n, w.q0, w.cnt);
+      GG_HIP(hipGetLastError());
+      GG_TRY(read_counters(ctx, w, &hc));
+      qlen = hc.qlen;
+      scout = 1;
+    } else {
+      edges_to_check -= scout;
+      ++level;
+      fill_words(ctx, w.next, w.words, 0u);
+      hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
+      unsigned nb = blocks_for((qlen << lg_push), kB);
+      const unsigned cap = (unsigned)ctx->num_cus * 32;
+      if (nb > cap) nb = cap;
+      {
+        HotTimer ht(ctx);
+        hipLaunchKernelGGL(bfs_push_kernel, dim3(nb), dim3(kB), 0, ctx->stream, A_out->ptr,
+                           A_out->idx, w.q0, qlen, lg_push, w.q1, w.next, w.visited, dist, level,
+                           w.cnt);
+      }
+      GG_HIP(hipGetLastError());
+      GG_TRY(read_counters(ctx, w, &hc));
+      qlen = hc.qlen;
+      scout = (double)hc.scout;
+      std::swap(w.q0, w.q1);
+      std::swap(w.front, w.next);
+    }
+  }
+  timer_end(ctx);
+  return GG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+// internal.hpp -- context, workspace and error plumbing shared by the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+
+#include "../../include/gpugraphs.h"
+#include "semiring.hpp"
+
+namespace gg {
+
+// Thread-local message of the last failure (gg_last_error).
+int set_error(int status, const char* fmt, ...);
+void clear_error();
+
+#define GG_HIP(call)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return ::gg::set_error(GG_ERR_HIP, "%s failed: %s (%s:%d)", #call,           \
+                             hipGetErrorString(e_), __FILE__, __LINE__);           \
+  } while (0)
+
+#define GG_TRY(expr)            \
+  do {                          \
+    int s_ = (expr);            \
+    if (s_ != GG_OK) return s_; \
+  } while (0)
+
+// Named workspace slots owned by a context.  A slot grows on demand (the
+// stream is synchronised before a reallocation) and is never shared by two
+// live uses within one entry point.
+enum WsSlot {
+  WS_PARTIAL = 0,   // heavy-row segment partials
+  WS_COUNTER,       // per-heavy-row arrival counters (kept zero between launches)
+  WS_FLAG,          // small device flags / counters
+  WS_BFS0, WS_BFS1, WS_BFS2, WS_BFS3, WS_BFS4, WS_BFS5,
+  WS_SSSP0, WS_SSSP1, WS_SSSP2,
+  WS_SPMSPV0, WS_SPMSPV1, WS_SPMSPV2,
+  WS_MS0, WS_MS1, WS_MS2, WS_MS3, WS_MS4,
+  WS_NSLOTS
+};
+
+}  // namespace gg
+
+struct gg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* ws[gg::WS_NSLOTS] = {};
+  size_t ws_bytes[gg::WS_NSLOTS] = {};
+  void* host_pinned = nullptr;  // 4 KiB pinned staging for flags / counts
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around a whole entry point
+  hipEvent_t ev2 = nullptr, ev3 = nullptr;  // around one hot-kernel launch
+  double last_ms = 0.0;
+  double hot_ms = 0.0;                       // summed hot-kernel launch durations
+  int64_t hot_launches = 0;
+  int num_cus = 256;
+};
+
+namespace gg {
+
+// Device pointer of at least `bytes` in `slot`; zero-filled when `zero`
+// and freshly (re)allocated.
+int ws_get(gg_ctx* ctx, int slot, size_t bytes, void** out, bool zero_new = false);
+
+// Time the launches between timer_begin / timer_end when the context asks.
+void timer_begin(gg_ctx* ctx);
+void timer_end(gg_ctx* ctx);
+
+// Times one launch of a hot kernel (the traversal / product kernel of an
+// entry point) with events on the context stream when timing is enabled.
+struct HotTimer {
+  gg_ctx* c;
+  explicit HotTimer(gg_ctx* ctx) : c(ctx) {
+    if (c->timing) (void)hipEventRecord(c->ev2, c->stream);
+  }
+  ~HotTimer() {
+    if (!c->timing) return;
+    (void)hipEventRecord(c->ev3, c->stream);
+    if (hipEventSynchronize(c->ev3) == hipSuccess) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, c->ev2, c->ev3) == hipSuccess) {
+        c->hot_ms += ms;
+        c->hot_launches += 1;
+      }
+    }
+  }
+};
+
+inline size_t dtype_size(int code) {
+  switch (code) {
+    case GG_BOOL: return 1;
+    case GG_INT32: case GG_FLOAT32: return 4;
+    case GG_INT64: case GG_FLOAT64: case GG_UINT64: return 8;
+  }
+  return 0;
+}
+
+inline unsigned blocks_for(int64_t work, int per_block) {
+  int64_t b = (work + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+// Validation shared by gpu_spmv!-style entry points (src/spmv/CSR.jl:358-408).
+int check_matrix(const gg_matrix* A);
+
+// Internal launchers (used by the algorithms; no argument checks).
+int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const void* B,
+                int b_dtype, int add, int mul, int accum, const void* mask, int mask_dtype,
+                int64_t r0, int64_t nrows, bool has_range);
+
+}  // namespace gg

@@ -1,0 +1,420 @@
+// sssp.hip -- Jacobi Bellman-Ford (min, safe_add) from one or many sources.
+//
+// src/algorithms/shortest_path.jl:34-83 iterates next = min(next, A_T
+// (min.safe_add) dist) and then dist .= next; since next == dist after every
+// copy, one iteration maps dist_in to dist_out[v] = min(dist_in[v],
+// min_u safe_add(w(u->v), dist_in[u])).  That is what one kernel does here,
+// reading one buffer and writing the other and counting the rows whose
+// distance fell; the loop stops at the first iteration without a change (the
+// reference checks every 8 iterations and stops at the same fixpoint, which
+// is unique: every relaxation order converges to the minimum over paths of the
+// left-fold sums, and min is exact).
+//
+// k sources (src/algorithms/shortest_path.jl:112-161) use a row-major m x k
+// working set: a group of lanes owns one row, one lane per source, so the k
+// distances of a neighbour are one contiguous read; the reference's
+// column-major result is written at the end.
+#include "frontier.hpp"
+
+namespace gg {
+
+constexpr int kT = 256;
+
+template <class TD> __device__ __forceinline__ TD sadd(TD w, TD d) {
+  const TD sat = Lim<TD>::max();
+  return jmax(w, d) == sat ? sat : jadd(w, d);  // GPUGraphs_safe_add, src/algebra.jl:47-48
+}
+
+__device__ __forceinline__ void count_changes(bool ch, unsigned int* cnt) {
+  const unsigned long long b = __ballot(ch);
+  if (b && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)b) - 1)
+    atomicAdd(cnt, (unsigned)__popcll(b));
+}
+
+// ----------------------------------------------------------- one source, CSR
+template <class TA, class TD>
+__global__ __launch_bounds__(kT) void bf_csr_kernel(const int32_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ colval,
+                                                    const TA* __restrict__ val, int64_t m,
+                                                    int logL, const TD* __restrict__ din,
+                                                    TD* __restrict__ dout, unsigned int* changed) {
+  const int L = 1 << logL;
+  const int wl = threadIdx.x & 63;
+  const int lane = wl & (L - 1);
+  const int64_t gpw = 64 >> logL;
+  const int64_t wave = ((int64_t)blockIdx.x * kT + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * kT) >> 6;
+  for (int64_t w0 = wave * gpw; w0 < m; w0 += nwaves * gpw) {
+    const int64_t row = w0 + (wl >> logL);
+    TD acc = Lim<TD>::max();
+    if (row < m) {
+      for (int32_t i = rowptr[row] - 1 + lane; i < rowptr[row + 1] - 1; i += L) {
+        acc = jmin(acc, sadd(TD(val[i]), din[colval[i] - 1]));
+      }
+    }
+    for (int o = L >> 1; o > 0; o >>= 1) acc = jmin(acc, shfl_xor(acc, o));
+    bool ch = false;
+    if (row < m && lane == 0) {
+      const TD dv = din[row];
+      const TD nv = jmin(dv, acc);
+      dout[row] = nv;
+      ch = nv < dv;
+    }
+    count_changes(ch, changed);
+  }
+}
+
+// ---------------------------------------------------------- one source, SELL
+template <class TA, class TD>
+__global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_blocks,
+                                                     const TD* __restrict__ din,
+                                                     TD* __restrict__ dout,
+                                                     unsigned int* changed) {
+  if ((int64_t)blockIdx.x < n_sell_blocks) {
+    int64_t row, base, len;
+    bool has_len;
+    bool ch = false;
+    const int64_t slot = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (sell_slot(g, slot, row, base, len, has_len)) {
+      const TA* val = (const TA*)g.val;
+      TD acc = Lim<TD>::max();
+      for (int64_t k = 0; k < len; ++k) {
+        const int64_t i = base + k * g.C;
+        const int32_t col = g.colval[i];
+        if (!has_len && col == -1) break;
+        acc = jmin(acc, sadd(TD(val[i]), din[col - 1]));
+      }
+      const TD dv = din[row];
+      const TD nv = jmin(dv, acc);
+      dout[row] = nv;
+      ch = nv < dv;
+    }
+    count_changes(ch, changed);
+    return;
+  }
+  // one block per heavy row
+  __shared__ TD red[kT / 64];
+  const int64_t h = (int64_t)blockIdx.x - n_sell_blocks;
+  const TA* hv = (const TA*)g.heavy_val;
+  TD acc = Lim<TD>::max();
+  for (int64_t i = g.heavy_ptr[h] + threadIdx.x; i < g.heavy_ptr[h + 1]; i += kT)
+    acc = jmin(acc, sadd(TD(hv[i]), din[g.heavy_col[i] - 1]));
+  for (int o = 32; o > 0; o >>= 1) acc = jmin(acc, shfl_xor(acc, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kT / 64; ++w) acc = jmin(acc, red[w]);
+    const int64_t row = g.heavy_row[h] - 1;
+    const TD dv = din[row];
+    const TD nv = jmin(dv, acc);
+    dout[row] = nv;
+    if (nv < dv) atomicAdd(changed, 1u);
+  }
+}
+
+// -------------------------------------------------- k sources, row-major set
+// Rows of A_T as (entry list) for either format: CSR row, SELL slot, or a
+// heavy row.  A group of Lk lanes (one per source, Lk >= k when k <= 64)
+// walks the row; heavy rows get a whole block whose groups split the entries.
+template <class TA, class TD>
+__global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t* __restrict__ rowptr,
+                                                      SellView g, int64_t n_light_groups,
+                                                      int64_t m, int64_t k, int logLk,
+                                                      const TD* __restrict__ din,
+                                                      TD* __restrict__ dout,
+                                                      unsigned int* changed) {
+  const int Lk = 1 << logLk;
+  const int wl = threadIdx.x & 63;
+  const int lane = wl & (Lk - 1);
+  const int64_t jblocks = (k + Lk - 1) >> logLk;
+  const int64_t gpb = kT >> logLk;  // groups per block
+  const int64_t light_blocks = (n_light_groups * jblocks + gpb - 1) / gpb;
+  if ((int64_t)blockIdx.x < light_blocks) {
+    const int64_t q = (int64_t)blockIdx.x * gpb + (threadIdx.x >> logLk);
+    const int64_t unit = q / jblocks;
+    const int64_t j = (q - unit * jblocks) * Lk + lane;
+    bool ch = false;
+    if (unit < n_light_groups && j < k) {
+      TD acc = Lim<TD>::max();
+      int64_t row = -1;
+      if (format == GG_CSR) {
+        row = unit;
+        const TA* val = (const TA*)g.val;
+        for (int32_t i = rowptr[row] - 1; i < rowptr[row + 1] - 1; ++i)
+          acc = jmin(acc, sadd(TD(val[i]), din[(int64_t)(g.colval[i] - 1) * k + j]));
+      } else {
+        int64_t base, len;
+        bool has_len;
+        if (sell_slot(g, unit, row, base, len, has_len)) {
+          const TA* val = (const TA*)g.val;
+          for (int64_t kk = 0; kk < len; ++kk) {
+            const int64_t i = base + kk * g.C;
+            const int32_t col = g.colval[i];
+            if (!has_len && col == -1) break;
+            acc = jmin(acc, sadd(TD(val[i]), din[(int64_t)(col - 1) * k + j]));
+          }
+        } else {
+          row = -1;
+        }
+      }
+      if (row >= 0) {
+        const TD dv = din[row * k + j];
+        const TD nv = jmin(dv, acc);
+        dout[row * k + j] = nv;
+        ch = nv < dv;
+      }
+    }
+    count_changes(ch, changed);
+    return;
+  }
+  // heavy rows: block (h, jblock); the block's groups split the row's entries
+  extern __shared__ unsigned char smem[];
+  TD* red = (TD*)smem;  // [gpb][Lk]
+  const int64_t hb = (int64_t)blockIdx.x - light_blocks;
+  const int64_t h = hb / jblocks;
+  const int64_t j = (hb - h * jblocks) * Lk + lane;
+  const int64_t grp = threadIdx.x >> logLk;
+  const TA* hv = (const TA*)g.heavy_val;
+  TD acc = Lim<TD>::max();
+  if (j < k)
+    for (int64_t i = g.heavy_ptr[h] + grp; i < g.heavy_ptr[h + 1]; i += gpb)
+      acc = jmin(acc, sadd(TD(hv[i]), din[(int64_t)(g.heavy_col[i] - 1) * k + j]));
+  red[grp * Lk + lane] = acc;
+  __syncthreads();
+  if (grp == 0) {
+    for (int64_t o = 1; o < gpb; ++o) acc = jmin(acc, red[o * Lk + lane]);
+    if (j < k) {
+      const int64_t row = g.heavy_row[h] - 1;
+      const TD dv = din[row * k + j];
+      const TD nv = jmin(dv, acc);
+      dout[row * k + j] = nv;
+      if (nv < dv) atomicAdd(changed, 1u);
+    }
+  }
+}
+
+template <class TD>
+__global__ void bf_multi_seed_kernel(TD* d, const int32_t* sources, int64_t k) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k) d[(int64_t)(sources[i] - 1) * k + i] = TD(0);
+}
+
+template <class TD>
+__global__ void fill_td_kernel(TD* p, int64_t n, TD v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <class TD>
+__global__ void seed_one_kernel(TD* p, int64_t s) {
+  p[s] = TD(0);
+}
+
+// row-major m x k -> column-major m x k
+template <class TD>
+__global__ void rm_to_cm_kernel(const TD* __restrict__ src, TD* __restrict__ dst, int64_t m,
+                                int64_t k) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m * k) return;
+  const int64_t r = i / k, j = i - r * k;
+  dst[r + j * m] = src[i];
+}
+
+__global__ void zero_u32_kernel(unsigned int* p) { *p = 0u; }
+
+static int log_lanes_for(double avg) {
+  int lg = 0;
+  while (lg < 6 && (double)(1 << lg) < avg) ++lg;
+  return lg;
+}
+
+// Dist type of shortest_path: Int32 for Bool values, else the value type
+// (src/algorithms/shortest_path.jl:18).
+static int dist_type_for(int val_dtype) { return val_dtype == GG_BOOL ? GG_INT32 : val_dtype; }
+
+template <class F> static int with_ad(int ta, int td, F&& f) {
+  if (td != dist_type_for(ta))
+    return set_error(GG_ERR_ARGUMENT, "dist element type must be %s of the matrix values",
+                     ta == GG_BOOL ? "Int32 for Bool" : "the type");
+  return with_type(ta, [&](auto a) -> int {
+    typedef typename decltype(a)::type TA;
+    if constexpr (std::is_same<TA, jbool>::value) return f(a, Tag<int32_t>{});
+    else return f(a, Tag<TA>{});
+  });
+}
+
+static int read_u32(gg_ctx* ctx, const unsigned int* d, unsigned int* h) {
+  GG_HIP(hipMemcpyAsync(ctx->host_pinned, d, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  GG_HIP(hipStreamSynchronize(ctx->stream));
+  *h = *(unsigned*)ctx->host_pinned;
+  return GG_OK;
+}
+
+// Iterate until no row changes; the check reads a device counter every
+// iteration for the first 16 and then every 8 (extra iterations are no-ops).
+template <class Step>
+static int iterate(gg_ctx* ctx, unsigned int* changed, int64_t max_iter, Step&& step,
+                   int64_t* iters, int64_t* parity) {
+  int64_t it = 0;
+  while (true) {
+    ++it;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, ctx->stream, changed);
+    GG_TRY(step(it));
+    if (it <= 16 || it % 8 == 0) {
+      unsigned h = 0;
+      GG_TRY(read_u32(ctx, changed, &h));
+      if (h == 0) break;
+    }
+    if (max_iter > 0 && it >= max_iter) {
+      if (iters) *iters = it;
+      *parity = it;
+      return set_error(GG_ERR_NOT_CONVERGED, "no fixpoint after %lld iterations (negative cycle?)",
+                       (long long)it);
+    }
+  }
+  if (iters) *iters = it;
+  *parity = it;
+  return GG_OK;
+}
+
+int sssp_one(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist, int dist_dtype,
+             int64_t* iterations) {
+  return with_ad(A->val_dtype, dist_dtype, [&](auto a, auto d) -> int {
+    typedef typename decltype(a)::type TA;
+    typedef typename decltype(d)::type TD;
+    const int64_t m = A->m;
+    void *tmp, *cnt;
+    GG_TRY(ws_get(ctx, WS_SSSP0, sizeof(TD) * m, &tmp));
+    GG_TRY(ws_get(ctx, WS_FLAG, 64, &cnt));
+    TD* buf[2] = {(TD*)dist, (TD*)tmp};
+    hipLaunchKernelGGL(fill_td_kernel<TD>, dim3(blocks_for(m, kT)), dim3(kT), 0, ctx->stream,
+                       buf[0], m, Lim<TD>::max());
+    hipLaunchKernelGGL(seed_one_kernel<TD>, dim3(1), dim3(1), 0, ctx->stream, buf[0], source - 1);
+    const int lg = log_lanes_for(m ? (double)A->nnz / m : 1.0);
+    SellView g = sell_view(A);
+    const int64_t nsb = (g.n_slots + kT - 1) / kT;
+    auto step = [&](int64_t it) -> int {
+      const TD* din = buf[(it - 1) & 1];
+      TD* dout = buf[it & 1];
+      HotTimer ht(ctx);
+      if (A->format == GG_CSR) {
+        unsigned nb = blocks_for(m << lg, kT);
+        const unsigned cap = (unsigned)ctx->num_cus * 32;
+        if (nb > cap) nb = cap;
+        hipLaunchKernelGGL((bf_csr_kernel<TA, TD>), dim3(nb), dim3(kT), 0, ctx->stream, A->ptr,
+                           A->idx, (const TA*)A->val, m, lg, din, dout, (unsigned*)cnt);
+      } else {
+        hipLaunchKernelGGL((bf_sell_kernel<TA, TD>), dim3((unsigned)(nsb + g.n_heavy)), dim3(kT),
+                           0, ctx->stream, g, nsb, din, dout, (unsigned*)cnt);
+      }
+      GG_HIP(hipGetLastError());
+      return GG_OK;
+    };
+    int64_t last = 0;
+    int st = iterate(ctx, (unsigned*)cnt, m + 16, step, iterations, &last);
+    if (st == GG_OK && (last & 1))  // the final buffer is the workspace one
+      GG_HIP(hipMemcpyAsync(dist, tmp, sizeof(TD) * m, hipMemcpyDeviceToDevice, ctx->stream));
+    GG_HIP(hipStreamSynchronize(ctx->stream));
+    return st;
+  });
+}
+
+// k-source Bellman-Ford into a row-major working set; result written
+// column-major into dist (or row-major when rowmajor_out).
+int sssp_multi(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_host, int64_t k,
+               void* dist, int dist_dtype, bool rowmajor_out, int64_t* iterations) {
+  return with_ad(A->val_dtype, dist_dtype, [&](auto a, auto d) -> int {
+    typedef typename decltype(a)::type TA;
+    typedef typename decltype(d)::type TD;
+    const int64_t m = A->m;
+    void *b0, *b1, *cnt, *src;
+    GG_TRY(ws_get(ctx, WS_MS0, sizeof(TD) * m * k, &b0));
+    GG_TRY(ws_get(ctx, WS_MS1, sizeof(TD) * m * k, &b1));
+    GG_TRY(ws_get(ctx, WS_MS2, sizeof(int32_t) * k, &src));
+    GG_TRY(ws_get(ctx, WS_FLAG, 64, &cnt));
+    GG_HIP(hipMemcpyAsync(src, sources_host, sizeof(int32_t) * k, hipMemcpyHostToDevice, ctx->stream));
+    TD* buf[2] = {(TD*)b0, (TD*)b1};
+    hipLaunchKernelGGL(fill_td_kernel<TD>, dim3(blocks_for(m * k, kT)), dim3(kT), 0, ctx->stream,
+                       buf[0], m * k, Lim<TD>::max());
+    hipLaunchKernelGGL(bf_multi_seed_kernel<TD>, dim3(blocks_for(k, kT)), dim3(kT), 0,
+                       ctx->stream, buf[0], (const int32_t*)src, k);
+    int lg = 0;
+    while (lg < 6 && (1LL << lg) < k) ++lg;
+    const int64_t Lk = 1LL << lg;
+    const int64_t jblocks = (k + Lk - 1) / Lk;
+    const int64_t gpb = kT / Lk;
+    SellView g = sell_view(A);
+    if (A->format == GG_CSR) { g.colval = A->idx; g.val = A->val; g.n_heavy = 0; }
+    const int64_t n_light = A->format == GG_CSR ? m : g.n_slots;
+    const int64_t light_blocks = (n_light * jblocks + gpb - 1) / gpb;
+    const int64_t nb = light_blocks + g.n_heavy * jblocks;
+    if (nb > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
+    auto step = [&](int64_t it) -> int {
+      HotTimer ht(ctx);
+      hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT),
+                         sizeof(TD) * kT, ctx->stream, A->format, A->ptr, g, n_light, m, k, lg,
+                         buf[(it - 1) & 1], buf[it & 1], (unsigned*)cnt);
+      GG_HIP(hipGetLastError());
+      return GG_OK;
+    };
+    int64_t last = 0;
+    int st = iterate(ctx, (unsigned*)cnt, m + 16, step, iterations, &last);
+    const TD* fin = buf[last & 1];
+    if (st == GG_OK) {
+      if (rowmajor_out) {
+        GG_HIP(hipMemcpyAsync(dist, fin, sizeof(TD) * m * k, hipMemcpyDeviceToDevice, ctx->stream));
+      } else {
+        hipLaunchKernelGGL(rm_to_cm_kernel<TD>, dim3(blocks_for(m * k, kT)), dim3(kT), 0,
+                           ctx->stream, fin, (TD*)dist, m, k);
+        GG_HIP(hipGetLastError());
+      }
+    }
+    GG_HIP(hipStreamSynchronize(ctx->stream));
+    return st;
+  });
+}
+
+static int check_sp(const gg_matrix* A, int64_t source) {
+  GG_TRY(check_matrix(A));
+  if (A->format == GG_CSC) return set_error(GG_ERR_ARGUMENT, "Bellman-Ford runs on CSR or SELL A_T");
+  if (A->m != A->n) return set_error(GG_ERR_DIMENSION, "A_T must be square");
+  if (source < 1 || source > A->m)
+    return set_error(GG_ERR_BOUNDS, "source %lld outside 1:%lld", (long long)source, (long long)A->m);
+  return GG_OK;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" {
+
+int gg_shortest_path(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, void* dist,
+                     int dist_dtype, int64_t* iterations) {
+  if (!ctx || !dist) return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  clear_error();
+  GG_TRY(check_sp(A_T, source));
+  timer_begin(ctx);
+  int st = sssp_one(ctx, A_T, source, dist, dist_dtype, iterations);
+  timer_end(ctx);
+  return st;
+}
+
+int gg_shortest_path_multi(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources, int64_t k,
+                           void* dist, int dist_dtype, int64_t* iterations) {
+  if (!ctx || !dist || (k > 0 && !sources)) return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  clear_error();
+  if (k < 1) return set_error(GG_ERR_ARGUMENT, "need at least one source");
+  for (int64_t i = 0; i < k; ++i) GG_TRY(check_sp(A_T, sources[i]));
+  timer_begin(ctx);
+  int st = sssp_multi(ctx, A_T, sources, k, dist, dist_dtype, false, iterations);
+  timer_end(ctx);
+  return st;
+}
+
+int gg_mssp_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources, int64_t k,
+                      void* dist, int dist_dtype, int64_t* iterations) {
+  return gg_shortest_path_multi(ctx, A_T, sources, k, dist, dist_dtype, iterations);
+}
+
+}  // extern "C"

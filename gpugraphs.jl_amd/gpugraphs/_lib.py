@@ -1,0 +1,204 @@
+"""ctypes binding of libgpugraphs_hip.so (the C ABI of include/gpugraphs.h).
+
+The library is loaded from ``gpugraphs.jl_amd/lib``; a missing library is an
+error, never a silent fallback.  Status codes become the exception types the
+reference throws for the same conditions (see ``errors``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libgpugraphs_hip.so")
+
+# gg_dtype
+BOOL, INT32, INT64, FLOAT32, FLOAT64, UINT64 = 1, 2, 3, 4, 5, 6
+# gg_format
+CSR, SELL, CSC = 1, 2, 3
+# gg_layout
+COL_MAJOR, ROW_MAJOR = 0, 1
+FLAG_UNIT_PATTERN = 1
+
+
+class DimensionMismatch(ValueError):
+    """Julia DimensionMismatch."""
+
+
+class ArgumentError(ValueError):
+    """Julia ArgumentError."""
+
+
+class BoundsError(IndexError):
+    """Julia BoundsError."""
+
+
+class MethodError(TypeError):
+    """No kernel for this operator / element-type combination."""
+
+
+class HIPError(RuntimeError):
+    """A HIP runtime call failed."""
+
+
+class NotConvergedError(RuntimeError):
+    """Bellman-Ford reached its iteration cap (negative cycle)."""
+
+
+class BudgetExceeded(RuntimeError):
+    """A request does not fit the serving memory budget."""
+
+
+_STATUS_EXC = {1: DimensionMismatch, 2: ArgumentError, 3: AssertionError, 4: BoundsError,
+               5: MethodError, 6: HIPError, 7: MemoryError, 8: NotConvergedError,
+               9: BudgetExceeded}
+
+
+class GGMatrix(C.Structure):
+    _fields_ = [
+        ("format", C.c_int32), ("val_dtype", C.c_int32),
+        ("m", C.c_int64), ("n", C.c_int64), ("nnz", C.c_int64),
+        ("ptr", C.c_void_p), ("idx", C.c_void_p), ("val", C.c_void_p),
+        ("slice_size", C.c_int64), ("nslices", C.c_int64), ("n_stored", C.c_int64),
+        ("perm", C.c_void_p), ("slot_len", C.c_void_p),
+        ("n_heavy", C.c_int64), ("heavy_row", C.c_void_p), ("heavy_ptr", C.c_void_p),
+        ("heavy_seg", C.c_void_p), ("heavy_seg_len", C.c_int64), ("heavy_nseg", C.c_int64),
+        ("heavy_col", C.c_void_p), ("heavy_val", C.c_void_p), ("flags", C.c_int64),
+    ]
+
+
+class GGSellHost(C.Structure):
+    _fields_ = [
+        ("m", C.c_int64), ("n", C.c_int64), ("nnz", C.c_int64), ("slice_size", C.c_int64),
+        ("sigma", C.c_int64), ("nslices", C.c_int64), ("n_stored", C.c_int64),
+        ("n_slots", C.c_int64), ("val_dtype", C.c_int32),
+        ("slice_ptr", C.c_void_p), ("colval", C.c_void_p), ("nzval", C.c_void_p),
+        ("perm", C.c_void_p), ("slot_len", C.c_void_p),
+        ("n_heavy", C.c_int64), ("heavy_nnz", C.c_int64), ("heavy_seg_len", C.c_int64),
+        ("n_seg", C.c_int64),
+        ("heavy_row", C.c_void_p), ("heavy_ptr", C.c_void_p), ("heavy_seg", C.c_void_p),
+        ("heavy_col", C.c_void_p), ("heavy_val", C.c_void_p),
+    ]
+
+
+class GGCsrHost(C.Structure):
+    _fields_ = [("m", C.c_int64), ("n", C.c_int64), ("nnz", C.c_int64),
+                ("val_dtype", C.c_int32), ("rowptr", C.c_void_p), ("colval", C.c_void_p),
+                ("nzval", C.c_void_p)]
+
+
+# every symbol of include/gpugraphs.h with its ctypes signature
+_vp, _i64, _i32, _u64, _d = C.c_void_p, C.c_int64, C.c_int, C.c_uint64, C.c_double
+_PM = C.POINTER(GGMatrix)
+SIGNATURES = {
+    "gg_abi_version": ([], _i32),
+    "gg_status_string": ([_i32], C.c_char_p),
+    "gg_last_error": ([], C.c_char_p),
+    "gg_ctx_create": ([_i32, _vp, C.POINTER(_vp)], _i32),
+    "gg_ctx_destroy": ([_vp], _i32),
+    "gg_ctx_synchronize": ([_vp], _i32),
+    "gg_ctx_set_stream": ([_vp, _vp], _i32),
+    "gg_malloc": ([_vp, C.c_size_t, C.POINTER(_vp)], _i32),
+    "gg_free": ([_vp, _vp], _i32),
+    "gg_memcpy_htod": ([_vp, _vp, _vp, C.c_size_t], _i32),
+    "gg_memcpy_dtoh": ([_vp, _vp, _vp, C.c_size_t], _i32),
+    "gg_ctx_enable_timing": ([_vp, _i32], _i32),
+    "gg_ctx_last_kernel_ms": ([_vp, C.POINTER(_d)], _i32),
+    "gg_ctx_hot_kernel_stats": ([_vp, C.POINTER(_d), C.POINTER(_i64), _i32], _i32),
+    "gg_csr_check": ([_i64, _i64, _vp, _i64, _vp, _i64, _i64], _i32),
+    "gg_sell_check": ([_i64, _i64, _i64, _i64, _vp, _i64, _i64], _i32),
+    "gg_sell_build": ([_i64, _i64, _vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64,
+                       C.POINTER(C.POINTER(GGSellHost))], _i32),
+    "gg_sell_host_free": ([C.POINTER(GGSellHost)], None),
+    "gg_csr_host_free": ([C.POINTER(GGCsrHost)], None),
+    "gg_gen_rmat": ([_i32, _i32, _d, _d, _d, _u64, _i32, _i32, _i32,
+                     C.POINTER(C.POINTER(GGCsrHost))], _i32),
+    "gg_gen_grid": ([_i64, _i64, _d, C.c_int32, C.c_int32, _u64, _i32,
+                     C.POINTER(C.POINTER(GGCsrHost))], _i32),
+    "gg_gen_uniform_rows": ([_i64, _i64, _u64, _i32, C.POINTER(C.POINTER(GGCsrHost))], _i32),
+    "gg_gen_random_graph": ([_i64, _i64, _u64, _i32, _i32, C.c_int32, C.c_int32,
+                             C.POINTER(C.POINTER(GGCsrHost))], _i32),
+    "gg_spmv": ([_vp, _vp, _i32, _i64, _PM, _vp, _i32, _i64, _i32, _i32, _i32, _vp, _i32,
+                 _i64, _i64, _i64], _i32),
+    "gg_spmm": ([_vp, _vp, _i32, _i64, _i64, _i32, _PM, _vp, _i32, _i64, _i64, _i32, _i32,
+                 _i32, _i32, _vp, _i32, _i64, _i64, _i64], _i32),
+    "gg_spmspv": ([_vp, _vp, _vp, C.POINTER(_i64), _i32, _i64, _PM, _vp, _vp, _i64, _i32, _i64,
+                   _i32, _i32, _vp, _i32, _i64, _i32], _i32),
+    "gg_bfs_distances": ([_vp, _PM, _i64, _vp, _i32], _i32),
+    "gg_bfs_parents": ([_vp, _PM, _i64, _vp, _i64], _i32),
+    "gg_bfs_distances_do": ([_vp, _PM, _PM, _i64, _vp, _d, _d], _i32),
+    "gg_shortest_path": ([_vp, _PM, _i64, _vp, _i32, C.POINTER(_i64)], _i32),
+    "gg_shortest_path_multi": ([_vp, _PM, _vp, _i64, _vp, _i32, C.POINTER(_i64)], _i32),
+    "gg_msbfs_distances": ([_vp, _PM, _vp, _i64, _vp, C.POINTER(_i64)], _i32),
+    "gg_mssp_distances": ([_vp, _PM, _vp, _i64, _vp, _i32, C.POINTER(_i64)], _i32),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """The loaded library (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise ImportError(
+                        f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                        "(make -C gpugraphs.jl_amd/csrc)")
+                L = C.CDLL(LIB_PATH)
+                for name, (args, res) in SIGNATURES.items():
+                    f = getattr(L, name)
+                    f.argtypes = args
+                    f.restype = res
+                _lib = L
+    return _lib
+
+
+def check(status: int, what: str = ""):
+    """Raise the exception that matches a gg_status."""
+    if status == 0:
+        return
+    msg = lib().gg_last_error().decode(errors="replace")
+    exc = _STATUS_EXC.get(status, RuntimeError)
+    raise exc(f"{what}: {msg}" if what else msg)
+
+
+# ------------------------------------------------------------------ contexts
+_ctxs: dict = {}
+
+
+def context(device: int = 0, stream=None):
+    """gg_ctx bound to (device, stream); the torch current stream by default."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("GPUGraphs kernels need a ROCm GPU; none is visible")
+    if stream is None:
+        stream = torch.cuda.current_stream(device)
+    key = (device, int(stream.cuda_stream))
+    ctx = _ctxs.get(key)
+    if ctx is None:
+        out = C.c_void_p()
+        check(lib().gg_ctx_create(device, C.c_void_p(key[1]), C.byref(out)), "gg_ctx_create")
+        ctx = out
+        _ctxs[key] = ctx
+    return ctx
+
+
+def last_kernel_ms(ctx) -> float:
+    ms = C.c_double()
+    check(lib().gg_ctx_last_kernel_ms(ctx, C.byref(ms)))
+    return ms.value
+
+
+def enable_timing(ctx, on: bool = True):
+    check(lib().gg_ctx_enable_timing(ctx, int(on)))
+
+
+def hot_kernel_stats(ctx, reset: bool = True):
+    """(summed ms, launches) of hot-kernel launches since the last reset."""
+    ms, n = C.c_double(), C.c_int64()
+    check(lib().gg_ctx_hot_kernel_stats(ctx, C.byref(ms), C.byref(n), int(reset)))
+    return ms.value, n.value

@@ -1,0 +1,100 @@
+"""Seeded synthetic graphs from the library's generators (gen.hip).
+
+Each returns a ``HostCSR`` of A_T (row v = in-neighbours of v, 1-based Int32
+arrays, sorted, deduplicated) that converts to the storage types.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import GGCsrHost, check, lib
+from .storage import _CODE2NP, dtype_code
+
+UNIT, UNIFORM01, INTEGER = 0, 1, 2
+
+
+@dataclass
+class HostCSR:
+    m: int
+    n: int
+    rowptr: np.ndarray  # int32 [m+1], 1-based
+    colval: np.ndarray  # int32 [nnz], 1-based
+    nzval: np.ndarray
+
+    @property
+    def nnz(self):
+        return len(self.colval)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.nzval, self.colval - 1, self.rowptr.astype(np.int64) - 1),
+                             shape=(self.m, self.n))
+
+    def to_csr(self, backend):
+        from .storage import SparseGPUMatrixCSR
+        return SparseGPUMatrixCSR(self.m, self.n, self.rowptr, self.colval, self.nzval, backend)
+
+    def to_sell(self, backend, slice_size=64, sigma=4096, heavy_threshold=0, heavy_seg_len=4096):
+        from .storage import SparseGPUMatrixSELL
+        M = SparseGPUMatrixSELL.__new__(SparseGPUMatrixSELL)
+        M._build(self.m, self.n, self.rowptr, self.colval, np.ascontiguousarray(self.nzval),
+                 backend, slice_size, sigma, heavy_threshold, heavy_seg_len)
+        return M
+
+    def degrees(self):
+        return np.diff(self.rowptr.astype(np.int64))
+
+
+def _take(out) -> HostCSR:
+    try:
+        h = out.contents
+        dt = _CODE2NP[h.val_dtype]
+
+        def arr(p, count, t):
+            if count == 0:
+                return np.zeros(0, t)
+            nb = count * np.dtype(t).itemsize
+            return np.frombuffer((C.c_char * nb).from_address(p), dtype=t).copy()
+
+        return HostCSR(int(h.m), int(h.n), arr(h.rowptr, h.m + 1, np.int32),
+                       arr(h.colval, h.nnz, np.int32),
+                       arr(h.nzval, h.nnz, np.uint8 if dt == np.bool_ else dt).astype(dt))
+    finally:
+        lib().gg_csr_host_free(out)
+
+
+def rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, permute=True, dtype=np.bool_,
+         values=UNIT) -> HostCSR:
+    """R-MAT graph (symmetrised, no self-loops or duplicates, ids permuted)."""
+    out = C.POINTER(GGCsrHost)()
+    check(lib().gg_gen_rmat(scale, edge_factor, a, b, c, seed, int(permute), dtype_code(dtype),
+                            values, C.byref(out)), "rmat")
+    return _take(out)
+
+
+def grid(rows, cols, long_range_frac=0.01, wmin=1, wmax=1000, seed=1, dtype=np.float32) -> HostCSR:
+    """4-neighbour grid plus random long-range edges; independent integer
+    weights per direction in [wmin, wmax]."""
+    out = C.POINTER(GGCsrHost)()
+    check(lib().gg_gen_grid(rows, cols, long_range_frac, wmin, wmax, seed, dtype_code(dtype),
+                            C.byref(out)), "grid")
+    return _take(out)
+
+
+def uniform_rows(n, per_row=16, seed=1, dtype=np.float32) -> HostCSR:
+    """n x n, exactly per_row distinct uniform columns per row, values U[0,1)."""
+    out = C.POINTER(GGCsrHost)()
+    check(lib().gg_gen_uniform_rows(n, per_row, seed, dtype_code(dtype), C.byref(out)),
+          "uniform_rows")
+    return _take(out)
+
+
+def random_graph(n, degree=16, seed=1, dtype=np.bool_, values=UNIT, wmin=1, wmax=1000) -> HostCSR:
+    """n*degree/2 uniform vertex pairs, symmetrised."""
+    out = C.POINTER(GGCsrHost)()
+    check(lib().gg_gen_random_graph(n, degree, seed, dtype_code(dtype), values, wmin, wmax,
+                                    C.byref(out)), "random_graph")
+    return _take(out)

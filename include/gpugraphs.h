@@ -118,9 +118,17 @@ typedef struct gg_matrix {
   const int64_t* heavy_ptr; /* [n_heavy+1] 0-based offsets */
   const int32_t* heavy_seg; /* [n_heavy+1] prefix count of segments */
   int64_t heavy_seg_len;
+  int64_t heavy_nseg;       /* heavy_seg[n_heavy] */
   const int32_t* heavy_col; /* 1-based columns */
   const void* heavy_val;
+  int64_t flags;            /* gg_matrix_flags */
 } gg_matrix;
+
+typedef enum gg_matrix_flags {
+  /* Every stored value is Bool true or an odd integer, so (value & x) == x
+   * for x in {0,1}: or-and BFS may skip loading values. */
+  GG_FLAG_UNIT_PATTERN = 1
+} gg_matrix_flags;
 
 /* Host-side SELL-C-sigma arrays produced by gg_sell_build (owned by the
  * library until gg_sell_host_free). */
@@ -157,10 +165,15 @@ int gg_malloc(gg_ctx* ctx, size_t bytes, void** out);
 int gg_free(gg_ctx* ctx, void* ptr);
 int gg_memcpy_htod(gg_ctx* ctx, void* dst, const void* src, size_t bytes);
 int gg_memcpy_dtoh(gg_ctx* ctx, void* dst, const void* src, size_t bytes);
-/* Milliseconds the most recent kernel launch of the named family took, timed
- * with HIP events on the context stream when timing is enabled. */
+/* Timing with HIP events on the context stream, when enabled (it adds a
+ * synchronisation after each timed launch):
+ *  - gg_ctx_last_kernel_ms: milliseconds of the most recent entry point;
+ *  - gg_ctx_hot_kernel_stats: summed milliseconds and count of the hot-kernel
+ *    launches (the SpMV / SpMM / traversal kernel of each entry point, not
+ *    its fills and bookkeeping) since the last reset. */
 int gg_ctx_enable_timing(gg_ctx* ctx, int on);
 int gg_ctx_last_kernel_ms(gg_ctx* ctx, double* ms);
+int gg_ctx_hot_kernel_stats(gg_ctx* ctx, double* total_ms, int64_t* launches, int reset);
 
 /* ---------------------------------------------------------------- storage */
 /* Validation of a CSR / CSC triple held in host memory; same checks and order
@@ -178,6 +191,40 @@ int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* co
                   const void* nzval, int val_dtype, int64_t slice_size, int64_t sigma,
                   int64_t heavy_threshold, int64_t heavy_seg_len, gg_sell_host** out);
 void gg_sell_host_free(gg_sell_host* h);
+
+/* ------------------------------------------------------------- generators */
+/* Seeded synthetic graphs as host CSR of A_T (row v = in-neighbours of v,
+ * 1-based, sorted, deduplicated).  Every random draw is a hash of the seed and
+ * the ids it decides, so a graph depends only on its parameters.  The
+ * reference draws its inputs with Julia's sprand / Graphs.jl generators
+ * (test/spmv.jl:12, test/bfs.jl:11-12, benchmark/benchmarks.jl:64-65). */
+typedef struct gg_csr_host {
+  int64_t m, n, nnz;
+  int32_t val_dtype;
+  int32_t* rowptr;  /* [m+1] */
+  int32_t* colval;  /* [nnz] */
+  void* nzval;      /* [nnz] */
+} gg_csr_host;
+void gg_csr_host_free(gg_csr_host* h);
+/* values: 0 = unit (Bool true / 1), 1 = U[0,1) */
+/* R-MAT (Chakrabarti et al. 2004): 2^scale vertices, edge_factor*2^scale
+ * draws with quadrant probabilities a, b, c, 1-a-b-c; ids randomly permuted
+ * when permute; symmetrised; self-loops and duplicates removed. */
+int gg_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                int permute, int val_dtype, int values, gg_csr_host** out);
+/* rows x cols 4-neighbour grid, both directions with independent integer
+ * weights in [wmin, wmax], plus round(long_range_frac * grid edges) random
+ * vertex pairs (u != v) added in both directions with independent weights. */
+int gg_gen_grid(int64_t rows, int64_t cols, double long_range_frac, int32_t wmin, int32_t wmax,
+                uint64_t seed, int val_dtype, gg_csr_host** out);
+/* n x n matrix whose rows hold exactly per_row distinct uniformly drawn
+ * columns, values U[0,1). */
+int gg_gen_uniform_rows(int64_t n, int64_t per_row, uint64_t seed, int val_dtype,
+                        gg_csr_host** out);
+/* Uniform random graph: n*degree/2 uniform vertex pairs, symmetrised.
+ * values: 0 unit, 1 U[0,1), 2 integer weights in [wmin, wmax]. */
+int gg_gen_random_graph(int64_t n, int64_t degree, uint64_t seed, int val_dtype, int values,
+                        int32_t wmin, int32_t wmax, gg_csr_host** out);
 
 /* ---------------------------------------------------------------- kernels */
 /* C = accum(C, A (add.mul) B) on rows selected by mask (mask[r] != 0) and the

@@ -275,7 +275,7 @@ inline T sell_row(const orc_matrix& A, int64_t row, const T* b, const Monoid<T>&
 
 template <class T, class TA, class TC>
 int spmv_typed(TC* c, const orc_matrix& A, const void* bv, int bdt, int add, int mul, int accum,
-               const void* mask, int mdt, int64_t r0, int64_t nrows, int nthreads) {
+               const void* mask, int mdt, int64_t r0, int64_t nrows, int nthreads, bool fast) {
   // B converted to the compute type once (promote_type(Tv, InputType)).
   const int64_t n = A.n;
   std::vector<T> b(n);
@@ -283,7 +283,10 @@ int spmv_typed(TC* c, const orc_matrix& A, const void* bv, int bdt, int add, int
   Monoid<T> mo;
   if (!make_monoid<T>(add, mo)) return S_UNSUP;
   if (!mul_supported<T>(mul) || !accum_supported<T>(accum)) return S_UNSUP;
-  const bool any_fast = (add == OP_ANY) && mask && A.format == F_CSR;
+  // any_dense_masked_csr_spmv_kernel! runs only for a mask without a range
+  // (_dense_mask_gpu_spmv!, src/spmv/CSR.jl:191-205); range + mask takes the
+  // generic kernel (src/spmv/CSR.jl:338-353) where any = max.
+  const bool any_fast = fast && (add == OP_ANY) && mask && A.format == F_CSR;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
   for (int64_t k = 0; k < nrows; ++k) {
     int64_t row = r0 + k;
@@ -317,28 +320,28 @@ int spmv_typed(TC* c, const orc_matrix& A, const void* bv, int bdt, int add, int
 template <class T>
 int spmv_dispatch_a(void* c, const orc_matrix& A, const void* b, int bdt, int add, int mul,
                     int accum, const void* mask, int mdt, int64_t r0, int64_t nrows, int nth,
-                    int cdt) {
+                    int cdt, bool fast) {
   if (cdt != dcode<T>()) return S_ARG;
   switch (A.val_dtype) {
-    case D_BOOL: return spmv_typed<T, jbool, T>((T*)c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth);
-    case D_I32: return spmv_typed<T, int32_t, T>((T*)c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth);
-    case D_I64: return spmv_typed<T, int64_t, T>((T*)c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth);
-    case D_F32: return spmv_typed<T, float, T>((T*)c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth);
-    case D_F64: return spmv_typed<T, double, T>((T*)c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth);
+    case D_BOOL: return spmv_typed<T, jbool, T>((T*)c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, fast);
+    case D_I32: return spmv_typed<T, int32_t, T>((T*)c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, fast);
+    case D_I64: return spmv_typed<T, int64_t, T>((T*)c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, fast);
+    case D_F32: return spmv_typed<T, float, T>((T*)c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, fast);
+    case D_F64: return spmv_typed<T, double, T>((T*)c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, fast);
   }
   return S_UNSUP;
 }
 
 int spmv_any(void* c, int cdt, const orc_matrix& A, const void* b, int bdt, int add, int mul,
-             int accum, const void* mask, int mdt, int64_t r0, int64_t nrows, int nth) {
+             int accum, const void* mask, int mdt, int64_t r0, int64_t nrows, int nth, bool fast) {
   int tc = promote(A.val_dtype, bdt);
   if (mul == OP_SECONDI && cdt == D_I32 && tc != D_F32 && tc != D_F64) tc = D_I32;
   switch (tc) {
-    case D_BOOL: return spmv_dispatch_a<jbool>(c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, cdt);
-    case D_I32: return spmv_dispatch_a<int32_t>(c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, cdt);
-    case D_I64: return spmv_dispatch_a<int64_t>(c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, cdt);
-    case D_F32: return spmv_dispatch_a<float>(c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, cdt);
-    case D_F64: return spmv_dispatch_a<double>(c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, cdt);
+    case D_BOOL: return spmv_dispatch_a<jbool>(c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, cdt, fast);
+    case D_I32: return spmv_dispatch_a<int32_t>(c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, cdt, fast);
+    case D_I64: return spmv_dispatch_a<int64_t>(c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, cdt, fast);
+    case D_F32: return spmv_dispatch_a<float>(c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, cdt, fast);
+    case D_F64: return spmv_dispatch_a<double>(c, A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nth, cdt, fast);
   }
   return S_UNSUP;
 }
@@ -497,7 +500,8 @@ int orc_spmv(void* c, int cdt, int64_t c_len, const orc_matrix* A, const void* b
     if (range_start < 1 || range_start + range_len - 1 > A->m) return S_BOUNDS;
     r0 = range_start; nrows = range_len;
   }
-  return spmv_any(c, cdt, *A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nthreads);
+  return spmv_any(c, cdt, *A, b, bdt, add, mul, accum, mask, mdt, r0, nrows, nthreads,
+                  range_len < 0);
 }
 
 // gpu_spmm! src/spmm/CSR.jl:111-204, src/spmm/SELL.jl:84-124 (col-major B, C).

@@ -1,0 +1,441 @@
+"""GPU parity: the HIP kernels through the C ABI against the CPU oracle on the
+same seeded inputs.  Integer, Bool and min-plus results must be bit-exact;
+fp32 plus-times must be within 1e-5 relative of an FP64 product (and of the
+oracle's fp32 result)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import print_case, random_csr
+import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+INT32_MAX = np.iinfo(np.int32).max
+RTOL = 1e-5  # fp32 plus-times vs FP64 product, relative
+
+
+@pytest.fixture(scope="module")
+def env(oracle):
+    import torch
+    import gpugraphs as gg
+    assert torch.cuda.is_available()
+    gg.lib()
+    return gg, torch, oracle
+
+
+def dev(torch, a):
+    a = np.ascontiguousarray(a)
+    return torch.from_numpy(a).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+FORMATS = ["csr", "sell_ref", "sell_hybrid"]
+
+
+def make(gg, S, fmt):
+    R = gg.ROCBackend(0)
+    if fmt == "csr":
+        return gg.SparseGPUMatrixCSR(S, R)
+    if fmt == "sell_ref":
+        return gg.SparseGPUMatrixSELL(S, R, 32)
+    # sigma-sorted C=64 slices with a heavy partition cut into 16-entry segments
+    return gg.SparseGPUMatrixSELL(S, R, 64, sigma=64, heavy_threshold=8, heavy_seg_len=16)
+
+
+def orc_mat(oracle, S, fmt):
+    A = oracle.from_scipy_csr(S)
+    return oracle.sell_from_csr(A, 32) if fmt != "csr" else A
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+@pytest.mark.parametrize("m,n,p", [(10, 10, 0.5), (1000, 1000, 0.2), (300, 700, 0.05),
+                                   (4096, 4096, 0.004)])
+def test_spmv_plus_times_f32(env, fmt, m, n, p):
+    gg, torch, orc = env
+    print_case("spmv plus-times", fmt, m, n, p)
+    rng = np.random.default_rng(m + n)
+    S = random_csr(rng, m, n, p)
+    x = rng.random(n, dtype=np.float32)
+    A = make(gg, S, fmt)
+    c = dev(torch, np.zeros(m, np.float32))
+    gg.gpu_spmv_(c, A, dev(torch, x))
+    torch.cuda.synchronize()
+    got = host(c)
+    ref64 = S.astype(np.float64) @ x.astype(np.float64)
+    np.testing.assert_allclose(got, ref64, rtol=RTOL, atol=1e-7)
+    co = np.zeros(m, np.float32)
+    orc.spmv(co, orc_mat(orc, S, "csr"), x)
+    np.testing.assert_allclose(got, co, rtol=RTOL, atol=1e-7)
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_spmv_power_law_rows_f32(env, fmt):
+    """Long rows (hubs) accumulate in segments; the tolerance still holds."""
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    print_case("spmv power law", fmt)
+    R = GEN.rmat(15, seed=3, dtype=np.float32, values=GEN.UNIFORM01)
+    S = R.to_scipy()
+    x = np.random.default_rng(1).random(R.n, dtype=np.float32)
+    if fmt == "sell_hybrid":
+        A = R.to_sell(gg.ROCBackend(0), 64, 4096, 256, 512)
+    elif fmt == "sell_ref":
+        A = R.to_sell(gg.ROCBackend(0), 64, 1, 0)
+    else:
+        A = R.to_csr(gg.ROCBackend(0))
+    c = dev(torch, np.zeros(R.m, np.float32))
+    gg.gpu_spmv_(c, A, dev(torch, x))
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(host(c), S.astype(np.float64) @ x.astype(np.float64), rtol=RTOL,
+                               atol=1e-7)
+
+
+SEMIRINGS = [
+    # (add, mul, accum, A dtype, x dtype, out dtype)
+    ("bor", "band", "second", np.bool_, np.bool_, np.bool_),
+    ("bor", "band", "second", np.int32, np.int32, np.int32),
+    ("min", "safe_add", "min", np.int32, np.int32, np.int32),
+    ("min", "safe_add", "min", np.float32, np.float32, np.float32),
+    ("min", "safe_add", "min", np.bool_, np.int32, np.int32),
+    ("min", "add", "second", np.int32, np.int32, np.int32),
+    ("add", "mul", "second", np.int32, np.int32, np.int32),
+    ("add", "mul", "add", np.int64, np.int64, np.int64),
+    ("any", "secondi", "second", np.int32, np.int32, np.int32),
+    ("lor", "land", "second", np.bool_, np.bool_, np.bool_),
+]
+
+
+def _ops(gg, name):
+    return {"bor": gg.GPUGraphs_bor, "band": gg.GPUGraphs_band, "min": gg.GPUGraphs_min,
+            "safe_add": gg.GPUGraphs_safe_add, "add": gg.GPUGraphs_add, "mul": gg.GPUGraphs_mul,
+            "any": gg.GPUGraphs_any, "secondi": gg.GPUGraphs_secondi,
+            "second": gg.GPUGraphs_second, "lor": gg.GPUGraphs_lor,
+            "land": gg.GPUGraphs_land}[name]
+
+
+def _values(rng, dt, size, lo=1, hi=1000):
+    if dt == np.bool_:
+        return rng.random(size) < 0.5
+    if dt == np.float32:
+        return rng.integers(lo, hi, size).astype(np.float32)
+    return rng.integers(lo, hi, size).astype(dt)
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+@pytest.mark.parametrize("sr", SEMIRINGS, ids=lambda s: "-".join(str(x) for x in s[:3]) + f"-{np.dtype(s[3]).name}")
+@pytest.mark.parametrize("variant", ["plain", "mask", "range", "range_mask"])
+def test_spmv_semirings_exact(env, fmt, sr, variant):
+    gg, torch, orc = env
+    add, mul, acc, ta, tx, tc = sr
+    print_case("spmv exact", fmt, sr, variant)
+    rng = np.random.default_rng(hash((fmt, variant, sr[:3])) % 2**32)
+    m = n = 700
+    S = random_csr(rng, m, n, 0.02)
+    S.data = _values(rng, ta, S.nnz)
+    if ta == np.bool_:
+        S.data[:] = True  # sparse() keeps only nonzeros
+    S = S.astype(ta)
+    x = _values(rng, tx, n)
+    if add == "any":
+        x = (rng.random(n) < 0.3).astype(tx)
+    c0 = _values(rng, tc, m) if acc != "second" or variant != "plain" else np.zeros(m, tc)
+    mask = (rng.random(m) < 0.5).astype(np.int32) if "mask" in variant else None
+    rng_ = range(101, 401) if "range" in variant else None
+    A = make(gg, S, fmt)
+    c = dev(torch, c0.copy())
+    kw = dict(mul=_ops(gg, mul), add=_ops(gg, add), accum=_ops(gg, acc))
+    if mask is not None:
+        kw["mask"] = dev(torch, mask)
+    if rng_ is not None:
+        kw["range_"] = rng_
+    if fmt != "csr" and add == "add" and ta in (np.int32, np.int64):
+        pass
+    gg.gpu_spmv_(c, A, dev(torch, x), **kw)
+    torch.cuda.synchronize()
+    co = c0.copy()
+    if co.dtype == np.bool_:
+        co = co.view(np.uint8)
+    orc.spmv(co, orc_mat(orc, S, fmt), x, add=getattr(orc, "OP_" + add.upper()),
+             mul=getattr(orc, "OP_" + mul.upper()), accum=getattr(orc, "OP_" + acc.upper()),
+             mask=mask, range_=None if rng_ is None else (rng_.start, len(rng_)))
+    got = host(c)
+    if got.dtype == np.bool_:
+        got = got.view(np.uint8)
+    np.testing.assert_array_equal(got, co)
+
+
+def test_spmv_errors(env):
+    gg, torch, orc = env
+    A = gg.SparseGPUMatrixCSR(sp.random(30, 40, density=0.1, format="csr", dtype=np.float32),
+                              gg.ROCBackend(0))
+    f32 = lambda k: dev(torch, np.zeros(k, np.float32))
+    with pytest.raises(gg.DimensionMismatch):
+        gg.gpu_spmv_(f32(30), A, f32(41))
+    with pytest.raises(gg.DimensionMismatch):
+        gg.gpu_spmv_(f32(31), A, f32(40))
+    with pytest.raises(gg.ArgumentError):
+        gg.gpu_spmv_(dev(torch, np.zeros(30, np.int32)), A, f32(40))
+    with pytest.raises(gg.DimensionMismatch):
+        gg.gpu_spmv_(f32(30), A, f32(40), mask=dev(torch, np.ones(29, np.int32)))
+    with pytest.raises(gg.BoundsError):
+        gg.gpu_spmv_(f32(30), A, f32(40), range_=range(20, 32))
+    with pytest.raises(gg.MethodError):
+        gg.gpu_spmv_(f32(30), A, f32(40), add=gg.GPUGraphs_max, mul=gg.GPUGraphs_min)
+    with pytest.raises(gg.ArgumentError):
+        gg.gpu_spmv_(f32(30), A, np.zeros(40, np.float32))  # host vector: not the same backend
+
+
+def _cm(torch, a):
+    """Column-major (Julia layout) device copy of a 2-D array."""
+    return dev(torch, np.ascontiguousarray(a.T)).T
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+@pytest.mark.parametrize("k", [1, 5, 20, 33])
+@pytest.mark.parametrize("layout", ["col", "row"])
+def test_spmm_plus_times_and_min_plus(env, fmt, k, layout):
+    gg, torch, orc = env
+    print_case("spmm", fmt, k, layout)
+    rng = np.random.default_rng(k * 3 + len(fmt))
+    S = random_csr(rng, 400, 400, 0.03)
+    B = rng.random((400, k), dtype=np.float32)
+    A = make(gg, S, fmt)
+    mk = (lambda a: _cm(torch, a)) if layout == "col" else (lambda a: dev(torch, a))
+    Cd = mk(np.zeros((400, k), np.float32))
+    gg.gpu_spmm_(Cd, A, mk(B))
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(host(Cd), S.astype(np.float64) @ B.astype(np.float64), rtol=RTOL,
+                               atol=1e-7)
+    # min-plus with integer weights, exact against the oracle
+    Si = S.copy()
+    Si.data = rng.integers(1, 1000, Si.nnz).astype(np.float32)
+    Bi = rng.integers(0, 5000, (400, k)).astype(np.float32)
+    Bi[rng.random((400, k)) < 0.3] = np.inf
+    Ai = make(gg, Si, fmt)
+    C0 = rng.integers(0, 9000, (400, k)).astype(np.float32)
+    Cd = mk(C0)
+    gg.gpu_spmm_(Cd, Ai, mk(Bi), mul=gg.GPUGraphs_safe_add, add=gg.GPUGraphs_min,
+                 accum=gg.GPUGraphs_min)
+    torch.cuda.synchronize()
+    Co = np.asfortranarray(C0.copy())
+    orc.spmm(Co, orc_mat(orc, Si, fmt), np.asfortranarray(Bi), add=orc.OP_MIN,
+             mul=orc.OP_SAFE_ADD, accum=orc.OP_MIN)
+    np.testing.assert_array_equal(host(Cd), Co)
+
+
+def test_spmm_csr_mask_range_and_asserts(env):
+    gg, torch, orc = env
+    rng = np.random.default_rng(7)
+    S = random_csr(rng, 300, 300, 0.05)
+    S.data = rng.integers(1, 100, S.nnz).astype(np.float32)
+    B = rng.integers(0, 100, (300, 6)).astype(np.float32)
+    A = gg.SparseGPUMatrixCSR(S, gg.ROCBackend(0))
+    mask = (rng.random(300) < 0.4).astype(np.int32)
+    for kw, okw in [({"mask": dev(torch, mask)}, {"mask": mask}),
+                    ({"range_": range(50, 151)}, {"range_": (50, 101)})]:
+        C0 = np.full((300, 6), -1.0, np.float32)
+        Cd = _cm(torch, C0)
+        gg.gpu_spmm_(Cd, A, _cm(torch, B), **kw)
+        torch.cuda.synchronize()
+        Co = np.asfortranarray(C0.copy())
+        orc.spmm(Co, orc_mat(orc, S, "csr"), np.asfortranarray(B), **okw)
+        np.testing.assert_array_equal(host(Cd), Co)
+    with pytest.raises(AssertionError):
+        gg.gpu_spmm_(_cm(torch, np.zeros((300, 5), np.float32)), A, _cm(torch, B))
+    with pytest.raises(AssertionError):
+        gg.gpu_spmm_(_cm(torch, np.zeros((300, 6), np.float32)), A, _cm(torch, B),
+                     range_=range(250, 320))
+
+
+def test_csc_spmv(env):
+    gg, torch, orc = env
+    rng = np.random.default_rng(2)
+    for m, p in ((10, 0.5), (1000, 0.2)):  # test/spmv.jl:16-68 sizes
+        S = random_csr(rng, m, m, p)
+        x = rng.random(m, dtype=np.float32)
+        A = gg.SparseGPUMatrixCSC(S, gg.ROCBackend(0))
+        c = dev(torch, np.zeros(m, np.float32))
+        gg.gpu_spmv_(c, A, dev(torch, x))
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(host(c), S.astype(np.float64) @ x, rtol=RTOL, atol=1e-6)
+
+
+@pytest.mark.parametrize("semiring", ["plus_times", "min_plus", "any_secondi", "or_and"])
+def test_spmspv(env, semiring):
+    gg, torch, orc = env
+    print_case("spmspv", semiring)
+    rng = np.random.default_rng(11)
+    S = random_csr(rng, 600, 500, 0.02).tocsc()
+    S.sort_indices()
+    nz = np.sort(rng.choice(500, 60, replace=False)) + 1
+    mask = (rng.random(600) < 0.5).astype(np.int32)
+    if semiring == "plus_times":
+        vals, bv, add, mul, odt = S.data, rng.random(60, dtype=np.float32), "ADD", "MUL", np.float32
+    elif semiring == "min_plus":
+        vals = rng.integers(1, 1000, S.nnz).astype(np.int32)
+        bv, add, mul, odt = rng.integers(0, 9999, 60).astype(np.int32), "MIN", "SAFE_ADD", np.int32
+    elif semiring == "any_secondi":
+        vals, bv, add, mul, odt = np.ones(S.nnz, np.int32), np.ones(60, np.int32), "ANY", "SECONDI", np.int32
+    else:
+        vals, bv, add, mul, odt = np.ones(S.nnz, np.bool_), np.ones(60, np.bool_), "BOR", "BAND", np.bool_
+    S2 = sp.csc_matrix((vals, S.indices, S.indptr), shape=S.shape)
+    A = gg.SparseGPUMatrixCSC(600, 500, S.indptr + 1, S.indices + 1, vals, gg.ROCBackend(0))
+    b = gg.SparseGPUVector(500, nz, bv, gg.ROCBackend(0))
+    ops = {"ADD": gg.GPUGraphs_add, "MUL": gg.GPUGraphs_mul, "MIN": gg.GPUGraphs_min,
+           "SAFE_ADD": gg.GPUGraphs_safe_add, "ANY": gg.GPUGraphs_any,
+           "SECONDI": gg.GPUGraphs_secondi, "BOR": gg.GPUGraphs_bor, "BAND": gg.GPUGraphs_band}
+    for comp in (False, True):
+        out = gg.gpu_spmspv(A, b, mul=ops[mul], add=ops[add], out_dtype=odt,
+                            mask=dev(torch, mask), mask_complement=comp)
+        torch.cuda.synchronize()
+        Ao = orc.csc(600, 500, S.indptr + 1, S.indices + 1, vals)
+        ci, cv = orc.spmspv(Ao, nz, bv, 500, getattr(orc, "OP_" + add), getattr(orc, "OP_" + mul),
+                            np.uint8 if odt == np.bool_ else odt, mask=mask, mask_complement=comp)
+        np.testing.assert_array_equal(host(out.nzind), ci)
+        gv = host(out.nzval)
+        if odt == np.float32:
+            np.testing.assert_allclose(gv, cv, rtol=RTOL)
+        else:
+            np.testing.assert_array_equal(gv.view(np.uint8) if odt == np.bool_ else gv, cv)
+
+
+def _test_graphs():
+    out = []
+    for seed in (1, 2):
+        out.append(("ba30", 30, G.barabasi_albert(30, 3, seed)))
+        out.append(("dm30", 30, G.dorogovtsev_mendes(30, seed)))
+    out.append(("ba2000", 2000, G.barabasi_albert(2000, 3, 5)))
+    out.append(("dm3000", 3000, G.dorogovtsev_mendes(3000, 6)))
+    return out
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+@pytest.mark.parametrize("dtype", [np.bool_, np.int32])
+def test_bfs_distances_vs_oracle(env, fmt, dtype):
+    """test/bfs.jl:10-46 (Bool and Int32 adjacency of BA / DM graphs)."""
+    gg, torch, orc = env
+    for name, n, edges in _test_graphs():
+        print_case("bfs", fmt, np.dtype(dtype).name, name)
+        Adj = G.undirected_adj(n, edges, dtype)
+        AT = Adj.T.tocsr()
+        A = make(gg, AT, fmt)
+        Ao = orc_mat(orc, AT, fmt)
+        for s in (1, n // 2, n):
+            d = gg.bfs_distances(A, s)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(host(d), orc.bfs_distances(Ao, s))
+
+
+def test_bfs_directed_unreachable_and_even_values(env):
+    """Directed graph with unreachable vertices; Int32 values that are even
+    are not edges under or-and (2 & 1 == 0), as in the reference."""
+    gg, torch, orc = env
+    rng = np.random.default_rng(12)
+    Adj = G.random_digraph(rng, 3000, 0.0015, wmax=4, dtype=np.int32)
+    AT = Adj.T.tocsr()
+    for fmt in FORMATS:
+        A = make(gg, AT, fmt)
+        Ao = orc_mat(orc, AT, fmt)
+        for s in (1, 777, 3000):
+            print_case("bfs directed", fmt, s)
+            np.testing.assert_array_equal(host(gg.bfs_distances(A, s)), orc.bfs_distances(Ao, s))
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_bfs_parents_vs_oracle(env, fmt):
+    """bfs_parents bit-exact including the reference's tie-break: CSR the first
+    frontier in-neighbour (min col on sorted rows), SELL the largest."""
+    gg, torch, orc = env
+    for name, n, edges in _test_graphs():
+        print_case("bfs parents", fmt, name)
+        Adj = G.undirected_adj(n, edges, np.int32)
+        AT = Adj.T.tocsr()
+        A = make(gg, AT, fmt)
+        Ao = orc_mat(orc, AT, fmt)
+        for s in (1, n):
+            for batch in (1, 8):
+                p = gg.bfs_parents(A, s, batch_size=batch)
+                torch.cuda.synchronize()
+                np.testing.assert_array_equal(host(p), orc.bfs_parents(Ao, s, batch_size=batch))
+
+
+def test_bfs_direction_optimising_equals_plain(env):
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    R = GEN.rmat(16, seed=4)
+    A = R.to_csr(gg.ROCBackend(0))
+    H = R.to_sell(gg.ROCBackend(0), 64, 4096, 256, 1024)
+    deg = R.degrees()
+    roots = np.nonzero(deg)[0][:5] + 1
+    for s in roots:
+        print_case("bfs do", s)
+        ref = host(gg.bfs_distances(A, int(s)))
+        np.testing.assert_array_equal(host(gg.bfs_distances(H, int(s))), ref)
+        np.testing.assert_array_equal(host(gg.bfs_distances_do(A, int(s))), ref)
+        np.testing.assert_array_equal(host(gg.bfs_distances_do(H, int(s), A_out=A)), ref)
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_shortest_path_vs_oracle(env, fmt):
+    """test/shortest_path.jl:31-80 (integer weights) on digraphs and a grid."""
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    rng = np.random.default_rng(21)
+    cases = [G.random_digraph(rng, 400, 0.01, wmax=1000, dtype=np.float32),
+             G.random_digraph(rng, 400, 0.01, wmax=1000, dtype=np.int32),
+             GEN.grid(40, 50, 0.01, 1, 1000, seed=2).to_scipy().T.tocsr().T]
+    for Adj in cases:
+        AT = sp.csr_matrix(Adj.T)
+        A = make(gg, AT, fmt)
+        Ao = orc_mat(orc, AT, fmt)
+        for s in (1, 123, AT.shape[0]):
+            print_case("sssp", fmt, AT.shape, AT.dtype, s)
+            d = gg.shortest_path(A, s)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(host(d), orc.shortest_path(Ao, s))
+
+
+def test_shortest_path_bool_graph_hops(env):
+    gg, torch, orc = env
+    for name, n, edges in _test_graphs()[:2]:
+        Adj = G.undirected_adj(n, edges, np.bool_)
+        A = gg.SparseGPUMatrixSELL(Adj.T.tocsr(), gg.ROCBackend(0))
+        d = gg.shortest_path(A, 1)
+        torch.cuda.synchronize()
+        assert d.dtype == torch.int32
+        np.testing.assert_array_equal(host(d), host(gg.bfs_distances(A, 1)))
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_shortest_path_multi_vs_oracle(env, fmt):
+    """test/shortest_path.jl:82-126: all n sources (n x n result)."""
+    gg, torch, orc = env
+    rng = np.random.default_rng(4)
+    Adj = G.random_digraph(rng, 60, 0.06, wmax=10, dtype=np.int32)
+    AT = Adj.T.tocsr()
+    A = make(gg, AT, fmt)
+    srcs = np.arange(1, 61, dtype=np.int32)
+    print_case("sssp multi", fmt)
+    D = gg.shortest_path(A, srcs)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(host(D), orc.shortest_path_multi(orc_mat(orc, AT, fmt), srcs))
+
+
+@pytest.mark.parametrize("k", [1, 64, 100, 256])
+def test_multi_source_bfs_equals_single(env, k):
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    R = GEN.rmat(13, seed=9)
+    rng = np.random.default_rng(k)
+    nzr = np.nonzero(R.degrees())[0] + 1
+    srcs = rng.choice(nzr, k, replace=True).astype(np.int32)
+    for A in (R.to_csr(gg.ROCBackend(0)), R.to_sell(gg.ROCBackend(0), 64, 4096, 128, 256)):
+        print_case("msbfs", k, type(A).__name__)
+        D = host(gg.multi_source_bfs(A, srcs))
+        Ao = orc.from_scipy_csr(R.to_scipy())
+        for i in rng.choice(k, min(k, 6), replace=False):
+            np.testing.assert_array_equal(D[:, i], orc.bfs_distances(Ao, int(srcs[i])))

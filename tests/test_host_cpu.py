@@ -1,0 +1,227 @@
+"""CPU tests of the product's host side: the C-ABI library loads and exports
+every symbol of include/gpugraphs.h, the storage constructors reproduce the
+reference's layouts and errors, the SELL-C-sigma hybrid keeps every row, and
+the generators produce the graphs they promise.  No kernel runs here."""
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import GOLDEN, ROOT
+
+import gpugraphs as gg
+from gpugraphs import _lib
+from gpugraphs import generators as G
+
+CPU = gg.CPUBackend()
+
+
+def test_library_exports_every_header_symbol():
+    hdr = open(os.path.join(ROOT, "include", "gpugraphs.h")).read()
+    names = set(re.findall(r"\b(gg_[a-z0-9_]+)\s*\(", hdr))
+    assert len(names) > 25
+    L = gg.lib()
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(_lib.SIGNATURES) == names, set(_lib.SIGNATURES) ^ names
+    assert L.gg_abi_version() == 1
+
+
+def test_status_strings_name_julia_exceptions():
+    L = gg.lib()
+    assert L.gg_status_string(1) == b"DimensionMismatch"
+    assert L.gg_status_string(2) == b"ArgumentError"
+    assert L.gg_status_string(3) == b"AssertionError"
+    assert L.gg_status_string(4) == b"BoundsError"
+
+
+def test_sell_known_answer_through_product_builder():
+    """test/structs.jl:176-190 through gg_sell_build, all three constructor routes."""
+    kat = json.load(open(os.path.join(GOLDEN, "sell_kat.json")))
+    dense = np.array(kat["dense"], np.float32)
+    e = kat["expected"]
+    routes = [gg.SparseGPUMatrixSELL(sp.csc_matrix(dense), CPU),
+              gg.SparseGPUMatrixSELL(sp.csc_matrix(dense.T).T, CPU),
+              gg.SparseGPUMatrixSELL(dense, CPU)]
+    for A in routes:
+        assert A.size() == (4, 4) and A.length() == 16 and A.nnz() == e["nnz"]
+        assert A.slice_size == e["slice_size"] and A.n_stored == e["n_stored"]
+        assert A.slice_ptr.tolist() == e["slice_ptr"]
+        assert A.colval.tolist() == e["colval"]
+        assert A.nzval.tolist() == e["nzval"]
+        for i in range(1, 5):
+            for j in range(1, 5):
+                assert A[i, j] == dense[i - 1, j - 1]
+        with pytest.raises(gg.BoundsError):
+            A[11, 1]
+
+
+def test_csr_constructor_errors_through_product():
+    """test/structs.jl:79-150 through gg_csr_check."""
+    cases = json.load(open(os.path.join(GOLDEN, "csr_errors.json")))["cases"]
+    for c in cases:
+        nz = np.arange(c["nzval_len"], dtype=np.float32)
+        if c["expect"] == "ArgumentError":
+            with pytest.raises(gg.ArgumentError):
+                gg.SparseGPUMatrixCSR(c["m"], c["n"], c["rowptr"], c["colval"], nz, CPU)
+        else:
+            A = gg.SparseGPUMatrixCSR(c["m"], c["n"], c["rowptr"], c["colval"], nz, CPU)
+            assert A.size() == (c["m"], c["n"])
+
+
+def test_csr_constructor_routes_match_fields():
+    """test/structs.jl:30-78: the four routes give the same fields."""
+    rng = np.random.default_rng(0)
+    Acsc = sp.random(10, 10, density=0.5, format="csc", random_state=rng, dtype=np.float32)
+    Acsr_t = Acsc.T.tocsc()  # sparse(transpose(A_csc))
+    ref_rowptr = Acsr_t.indptr + 1
+    ref_colval = Acsr_t.indices + 1
+    ref_nzval = Acsr_t.data
+    B0 = gg.SparseGPUMatrixCSR(10, 10, ref_rowptr, ref_colval, ref_nzval, CPU)
+    B1 = gg.SparseGPUMatrixCSR(Acsc, CPU)
+    B2 = gg.SparseGPUMatrixCSR(Acsr_t.T, CPU)
+    B3 = gg.SparseGPUMatrixCSR(Acsc.toarray(), CPU)
+    for B in (B0, B1, B2, B3):
+        assert B.size() == (10, 10) and B.length() == 100 and B.nnz() == Acsc.nnz
+        assert B.rowptr.tolist() == ref_rowptr.tolist()
+        assert B.colval.tolist() == ref_colval.tolist()
+        np.testing.assert_array_equal(B.nzval, ref_nzval)
+        D = Acsc.toarray()
+        assert all(B[i, j] == D[i - 1, j - 1] for i in range(1, 11) for j in range(1, 11))
+        with pytest.raises(gg.BoundsError):
+            B[11, 1]
+    E = gg.SparseGPUMatrixCSR(np.float32, np.int32, CPU)
+    assert E.size() == (0, 0) and E.nnz() == 0 and E.length() == 0
+    R = gg.sprand_gpu(np.float32, 10, 10, 0.5, CPU, seed=1)
+    assert R.size() == (10, 10)
+
+
+def test_csc_constructor_and_vector():
+    rng = np.random.default_rng(1)
+    Acsc = sp.random(10, 10, density=0.5, format="csc", random_state=rng, dtype=np.float32)
+    A = gg.SparseGPUMatrixCSC(Acsc, CPU)
+    assert A.colptr.tolist() == (Acsc.indptr + 1).tolist()
+    assert A.rowval.tolist() == (Acsc.indices + 1).tolist()
+    D = Acsc.toarray()
+    assert all(A[i, j] == D[i - 1, j - 1] for i in range(1, 11) for j in range(1, 11))
+    v = sp.random(1, 10, density=0.5, format="csr", random_state=rng, dtype=np.float32)
+    for V in (gg.SparseGPUVector(10, v.indices + 1, v.data, CPU), gg.SparseGPUVector(v, CPU),
+              gg.SparseGPUVector(v.toarray()[0], CPU)):
+        assert V.size() == (10,) and V.length() == 10
+        assert V.nzind.tolist() == (v.indices + 1).tolist()
+        np.testing.assert_array_equal(V.nzval, v.data)
+    assert gg.SparseGPUVector(np.float32, np.int32, CPU).length() == 0
+    with pytest.raises(gg.ArgumentError):
+        gg.SparseGPUVector(10, [1, 2], [1.0], CPU)
+    V = gg.SparseGPUVector(10, [2, 5], np.array([1.0, 2.0], np.float32), CPU)
+    with pytest.raises(gg.ArgumentError):
+        V[3] = 1.0  # adding entries is not supported (src/GPUGraphsVector.jl:113-119)
+
+
+def _rows_of_sell(A):
+    """Rebuild {row: sorted (col, val)} from a SELL hybrid's host arrays."""
+    rows = {}
+    sp_, cv, nz = A.slice_ptr, A.colval, A.nzval
+    perm = np.asarray(A.perm)
+    slen = None if A.slot_len is None else np.asarray(A.slot_len)
+    C = A.slice_size
+    for slot, row in enumerate(perm):
+        if row == 0:
+            continue
+        s, r = divmod(slot, C)
+        width = (sp_[s + 1] - sp_[s]) // C
+        ent = []
+        for k in range(width if slen is None else slen[slot]):
+            p = sp_[s] - 1 + k * C + r
+            if cv[p] == -1:
+                break
+            ent.append((cv[p], nz[p]))
+        rows[int(row)] = ent
+    if A.n_heavy:
+        for h, row in enumerate(A.heavy_row):
+            a, b = A.heavy_ptr[h], A.heavy_ptr[h + 1]
+            rows[int(row)] = list(zip(A.heavy_col[a:b], A.heavy_val[a:b]))
+    return rows
+
+
+@pytest.mark.parametrize("sigma,heavy", [(1, 0), (64, 0), (4096, 0), (4096, 100), (7, 30)])
+def test_sell_hybrid_keeps_every_row(sigma, heavy):
+    R = G.rmat(12, seed=5, dtype=np.float32, values=G.UNIFORM01)
+    A = R.to_sell(CPU, 64, sigma, heavy, heavy_seg_len=64)
+    rows = _rows_of_sell(A)
+    S = R.to_scipy()
+    assert len(rows) == R.m
+    for v in range(R.m):
+        a, b = S.indptr[v], S.indptr[v + 1]
+        exp = list(zip(S.indices[a:b] + 1, S.data[a:b]))
+        assert rows[v + 1] == exp
+    if sigma > 1:
+        # inside a window, slots are ordered by decreasing length
+        lens = np.asarray(A.slot_len)
+        for w0 in range(0, len(lens), sigma):
+            seg = [l for l, p in zip(lens[w0:w0 + sigma], A.perm[w0:w0 + sigma]) if p]
+            assert seg == sorted(seg, reverse=True)
+    if heavy:
+        assert all(len(rows[int(r)]) > heavy for r in A.heavy_row)
+        assert A.n_seg == sum(-(-len(rows[int(r)]) // 64) for r in A.heavy_row)
+
+
+def test_rmat_generator_properties():
+    R = G.rmat(14, edge_factor=16, seed=7)
+    S = R.to_scipy()
+    assert R.m == 2**14
+    assert (S != S.T).nnz == 0                       # symmetrised
+    assert S.diagonal().sum() == 0                   # no self-loops
+    assert all(np.all(np.diff(R.colval[R.rowptr[v] - 1:R.rowptr[v + 1] - 1]) > 0)
+               for v in range(0, R.m, 97))           # sorted, no duplicates
+    deg = R.degrees()
+    assert deg.max() > 20 * deg.mean()               # power law (a=.57)
+    R2 = G.rmat(14, edge_factor=16, seed=7)
+    assert np.array_equal(R.colval, R2.colval)       # deterministic
+    assert not np.array_equal(G.rmat(14, seed=8).colval[:1000], R.colval[:1000])
+
+
+def test_grid_generator_properties():
+    g = G.grid(50, 40, long_range_frac=0.01, wmin=1, wmax=1000, seed=3)
+    S = g.to_scipy()
+    und = 50 * 39 + 40 * 49
+    lr = round(0.01 * und)
+    assert S.nnz <= 2 * (und + lr) and S.nnz >= 2 * und
+    assert S.data.min() >= 1 and S.data.max() <= 1000
+    assert np.all(S.data == np.round(S.data))
+    P = (S != 0).astype(np.int8)
+    assert (P != P.T).nnz == 0                        # both directions present
+    assert not np.array_equal(S.data, S.T.tocsr().data)  # independent weights per direction
+
+
+def test_uniform_rows_generator():
+    u = G.uniform_rows(4096, 16, seed=9)
+    assert np.all(np.diff(u.rowptr) == 16)
+    for v in range(0, 4096, 101):
+        c = u.colval[v * 16:(v + 1) * 16]
+        assert np.all(np.diff(c) > 0) and c.min() >= 1 and c.max() <= 4096
+    assert u.nzval.min() >= 0 and u.nzval.max() < 1
+
+
+def test_algebra_identities():
+    assert gg.monoid_neutral(np.float32, gg.GPUGraphs_min) == np.inf
+    assert gg.monoid_absorb(np.int32, gg.GPUGraphs_min) == np.iinfo(np.int32).min
+    assert gg.monoid_neutral(np.int32, gg.GPUGraphs_band) == -1
+    assert gg.monoid_absorb(np.bool_, gg.GPUGraphs_bor) is True
+    assert gg.monoid_absorb(np.int32, gg.GPUGraphs_any) is None
+    imax = np.iinfo(np.int32).max
+    assert gg.GPUGraphs_safe_add(np.int32(5), np.int32(imax)) == imax
+    assert gg.GPUGraphs_safe_add(np.float32(2), np.float32(3)) == 5
+    assert gg.GPUGraphs_secondi(0, 1, 1, 1, 7, 1) == 7
+    with pytest.raises(gg.MethodError):
+        from gpugraphs.algebra import op_code
+        op_code(lambda x, y: x)
+
+
+def test_kernels_refuse_host_arrays():
+    A = gg.SparseGPUMatrixCSR(np.eye(4, dtype=np.float32), CPU)
+    with pytest.raises(RuntimeError):
+        gg.gpu_spmv_(np.zeros(4, np.float32), A, np.ones(4, np.float32))

@@ -140,6 +140,10 @@ def test_algebra_special_cases(oracle):
     oracle.spmv(c, oracle.sell_from_csr(A, 32), curr, add=oracle.OP_ANY, mul=oracle.OP_SECONDI,
                 mask=mask)
     assert c[0] == 4
+    # range + mask on CSR takes the generic kernel: any = max (src/spmv/CSR.jl:338-353)
+    c = np.zeros(1, np.int32)
+    oracle.spmv(c, A, curr, add=oracle.OP_ANY, mul=oracle.OP_SECONDI, mask=mask, range_=(1, 1))
+    assert c[0] == 4
     # or-and on Bool, absorbing true
     A = oracle.csr(2, 2, [1, 3, 3], [1, 2], np.array([1, 1], np.uint8))
     c = np.zeros(2, np.uint8)
