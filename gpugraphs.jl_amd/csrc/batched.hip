@@ -13,27 +13,17 @@
 //
 // dist[:, s] equals the single-source BFS levels from sources[s]: the
 // bit-parallel traversal is k independent level-synchronous BFS runs.
-#include "frontier.hpp"
+#include "batched.hpp"
 
 namespace gg {
 
 constexpr int kM = 256;
 
-struct MsArgs {
-  int64_t m, k;
-  int layout;                  // dist layout: GG_COL_MAJOR (m x k) or GG_ROW_MAJOR
-  const uint64_t* front;       // [m*W]
-  const uint32_t* front_any;   // [ceil(m/32)]
-  uint64_t* next;              // [m*W]
-  uint32_t* next_any;
-  uint64_t* seen;              // [m*W]
-  int32_t* dist;
-  int32_t level;
-  unsigned int* found;         // rows with a new bit this level
-};
+// MsArgs: batched.hpp
+
 
 // Bits of word w that belong to one of the k sources.
-template <int W> __device__ __forceinline__ uint64_t last_mask(int64_t k, int w) {
+inline uint64_t word_mask(int64_t k, int w) {
   const int64_t lo = (int64_t)w * 64;
   if (lo >= k) return 0ull;
   if (lo + 64 <= k) return ~0ull;
@@ -42,12 +32,11 @@ template <int W> __device__ __forceinline__ uint64_t last_mask(int64_t k, int w)
 
 template <int W>
 __device__ __forceinline__ bool finish_row(const MsArgs& a, int64_t row, const uint64_t* acc,
-                                           const uint64_t* sn) {
+                                           const uint64_t* sn, uint64_t* nw) {
   bool any = false;
-  uint64_t nw[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) {
-    nw[w] = acc[w] & ~sn[w] & last_mask<W>(a.k, w);
+    nw[w] = acc[w] & ~sn[w] & a.live[w];
     any |= nw[w] != 0;
   }
 #pragma unroll
@@ -63,8 +52,9 @@ __device__ __forceinline__ bool finish_row(const MsArgs& a, int64_t row, const u
       while (b) {
         const int s = w * 64 + __ffsll((unsigned long long)b) - 1;
         b &= b - 1;
-        if (a.layout == GG_ROW_MAJOR) a.dist[row * a.k + s] = a.level;
-        else a.dist[row + (int64_t)s * a.m] = a.level;
+        const int32_t d = a.start ? a.level - a.start[s] : a.level;
+        if (a.layout == GG_ROW_MAJOR) a.dist[row * a.k + s] = d;
+        else a.dist[row + (int64_t)s * a.m] = d;
       }
     }
   }
@@ -72,11 +62,23 @@ __device__ __forceinline__ bool finish_row(const MsArgs& a, int64_t row, const u
 }
 
 template <int W>
-__device__ __forceinline__ bool saturated(const uint64_t* acc, const uint64_t* sn, int64_t k) {
+__device__ __forceinline__ bool saturated(const uint64_t* acc, const uint64_t* sn,
+                                          const MsArgs& a) {
   bool full = true;
 #pragma unroll
-  for (int w = 0; w < W; ++w) full &= ((acc[w] | sn[w]) & last_mask<W>(k, w)) == last_mask<W>(k, w);
+  for (int w = 0; w < W; ++w) full &= ((acc[w] | sn[w]) & a.live[w]) == a.live[w];
   return full;
+}
+
+// OR of the wavefront's next words into a.alive (all lanes active).
+template <int W> __device__ __forceinline__ void alive_wave(const MsArgs& a, const uint64_t* nw) {
+  if (!a.alive) return;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    uint64_t v = nw[w];
+    for (int o = 32; o > 0; o >>= 1) v |= shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0 && v) atomicOr(&a.alive[w], (unsigned long long)v);
+  }
 }
 
 template <class TA, bool UNIT, int W>
@@ -85,12 +87,15 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
     int64_t row, base, len;
     bool has_len;
     bool newly = false;
+    uint64_t nw[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) nw[w] = 0;
     const int64_t slot = (int64_t)blockIdx.x * kM + threadIdx.x;
     if (sell_slot(g, slot, row, base, len, has_len)) {
       uint64_t sn[W], acc[W];
 #pragma unroll
       for (int w = 0; w < W; ++w) { sn[w] = a.seen[row * W + w]; acc[w] = 0; }
-      if (!saturated<W>(acc, sn, a.k)) {
+      if (!saturated<W>(acc, sn, a)) {
         const TA* val = (const TA*)g.val;
         for (int64_t kk = 0; kk < len; ++kk) {
           const int64_t i = base + kk * g.C;
@@ -99,9 +104,9 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
           if (!bit_get(a.front_any, col - 1) || !(UNIT || edge_on(val, i))) continue;
 #pragma unroll
           for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
-          if (saturated<W>(acc, sn, a.k)) break;
+          if (saturated<W>(acc, sn, a)) break;
         }
-        newly = finish_row<W>(a, row, acc, sn);
+        newly = finish_row<W>(a, row, acc, sn, nw);
       } else {
 #pragma unroll
         for (int w = 0; w < W; ++w) a.next[row * W + w] = 0;
@@ -110,6 +115,7 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
     const unsigned long long b = __ballot(newly);
     if (b && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)b) - 1)
       atomicAdd(a.found, (unsigned)__popcll(b));
+    alive_wave<W>(a, nw);
     return;
   }
   // one block per heavy row: OR across the block, early exit once saturated
@@ -120,7 +126,7 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
   uint64_t sn[W], acc[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) { sn[w] = a.seen[row * W + w]; acc[w] = 0; }
-  if (threadIdx.x == 0) s_full = saturated<W>(acc, sn, a.k);
+  if (threadIdx.x == 0) s_full = saturated<W>(acc, sn, a);
   __syncthreads();
   if (!s_full) {
     const TA* hv = (const TA*)g.heavy_val;
@@ -130,7 +136,7 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
       if (!bit_get(a.front_any, col - 1) || !(UNIT || edge_on(hv, i))) continue;
 #pragma unroll
       for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
-      if (saturated<W>(acc, sn, a.k)) __atomic_store_n(&s_full, 1, __ATOMIC_RELAXED);
+      if (saturated<W>(acc, sn, a)) __atomic_store_n(&s_full, 1, __ATOMIC_RELAXED);
     }
   }
 #pragma unroll
@@ -144,7 +150,12 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
 #pragma unroll
     for (int w = 0; w < W; ++w)
       for (int q = 1; q < kM / 64; ++q) acc[w] |= red[q][w];
-    if (finish_row<W>(a, row, acc, sn)) atomicAdd(a.found, 1u);
+    uint64_t nw[W];
+    if (finish_row<W>(a, row, acc, sn, nw)) atomicAdd(a.found, 1u);
+    if (a.alive)
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        if (nw[w]) atomicOr(&a.alive[w], (unsigned long long)nw[w]);
   }
 }
 
@@ -154,19 +165,22 @@ __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict
                                                        const TA* __restrict__ val, MsArgs a) {
   const int64_t row = (int64_t)blockIdx.x * kM + threadIdx.x;
   bool newly = false;
+  uint64_t nw[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) nw[w] = 0;
   if (row < a.m) {
     uint64_t sn[W], acc[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) { sn[w] = a.seen[row * W + w]; acc[w] = 0; }
-    if (!saturated<W>(acc, sn, a.k)) {
+    if (!saturated<W>(acc, sn, a)) {
       for (int32_t i = rowptr[row] - 1; i < rowptr[row + 1] - 1; ++i) {
         const int32_t col = colval[i];
         if (!bit_get(a.front_any, col - 1) || !(UNIT || edge_on(val, i))) continue;
 #pragma unroll
         for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
-        if (saturated<W>(acc, sn, a.k)) break;
+        if (saturated<W>(acc, sn, a)) break;
       }
-      newly = finish_row<W>(a, row, acc, sn);
+      newly = finish_row<W>(a, row, acc, sn, nw);
     } else {
 #pragma unroll
       for (int w = 0; w < W; ++w) a.next[row * W + w] = 0;
@@ -175,6 +189,7 @@ __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict
   const unsigned long long b = __ballot(newly);
   if (b && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)b) - 1)
     atomicAdd(a.found, (unsigned)__popcll(b));
+  alive_wave<W>(a, nw);
 }
 
 __global__ void msbfs_seed_kernel(const int32_t* sources, int64_t k, int64_t m, int W, int layout,
@@ -197,6 +212,43 @@ __global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
   if (i < n) p[i] = v;
 }
 __global__ void ms_zero_kernel(unsigned int* p) { *p = 0u; }
+
+int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc) {
+  const bool unit = (A->flags & GG_FLAG_UNIT_PATTERN) != 0;
+  const int64_t m = A->m;
+  return with_type(A->val_dtype, [&](auto ta) -> int {
+    typedef typename decltype(ta)::type TA;
+    auto go = [&](auto ut, auto wt) -> int {
+      constexpr bool U = decltype(ut)::value;
+      constexpr int WW = decltype(wt)::value;
+      if (A->format == GG_CSR) {
+        hipLaunchKernelGGL((msbfs_csr_kernel<TA, U, WW>), dim3(blocks_for(m, kM)), dim3(kM), 0,
+                           stream, A->ptr, A->idx, (const TA*)A->val, a);
+      } else {
+        SellView g = sell_view(A);
+        const int64_t nsb = (g.n_slots + kM - 1) / kM;
+        hipLaunchKernelGGL((msbfs_sell_kernel<TA, U, WW>), dim3((unsigned)(nsb + g.n_heavy)),
+                           dim3(kM), 0, stream, g, nsb, a);
+      }
+      GG_HIP(hipGetLastError());
+      return GG_OK;
+    };
+    auto pick_w = [&](auto ut) -> int {
+      switch (Wc) {
+        case 1: return go(ut, std::integral_constant<int, 1>{});
+        case 2: return go(ut, std::integral_constant<int, 2>{});
+        case 4: return go(ut, std::integral_constant<int, 4>{});
+        default: return go(ut, std::integral_constant<int, 8>{});
+      }
+    };
+    if (unit) return pick_w(std::true_type{});
+    if constexpr (std::is_floating_point<TA>::value) {
+      return set_error(GG_ERR_ARGUMENT, "or-and BFS needs Bool or integer matrix values");
+    } else {
+      return pick_w(std::false_type{});
+    }
+  });
+}
 
 // Runs one batch of k <= 512 sources.
 int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64_t k,
@@ -230,11 +282,13 @@ int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64
   a.seen = (uint64_t*)sn;
   a.dist = dist;
   a.found = (unsigned*)cnt;
+  for (int w = 0; w < 8; ++w) a.live[w] = w < Wc ? word_mask(k, w) : 0ull;
+  a.start = nullptr;
+  a.alive = nullptr;
   uint64_t* fr = (uint64_t*)f;
   uint64_t* nxp = (uint64_t*)nx;
   uint32_t* fany = (uint32_t*)fa;
   uint32_t* nany = (uint32_t*)na;
-  const bool unit = (A->flags & GG_FLAG_UNIT_PATTERN) != 0;
   int32_t level = 0;
   while (true) {
     ++level;
@@ -246,43 +300,10 @@ int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, ctx->stream,
                        nany, words, 0u);
     hipLaunchKernelGGL(ms_zero_kernel, dim3(1), dim3(1), 0, ctx->stream, (unsigned*)cnt);
-    int st;
     {
-    HotTimer ht(ctx);
-    st = with_type(A->val_dtype, [&](auto ta) -> int {
-      typedef typename decltype(ta)::type TA;
-      auto go = [&](auto ut, auto wt) -> int {
-        constexpr bool U = decltype(ut)::value;
-        constexpr int WW = decltype(wt)::value;
-        if (A->format == GG_CSR) {
-          hipLaunchKernelGGL((msbfs_csr_kernel<TA, U, WW>), dim3(blocks_for(m, kM)), dim3(kM), 0,
-                             ctx->stream, A->ptr, A->idx, (const TA*)A->val, a);
-        } else {
-          SellView g = sell_view(A);
-          const int64_t nsb = (g.n_slots + kM - 1) / kM;
-          hipLaunchKernelGGL((msbfs_sell_kernel<TA, U, WW>), dim3((unsigned)(nsb + g.n_heavy)),
-                             dim3(kM), 0, ctx->stream, g, nsb, a);
-        }
-        GG_HIP(hipGetLastError());
-        return GG_OK;
-      };
-      auto pick_w = [&](auto ut) -> int {
-        switch (Wc) {
-          case 1: return go(ut, std::integral_constant<int, 1>{});
-          case 2: return go(ut, std::integral_constant<int, 2>{});
-          case 4: return go(ut, std::integral_constant<int, 4>{});
-          default: return go(ut, std::integral_constant<int, 8>{});
-        }
-      };
-      if (unit) return pick_w(std::true_type{});
-      if constexpr (std::is_floating_point<TA>::value) {
-        return set_error(GG_ERR_ARGUMENT, "or-and BFS needs Bool or integer matrix values");
-      } else {
-        return pick_w(std::false_type{});
-      }
-    });
+      HotTimer ht(ctx);
+      GG_TRY(msbfs_level(ctx->stream, A, a, Wc));
     }
-    GG_TRY(st);
     GG_HIP(hipMemcpyAsync(ctx->host_pinned, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
     GG_HIP(hipStreamSynchronize(ctx->stream));
     if (*(unsigned*)ctx->host_pinned == 0) break;

@@ -1,0 +1,36 @@
+// batched.hpp -- launchers of the multi-source traversal steps, shared by the
+// batched entry points and the serving engine.
+#pragma once
+
+#include "frontier.hpp"
+
+namespace gg {
+
+struct MsArgs {
+  int64_t m, k;
+  int layout;                  // dist layout: GG_COL_MAJOR (m x k) or GG_ROW_MAJOR
+  const uint64_t* front;       // [m*W]
+  const uint32_t* front_any;   // [ceil(m/32)]
+  uint64_t* next;              // [m*W]
+  uint32_t* next_any;
+  uint64_t* seen;              // [m*W]
+  int32_t* dist;
+  int32_t level;
+  unsigned int* found;         // rows with a new bit this level
+  uint64_t live[8];            // bits of the sources being traversed, per word
+  const int32_t* start;        // per source: level at which it was seeded (NULL: 0)
+  unsigned long long* alive;   // OR of every next word, per word (NULL: not kept)
+};
+
+// One or-and level over A_T with W = Wc words per vertex (1, 2, 4 or 8).
+int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc);
+
+// One Jacobi min-plus step of k sources over a row-major working set:
+// dout[v, j] = min(din[v, j], min_u safe_add(w(u->v), din[u, j])).  Rows
+// whose value fell are counted in *changed; when changed_mask is given, bit j
+// is set for every column j that changed (k <= 64).
+int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k,
+                  const void* din, void* dout, unsigned int* changed,
+                  unsigned long long* changed_mask);
+
+}  // namespace gg

@@ -57,7 +57,7 @@ inline SellView sell_view(const gg_matrix* A) {
     while ((1LL << l) < A->slice_size) ++l;
     v.logC = l;
   }
-  v.n_slots = A->perm ? A->nslices * A->slice_size : A->m;
+  v.n_slots = sell_slots(A);
   v.m = A->m;
   v.n_heavy = A->n_heavy;
   v.heavy_row = A->heavy_row;

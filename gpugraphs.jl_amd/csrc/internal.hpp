@@ -108,6 +108,15 @@ inline unsigned blocks_for(int64_t work, int per_block) {
 // Validation shared by gpu_spmv!-style entry points (src/spmv/CSR.jl:358-408).
 int check_matrix(const gg_matrix* A);
 
+// Number of SELL slots a kernel walks: every slot of every slice when a
+// permutation is given (empty slots hold perm 0), else the identity layout's
+// rows, which the slices must cover.  Zero slices means zero slots whatever
+// perm is (a hybrid whose rows are all in the heavy partition).
+inline int64_t sell_slots(const gg_matrix* A) {
+  const int64_t all = A->nslices * A->slice_size;
+  return A->perm ? all : (A->m < all ? A->m : all);
+}
+
 // Internal launchers (used by the algorithms; no argument checks).
 int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const void* B,
                 int b_dtype, int add, int mul, int accum, const void* mask, int mask_dtype,

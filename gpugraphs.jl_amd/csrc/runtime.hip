@@ -20,7 +20,7 @@ int set_error(int status, const char* fmt, ...) {
 void clear_error() { g_err[0] = 0; }
 
 int ws_get(gg_ctx* ctx, int slot, size_t bytes, void** out, bool zero_new) {
-  if (bytes == 0) bytes = 256;
+  if (bytes < 256) bytes = 256;  // whole words for byte-wide atomics, never zero
   if (ctx->ws_bytes[slot] < bytes) {
     if (ctx->ws[slot]) {
       GG_HIP(hipStreamSynchronize(ctx->stream));
@@ -66,6 +66,20 @@ int check_matrix(const gg_matrix* A) {
     case GG_SELL:
       if (A->slice_size < 1 && A->m > 0)
         return set_error(GG_ERR_ARGUMENT, "SELL slice_size must be >= 1");
+      if (A->nslices < 0 || A->n_heavy < 0 || A->n_heavy > A->m)
+        return set_error(GG_ERR_ARGUMENT, "SELL slice / heavy counts out of range");
+      if (A->nslices > 0 && (!A->ptr || (A->n_stored > 0 && (!A->idx || !A->val))))
+        return set_error(GG_ERR_ARGUMENT, "SELL slices without arrays");
+      if (!A->perm) {
+        // identity layout: the slices hold every row and nothing is heavy
+        if (A->n_heavy > 0 && A->nslices > 0)
+          return set_error(GG_ERR_ARGUMENT, "SELL heavy rows need a slot permutation");
+        if (A->n_heavy == 0 && A->nslices * A->slice_size < A->m)
+          return set_error(GG_ERR_ARGUMENT, "SELL slices do not cover the %lld rows",
+                           (long long)A->m);
+        if (A->slot_len)
+          return set_error(GG_ERR_ARGUMENT, "SELL slot lengths need a slot permutation");
+      }
       if (A->n_heavy > 0 && (!A->heavy_row || !A->heavy_ptr || !A->heavy_seg ||
                              !A->heavy_col || A->heavy_seg_len < 1 || A->heavy_nseg < A->n_heavy))
         return set_error(GG_ERR_ARGUMENT, "SELL heavy partition incomplete");
@@ -81,6 +95,18 @@ using namespace gg;
 extern "C" {
 
 int gg_abi_version(void) { return GG_ABI_VERSION; }
+
+int gg_matrix_check(const gg_matrix* A) {
+  clear_error();
+  return check_matrix(A);
+}
+
+int gg_matrix_slots(const gg_matrix* A, int64_t* n_slots) {
+  if (!A || !n_slots) return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  GG_TRY(check_matrix(A));
+  *n_slots = A->format == GG_SELL ? sell_slots(A) : 0;
+  return GG_OK;
+}
 
 const char* gg_status_string(int status) {
   switch (status) {

@@ -286,7 +286,7 @@ int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const voi
     g.slot_len = A->slot_len;
     g.C = A->slice_size;
     g.logC = ilog2_exact(A->slice_size);
-    g.n_slots = A->perm ? A->nslices * A->slice_size : A->m;
+    g.n_slots = sell_slots(A);
     g.m = A->m;
     g.r_lo = r0;
     g.r_hi = r0 + nrows;

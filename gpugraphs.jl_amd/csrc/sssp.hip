@@ -14,7 +14,7 @@
 // working set: a group of lanes owns one row, one lane per source, so the k
 // distances of a neighbour are one contiguous read; the reference's
 // column-major result is written at the end.
-#include "frontier.hpp"
+#include "batched.hpp"
 
 namespace gg {
 
@@ -122,7 +122,8 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
                                                       int64_t m, int64_t k, int logLk,
                                                       const TD* __restrict__ din,
                                                       TD* __restrict__ dout,
-                                                      unsigned int* changed) {
+                                                      unsigned int* changed,
+                                                      unsigned long long* changed_mask) {
   const int Lk = 1 << logLk;
   const int wl = threadIdx.x & 63;
   const int lane = wl & (Lk - 1);
@@ -165,6 +166,13 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
       }
     }
     count_changes(ch, changed);
+    if (changed_mask) {  // k <= 64: lane l of a group is column l
+      const unsigned long long b = __ballot(ch);
+      unsigned long long cm = 0;
+      const unsigned long long low = Lk == 64 ? ~0ull : ((1ull << Lk) - 1ull);
+      for (int o = 0; o < 64; o += Lk) cm |= (b >> o) & low;
+      if (wl == 0 && cm) atomicOr(changed_mask, cm);
+    }
     return;
   }
   // heavy rows: block (h, jblock); the block's groups split the row's entries
@@ -188,7 +196,10 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
       const TD dv = din[row * k + j];
       const TD nv = jmin(dv, acc);
       dout[row * k + j] = nv;
-      if (nv < dv) atomicAdd(changed, 1u);
+      if (nv < dv) {
+        atomicAdd(changed, 1u);
+        if (changed_mask) atomicOr(changed_mask, 1ull << j);
+      }
     }
   }
 }
@@ -319,6 +330,33 @@ int sssp_one(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist, int di
   });
 }
 
+int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k,
+                  const void* din, void* dout, unsigned int* changed,
+                  unsigned long long* changed_mask) {
+  if (changed_mask && k > 64) return set_error(GG_ERR_ARGUMENT, "column masks need k <= 64");
+  return with_ad(A->val_dtype, dist_dtype, [&](auto a, auto d) -> int {
+    typedef typename decltype(a)::type TA;
+    typedef typename decltype(d)::type TD;
+    const int64_t m = A->m;
+    int lg = 0;
+    while (lg < 6 && (1LL << lg) < k) ++lg;
+    const int64_t Lk = 1LL << lg;
+    const int64_t jblocks = (k + Lk - 1) / Lk;
+    const int64_t gpb = kT / Lk;
+    SellView g = sell_view(A);
+    if (A->format == GG_CSR) { g.colval = A->idx; g.val = A->val; g.n_heavy = 0; }
+    const int64_t n_light = A->format == GG_CSR ? m : g.n_slots;
+    const int64_t light_blocks = (n_light * jblocks + gpb - 1) / gpb;
+    const int64_t nb = light_blocks + g.n_heavy * jblocks;
+    if (nb > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
+    hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
+                       stream, A->format, A->ptr, g, n_light, m, k, lg, (const TD*)din, (TD*)dout,
+                       changed, changed_mask);
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+  });
+}
+
 // k-source Bellman-Ford into a row-major working set; result written
 // column-major into dist (or row-major when rowmajor_out).
 int sssp_multi(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_host, int64_t k,
@@ -338,24 +376,10 @@ int sssp_multi(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_host, int
                        buf[0], m * k, Lim<TD>::max());
     hipLaunchKernelGGL(bf_multi_seed_kernel<TD>, dim3(blocks_for(k, kT)), dim3(kT), 0,
                        ctx->stream, buf[0], (const int32_t*)src, k);
-    int lg = 0;
-    while (lg < 6 && (1LL << lg) < k) ++lg;
-    const int64_t Lk = 1LL << lg;
-    const int64_t jblocks = (k + Lk - 1) / Lk;
-    const int64_t gpb = kT / Lk;
-    SellView g = sell_view(A);
-    if (A->format == GG_CSR) { g.colval = A->idx; g.val = A->val; g.n_heavy = 0; }
-    const int64_t n_light = A->format == GG_CSR ? m : g.n_slots;
-    const int64_t light_blocks = (n_light * jblocks + gpb - 1) / gpb;
-    const int64_t nb = light_blocks + g.n_heavy * jblocks;
-    if (nb > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
     auto step = [&](int64_t it) -> int {
       HotTimer ht(ctx);
-      hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT),
-                         sizeof(TD) * kT, ctx->stream, A->format, A->ptr, g, n_light, m, k, lg,
-                         buf[(it - 1) & 1], buf[it & 1], (unsigned*)cnt);
-      GG_HIP(hipGetLastError());
-      return GG_OK;
+      return bf_multi_step(ctx->stream, A, dist_dtype, k, buf[(it - 1) & 1], buf[it & 1],
+                           (unsigned*)cnt, nullptr);
     };
     int64_t last = 0;
     int st = iterate(ctx, (unsigned*)cnt, m + 16, step, iterations, &last);

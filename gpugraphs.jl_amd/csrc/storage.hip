@@ -63,6 +63,10 @@ int gg_csr_check(int64_t m, int64_t n, const int64_t* rowptr, int64_t rowptr_len
   if (rowptr[0] != 1 || rowptr[m] != colval_len + 1)
     return set_error(GG_ERR_ARGUMENT,
                      "rowptr[1] must be equal to 1 and rowptr[m+1] must be equal to length(colval) + 1");
+  // Beyond the reference's checks: the kernels index colval through rowptr,
+  // so a decreasing pointer would send them outside the arrays.
+  for (int64_t i = 0; i < m; ++i)
+    if (rowptr[i + 1] < rowptr[i]) return set_error(GG_ERR_ARGUMENT, "rowptr must be non-decreasing");
   return GG_OK;
 }
 

@@ -88,6 +88,18 @@ class GGCsrHost(C.Structure):
                 ("nzval", C.c_void_p)]
 
 
+class QueryInfo(C.Structure):
+    _fields_ = [("request_id", C.c_int64), ("graph", C.c_int32), ("kind", C.c_int32),
+                ("source", C.c_int64), ("status", C.c_int32), ("arrival_s", C.c_double),
+                ("admit_s", C.c_double), ("complete_s", C.c_double)]
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("failed", C.c_int64), ("waiting", C.c_int64),
+                ("running", C.c_int64), ("steps", C.c_int64), ("arena_bytes", C.c_size_t),
+                ("arena_used", C.c_size_t)]
+
+
 # every symbol of include/gpugraphs.h with its ctypes signature
 _vp, _i64, _i32, _u64, _d = C.c_void_p, C.c_int64, C.c_int, C.c_uint64, C.c_double
 _PM = C.POINTER(GGMatrix)
@@ -106,6 +118,8 @@ SIGNATURES = {
     "gg_ctx_enable_timing": ([_vp, _i32], _i32),
     "gg_ctx_last_kernel_ms": ([_vp, C.POINTER(_d)], _i32),
     "gg_ctx_hot_kernel_stats": ([_vp, C.POINTER(_d), C.POINTER(_i64), _i32], _i32),
+    "gg_matrix_check": ([_PM], _i32),
+    "gg_matrix_slots": ([_PM, C.POINTER(_i64)], _i32),
     "gg_csr_check": ([_i64, _i64, _vp, _i64, _vp, _i64, _i64], _i32),
     "gg_sell_check": ([_i64, _i64, _i64, _i64, _vp, _i64, _i64], _i32),
     "gg_sell_build": ([_i64, _i64, _vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64,
@@ -132,6 +146,17 @@ SIGNATURES = {
     "gg_shortest_path_multi": ([_vp, _PM, _vp, _i64, _vp, _i32, C.POINTER(_i64)], _i32),
     "gg_msbfs_distances": ([_vp, _PM, _vp, _i64, _vp, C.POINTER(_i64)], _i32),
     "gg_mssp_distances": ([_vp, _PM, _vp, _i64, _vp, _i32, C.POINTER(_i64)], _i32),
+    "gg_server_create": ([_i32, C.c_size_t, C.POINTER(_vp)], _i32),
+    "gg_server_add_graph": ([_vp, _PM, C.POINTER(_i32)], _i32),
+    "gg_server_start": ([_vp], _i32),
+    "gg_server_submit": ([_vp, _i32, _i32, _i64, C.POINTER(_i64)], _i32),
+    "gg_server_poll": ([_vp, C.POINTER(QueryInfo), _i64, C.POINTER(_i64), _d], _i32),
+    "gg_server_fetch": ([_vp, _i64, _vp, C.c_size_t, _d], _i32),
+    "gg_server_fetch_device": ([_vp, _i64, _vp, C.c_size_t, _d], _i32),
+    "gg_server_release": ([_vp, _i64, _d], _i32),
+    "gg_server_stats": ([_vp, C.POINTER(ServerCounters)], _i32),
+    "gg_server_stop": ([_vp], _i32),
+    "gg_server_destroy": ([_vp], _i32),
 }
 
 _lib = None

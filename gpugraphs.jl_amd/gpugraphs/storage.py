@@ -118,11 +118,16 @@ def to_host(a) -> np.ndarray:
     return np.asarray(a)
 
 
-def ptr(a) -> int:
+def ptr(a, host_ok: bool = False) -> int:
+    """Address of an array's data (0 for None or an empty array).  Host
+    arrays are accepted only for host-side checks (host_ok)."""
     if a is None:
         return 0
     if is_device_array(a):
         return a.data_ptr()
+    if host_ok:
+        a = np.asarray(a)
+        return a.ctypes.data if a.size else 0
     raise RuntimeError("GPUGraphs kernels need device arrays (ROCBackend); got host memory")
 
 
@@ -255,12 +260,15 @@ class SparseGPUMatrixCSR(AbstractSparseGPUMatrix):
     def nnz(self):
         return len(self.nzval)
 
-    def descriptor(self) -> GGMatrix:
+    def descriptor(self, host_ok: bool = False) -> GGMatrix:
+        def P(a):
+            return ptr(a, host_ok)
+
         d = GGMatrix()
         d.format = _lib.CSR
         d.val_dtype = dtype_code(np_dtype_of(self.nzval))
         d.m, d.n, d.nnz = self.m, self.n, self.nnz()
-        d.ptr, d.idx, d.val = ptr(self.rowptr), ptr(self.colval), ptr(self.nzval)
+        d.ptr, d.idx, d.val = P(self.rowptr), P(self.colval), P(self.nzval)
         d.flags = _lib.FLAG_UNIT_PATTERN if self.unit else 0
         return d
 
@@ -325,12 +333,15 @@ class SparseGPUMatrixCSC(AbstractSparseGPUMatrix):
     def nnz(self):
         return len(self.nzval)
 
-    def descriptor(self) -> GGMatrix:
+    def descriptor(self, host_ok: bool = False) -> GGMatrix:
+        def P(a):
+            return ptr(a, host_ok)
+
         d = GGMatrix()
         d.format = _lib.CSC
         d.val_dtype = dtype_code(np_dtype_of(self.nzval))
         d.m, d.n, d.nnz = self.m, self.n, self.nnz()
-        d.ptr, d.idx, d.val = ptr(self.colptr), ptr(self.rowval), ptr(self.nzval)
+        d.ptr, d.idx, d.val = P(self.colptr), P(self.rowval), P(self.nzval)
         d.flags = _lib.FLAG_UNIT_PATTERN if self.unit else 0
         return d
 
@@ -442,6 +453,12 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
         check(lib().gg_sell_check(int(n), int(nslices), int(n_stored), len(slice_ptr),
                                   cv.ctypes.data, len(cv), len(nzval)), "SparseGPUMatrixSELL")
         vdt = np_dtype_of(nzval)
+        # beyond the reference's checks: the kernels index through slice_ptr
+        spt = np.asarray(to_host(slice_ptr), dtype=np.int64)
+        if (len(spt) and (spt[0] != 1 or spt[-1] != int(n_stored) + 1 or np.any(np.diff(spt) < 0)
+                          or np.any(np.diff(spt) % max(int(slice_size), 1)))) \
+                or int(nslices) * int(slice_size) < int(m):
+            raise ArgumentError("slice_ptr is not a SELL layout of these sizes")
         self.m, self.n, self.backend = int(m), int(n), backend
         self.perm = np.asarray(perm)
         if not np.array_equal(self.perm, np.arange(1, m + 1)):
@@ -460,21 +477,24 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
     def nnz(self):
         return self._nnz   # true nonzeros, src/storage/SELL.jl:431
 
-    def descriptor(self) -> GGMatrix:
+    def descriptor(self, host_ok: bool = False) -> GGMatrix:
+        def P(a):
+            return ptr(a, host_ok)
+
         d = GGMatrix()
         d.format = _lib.SELL
         d.val_dtype = dtype_code(np_dtype_of(self.nzval))
         d.m, d.n, d.nnz = self.m, self.n, self._nnz
-        d.ptr, d.idx, d.val = ptr(self.slice_ptr), ptr(self.colval), ptr(self.nzval)
+        d.ptr, d.idx, d.val = P(self.slice_ptr), P(self.colval), P(self.nzval)
         d.slice_size, d.nslices, d.n_stored = self.slice_size, self.nslices, self.n_stored
-        d.perm = ptr(self._perm_dev) if self._perm_dev is not None else 0
-        d.slot_len = ptr(self.slot_len) if self.slot_len is not None else 0
+        d.perm = P(self._perm_dev) if self._perm_dev is not None else 0
+        d.slot_len = P(self.slot_len) if self.slot_len is not None else 0
         d.n_heavy = self.n_heavy
         if self.n_heavy:
-            d.heavy_row, d.heavy_ptr = ptr(self.heavy_row), ptr(self.heavy_ptr)
-            d.heavy_seg, d.heavy_seg_len = ptr(self.heavy_seg), self.heavy_seg_len
+            d.heavy_row, d.heavy_ptr = P(self.heavy_row), P(self.heavy_ptr)
+            d.heavy_seg, d.heavy_seg_len = P(self.heavy_seg), self.heavy_seg_len
             d.heavy_nseg = self.n_seg
-            d.heavy_col, d.heavy_val = ptr(self.heavy_col), ptr(self.heavy_val)
+            d.heavy_col, d.heavy_val = P(self.heavy_col), P(self.heavy_val)
         d.flags = _lib.FLAG_UNIT_PATTERN if self.unit else 0
         return d
 

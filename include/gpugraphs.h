@@ -175,6 +175,13 @@ int gg_ctx_enable_timing(gg_ctx* ctx, int on);
 int gg_ctx_last_kernel_ms(gg_ctx* ctx, double* ms);
 int gg_ctx_hot_kernel_stats(gg_ctx* ctx, double* total_ms, int64_t* launches, int reset);
 
+/* Host-side consistency check of a descriptor (no device access): the checks
+ * every entry point runs before launching. */
+int gg_matrix_check(const gg_matrix* A);
+/* SELL slots the kernels walk: nslices*slice_size with a permutation, else
+ * min(m, nslices*slice_size) (identity layout); 0 for CSR / CSC. */
+int gg_matrix_slots(const gg_matrix* A, int64_t* n_slots);
+
 /* ---------------------------------------------------------------- storage */
 /* Validation of a CSR / CSC triple held in host memory; same checks and order
  * as the inner constructors src/storage/CSR.jl:48-63, src/storage/CSC.jl:31-46. */
@@ -291,6 +298,47 @@ int gg_msbfs_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources
  * m x k column-major like gg_shortest_path_multi. */
 int gg_mssp_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources, int64_t k,
                       void* dist, int dist_dtype, int64_t* iterations);
+
+/* ---------------------------------------------------------------- serving */
+/* Continuous-batching query server over HBM-resident graphs (no counterpart
+ * in the reference, which answers one call at a time).  Each graph gets a BFS
+ * engine (64 bitmask slots) and an SSSP engine (32 min-plus columns) on
+ * streams of their own; a query joins a free slot between two steps of its
+ * engine, its result is identical to gg_bfs_distances / gg_shortest_path
+ * from its source, and it is admitted only when an n-element result block
+ * fits in the result arena of budget_bytes (else it waits; a query that can
+ * never fit fails with GG_ERR_BUDGET).  Graph arrays stay owned by the
+ * caller and must outlive the server. */
+typedef struct gg_server gg_server;
+typedef enum gg_query_kind { GG_QUERY_BFS = 1, GG_QUERY_SSSP = 2 } gg_query_kind;
+typedef struct gg_query_info {
+  int64_t request_id;
+  int32_t graph, kind;
+  int64_t source;
+  int32_t status;      /* GG_OK or GG_ERR_BUDGET */
+  double arrival_s;    /* host steady clock, seconds */
+  double admit_s;
+  double complete_s;
+} gg_query_info;
+typedef struct gg_server_counters {
+  int64_t completed, failed, waiting, running, steps;
+  size_t arena_bytes, arena_used;
+} gg_server_counters;
+int gg_server_create(int device, size_t budget_bytes, gg_server** out);
+int gg_server_add_graph(gg_server* s, const gg_matrix* A_T, int* graph_id);
+int gg_server_start(gg_server* s);
+int gg_server_submit(gg_server* s, int graph_id, int kind, int64_t source, int64_t* request_id);
+/* Finished (or failed) queries not reported before; waits up to wait_s for one. */
+int gg_server_poll(gg_server* s, gg_query_info* out, int64_t max, int64_t* n_out, double wait_s);
+/* Copy a finished query's n-element result (Int32 levels for BFS; the dist
+ * type of gg_shortest_path for SSSP) and free its block.  Waits up to wait_s. */
+int gg_server_fetch(gg_server* s, int64_t request_id, void* host_dst, size_t bytes, double wait_s);
+int gg_server_fetch_device(gg_server* s, int64_t request_id, void* dev_dst, size_t bytes,
+                           double wait_s);
+int gg_server_release(gg_server* s, int64_t request_id, double wait_s);
+int gg_server_stats(gg_server* s, gg_server_counters* out);
+int gg_server_stop(gg_server* s);   /* finish the steps in flight and join */
+int gg_server_destroy(gg_server* s);
 
 #ifdef __cplusplus
 }

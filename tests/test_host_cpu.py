@@ -169,6 +169,84 @@ def test_sell_hybrid_keeps_every_row(sigma, heavy):
         assert A.n_seg == sum(-(-len(rows[int(r)]) // 64) for r in A.heavy_row)
 
 
+def _walk_sell_indexing(A):
+    """Every array index the SELL kernels form (sell_slot / heavy segments),
+    checked against the buffer lengths on the host."""
+    import ctypes as C
+    d = A.descriptor(host_ok=True)
+    assert gg.lib().gg_matrix_check(C.byref(d)) == 0, gg.lib().gg_last_error()
+    n_slots = C.c_int64()
+    assert gg.lib().gg_matrix_slots(C.byref(d), C.byref(n_slots)) == 0
+    n_slots = n_slots.value
+    Cs = A.slice_size
+    sp_ = np.asarray(A.slice_ptr)
+    perm = None if not d.perm else np.asarray(A.perm)
+    slen = None if not d.slot_len else np.asarray(A.slot_len)
+    assert n_slots <= A.nslices * Cs
+    if perm is not None:
+        assert len(perm) == n_slots and len(slen) == n_slots
+    for slot in range(n_slots):
+        sl, r = divmod(slot, Cs)
+        assert sl + 1 < len(sp_)                       # slice_ptr[sl], slice_ptr[sl+1]
+        row = (perm[slot] - 1) if perm is not None else slot
+        if perm is not None and perm[slot] == 0:
+            continue
+        assert 0 <= row < A.m
+        width = (sp_[sl + 1] - sp_[sl]) // Cs
+        ln = slen[slot] if slen is not None else width
+        assert ln <= width
+        if ln:
+            assert sp_[sl] - 1 + r + (ln - 1) * Cs < A.n_stored
+    if A.n_heavy:
+        hseg = np.asarray(A.heavy_seg)
+        hptr = np.asarray(A.heavy_ptr)
+        assert len(hseg) == A.n_heavy + 1 and hseg[-1] == A.n_seg == d.heavy_nseg
+        assert hptr[-1] == len(A.heavy_col)
+        for g in range(A.n_seg):
+            h = np.searchsorted(hseg, g, side="right") - 1
+            assert 0 <= h < A.n_heavy
+            s = hptr[h] + (g - hseg[h]) * A.heavy_seg_len
+            assert s < hptr[h + 1]
+
+
+@pytest.mark.parametrize("case", ["all_heavy", "mixed", "reference", "reference_c64", "empty_rows"])
+def test_sell_descriptor_slots_and_indexing(case):
+    """Regression: a hybrid whose rows are all heavy has no slots (its empty
+    permutation must not read as the identity layout)."""
+    rng = np.random.default_rng(3)
+    if case == "all_heavy":
+        S = sp.random(1000, 1000, density=0.2, format="csr", random_state=rng, dtype=np.float32)
+        A = gg.SparseGPUMatrixSELL(S, CPU, 64, sigma=64, heavy_threshold=8, heavy_seg_len=16)
+        assert A.nslices == 0 and A.n_heavy == 1000
+    elif case == "mixed":
+        S = G.rmat(11, seed=2, dtype=np.float32, values=G.UNIFORM01).to_scipy()
+        A = gg.SparseGPUMatrixSELL(S, CPU, 64, sigma=64, heavy_threshold=8, heavy_seg_len=16)
+        assert A.nslices > 0 and A.n_heavy > 0
+    elif case == "reference":
+        S = sp.random(300, 300, density=0.05, format="csr", random_state=rng, dtype=np.float32)
+        A = gg.SparseGPUMatrixSELL(S, CPU, 32)
+    elif case == "reference_c64":
+        S = sp.random(1000, 1000, density=0.01, format="csr", random_state=rng, dtype=np.float32)
+        A = gg.SparseGPUMatrixSELL(S, CPU, 64)
+    else:
+        S = sp.csr_matrix((500, 500), dtype=np.float32)
+        S[3, 4] = 1.0
+        A = gg.SparseGPUMatrixSELL(S, CPU, 64, sigma=64, heavy_threshold=8)
+    _walk_sell_indexing(A)
+
+
+def test_descriptor_check_rejects_uncovered_identity_layout():
+    import ctypes as C
+    from gpugraphs import _lib as L
+    d = L.GGMatrix()
+    d.format, d.val_dtype, d.m, d.n = L.SELL, L.FLOAT32, 100, 100
+    d.slice_size, d.nslices = 32, 2      # 64 slots for 100 rows
+    d.ptr = d.idx = d.val = 8
+    assert gg.lib().gg_matrix_check(C.byref(d)) == 2
+    d.nslices = 4
+    assert gg.lib().gg_matrix_check(C.byref(d)) == 0
+
+
 def test_rmat_generator_properties():
     R = G.rmat(14, edge_factor=16, seed=7)
     S = R.to_scipy()
