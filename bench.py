@@ -3,8 +3,9 @@
 
 Workload (one GPU): R-MAT scale 22, edge factor 16, a/b/c/d = .57/.19/.19/.05,
 symmetrised, self-loops and duplicates removed, ids permuted, Int32 indices;
-SELL-C-sigma with C = 64, sigma = 4096 (rows longer than the heavy threshold in
-the segmented partition).  A step is one batched BFS from 64 random
+SELL-C-sigma with C = 64, rows relabelled by decreasing degree (internal ids;
+rows longer than the heavy threshold in the segmented partition), or
+sigma = 4096 windows without relabelling (--no-relabel).  A step is one batched BFS from 64 random
 non-isolated roots (or-and SpMM over 64-bit source masks); the plain
 one-root-at-a-time BFS, the direction-optimising BFS and the fp32 plus-times
 SELL SpMV are measured beside it, with the oracle on the host cores as the
@@ -40,13 +41,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--roots", type=int, default=64)
-    ap.add_argument("--heavy", type=int, default=256, help="SELL heavy-row threshold")
+    ap.add_argument("--heavy", type=int, default=None,
+                    help="SELL heavy-row threshold (default 1024 relabelled, 256 sigma windows)")
+    ap.add_argument("--no-relabel", action="store_true",
+                    help="sigma = 4096 windows in original ids instead of degree relabelling")
     ap.add_argument("--seg", type=int, default=4096, help="heavy segment length")
     ap.add_argument("--plain-roots", type=int, default=64)
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default="", help="profile one part: spmv | msbfs | bfs")
     args = ap.parse_args()
+    relabel = not args.no_relabel
+    sigma = 1 if relabel else 4096
+    if args.heavy is None:
+        args.heavy = 1024 if relabel else 256
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
               "(the per-GPU shard and how it would shard are described in DESIGN.md)",
@@ -72,9 +80,9 @@ def main():
     log(f"rmat{args.scale}: n={n} nnz={nnz} max_deg={deg.max()} gen {time.time() - t:.1f}s")
 
     t = time.time()
-    A_f = Rg.to_sell(R0, 64, 4096, args.heavy, args.seg)          # fp32 values for SpMV
+    A_f = Rg.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel)   # fp32 values for SpMV
     Rb = G.HostCSR(n, n, Rg.rowptr, Rg.colval, np.ones(nnz, np.bool_))
-    A_b = Rb.to_sell(R0, 64, 4096, args.heavy, args.seg)          # Bool pattern for BFS
+    A_b = Rb.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel)   # Bool pattern for BFS
     A_csr = Rb.to_csr(R0)                                          # out-edges for push (symmetric)
     torch.cuda.synchronize()
     log(f"SELL builds + upload {time.time() - t:.1f}s; stored/nnz {(A_f.n_stored + (int(A_f.heavy_ptr[-1].item()) if A_f.n_heavy else 0)) / nnz:.3f}")
@@ -242,9 +250,11 @@ def main():
         "data": "synthetic (seeded R-MAT generator)",
         "config": {"workload": f"rmat{args.scale} ef16 a.57 b.19 c.19 symmetrised permuted; "
                                f"batched BFS from {args.roots} random non-isolated roots "
-                               "(one step = one k=64 or-and SpMM traversal); SELL-64-sigma4096",
-                   "n": n, "nnz": nnz, "roots": args.roots, "slice_size": 64, "sigma": 4096,
-                   "heavy_threshold": args.heavy, "parallelism": "single GPU"},
+                               "(one step = one k=64 or-and SpMM traversal); SELL-64 "
+                               + ("degree-relabelled" if relabel else "sigma4096"),
+                   "n": n, "nnz": nnz, "roots": args.roots, "slice_size": 64, "sigma": sigma,
+                   "relabel": relabel, "heavy_threshold": args.heavy,
+                   "parallelism": "single GPU"},
         "p50_ms": None if not ms else round(ms["p50_ms"], 3),
         "p99_ms": None if not ms else round(ms["p99_ms"], 3),
         "plain_qps": round(bfs["plain_qps"], 1) if bfs else None,

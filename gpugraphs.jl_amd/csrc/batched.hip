@@ -31,8 +31,8 @@ inline uint64_t word_mask(int64_t k, int w) {
 }
 
 template <int W>
-__device__ __forceinline__ bool finish_row(const MsArgs& a, int64_t row, const uint64_t* acc,
-                                           const uint64_t* sn, uint64_t* nw) {
+__device__ __forceinline__ bool finish_row(const MsArgs& a, int64_t rid, int64_t row,
+                                           const uint64_t* acc, const uint64_t* sn, uint64_t* nw) {
   bool any = false;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
@@ -41,11 +41,11 @@ __device__ __forceinline__ bool finish_row(const MsArgs& a, int64_t row, const u
   }
 #pragma unroll
   for (int w = 0; w < W; ++w) {
-    a.next[row * W + w] = nw[w];
-    if (nw[w]) a.seen[row * W + w] = sn[w] | nw[w];
+    a.next[rid * W + w] = nw[w];
+    if (nw[w]) a.seen[rid * W + w] = sn[w] | nw[w];
   }
   if (any) {
-    bit_claim(a.next_any, row);
+    bit_claim(a.next_any, rid);
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       uint64_t b = nw[w];
@@ -92,9 +92,10 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
     for (int w = 0; w < W; ++w) nw[w] = 0;
     const int64_t slot = (int64_t)blockIdx.x * kM + threadIdx.x;
     if (sell_slot(g, slot, row, base, len, has_len)) {
+      const int64_t rid = g.internal ? slot : row;
       uint64_t sn[W], acc[W];
 #pragma unroll
-      for (int w = 0; w < W; ++w) { sn[w] = a.seen[row * W + w]; acc[w] = 0; }
+      for (int w = 0; w < W; ++w) { sn[w] = a.seen[rid * W + w]; acc[w] = 0; }
       if (!saturated<W>(acc, sn, a)) {
         const TA* val = (const TA*)g.val;
         for (int64_t kk = 0; kk < len; ++kk) {
@@ -106,10 +107,10 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
           for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
           if (saturated<W>(acc, sn, a)) break;
         }
-        newly = finish_row<W>(a, row, acc, sn, nw);
+        newly = finish_row<W>(a, rid, row, acc, sn, nw);
       } else {
 #pragma unroll
-        for (int w = 0; w < W; ++w) a.next[row * W + w] = 0;
+        for (int w = 0; w < W; ++w) a.next[rid * W + w] = 0;
       }
     }
     const unsigned long long b = __ballot(newly);
@@ -123,9 +124,10 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
   __shared__ int s_full;
   const int64_t h = (int64_t)blockIdx.x - n_sell_blocks;
   const int64_t row = g.heavy_row[h] - 1;
+  const int64_t rid = heavy_rid(g, h);
   uint64_t sn[W], acc[W];
 #pragma unroll
-  for (int w = 0; w < W; ++w) { sn[w] = a.seen[row * W + w]; acc[w] = 0; }
+  for (int w = 0; w < W; ++w) { sn[w] = a.seen[rid * W + w]; acc[w] = 0; }
   if (threadIdx.x == 0) s_full = saturated<W>(acc, sn, a);
   __syncthreads();
   if (!s_full) {
@@ -151,7 +153,7 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
     for (int w = 0; w < W; ++w)
       for (int q = 1; q < kM / 64; ++q) acc[w] |= red[q][w];
     uint64_t nw[W];
-    if (finish_row<W>(a, row, acc, sn, nw)) atomicAdd(a.found, 1u);
+    if (finish_row<W>(a, rid, row, acc, sn, nw)) atomicAdd(a.found, 1u);
     if (a.alive)
 #pragma unroll
       for (int w = 0; w < W; ++w)
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict
         for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
         if (saturated<W>(acc, sn, a)) break;
       }
-      newly = finish_row<W>(a, row, acc, sn, nw);
+      newly = finish_row<W>(a, row, row, acc, sn, nw);
     } else {
 #pragma unroll
       for (int w = 0; w < W; ++w) a.next[row * W + w] = 0;
@@ -193,15 +195,16 @@ __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict
 }
 
 __global__ void msbfs_seed_kernel(const int32_t* sources, int64_t k, int64_t m, int W, int layout,
-                                  uint64_t* front, uint32_t* front_any, uint64_t* seen,
-                                  int32_t* dist) {
+                                  const int32_t* vinv, uint64_t* front, uint32_t* front_any,
+                                  uint64_t* seen, int32_t* dist) {
   // one thread: several sources may share a vertex
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   for (int64_t s = 0; s < k; ++s) {
     const int64_t v = sources[s] - 1;
-    front[v * W + s / 64] |= 1ull << (s & 63);
-    seen[v * W + s / 64] |= 1ull << (s & 63);
-    front_any[v >> 5] |= 1u << (v & 31);
+    const int64_t r = vinv ? (int64_t)vinv[v] - 1 : v;  // state index
+    front[r * W + s / 64] |= 1ull << (s & 63);
+    seen[r * W + s / 64] |= 1ull << (s & 63);
+    front_any[r >> 5] |= 1u << (r & 31);
     if (layout == GG_ROW_MAJOR) dist[v * k + s] = 0;
     else dist[v + s * m] = 0;
   }
@@ -254,26 +257,28 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc)
 int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64_t k,
               int32_t* dist, int layout, int64_t* levels_out) {
   const int64_t m = A->m;
+  const int64_t ns = state_size(A);  // rows of the traversal state
   int W = (int)((k + 63) / 64);
   int Wc = W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : 8;
-  const int64_t words = (m + 31) / 32;
+  const int64_t words = (ns + 31) / 32;
   void *f, *nx, *sn, *fa, *na, *cnt;
-  GG_TRY(ws_get(ctx, WS_MS0, sizeof(uint64_t) * m * Wc, &f));
-  GG_TRY(ws_get(ctx, WS_MS1, sizeof(uint64_t) * m * Wc, &nx));
-  GG_TRY(ws_get(ctx, WS_MS2, sizeof(uint64_t) * m * Wc, &sn));
+  GG_TRY(ws_get(ctx, WS_MS0, sizeof(uint64_t) * ns * Wc, &f));
+  GG_TRY(ws_get(ctx, WS_MS1, sizeof(uint64_t) * ns * Wc, &nx));
+  GG_TRY(ws_get(ctx, WS_MS2, sizeof(uint64_t) * ns * Wc, &sn));
   GG_TRY(ws_get(ctx, WS_BFS1, sizeof(uint32_t) * words, &fa));
   GG_TRY(ws_get(ctx, WS_BFS2, sizeof(uint32_t) * words, &na));
   GG_TRY(ws_get(ctx, WS_FLAG, 64, &cnt));
-  hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(m * Wc, kM)), dim3(kM), 0, ctx->stream,
-                     (uint64_t*)f, m * Wc, 0ull);
-  hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(m * Wc, kM)), dim3(kM), 0, ctx->stream,
-                     (uint64_t*)sn, m * Wc, 0ull);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(ns * Wc, kM)), dim3(kM), 0, ctx->stream,
+                     (uint64_t*)f, ns * Wc, 0ull);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(ns * Wc, kM)), dim3(kM), 0, ctx->stream,
+                     (uint64_t*)sn, ns * Wc, 0ull);
   hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, ctx->stream,
                      (uint32_t*)fa, words, 0u);
   hipLaunchKernelGGL(fill_i32v_kernel, dim3(blocks_for(m * k, kM)), dim3(kM), 0, ctx->stream,
                      dist, m * k, (int32_t)2147483647);
+  const int32_t* vinv = (A->flags & GG_FLAG_INTERNAL_IDS) ? A->vinv : nullptr;
   hipLaunchKernelGGL(msbfs_seed_kernel, dim3(1), dim3(1), 0, ctx->stream, sources_dev, k, m, Wc,
-                     layout, (uint64_t*)f, (uint32_t*)fa, (uint64_t*)sn, dist);
+                     layout, vinv, (uint64_t*)f, (uint32_t*)fa, (uint64_t*)sn, dist);
   GG_HIP(hipGetLastError());
   MsArgs a{};
   a.m = m;

@@ -6,15 +6,17 @@
 
 namespace gg {
 
+// The masks are indexed by state index (the internal id of a relabelled SELL,
+// else the row); dist by original row.
 struct MsArgs {
   int64_t m, k;
   int layout;                  // dist layout: GG_COL_MAJOR (m x k) or GG_ROW_MAJOR
-  const uint64_t* front;       // [m*W]
-  const uint32_t* front_any;   // [ceil(m/32)]
-  uint64_t* next;              // [m*W]
+  const uint64_t* front;       // [ns*W], ns = state_size(A)
+  const uint32_t* front_any;   // [ceil(ns/32)]
+  uint64_t* next;              // [ns*W]
   uint32_t* next_any;
-  uint64_t* seen;              // [m*W]
-  int32_t* dist;
+  uint64_t* seen;              // [ns*W]
+  int32_t* dist;               // original rows
   int32_t level;
   unsigned int* found;         // rows with a new bit this level
   uint64_t live[8];            // bits of the sources being traversed, per word

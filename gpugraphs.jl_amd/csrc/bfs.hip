@@ -42,11 +42,14 @@ struct BfsCounters {
   unsigned long long scout;  // push: out-degree sum of the claimed vertices
 };
 
+// out at the original source s; the bitmaps at its state index.
 __global__ void bfs_seed_kernel(int32_t* out, int32_t out_val, uint32_t* visited,
-                                uint32_t* frontier, int32_t* queue, int64_t s) {
+                                uint32_t* frontier, int32_t* queue, int64_t s,
+                                const int32_t* vinv) {
+  const int64_t r = vinv ? (int64_t)vinv[s] - 1 : s;
   out[s] = out_val;
-  visited[s >> 5] |= 1u << (s & 31);
-  frontier[s >> 5] |= 1u << (s & 31);
+  visited[r >> 5] |= 1u << (r & 31);
+  frontier[r >> 5] |= 1u << (r & 31);
   if (queue) queue[0] = (int32_t)s;
 }
 
@@ -71,22 +74,38 @@ __global__ __launch_bounds__(kB) void bfs_pull_sell_kernel(
     int64_t row, base, len;
     bool has_len;
     const int64_t slot = (int64_t)blockIdx.x * kB + threadIdx.x;
-    if (sell_slot(g, slot, row, base, len, has_len) && !bit_get(visited, row)) {
-      const TA* val = (const TA*)g.val;
-      int32_t hit = 0;
-      for (int64_t k = 0; k < len; ++k) {
-        const int64_t i = base + k * g.C;
-        const int32_t col = g.colval[i];
-        if (!has_len && col == -1) break;
-        if (bit_get(frontier, col - 1) && (PARENTS || UNIT || edge_on(val, i))) {
-          if (!PARENTS) { hit = col; break; }
-          hit = max(hit, col);
+    if (sell_slot(g, slot, row, base, len, has_len)) {
+      const int64_t rid = g.internal ? slot : row;  // index of the row's traversal state
+      if (!bit_get(visited, rid)) {
+        const TA* val = (const TA*)g.val;
+        int32_t hit = 0;
+        int64_t k = 0;
+        if (!PARENTS && has_len) {  // batches of 4 independent loads, first hit wins
+          for (; k + 4 <= len && !hit; k += 4) {
+            int32_t c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) c[u] = g.colval[base + (k + u) * g.C];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (!hit && bit_get(frontier, c[u] - 1) &&
+                  (UNIT || edge_on(val, base + (k + u) * g.C)))
+                hit = c[u];
+          }
         }
-      }
-      if (hit) {
-        out[row] = PARENTS ? hit : level;
-        bit_claim(next, row);
-        newly = bit_claim(visited, row);
+        for (; k < len && !(hit && !PARENTS); ++k) {
+          const int64_t i = base + k * g.C;
+          const int32_t col = g.colval[i];
+          if (!has_len && col == -1) break;
+          if (bit_get(frontier, col - 1) && (PARENTS || UNIT || edge_on(val, i))) {
+            // parents: GPUGraphs_any keeps the largest original column
+            hit = PARENTS ? max(hit, (int32_t)orig_of(g, col - 1) + 1) : col;
+          }
+        }
+        if (hit) {
+          out[row] = PARENTS ? hit : level;
+          bit_claim(next, rid);
+          newly = bit_claim(visited, rid);
+        }
       }
     }
     count_wave(newly, &cnt->found);
@@ -112,8 +131,9 @@ __global__ __launch_bounds__(kB) void bfs_pull_sell_kernel(
     e = min(s + g.seg_len, g.heavy_ptr[h + 1]);
   }
   const int64_t row = g.heavy_row[h] - 1;
+  const int64_t rid = heavy_rid(g, h);
   if (threadIdx.x == 0) {
-    s_skip = bit_get(visited, row);
+    s_skip = bit_get(visited, rid);
     s_hit = 0;
   }
   __syncthreads();
@@ -128,7 +148,7 @@ __global__ __launch_bounds__(kB) void bfs_pull_sell_kernel(
         __atomic_store_n(&s_hit, 1, __ATOMIC_RELAXED);
         break;
       }
-      best = max(best, col);
+      best = max(best, (int32_t)orig_of(g, col - 1) + 1);
     }
   }
   if (PARENTS) {
@@ -145,8 +165,8 @@ __global__ __launch_bounds__(kB) void bfs_pull_sell_kernel(
     }
     if (hit) {
       out[row] = PARENTS ? hit : level;
-      bit_claim(next, row);
-      if (bit_claim(visited, row)) atomicAdd(&cnt->found, 1u);
+      bit_claim(next, rid);
+      if (bit_claim(visited, rid)) atomicAdd(&cnt->found, 1u);
     }
   }
 }
@@ -208,7 +228,7 @@ __global__ __launch_bounds__(kB) void bfs_push_kernel(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
     const int32_t* __restrict__ queue, int64_t qlen, int logL, int32_t* __restrict__ next_queue,
     uint32_t* __restrict__ next, uint32_t* __restrict__ visited, int32_t* __restrict__ dist,
-    int32_t level, BfsCounters* cnt) {
+    int32_t level, const int32_t* __restrict__ vinv, BfsCounters* cnt) {
   const int L = 1 << logL;
   const int wl = threadIdx.x & 63;
   const int lane = wl & (L - 1);
@@ -231,10 +251,11 @@ __global__ __launch_bounds__(kB) void bfs_push_kernel(
       int32_t v = 0;
       if (i < e) {
         v = colval[i] - 1;
-        if (!bit_get(visited, v) && bit_claim(visited, v)) {
+        const int64_t r = vinv ? (int64_t)vinv[v] - 1 : v;  // bitmaps in state order
+        if (!bit_get(visited, r) && bit_claim(visited, r)) {
           newly = true;
           dist[v] = level;
-          bit_claim(next, v);
+          bit_claim(next, r);
         }
       }
       const unsigned long long b = __ballot(newly);
@@ -255,9 +276,9 @@ __global__ __launch_bounds__(kB) void bfs_push_kernel(
   }
 }
 
-// Bitmap -> queue of set bits (ascending order is not required).
+// Bitmap -> queue of the set bits' original vertices (order not required).
 __global__ __launch_bounds__(kB) void bitmap_to_queue_kernel(const uint32_t* __restrict__ bm,
-                                                             int64_t words, int64_t n,
+                                                             int64_t words, int64_t n, SellView g,
                                                              int32_t* __restrict__ queue,
                                                              BfsCounters* cnt) {
   const int64_t w = (int64_t)blockIdx.x * kB + threadIdx.x;
@@ -278,7 +299,7 @@ __global__ __launch_bounds__(kB) void bitmap_to_queue_kernel(const uint32_t* __r
   while (bits) {
     const int b = __ffs(bits) - 1;
     bits &= bits - 1;
-    queue[pos++] = (int32_t)(w * 32 + b);
+    queue[pos++] = (int32_t)orig_of(g, w * 32 + b);
   }
 }
 
@@ -378,15 +399,19 @@ static int pull_level(gg_ctx* ctx, const gg_matrix* A, bool parents, BfsWork& w,
   });
 }
 
+static const int32_t* vinv_of(const gg_matrix* A) {
+  return (A->format == GG_SELL && (A->flags & GG_FLAG_INTERNAL_IDS)) ? A->vinv : nullptr;
+}
+
 static int bfs_begin(gg_ctx* ctx, const gg_matrix* A, int64_t source, int32_t* out,
                      int32_t src_val, BfsWork& w, bool queues) {
-  GG_TRY(bfs_work(ctx, A->m, queues, w));
+  GG_TRY(bfs_work(ctx, state_size(A), queues, w));
   hipLaunchKernelGGL(fill_i32v_kernel, dim3(blocks_for(A->m, kB)), dim3(kB), 0, ctx->stream, out,
                      A->m, (int32_t)2147483647);
   fill_words(ctx, w.visited, w.words, 0u);
   fill_words(ctx, w.front, w.words, 0u);
   hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(1), 0, ctx->stream, out, src_val, w.visited,
-                     w.front, queues ? w.q0 : nullptr, source - 1);
+                     w.front, queues ? w.q0 : nullptr, source - 1, vinv_of(A));
   GG_HIP(hipGetLastError());
   return GG_OK;
 }
@@ -477,6 +502,9 @@ int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_ou
   BfsWork w;
   GG_TRY(bfs_begin(ctx, A_T, source, dist, 0, w, true));
   const int64_t n = A_T->m;
+  const int32_t* vinv = vinv_of(A_T);
+  SellView g{};  // state index -> original vertex for the queue rebuild
+  if (A_T->format == GG_SELL) g = sell_view(A_T);
   BfsCounters hc{};
   hipLaunchKernelGGL(degree_kernel, dim3(1), dim3(1), 0, ctx->stream, A_out->ptr, source - 1, w.cnt);
   GG_TRY(read_counters(ctx, w, &hc));
@@ -501,7 +529,7 @@ int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_ou
       if (awake == 0) break;
       hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
       hipLaunchKernelGGL(bitmap_to_queue_kernel, dim3(blocks_for(w.words, kB)), dim3(kB), 0,
-                         ctx->stream, w.front, w.words, n, w.q0, w.cnt);
+                         ctx->stream, w.front, w.words, w.m, g, w.q0, w.cnt);
       GG_HIP(hipGetLastError());
       GG_TRY(read_counters(ctx, w, &hc));
       qlen = hc.qlen;
@@ -518,7 +546,7 @@ int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_ou
         HotTimer ht(ctx);
         hipLaunchKernelGGL(bfs_push_kernel, dim3(nb), dim3(kB), 0, ctx->stream, A_out->ptr,
                            A_out->idx, w.q0, qlen, lg_push, w.q1, w.next, w.visited, dist, level,
-                           w.cnt);
+                           vinv, w.cnt);
       }
       GG_HIP(hipGetLastError());
       GG_TRY(read_counters(ctx, w, &hc));

@@ -41,7 +41,27 @@ struct SellView {
   int64_t seg_len, n_seg;
   const int32_t* heavy_col;
   const void* heavy_val;
+  // internal ids (GG_FLAG_INTERNAL_IDS): state arrays indexed by slot, then
+  // slot_base + heavy index; otherwise by original row
+  bool internal;
+  int64_t slot_base;  // nslices * slice_size
+  int64_t n_int;      // size of the state index space
+  const int32_t* vinv;
 };
+
+// State index of a heavy row.
+__device__ __forceinline__ int64_t heavy_rid(const SellView& g, int64_t h) {
+  return g.internal ? g.slot_base + h : (int64_t)g.heavy_row[h] - 1;
+}
+// Original (0-based) row of a state index.
+__device__ __forceinline__ int64_t orig_of(const SellView& g, int64_t id) {
+  if (!g.internal) return id;
+  return id < g.slot_base ? (int64_t)g.perm[id] - 1 : (int64_t)g.heavy_row[id - g.slot_base] - 1;
+}
+// State index of an original (0-based) row.
+__device__ __forceinline__ int64_t rid_of(const SellView& g, int64_t orig) {
+  return g.internal ? (int64_t)g.vinv[orig] - 1 : orig;
+}
 
 inline SellView sell_view(const gg_matrix* A) {
   SellView v{};
@@ -67,7 +87,18 @@ inline SellView sell_view(const gg_matrix* A) {
   v.n_seg = A->n_heavy > 0 ? A->heavy_nseg : 0;
   v.heavy_col = A->heavy_col;
   v.heavy_val = A->heavy_val;
+  v.internal = (A->flags & GG_FLAG_INTERNAL_IDS) != 0;
+  v.slot_base = A->nslices * A->slice_size;
+  v.n_int = v.internal ? v.slot_base + A->n_heavy : A->m;
+  v.vinv = A->vinv;
   return v;
+}
+
+// Size of the index space traversal state lives in (rows of A_T for CSR).
+inline int64_t state_size(const gg_matrix* A) {
+  if (A->format == GG_SELL && (A->flags & GG_FLAG_INTERNAL_IDS))
+    return A->nslices * A->slice_size + A->n_heavy;
+  return A->m;
 }
 
 // Slot -> (row, first entry, entry count); false for an empty / out-of-range slot.
@@ -90,6 +121,24 @@ __device__ __forceinline__ bool sell_slot(const SellView& g, int64_t slot, int64
   has_len = g.slot_len != nullptr;
   len = has_len ? (int64_t)g.slot_len[slot] : ((int64_t)g.slice_ptr[sl + 1] - g.slice_ptr[sl]) / g.C;
   return true;
+}
+
+// x in internal order: xp[slot] = x[perm[slot]] (0 for an empty slot),
+// xp[slot_base + h] = x[heavy_row[h]]; rows of k values when k > 1 (B is
+// m x k in b_layout, xp is row-major).
+template <class T>
+__global__ void gather_internal_kernel(const T* __restrict__ x, const int32_t* __restrict__ perm,
+                                       int64_t slot_base, const int32_t* __restrict__ heavy_row,
+                                       int64_t n_heavy, int64_t m, int64_t k, int b_rowmajor,
+                                       T* __restrict__ xp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n_int = slot_base + n_heavy;
+  if (t >= n_int * k) return;
+  const int64_t i = t / k, j = t - i * k;
+  int64_t o;
+  if (i < slot_base) o = perm ? (int64_t)perm[i] - 1 : -1;
+  else o = (int64_t)heavy_row[i - slot_base] - 1;
+  xp[t] = o < 0 ? T(0) : x[b_rowmajor ? o * k + j : o + j * m];
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v);

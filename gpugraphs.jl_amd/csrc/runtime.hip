@@ -62,6 +62,8 @@ int check_matrix(const gg_matrix* A) {
     return set_error(GG_ERR_ARGUMENT, "unknown matrix value type %d", A->val_dtype);
   switch (A->format) {
     case GG_CSR: case GG_CSC:
+      if (A->flags & GG_FLAG_INTERNAL_IDS)
+        return set_error(GG_ERR_ARGUMENT, "internal ids exist only for a relabelled SELL");
       return GG_OK;
     case GG_SELL:
       if (A->slice_size < 1 && A->m > 0)
@@ -83,6 +85,9 @@ int check_matrix(const gg_matrix* A) {
       if (A->n_heavy > 0 && (!A->heavy_row || !A->heavy_ptr || !A->heavy_seg ||
                              !A->heavy_col || A->heavy_seg_len < 1 || A->heavy_nseg < A->n_heavy))
         return set_error(GG_ERR_ARGUMENT, "SELL heavy partition incomplete");
+      if ((A->flags & GG_FLAG_INTERNAL_IDS) &&
+          (!A->vinv || A->m != A->n || (A->nslices > 0 && !A->perm)))
+        return set_error(GG_ERR_ARGUMENT, "relabelled SELL needs vinv, a permutation and m == n");
       return GG_OK;
   }
   return set_error(GG_ERR_ARGUMENT, "unknown matrix format %d", A->format);

@@ -62,11 +62,14 @@ __global__ void serve_bfs_fill_kernel(int32_t* dist, int64_t m, SeedList sl) {
   dist[v + (int64_t)sl.slot[q] * m] = v == sl.src[q] ? 0 : 2147483647;
 }
 
-__global__ void serve_bfs_seed_kernel(SeedList sl, uint64_t* front, uint64_t* seen,
-                                      uint32_t* fany, int32_t* start, int32_t level) {
+// Masks at state indices (vinv maps an original vertex to its internal id + 1
+// for a relabelled SELL; NULL otherwise).
+__global__ void serve_bfs_seed_kernel(SeedList sl, const int32_t* vinv, uint64_t* front,
+                                      uint64_t* seen, uint32_t* fany, int32_t* start,
+                                      int32_t level) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   for (int q = 0; q < sl.n; ++q) {
-    const int64_t v = sl.src[q];
+    const int64_t v = vinv ? (int64_t)vinv[sl.src[q]] - 1 : sl.src[q];
     const uint64_t b = 1ull << sl.slot[q];
     front[v] |= b;
     seen[v] |= b;
@@ -80,18 +83,22 @@ __global__ void serve_zero_kernel(unsigned int* found, unsigned long long* mask)
   *mask = 0ull;
 }
 
+// Working set rows are state indices (ns of them).
 template <class TD>
-__global__ void serve_sssp_seed_kernel(TD* d, int64_t m, int k, SeedList sl) {
+__global__ void serve_sssp_seed_kernel(TD* d, int64_t ns, int k, SeedList sl,
+                                       const int32_t* vinv) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m * sl.n) return;
-  const int64_t q = i / m, v = i - q * m;
-  d[v * k + sl.slot[q]] = v == sl.src[q] ? TD(0) : Lim<TD>::max();
+  if (i >= ns * sl.n) return;
+  const int64_t q = i / ns, r = i - q * ns;
+  const int64_t rs = vinv ? (int64_t)vinv[sl.src[q]] - 1 : sl.src[q];
+  d[r * k + sl.slot[q]] = r == rs ? TD(0) : Lim<TD>::max();
 }
 
 template <class TD>
-__global__ void serve_extract_kernel(const TD* d, int64_t m, int k, int slot, TD* out) {
+__global__ void serve_extract_kernel(const TD* d, int64_t m, int k, int slot,
+                                     const int32_t* vinv, TD* out) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < m) out[v] = d[v * k + slot];
+  if (v < m) out[v] = d[(vinv ? (int64_t)vinv[v] - 1 : v) * k + slot];
 }
 
 template <class TD> __global__ void serve_fill_kernel(TD* p, int64_t n, TD v) {
@@ -167,7 +174,9 @@ struct Engine {
   int kind;  // GG_QUERY_BFS / GG_QUERY_SSSP
   int slots;
   gg_matrix A;
-  int64_t m;
+  int64_t m;                    // vertices (result length)
+  int64_t ns;                   // rows of the traversal state
+  const int32_t* vinv = nullptr;  // original -> internal id + 1 (relabelled SELL)
   hipStream_t stream = nullptr;
   hipEvent_t step_done = nullptr;
   bool inflight = false;
@@ -228,27 +237,29 @@ static int make_engine(gg_server* s, const gg_matrix* A, int kind, std::unique_p
   e->kind = kind;
   e->A = *A;
   e->m = A->m;
+  e->ns = state_size(A);
+  e->vinv = (A->format == GG_SELL && (A->flags & GG_FLAG_INTERNAL_IDS)) ? A->vinv : nullptr;
   e->slots = kind == GG_QUERY_BFS ? 64 : 32;
   e->slot_q.assign(e->slots, -1);
-  const int64_t m = A->m;
+  const int64_t m = A->m, ns = e->ns;
   GG_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   GG_HIP(hipEventCreateWithFlags(&e->step_done, hipEventDisableTiming));
   GG_HIP(hipHostMalloc((void**)&e->mask_h, 64, hipHostMallocDefault));
   void* p;
   GG_TRY(engine_alloc(s, *e, 64, &p)); e->mask_d = (unsigned long long*)p;
   GG_TRY(engine_alloc(s, *e, 64, &p)); e->cnt_d = (unsigned int*)p;
-  const unsigned nb = blocks_for(m, 256);
+  const unsigned nb = blocks_for(ns, 256);
   if (kind == GG_QUERY_BFS) {
-    const int64_t words = (m + 31) / 32;
-    GG_TRY(engine_alloc(s, *e, 8 * m, &p)); e->front = (uint64_t*)p;
-    GG_TRY(engine_alloc(s, *e, 8 * m, &p)); e->next = (uint64_t*)p;
-    GG_TRY(engine_alloc(s, *e, 8 * m, &p)); e->seen = (uint64_t*)p;
+    const int64_t words = (ns + 31) / 32;
+    GG_TRY(engine_alloc(s, *e, 8 * ns, &p)); e->front = (uint64_t*)p;
+    GG_TRY(engine_alloc(s, *e, 8 * ns, &p)); e->next = (uint64_t*)p;
+    GG_TRY(engine_alloc(s, *e, 8 * ns, &p)); e->seen = (uint64_t*)p;
     GG_TRY(engine_alloc(s, *e, 4 * words, &p)); e->fany = (uint32_t*)p;
     GG_TRY(engine_alloc(s, *e, 4 * words, &p)); e->nany = (uint32_t*)p;
     GG_TRY(engine_alloc(s, *e, 4 * m * 64, &p)); e->dist = (int32_t*)p;
     GG_TRY(engine_alloc(s, *e, 4 * 64, &p)); e->start = (int32_t*)p;
-    hipLaunchKernelGGL(serve_fill_kernel<uint64_t>, dim3(nb), dim3(256), 0, e->stream, e->front, m, 0ull);
-    hipLaunchKernelGGL(serve_fill_kernel<uint64_t>, dim3(nb), dim3(256), 0, e->stream, e->seen, m, 0ull);
+    hipLaunchKernelGGL(serve_fill_kernel<uint64_t>, dim3(nb), dim3(256), 0, e->stream, e->front, ns, 0ull);
+    hipLaunchKernelGGL(serve_fill_kernel<uint64_t>, dim3(nb), dim3(256), 0, e->stream, e->seen, ns, 0ull);
     hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(words, 256)), dim3(256), 0,
                        e->stream, e->fany, words, 0u);
   } else {
@@ -258,12 +269,12 @@ static int make_engine(gg_server* s, const gg_matrix* A, int kind, std::unique_p
         e->dist_dtype != GG_FLOAT64)
       return set_error(GG_ERR_ARGUMENT, "unsupported SSSP value type");
     for (int b = 0; b < 2; ++b) {
-      GG_TRY(engine_alloc(s, *e, e->esz * m * 32, &p));
+      GG_TRY(engine_alloc(s, *e, e->esz * ns * 32, &p));
       e->buf[b] = p;
       GG_TRY(with_type(e->dist_dtype, [&](auto t) -> int {
         typedef typename decltype(t)::type TD;
-        hipLaunchKernelGGL(serve_fill_kernel<TD>, dim3(blocks_for(m * 32, 256)), dim3(256), 0,
-                           e->stream, (TD*)p, m * 32, Lim<TD>::max());
+        hipLaunchKernelGGL(serve_fill_kernel<TD>, dim3(blocks_for(ns * 32, 256)), dim3(256), 0,
+                           e->stream, (TD*)p, ns * 32, Lim<TD>::max());
         return GG_OK;
       }));
     }
@@ -312,7 +323,7 @@ static int retire(gg_server* s, Engine& e) {
       GG_TRY(with_type(e.dist_dtype, [&](auto tt) -> int {
         typedef typename decltype(tt)::type TD;
         hipLaunchKernelGGL(serve_extract_kernel<TD>, dim3(blocks_for(e.m, 256)), dim3(256), 0,
-                           e.stream, (const TD*)e.buf[e.parity], e.m, 32, sl, (TD*)dst);
+                           e.stream, (const TD*)e.buf[e.parity], e.m, 32, sl, e.vinv, (TD*)dst);
         return GG_OK;
       }));
     }
@@ -351,7 +362,7 @@ static void admit(gg_server* s, Engine& e) {
 }
 
 static int launch_step(gg_server* s, Engine& e) {
-  const int64_t m = e.m;
+  const int64_t m = e.m, ns = e.ns;
   SeedList sl{};
   sl.n = (int)e.seeds.size();
   uint64_t seedmask = 0;
@@ -363,14 +374,14 @@ static int launch_step(gg_server* s, Engine& e) {
   e.seeds.clear();
   hipLaunchKernelGGL(serve_zero_kernel, dim3(1), dim3(1), 0, e.stream, e.cnt_d, e.mask_d);
   if (e.kind == GG_QUERY_BFS) {
-    const int64_t words = (m + 31) / 32;
+    const int64_t words = (ns + 31) / 32;
     if (sl.n) {
-      hipLaunchKernelGGL(serve_bfs_clear_kernel, dim3(blocks_for(m, 256)), dim3(256), 0,
-                         e.stream, e.seen, e.front, m, seedmask);
+      hipLaunchKernelGGL(serve_bfs_clear_kernel, dim3(blocks_for(ns, 256)), dim3(256), 0,
+                         e.stream, e.seen, e.front, ns, seedmask);
       hipLaunchKernelGGL(serve_bfs_fill_kernel, dim3(blocks_for(m * sl.n, 256)), dim3(256), 0,
                          e.stream, e.dist, m, sl);
-      hipLaunchKernelGGL(serve_bfs_seed_kernel, dim3(1), dim3(1), 0, e.stream, sl, e.front,
-                         e.seen, e.fany, e.start, e.level);
+      hipLaunchKernelGGL(serve_bfs_seed_kernel, dim3(1), dim3(1), 0, e.stream, sl, e.vinv,
+                         e.front, e.seen, e.fany, e.start, e.level);
     }
     hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(words, 256)), dim3(256), 0,
                        e.stream, e.nany, words, 0u);
@@ -397,8 +408,8 @@ static int launch_step(gg_server* s, Engine& e) {
     if (sl.n) {
       GG_TRY(with_type(e.dist_dtype, [&](auto tt) -> int {
         typedef typename decltype(tt)::type TD;
-        hipLaunchKernelGGL(serve_sssp_seed_kernel<TD>, dim3(blocks_for(m * sl.n, 256)), dim3(256),
-                           0, e.stream, (TD*)e.buf[e.parity], m, 32, sl);
+        hipLaunchKernelGGL(serve_sssp_seed_kernel<TD>, dim3(blocks_for(ns * sl.n, 256)), dim3(256),
+                           0, e.stream, (TD*)e.buf[e.parity], ns, 32, sl, e.vinv);
         return GG_OK;
       }));
     }
@@ -535,6 +546,9 @@ int gg_server_submit(gg_server* s, int graph_id, int kind, int64_t source, int64
   if (kind != GG_QUERY_BFS && kind != GG_QUERY_SSSP)
     return set_error(GG_ERR_ARGUMENT, "unknown query kind %d", kind);
   Engine& e = *s->engines[2 * graph_id + (kind == GG_QUERY_BFS ? 0 : 1)];
+  if (kind == GG_QUERY_BFS && (e.A.val_dtype == GG_FLOAT32 || e.A.val_dtype == GG_FLOAT64) &&
+      !(e.A.flags & GG_FLAG_UNIT_PATTERN))
+    return set_error(GG_ERR_ARGUMENT, "or-and BFS needs Bool or integer matrix values");
   if (source < 1 || source > e.m)
     return set_error(GG_ERR_BOUNDS, "source %lld outside 1:%lld", (long long)source, (long long)e.m);
   {

@@ -11,7 +11,7 @@
 // load) and, with a row-major operand, k consecutive B values of the same
 // vertex (one coalesced line); the column-major layout of the reference's
 // Julia matrices is accepted as well.
-#include "internal.hpp"
+#include "frontier.hpp"
 
 namespace gg {
 
@@ -42,7 +42,22 @@ struct MmArgs {
   const void* heavy_val;
   int b_rowmajor, c_rowmajor;
   int64_t r0, nrows;
+  // internal ids: columns are slot / slot_base + heavy index; secondi maps
+  // them back through perm / heavy_row
+  int internal;
+  int64_t slot_base;
+  const int32_t* perm;
+  const int32_t* heavy_row;
 };
+
+template <class S>
+__device__ __forceinline__ int32_t mm_col(const MmArgs& g, int32_t c) {
+  if constexpr (S::MUL == GG_OP_SECONDI) {
+    if (g.internal)
+      return c - 1 < g.slot_base ? g.perm[c - 1] : g.heavy_row[c - 1 - g.slot_base];
+  }
+  return c;
+}
 
 template <class S, class TA, class TB>
 __global__ __launch_bounds__(kBlk) void spmm_kernel(typename MmTypes<S, TA, TB>::TC* __restrict__ Cm,
@@ -80,7 +95,7 @@ __global__ __launch_bounds__(kBlk) void spmm_kernel(typename MmTypes<S, TA, TB>:
         for (int64_t i = g.heavy_ptr[h]; i < g.heavy_ptr[h + 1]; ++i) {
           if (T::MO::has_absorb && acc == T::MO::absorb()) break;
           const int32_t col = g.heavy_col[i];
-          acc = T::MO::op(acc, A(T::MU::ap(CT(hv[i]), bval(col), col)));
+          acc = T::MO::op(acc, A(T::MU::ap(CT(hv[i]), bval(col), mm_col<S>(g, col))));
         }
       } else {
         const int64_t slot = g.slot_of_row ? g.slot_of_row[row] : row;
@@ -92,7 +107,7 @@ __global__ __launch_bounds__(kBlk) void spmm_kernel(typename MmTypes<S, TA, TB>:
           const int64_t i = base + kk * g.C;
           const int32_t col = g.idx[i];
           if (col == -1 || (T::MO::has_absorb && acc == T::MO::absorb())) break;
-          acc = T::MO::op(acc, A(T::MU::ap(CT(val[i]), bval(col), col)));
+          acc = T::MO::op(acc, A(T::MU::ap(CT(val[i]), bval(col), mm_col<S>(g, col))));
         }
       }
     }
@@ -179,6 +194,22 @@ int launch_spmm(gg_ctx* ctx, void* C, int c_dtype, int c_layout, const gg_matrix
       return set_error(GG_ERR_ARGUMENT, "result element type must equal promote_type(eltype(A), eltype(B))");
     if (!accum_ok(accum, std::is_floating_point<TC>::value))
       return set_error(GG_ERR_UNSUPPORTED, "accum operator %d not supported for this type", accum);
+    if (A->format == GG_SELL && (A->flags & GG_FLAG_INTERNAL_IDS)) {
+      // columns are internal ids: B rows gathered into internal order, row-major
+      const int64_t n_int = A->nslices * A->slice_size + A->n_heavy;
+      void* bp;
+      GG_TRY(ws_get(ctx, WS_XPERM, sizeof(TB) * (size_t)(n_int * k), &bp));
+      hipLaunchKernelGGL(gather_internal_kernel<TB>, dim3(blocks_for(n_int * k, kBlk)), dim3(kBlk),
+                         0, ctx->stream, (const TB*)B, A->perm, A->nslices * A->slice_size,
+                         A->heavy_row, A->n_heavy, A->n, k, g.b_rowmajor, (TB*)bp);
+      B = bp;
+      g.b_rowmajor = 1;
+      g.n = n_int;
+      g.internal = 1;
+      g.slot_base = A->nslices * A->slice_size;
+      g.perm = A->perm;
+      g.heavy_row = A->heavy_row;
+    }
     const int64_t groups = nrows * ((k + (1 << lg) - 1) >> lg);
     unsigned nb = blocks_for(groups << lg, kBlk);
     const unsigned cap = (unsigned)ctx->num_cus * 64;

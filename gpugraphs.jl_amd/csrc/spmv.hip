@@ -22,11 +22,13 @@
 // CSC (src/spmv/CSC.jl:2-21): column-parallel atomic scatter
 // c[row] += b[col] * a, plus-times only, add / accum ignored as in the
 // reference.
-#include "internal.hpp"
+#include "frontier.hpp"
 
 namespace gg {
 
 constexpr int kBlock = 256;
+constexpr int kU = 4;  // entries per batch of a SELL slot walk
+constexpr int kH = 8;  // entries per thread per batch of a heavy segment
 
 template <class S, class TA, class TX>
 struct Types {
@@ -106,7 +108,20 @@ struct SellArgs {
   const int32_t* heavy_col;
   const void* heavy_val;
   void* partial;  // [n_seg] accumulators of multi-segment rows
+  int internal;       // columns are internal ids (slot, then slot_base + heavy index)
+  int64_t slot_base;  // nslices * C
 };
+
+// Column handed to the multiply: secondi reads it, and with internal ids it
+// must be the original column (perm / heavy_row of the internal id).
+template <class S>
+__device__ __forceinline__ int32_t mul_col(const SellArgs& g, int32_t c) {
+  if constexpr (S::MUL == GG_OP_SECONDI) {
+    if (g.internal)
+      return c - 1 < g.slot_base ? g.perm[c - 1] : g.heavy_row[c - 1 - g.slot_base];
+  }
+  return c;
+}
 
 template <class MO, class A>
 __device__ __forceinline__ A block_reduce(A v, A* lds) {
@@ -146,18 +161,35 @@ __global__ __launch_bounds__(kBlock) void sell_spmv_kernel(
     const TA* __restrict__ val = (const TA*)g.val;
     A acc = T::MO::neutral();
     if (g.slot_len) {
+      // batches of kU entries: all index / value loads, then all gathers, then
+      // the folds in storage order, so kU loads of each kind are in flight
       const int32_t len = g.slot_len[slot];
       int64_t i = base;
-      for (int32_t k = 0; k < len; ++k, i += g.C) {
+      int32_t k = 0;
+      for (; k + kU <= len; k += kU, i += kU * g.C) {
+        int32_t c[kU];
+        TA v[kU];
+        TX xv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) c[u] = g.colval[i + u * g.C];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) v[u] = val[i + u * g.C];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) xv[u] = x[c[u] - 1];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+          acc = T::MO::op(acc, term<S, TA, TX>(v[u], xv[u], mul_col<S>(g, c[u])));
+      }
+      for (; k < len; ++k, i += g.C) {
         const int32_t col = g.colval[i];
-        acc = T::MO::op(acc, term<S, TA, TX>(val[i], x[col - 1], col));
+        acc = T::MO::op(acc, term<S, TA, TX>(val[i], x[col - 1], mul_col<S>(g, col)));
       }
     } else {
       const int64_t end = (int64_t)g.slice_ptr[sl + 1] - 1;
       for (int64_t i = base; i < end; i += g.C) {
         const int32_t col = g.colval[i];
         if (col == -1) break;
-        acc = T::MO::op(acc, term<S, TA, TX>(val[i], x[col - 1], col));
+        acc = T::MO::op(acc, term<S, TA, TX>(val[i], x[col - 1], mul_col<S>(g, col)));
       }
     }
     store_acc<typename T::TC, typename T::CT>(c, row, accum, acc);
@@ -178,9 +210,22 @@ __global__ __launch_bounds__(kBlock) void sell_spmv_kernel(
   const int64_t e = min(s + g.seg_len, g.heavy_ptr[lo + 1]);
   const TA* __restrict__ hv = (const TA*)g.heavy_val;
   A acc = T::MO::neutral();
-  for (int64_t i = s + threadIdx.x; i < e; i += kBlock) {
-    const int32_t col = g.heavy_col[i];
-    acc = T::MO::op(acc, term<S, TA, TX>(hv[i], x[col - 1], col));
+  for (int64_t i0 = s + threadIdx.x; i0 < e; i0 += kH * kBlock) {
+    int32_t c[kH];
+    TA v[kH];
+    TX xv[kH];
+#pragma unroll
+    for (int u = 0; u < kH; ++u) {
+      const int64_t i = i0 + (int64_t)u * kBlock;
+      c[u] = i < e ? g.heavy_col[i] : 1;  // column 1 is a valid, unused address
+      v[u] = i < e ? hv[i] : TA(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kH; ++u) xv[u] = x[c[u] - 1];
+#pragma unroll
+    for (int u = 0; u < kH; ++u)
+      if (i0 + (int64_t)u * kBlock < e)
+        acc = T::MO::op(acc, term<S, TA, TX>(v[u], xv[u], mul_col<S>(g, c[u])));
   }
   acc = block_reduce<typename T::MO>(acc, lds);
   if (threadIdx.x == 0) {
@@ -278,6 +323,16 @@ int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const voi
       return GG_OK;
     }
     // SELL
+    const TX* xin = (const TX*)B;
+    if (A->flags & GG_FLAG_INTERNAL_IDS) {  // columns are internal ids: gather x first
+      const int64_t n_int = A->nslices * A->slice_size + A->n_heavy;
+      void* xp;
+      GG_TRY(ws_get(ctx, WS_XPERM, sizeof(TX) * (size_t)n_int, &xp));
+      hipLaunchKernelGGL(gather_internal_kernel<TX>, dim3(blocks_for(n_int, kBlock)), dim3(kBlock),
+                         0, ctx->stream, (const TX*)B, A->perm, A->nslices * A->slice_size,
+                         A->heavy_row, A->n_heavy, A->m, (int64_t)1, 1, (TX*)xp);
+      xin = (const TX*)xp;
+    }
     SellArgs g{};
     g.slice_ptr = A->ptr;
     g.colval = A->idx;
@@ -288,6 +343,8 @@ int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const voi
     g.logC = ilog2_exact(A->slice_size);
     g.n_slots = sell_slots(A);
     g.m = A->m;
+    g.internal = (A->flags & GG_FLAG_INTERNAL_IDS) ? 1 : 0;
+    g.slot_base = A->nslices * A->slice_size;
     g.r_lo = r0;
     g.r_hi = r0 + nrows;
     g.n_sell_blocks = (g.n_slots + kBlock - 1) / kBlock;
@@ -308,7 +365,7 @@ int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const voi
     {
       HotTimer ht(ctx);
       hipLaunchKernelGGL((sell_spmv_kernel<S, TA, TX>), dim3((unsigned)nb), dim3(kBlock), 0,
-                         ctx->stream, (TC*)C, (const TX*)B, g, accum, mask, mask_dtype);
+                         ctx->stream, (TC*)C, xin, g, accum, mask, mask_dtype);
     }
     GG_HIP(hipGetLastError());
     if (n_seg > A->n_heavy) {

@@ -84,9 +84,10 @@ __global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_
         if (!has_len && col == -1) break;
         acc = jmin(acc, sadd(TD(val[i]), din[col - 1]));
       }
-      const TD dv = din[row];
+      const int64_t rid = g.internal ? slot : row;
+      const TD dv = din[rid];
       const TD nv = jmin(dv, acc);
-      dout[row] = nv;
+      dout[rid] = nv;
       ch = nv < dv;
     }
     count_changes(ch, changed);
@@ -104,10 +105,10 @@ __global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < kT / 64; ++w) acc = jmin(acc, red[w]);
-    const int64_t row = g.heavy_row[h] - 1;
-    const TD dv = din[row];
+    const int64_t rid = heavy_rid(g, h);
+    const TD dv = din[rid];
     const TD nv = jmin(dv, acc);
-    dout[row] = nv;
+    dout[rid] = nv;
     if (nv < dv) atomicAdd(changed, 1u);
   }
 }
@@ -147,6 +148,7 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
         int64_t base, len;
         bool has_len;
         if (sell_slot(g, unit, row, base, len, has_len)) {
+          if (g.internal) row = unit;  // working set in internal order
           const TA* val = (const TA*)g.val;
           for (int64_t kk = 0; kk < len; ++kk) {
             const int64_t i = base + kk * g.C;
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
   if (grp == 0) {
     for (int64_t o = 1; o < gpb; ++o) acc = jmin(acc, red[o * Lk + lane]);
     if (j < k) {
-      const int64_t row = g.heavy_row[h] - 1;
+      const int64_t row = heavy_rid(g, h);
       const TD dv = din[row * k + j];
       const TD nv = jmin(dv, acc);
       dout[row * k + j] = nv;
@@ -205,9 +207,12 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
 }
 
 template <class TD>
-__global__ void bf_multi_seed_kernel(TD* d, const int32_t* sources, int64_t k) {
+__global__ void bf_multi_seed_kernel(TD* d, const int32_t* sources, int64_t k,
+                                     const int32_t* vinv) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < k) d[(int64_t)(sources[i] - 1) * k + i] = TD(0);
+  if (i >= k) return;
+  const int64_t v = sources[i] - 1;
+  d[(vinv ? (int64_t)vinv[v] - 1 : v) * k + i] = TD(0);
 }
 
 template <class TD>
@@ -217,18 +222,21 @@ __global__ void fill_td_kernel(TD* p, int64_t n, TD v) {
 }
 
 template <class TD>
-__global__ void seed_one_kernel(TD* p, int64_t s) {
-  p[s] = TD(0);
+__global__ void seed_one_kernel(TD* p, int64_t s, const int32_t* vinv) {
+  p[vinv ? (int64_t)vinv[s] - 1 : s] = TD(0);
 }
 
-// row-major m x k -> column-major m x k
+// Row-major working set (state rows, k columns) -> result at original rows,
+// column-major m x k (or row-major when rowmajor).  State row of original v:
+// vinv[v] - 1 for a relabelled SELL, else v.
 template <class TD>
-__global__ void rm_to_cm_kernel(const TD* __restrict__ src, TD* __restrict__ dst, int64_t m,
-                                int64_t k) {
+__global__ void state_to_result_kernel(const TD* __restrict__ src, const int32_t* __restrict__ vinv,
+                                       int64_t m, int64_t k, int rowmajor, TD* __restrict__ dst) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m * k) return;
-  const int64_t r = i / k, j = i - r * k;
-  dst[r + j * m] = src[i];
+  const int64_t v = i / k, j = i - v * k;
+  const int64_t r = vinv ? (int64_t)vinv[v] - 1 : v;
+  dst[rowmajor ? i : v + j * m] = src[r * k + j];
 }
 
 __global__ void zero_u32_kernel(unsigned int* p) { *p = 0u; }
@@ -294,13 +302,19 @@ int sssp_one(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist, int di
     typedef typename decltype(a)::type TA;
     typedef typename decltype(d)::type TD;
     const int64_t m = A->m;
-    void *tmp, *cnt;
-    GG_TRY(ws_get(ctx, WS_SSSP0, sizeof(TD) * m, &tmp));
+    // A relabelled SELL iterates in internal order in two workspace buffers
+    // and scatters the fixpoint to dist; otherwise dist is one of the two.
+    const int32_t* vinv = (A->flags & GG_FLAG_INTERNAL_IDS) ? A->vinv : nullptr;
+    const int64_t ns = state_size(A);
+    void *tmp, *tmp2 = nullptr, *cnt;
+    GG_TRY(ws_get(ctx, WS_SSSP0, sizeof(TD) * ns, &tmp));
+    if (vinv) GG_TRY(ws_get(ctx, WS_SSSP1, sizeof(TD) * ns, &tmp2));
     GG_TRY(ws_get(ctx, WS_FLAG, 64, &cnt));
-    TD* buf[2] = {(TD*)dist, (TD*)tmp};
-    hipLaunchKernelGGL(fill_td_kernel<TD>, dim3(blocks_for(m, kT)), dim3(kT), 0, ctx->stream,
-                       buf[0], m, Lim<TD>::max());
-    hipLaunchKernelGGL(seed_one_kernel<TD>, dim3(1), dim3(1), 0, ctx->stream, buf[0], source - 1);
+    TD* buf[2] = {vinv ? (TD*)tmp2 : (TD*)dist, (TD*)tmp};
+    hipLaunchKernelGGL(fill_td_kernel<TD>, dim3(blocks_for(ns, kT)), dim3(kT), 0, ctx->stream,
+                       buf[0], ns, Lim<TD>::max());
+    hipLaunchKernelGGL(seed_one_kernel<TD>, dim3(1), dim3(1), 0, ctx->stream, buf[0], source - 1,
+                       vinv);
     const int lg = log_lanes_for(m ? (double)A->nnz / m : 1.0);
     SellView g = sell_view(A);
     const int64_t nsb = (g.n_slots + kT - 1) / kT;
@@ -323,8 +337,13 @@ int sssp_one(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist, int di
     };
     int64_t last = 0;
     int st = iterate(ctx, (unsigned*)cnt, m + 16, step, iterations, &last);
-    if (st == GG_OK && (last & 1))  // the final buffer is the workspace one
+    if (st == GG_OK && vinv) {
+      hipLaunchKernelGGL(state_to_result_kernel<TD>, dim3(blocks_for(m, kT)), dim3(kT), 0,
+                         ctx->stream, buf[last & 1], vinv, m, (int64_t)1, 1, (TD*)dist);
+      GG_HIP(hipGetLastError());
+    } else if (st == GG_OK && (last & 1)) {  // the final buffer is the workspace one
       GG_HIP(hipMemcpyAsync(dist, tmp, sizeof(TD) * m, hipMemcpyDeviceToDevice, ctx->stream));
+    }
     GG_HIP(hipStreamSynchronize(ctx->stream));
     return st;
   });
@@ -337,7 +356,6 @@ int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_
   return with_ad(A->val_dtype, dist_dtype, [&](auto a, auto d) -> int {
     typedef typename decltype(a)::type TA;
     typedef typename decltype(d)::type TD;
-    const int64_t m = A->m;
     int lg = 0;
     while (lg < 6 && (1LL << lg) < k) ++lg;
     const int64_t Lk = 1LL << lg;
@@ -345,12 +363,12 @@ int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_
     const int64_t gpb = kT / Lk;
     SellView g = sell_view(A);
     if (A->format == GG_CSR) { g.colval = A->idx; g.val = A->val; g.n_heavy = 0; }
-    const int64_t n_light = A->format == GG_CSR ? m : g.n_slots;
+    const int64_t n_light = A->format == GG_CSR ? A->m : g.n_slots;
     const int64_t light_blocks = (n_light * jblocks + gpb - 1) / gpb;
     const int64_t nb = light_blocks + g.n_heavy * jblocks;
     if (nb > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
     hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
-                       stream, A->format, A->ptr, g, n_light, m, k, lg, (const TD*)din, (TD*)dout,
+                       stream, A->format, A->ptr, g, n_light, A->m, k, lg, (const TD*)din, (TD*)dout,
                        changed, changed_mask);
     GG_HIP(hipGetLastError());
     return GG_OK;
@@ -361,21 +379,22 @@ int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_
 // column-major into dist (or row-major when rowmajor_out).
 int sssp_multi(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_host, int64_t k,
                void* dist, int dist_dtype, bool rowmajor_out, int64_t* iterations) {
-  return with_ad(A->val_dtype, dist_dtype, [&](auto a, auto d) -> int {
-    typedef typename decltype(a)::type TA;
+  return with_ad(A->val_dtype, dist_dtype, [&](auto, auto d) -> int {
     typedef typename decltype(d)::type TD;
     const int64_t m = A->m;
+    const int64_t ns = state_size(A);
+    const int32_t* vinv = (A->flags & GG_FLAG_INTERNAL_IDS) ? A->vinv : nullptr;
     void *b0, *b1, *cnt, *src;
-    GG_TRY(ws_get(ctx, WS_MS0, sizeof(TD) * m * k, &b0));
-    GG_TRY(ws_get(ctx, WS_MS1, sizeof(TD) * m * k, &b1));
+    GG_TRY(ws_get(ctx, WS_MS0, sizeof(TD) * ns * k, &b0));
+    GG_TRY(ws_get(ctx, WS_MS1, sizeof(TD) * ns * k, &b1));
     GG_TRY(ws_get(ctx, WS_MS2, sizeof(int32_t) * k, &src));
     GG_TRY(ws_get(ctx, WS_FLAG, 64, &cnt));
     GG_HIP(hipMemcpyAsync(src, sources_host, sizeof(int32_t) * k, hipMemcpyHostToDevice, ctx->stream));
     TD* buf[2] = {(TD*)b0, (TD*)b1};
-    hipLaunchKernelGGL(fill_td_kernel<TD>, dim3(blocks_for(m * k, kT)), dim3(kT), 0, ctx->stream,
-                       buf[0], m * k, Lim<TD>::max());
+    hipLaunchKernelGGL(fill_td_kernel<TD>, dim3(blocks_for(ns * k, kT)), dim3(kT), 0, ctx->stream,
+                       buf[0], ns * k, Lim<TD>::max());
     hipLaunchKernelGGL(bf_multi_seed_kernel<TD>, dim3(blocks_for(k, kT)), dim3(kT), 0,
-                       ctx->stream, buf[0], (const int32_t*)src, k);
+                       ctx->stream, buf[0], (const int32_t*)src, k, vinv);
     auto step = [&](int64_t it) -> int {
       HotTimer ht(ctx);
       return bf_multi_step(ctx->stream, A, dist_dtype, k, buf[(it - 1) & 1], buf[it & 1],
@@ -385,11 +404,11 @@ int sssp_multi(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_host, int
     int st = iterate(ctx, (unsigned*)cnt, m + 16, step, iterations, &last);
     const TD* fin = buf[last & 1];
     if (st == GG_OK) {
-      if (rowmajor_out) {
+      if (rowmajor_out && !vinv) {
         GG_HIP(hipMemcpyAsync(dist, fin, sizeof(TD) * m * k, hipMemcpyDeviceToDevice, ctx->stream));
       } else {
-        hipLaunchKernelGGL(rm_to_cm_kernel<TD>, dim3(blocks_for(m * k, kT)), dim3(kT), 0,
-                           ctx->stream, fin, (TD*)dist, m, k);
+        hipLaunchKernelGGL(state_to_result_kernel<TD>, dim3(blocks_for(m * k, kT)), dim3(kT), 0,
+                           ctx->stream, fin, vinv, m, k, rowmajor_out ? 1 : 0, (TD*)dist);
         GG_HIP(hipGetLastError());
       }
     }

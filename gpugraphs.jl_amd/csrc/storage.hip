@@ -100,18 +100,22 @@ void gg_sell_host_free(gg_sell_host* h) {
   free(h->heavy_seg);
   free(h->heavy_col);
   free(h->heavy_val);
+  free(h->vinv);
   delete h;
 }
 
 int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* colval,
                   const void* nzval, int val_dtype, int64_t slice_size, int64_t sigma,
-                  int64_t heavy_threshold, int64_t heavy_seg_len, gg_sell_host** out) {
+                  int64_t heavy_threshold, int64_t heavy_seg_len, int flags,
+                  gg_sell_host** out) {
   if (!out || (m > 0 && !rowptr)) return set_error(GG_ERR_ARGUMENT, "NULL argument");
   const size_t esz = dtype_size(val_dtype);
   if (!esz || val_dtype == GG_UINT64) return set_error(GG_ERR_ARGUMENT, "bad value type");
   if (slice_size < 1) return set_error(GG_ERR_ARGUMENT, "slice_size must be >= 1");
   if (rowptr && rowptr[0] != 1) return set_error(GG_ERR_ARGUMENT, "rowptr[1] must be 1");
-  const bool reference_layout = sigma <= 1 && heavy_threshold <= 0;
+  const bool relabel = (flags & GG_SELL_RELABEL) != 0;
+  if (relabel && m != n) return set_error(GG_ERR_ARGUMENT, "relabelling needs a square matrix");
+  const bool reference_layout = sigma <= 1 && heavy_threshold <= 0 && !relabel;
   if (heavy_threshold > 0 && heavy_seg_len < 1)
     return set_error(GG_ERR_ARGUMENT, "heavy_seg_len must be >= 1");
 
@@ -138,12 +142,18 @@ int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* co
     std::iota(slot_rows.begin(), slot_rows.end(), 1);
   } else {
     slot_rows.reserve(m);
-    for (int64_t r = 1; r <= m; ++r) {
+    std::vector<int32_t> order(m);
+    std::iota(order.begin(), order.end(), 1);
+    if (relabel)  // every row by decreasing length, ties in row order
+      std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return (rowptr[x] - rowptr[x - 1]) > (rowptr[y] - rowptr[y - 1]);
+      });
+    for (int32_t r : order) {
       int64_t len = rowptr[r] - rowptr[r - 1];
       if (heavy_threshold > 0 && len > heavy_threshold) heavy.push_back((int32_t)r);
       else slot_rows.push_back((int32_t)r);
     }
-    if (sigma > 1) {
+    if (sigma > 1 && !relabel) {
       const int64_t nl = (int64_t)slot_rows.size();
       parallel_for((nl + sigma - 1) / sigma, 64, [&](int64_t w0, int64_t w1) {
         for (int64_t w = w0; w < w1; ++w) {
@@ -161,6 +171,35 @@ int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* co
   h->slice_size = C;
   h->nslices = ns;
   h->n_slots = ns * C;
+
+  // Relabelled ids: a light row's internal id is its slot (1-based), a heavy
+  // row's is n_slots + its heavy index; stored columns use internal ids, so
+  // the hubs' entries of x sit together at the front of the slot space.
+  std::vector<int32_t> vinv;
+  const int64_t nh_all = (int64_t)heavy.size();
+  if (relabel) {
+    if (ns * C + nh_all > 2147483647LL) return fail(GG_ERR_ARGUMENT, "internal ids exceed Int32");
+    vinv.assign(m, 0);
+    for (int64_t q = 0; q < nl; ++q) vinv[slot_rows[q] - 1] = (int32_t)(q + 1);
+    for (int64_t i = 0; i < nh_all; ++i) vinv[heavy[i] - 1] = (int32_t)(ns * C + i + 1);
+    h->vinv = (int32_t*)malloc(sizeof(int32_t) * std::max<int64_t>(1, m));
+    if (!h->vinv) return fail(GG_ERR_NOMEM, "host allocation failed");
+    memcpy(h->vinv, vinv.data(), sizeof(int32_t) * m);
+    h->n_internal = ns * C + nh_all;
+  } else {
+    h->n_internal = m;
+  }
+  // entries of a row in storage order: (stored column, source position); with
+  // relabelled ids the columns are mapped but keep the row's input order, so
+  // every fold visits a row's terms in the order the reference's SELL does
+  auto row_entries = [&](int32_t row, std::vector<std::pair<int32_t, int64_t>>& ent) {
+    const int64_t start = rowptr[row - 1] - 1, len = rowptr[row] - rowptr[row - 1];
+    ent.resize(len);
+    for (int64_t t = 0; t < len; ++t) {
+      const int32_t c = colval[start + t];
+      ent[t] = {relabel ? vinv[c - 1] : c, start + t};
+    }
+  };
 
   // slice widths and pointers (src/storage/SELL.jl:309-321)
   std::vector<int64_t> width(ns, 0);
@@ -196,15 +235,17 @@ int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* co
   const char* src_val = (const char*)nzval;
   char* dst_val = (char*)h->nzval;
   parallel_for(ns, 256, [&](int64_t s0, int64_t s1) {
+    std::vector<std::pair<int32_t, int64_t>> ent;
     for (int64_t s = s0; s < s1; ++s) {
       const int64_t base = h->slice_ptr[s] - 1;
       for (int64_t r = 0; r < C; ++r) {
         const int64_t q = s * C + r;
-        int64_t len = 0, start = 0;
+        int64_t len = 0;
+        ent.clear();
         if (q < nl) {
           int32_t row = slot_rows[q];
-          start = rowptr[row - 1] - 1;
-          len = rowptr[row] - rowptr[row - 1];
+          row_entries(row, ent);
+          len = (int64_t)ent.size();
           if (h->perm) {
             h->perm[q] = row;
             h->slot_len[q] = (int32_t)len;
@@ -213,8 +254,8 @@ int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* co
         for (int64_t k = 0; k < width[s]; ++k) {
           const int64_t pos = base + k * C + r;  // column-major inside the slice
           if (k < len) {
-            h->colval[pos] = colval[start + k];
-            memcpy(dst_val + pos * esz, src_val + (start + k) * esz, esz);
+            h->colval[pos] = ent[k].first;
+            memcpy(dst_val + pos * esz, src_val + ent[k].second * esz, esz);
           } else {
             h->colval[pos] = -1;
             memset(dst_val + pos * esz, 0, esz);
@@ -249,11 +290,14 @@ int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* co
     h->heavy_val = malloc(esz * std::max<int64_t>(1, h->heavy_nnz));
     if (!h->heavy_col || !h->heavy_val) return fail(GG_ERR_NOMEM, "host allocation failed");
     parallel_for(nh, 64, [&](int64_t i0, int64_t i1) {
+      std::vector<std::pair<int32_t, int64_t>> ent;
       for (int64_t i = i0; i < i1; ++i) {
-        int32_t r = heavy[i];
-        int64_t start = rowptr[r - 1] - 1, len = rowptr[r] - rowptr[r - 1];
-        memcpy(h->heavy_col + h->heavy_ptr[i], colval + start, sizeof(int32_t) * len);
-        memcpy((char*)h->heavy_val + h->heavy_ptr[i] * esz, src_val + start * esz, esz * len);
+        row_entries(heavy[i], ent);
+        const int64_t o = h->heavy_ptr[i];
+        for (size_t t = 0; t < ent.size(); ++t) {
+          h->heavy_col[o + t] = ent[t].first;
+          memcpy((char*)h->heavy_val + (o + t) * esz, src_val + ent[t].second * esz, esz);
+        }
       }
     });
   }

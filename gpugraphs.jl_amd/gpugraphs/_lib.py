@@ -20,6 +20,8 @@ CSR, SELL, CSC = 1, 2, 3
 # gg_layout
 COL_MAJOR, ROW_MAJOR = 0, 1
 FLAG_UNIT_PATTERN = 1
+FLAG_INTERNAL_IDS = 2
+SELL_RELABEL = 1
 
 
 class DimensionMismatch(ValueError):
@@ -65,6 +67,7 @@ class GGMatrix(C.Structure):
         ("n_heavy", C.c_int64), ("heavy_row", C.c_void_p), ("heavy_ptr", C.c_void_p),
         ("heavy_seg", C.c_void_p), ("heavy_seg_len", C.c_int64), ("heavy_nseg", C.c_int64),
         ("heavy_col", C.c_void_p), ("heavy_val", C.c_void_p), ("flags", C.c_int64),
+        ("vinv", C.c_void_p),
     ]
 
 
@@ -79,6 +82,7 @@ class GGSellHost(C.Structure):
         ("n_seg", C.c_int64),
         ("heavy_row", C.c_void_p), ("heavy_ptr", C.c_void_p), ("heavy_seg", C.c_void_p),
         ("heavy_col", C.c_void_p), ("heavy_val", C.c_void_p),
+        ("vinv", C.c_void_p), ("n_internal", C.c_int64),
     ]
 
 
@@ -122,7 +126,7 @@ SIGNATURES = {
     "gg_matrix_slots": ([_PM, C.POINTER(_i64)], _i32),
     "gg_csr_check": ([_i64, _i64, _vp, _i64, _vp, _i64, _i64], _i32),
     "gg_sell_check": ([_i64, _i64, _i64, _i64, _vp, _i64, _i64], _i32),
-    "gg_sell_build": ([_i64, _i64, _vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64,
+    "gg_sell_build": ([_i64, _i64, _vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64, _i32,
                        C.POINTER(C.POINTER(GGSellHost))], _i32),
     "gg_sell_host_free": ([C.POINTER(GGSellHost)], None),
     "gg_csr_host_free": ([C.POINTER(GGCsrHost)], None),

@@ -37,11 +37,12 @@ class HostCSR:
         from .storage import SparseGPUMatrixCSR
         return SparseGPUMatrixCSR(self.m, self.n, self.rowptr, self.colval, self.nzval, backend)
 
-    def to_sell(self, backend, slice_size=64, sigma=4096, heavy_threshold=0, heavy_seg_len=4096):
+    def to_sell(self, backend, slice_size=64, sigma=4096, heavy_threshold=0, heavy_seg_len=4096,
+                relabel=False):
         from .storage import SparseGPUMatrixSELL
         M = SparseGPUMatrixSELL.__new__(SparseGPUMatrixSELL)
         M._build(self.m, self.n, self.rowptr, self.colval, np.ascontiguousarray(self.nzval),
-                 backend, slice_size, sigma, heavy_threshold, heavy_seg_len)
+                 backend, slice_size, sigma, heavy_threshold, heavy_seg_len, relabel)
         return M
 
     def degrees(self):

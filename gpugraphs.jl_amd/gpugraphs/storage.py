@@ -372,10 +372,14 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
     slice_size clipped to the row count, src/storage/SELL.jl:297,304).
     sigma > 1 sorts rows by length inside windows of sigma rows; rows longer
     than heavy_threshold are held in a CSR partition processed in segments of
-    heavy_seg_len entries.
+    heavy_seg_len entries.  relabel=True (square matrices) sorts all rows by
+    length and stores columns as internal ids (slot index, then heavy index),
+    so a row's neighbours are gathered from the compact, degree-ordered state
+    arrays; every operation still takes and returns vectors in original order.
     """
 
-    def __init__(self, *args, slice_size=None, sigma=1, heavy_threshold=0, heavy_seg_len=4096):
+    def __init__(self, *args, slice_size=None, sigma=1, heavy_threshold=0, heavy_seg_len=4096,
+                 relabel=False):
         if len(args) == 11:
             self._init_raw(*args)
             return
@@ -399,17 +403,18 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
             colval = (S.indices + 1).astype(np.int32)
             nzval = np.ascontiguousarray(S.data)
         self._build(m, n, rowptr, colval, nzval, backend, int(slice_size), int(sigma),
-                    int(heavy_threshold), int(heavy_seg_len))
+                    int(heavy_threshold), int(heavy_seg_len), bool(relabel))
 
     def _build(self, m, n, rowptr, colval, nzval, backend, slice_size, sigma, heavy_threshold,
-               heavy_seg_len):
+               heavy_seg_len, relabel=False):
         vdt = nzval.dtype
         if vdt == np.bool_:
             nzval = nzval.view(np.uint8)
         out = C.POINTER(GGSellHost)()
         check(lib().gg_sell_build(m, n, rowptr.ctypes.data, colval.ctypes.data, nzval.ctypes.data,
                                   dtype_code(vdt), slice_size, sigma, heavy_threshold,
-                                  heavy_seg_len, C.byref(out)), "SparseGPUMatrixSELL")
+                                  heavy_seg_len, _lib.SELL_RELABEL if relabel else 0,
+                                  C.byref(out)), "SparseGPUMatrixSELL")
         try:
             h = out.contents
 
@@ -442,6 +447,11 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
             else:
                 self.heavy_row = self.heavy_ptr = self.heavy_seg = None
                 self.heavy_col = self.heavy_val = None
+            if h.vinv:   # original row -> internal id (1-based), host and device copies
+                self.vinv = arr(h.vinv, m, np.int32)
+                self._vinv_dev = to_backend(self.vinv, np.int32, backend)
+            else:
+                self.vinv = self._vinv_dev = None
         finally:
             lib().gg_sell_host_free(out)
         self.unit = unit_pattern(nzval.view(np.bool_) if vdt == np.bool_ else nzval)
@@ -469,6 +479,7 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
         self.colval = to_backend(colval, np.int32, backend)
         self.nzval = to_backend(nzval, vdt, backend)
         self.slot_len = self._perm_dev = None
+        self.vinv = self._vinv_dev = None
         self.n_heavy, self.heavy_seg_len, self.n_seg = 0, 0, 0
         self.heavy_row = self.heavy_ptr = self.heavy_seg = self.heavy_col = self.heavy_val = None
         h = to_host(self.nzval)
@@ -496,11 +507,16 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
             d.heavy_nseg = self.n_seg
             d.heavy_col, d.heavy_val = P(self.heavy_col), P(self.heavy_val)
         d.flags = _lib.FLAG_UNIT_PATTERN if self.unit else 0
+        if self._vinv_dev is not None:
+            d.flags |= _lib.FLAG_INTERNAL_IDS
+            d.vinv = P(self._vinv_dev)
         return d
 
     def __getitem__(self, ij):
         i, j = ij
         self._check_bounds(i, j)
+        if self.vinv is not None:
+            j = int(self.vinv[j - 1])   # columns are stored as internal ids
         sp_ = to_host(self.slice_ptr)
         cv, nz = to_host(self.colval), to_host(self.nzval)
         if self.n_heavy:

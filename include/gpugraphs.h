@@ -122,12 +122,18 @@ typedef struct gg_matrix {
   const int32_t* heavy_col; /* 1-based columns */
   const void* heavy_val;
   int64_t flags;            /* gg_matrix_flags */
+  /* GG_FLAG_INTERNAL_IDS: columns (colval, heavy_col) are internal ids
+   * 1..nslices*slice_size + n_heavy (slot index, then n_slots + heavy index);
+   * vinv[original row] is its internal id. */
+  const int32_t* vinv;
 } gg_matrix;
 
 typedef enum gg_matrix_flags {
   /* Every stored value is Bool true or an odd integer, so (value & x) == x
    * for x in {0,1}: or-and BFS may skip loading values. */
-  GG_FLAG_UNIT_PATTERN = 1
+  GG_FLAG_UNIT_PATTERN = 1,
+  /* A relabelled SELL (gg_sell_build with GG_SELL_RELABEL). */
+  GG_FLAG_INTERNAL_IDS = 2
 } gg_matrix_flags;
 
 /* Host-side SELL-C-sigma arrays produced by gg_sell_build (owned by the
@@ -146,7 +152,19 @@ typedef struct gg_sell_host {
   int32_t* heavy_seg;  /* [n_heavy+1] */
   int32_t* heavy_col;  /* [heavy_nnz] */
   void* heavy_val;     /* [heavy_nnz] */
+  int32_t* vinv;       /* [m] original row -> internal id, or NULL (GG_SELL_RELABEL) */
+  int64_t n_internal;  /* size of the internal id space */
 } gg_sell_host;
+
+/* gg_sell_build flags */
+typedef enum gg_sell_flags {
+  /* Relabel vertices by decreasing degree: rows are laid out in that order
+   * (the sigma windows become moot), a light row's internal id is its slot,
+   * a heavy row's is n_slots + its heavy index, and stored columns use
+   * internal ids.  Entry points gather x into internal order first; results
+   * are written at the original row ids.  Needs a square matrix. */
+  GG_SELL_RELABEL = 1
+} gg_sell_flags;
 
 typedef struct gg_ctx gg_ctx;
 
@@ -196,7 +214,8 @@ int gg_sell_check(int64_t n, int64_t nslices, int64_t n_stored, int64_t slice_pt
  * heavy_threshold > 0 moves rows longer than it into the heavy partition. */
 int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* colval,
                   const void* nzval, int val_dtype, int64_t slice_size, int64_t sigma,
-                  int64_t heavy_threshold, int64_t heavy_seg_len, gg_sell_host** out);
+                  int64_t heavy_threshold, int64_t heavy_seg_len, int flags,
+                  gg_sell_host** out);
 void gg_sell_host_free(gg_sell_host* h);
 
 /* ------------------------------------------------------------- generators */

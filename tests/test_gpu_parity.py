@@ -33,7 +33,7 @@ def host(t):
     return t.cpu().numpy()
 
 
-FORMATS = ["csr", "sell_ref", "sell_hybrid"]
+FORMATS = ["csr", "sell_ref", "sell_hybrid", "sell_relabel"]
 
 
 def make(gg, S, fmt):
@@ -42,6 +42,11 @@ def make(gg, S, fmt):
         return gg.SparseGPUMatrixCSR(S, R)
     if fmt == "sell_ref":
         return gg.SparseGPUMatrixSELL(S, R, 32)
+    if fmt == "sell_relabel":
+        # degree-relabelled rows (internal column ids) with a heavy partition
+        if S.shape[0] != S.shape[1]:
+            pytest.skip("relabelling needs a square matrix")
+        return gg.SparseGPUMatrixSELL(S, R, 64, heavy_threshold=8, heavy_seg_len=16, relabel=True)
     # sigma-sorted C=64 slices with a heavy partition cut into 16-entry segments
     return gg.SparseGPUMatrixSELL(S, R, 64, sigma=64, heavy_threshold=8, heavy_seg_len=16)
 
@@ -83,6 +88,8 @@ def test_spmv_power_law_rows_f32(env, fmt):
     x = np.random.default_rng(1).random(R.n, dtype=np.float32)
     if fmt == "sell_hybrid":
         A = R.to_sell(gg.ROCBackend(0), 64, 4096, 256, 512)
+    elif fmt == "sell_relabel":
+        A = R.to_sell(gg.ROCBackend(0), 64, 1, 256, 512, relabel=True)
     elif fmt == "sell_ref":
         A = R.to_sell(gg.ROCBackend(0), 64, 1, 0)
     else:
@@ -338,7 +345,7 @@ def test_bfs_directed_unreachable_and_even_values(env):
     Adj = G.random_digraph(rng, 3000, 0.0015, wmax=4, dtype=np.int32)
     AT = Adj.T.tocsr()
     for fmt in FORMATS:
-        A = make(gg, AT, fmt)
+        A = make(gg, AT, fmt)   # square: no skip
         Ao = orc_mat(orc, AT, fmt)
         for s in (1, 777, 3000):
             print_case("bfs directed", fmt, s)
@@ -369,14 +376,17 @@ def test_bfs_direction_optimising_equals_plain(env):
     R = GEN.rmat(16, seed=4)
     A = R.to_csr(gg.ROCBackend(0))
     H = R.to_sell(gg.ROCBackend(0), 64, 4096, 256, 1024)
+    L = R.to_sell(gg.ROCBackend(0), 64, 1, 1024, 1024, relabel=True)
     deg = R.degrees()
     roots = np.nonzero(deg)[0][:5] + 1
     for s in roots:
         print_case("bfs do", s)
         ref = host(gg.bfs_distances(A, int(s)))
         np.testing.assert_array_equal(host(gg.bfs_distances(H, int(s))), ref)
+        np.testing.assert_array_equal(host(gg.bfs_distances(L, int(s))), ref)
         np.testing.assert_array_equal(host(gg.bfs_distances_do(A, int(s))), ref)
         np.testing.assert_array_equal(host(gg.bfs_distances_do(H, int(s), A_out=A)), ref)
+        np.testing.assert_array_equal(host(gg.bfs_distances_do(L, int(s), A_out=A)), ref)
 
 
 @pytest.mark.parametrize("fmt", FORMATS)
@@ -433,8 +443,9 @@ def test_multi_source_bfs_equals_single(env, k):
     rng = np.random.default_rng(k)
     nzr = np.nonzero(R.degrees())[0] + 1
     srcs = rng.choice(nzr, k, replace=True).astype(np.int32)
-    for A in (R.to_csr(gg.ROCBackend(0)), R.to_sell(gg.ROCBackend(0), 64, 4096, 128, 256)):
-        print_case("msbfs", k, type(A).__name__)
+    for A in (R.to_csr(gg.ROCBackend(0)), R.to_sell(gg.ROCBackend(0), 64, 4096, 128, 256),
+              R.to_sell(gg.ROCBackend(0), 64, 1, 128, 256, relabel=True)):
+        print_case("msbfs", k, type(A).__name__, A.vinv is not None if hasattr(A, "vinv") else "")
         D = host(gg.multi_source_bfs(A, srcs))
         Ao = orc.from_scipy_csr(R.to_scipy())
         for i in rng.choice(k, min(k, 6), replace=False):

@@ -169,6 +169,46 @@ def test_sell_hybrid_keeps_every_row(sigma, heavy):
         assert A.n_seg == sum(-(-len(rows[int(r)]) // 64) for r in A.heavy_row)
 
 
+@pytest.mark.parametrize("heavy", [0, 40, 1])
+def test_sell_relabel_keeps_every_row(heavy):
+    """Degree relabelling: vinv inverts (perm, heavy_row) into internal ids
+    (slot index, then slots + heavy index), slot lengths fall over the whole
+    matrix, and mapping the stored internal columns back gives every row."""
+    R = G.rmat(12, seed=6, dtype=np.float32, values=G.UNIFORM01)
+    A = R.to_sell(CPU, 64, 1, heavy, heavy_seg_len=64, relabel=True)
+    base = A.nslices * A.slice_size
+    assert A.vinv is not None and len(A.vinv) == R.m
+    internal_to_orig = np.zeros(base + A.n_heavy + 1, np.int64)
+    for slot, p in enumerate(A.perm):
+        if p:
+            assert A.vinv[p - 1] == slot + 1
+            internal_to_orig[slot + 1] = p
+    for h, r in enumerate(A.heavy_row if A.n_heavy else []):
+        assert A.vinv[r - 1] == base + h + 1
+        internal_to_orig[base + h + 1] = r
+    assert sorted(internal_to_orig[internal_to_orig > 0]) == list(range(1, R.m + 1))
+    lens = [l for l, p in zip(A.slot_len, A.perm) if p]
+    assert lens == sorted(lens, reverse=True)
+    rows = _rows_of_sell(A)
+    S = R.to_scipy()
+    for v in range(R.m):
+        a, b = S.indptr[v], S.indptr[v + 1]
+        back = [(int(internal_to_orig[c]), x) for c, x in rows[v + 1]]   # input order kept
+        assert back == list(zip(S.indices[a:b] + 1, S.data[a:b]))
+    d = A.descriptor(host_ok=True)
+    assert d.flags & 2 and d.vinv
+    for i, j in [(1, 1), (5, int(S.indices[S.indptr[4]]) + 1), (R.m, R.m)]:
+        assert A[i, j] == S[i - 1, j - 1]
+    _walk_sell_indexing(A)
+
+
+def test_sell_relabel_needs_square():
+    S = sp.random(30, 40, density=0.2, format="csr", random_state=np.random.default_rng(1),
+                  dtype=np.float32)
+    with pytest.raises(gg.ArgumentError):
+        gg.SparseGPUMatrixSELL(S, CPU, 64, relabel=True)
+
+
 def _walk_sell_indexing(A):
     """Every array index the SELL kernels form (sell_slot / heavy segments),
     checked against the buffer lengths on the host."""
