@@ -34,6 +34,110 @@ def pct(xs, q):
     return float(np.percentile(np.asarray(xs), q))
 
 
+def sssp_certificate(rowptr, colval, w, d, src):
+    """Exact shortest-path certificate at any size: d[src] = 0, no in-edge can
+    lower a distance (d[v] <= d[u] + w(u->v) for every edge), and every other
+    reached vertex has a tight in-edge (d[v] == min_u d[u] + w(u->v)).  With
+    integer-valued weights and sums below 2^24 the float64 check is exact."""
+    import numpy as np
+    d64 = d.astype(np.float64)
+    if np.issubdtype(d.dtype, np.integer):
+        d64[d == np.iinfo(d.dtype).max] = np.inf
+    cand = d64[colval - 1] + w.astype(np.float64)
+    starts = rowptr[:-1].astype(np.int64) - 1
+    nz = np.diff(rowptr.astype(np.int64)) > 0
+    best = np.full(len(d), np.inf)
+    best[nz] = np.minimum.reduceat(cand, starts[nz])
+    reached = np.isfinite(d64)
+    reached[src - 1] = False
+    return bool(d64[src - 1] == 0 and np.all(d64 <= best) and np.all(best[reached] == d64[reached]))
+
+
+def _time_queries(torch, fn, items):
+    lat, outs = [], []
+    for it in items:
+        t0 = time.perf_counter()
+        r = fn(it)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+        outs.append(r)
+    return lat, outs
+
+
+def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma, relabel):
+    """Bellman-Ford one source at a time vs k sources in one multi-source run,
+    on the 4-neighbour grid (+1% long-range edges, integer weights 1..1000 as
+    fp32) and on the R-MAT graph with the same kind of weights."""
+    from gpugraphs.algorithms import shortest_path_
+    out = {}
+    rng = np.random.default_rng(seed + 1)
+    t = time.time()
+    Gd = G.grid(args.grid, args.grid, 0.01, 1, 1000, seed=seed, dtype=np.float32)
+    w_r = rng.integers(1, 1001, Rg.nnz).astype(np.float32)
+    Rw = G.HostCSR(Rg.m, Rg.n, Rg.rowptr, Rg.colval, w_r)
+    cases = [("grid", Gd, Gd.to_sell(R0, 64, 64, 0, args.seg),
+              (rng.choice(Gd.m, args.sssp_sources, replace=False) + 1).astype(np.int32)),
+             ("rmat", Rw, Rw.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel),
+              roots[:args.sssp_k].astype(np.int32))]
+    torch.cuda.synchronize()
+    log(f"SSSP graphs: grid{args.grid} n={Gd.m} nnz={Gd.nnz}, weighted rmat; {time.time() - t:.1f}s")
+    for name, H, A, srcs in cases:
+        m, k = H.m, len(srcs)
+        dist = torch.empty(m, dtype=torch.float32, device=dev)
+        shortest_path_(A, int(srcs[0]), dist)
+        torch.cuda.synchronize()
+        iters = []
+
+        def one(s):
+            iters.append(shortest_path_(A, int(s), dist))
+            return dist.clone()
+        lat, plain = _time_queries(torch, one, srcs)
+        _lib.enable_timing(ctx, True)
+        _lib.hot_kernel_stats(ctx, reset=True)
+        shortest_path_(A, int(srcs[0]), dist)
+        kms, kn = _lib.hot_kernel_stats(ctx, reset=True)
+        _lib.enable_timing(ctx, False)
+        Dm = torch.empty(k * m, dtype=torch.float32, device=dev).view(k, m).T
+        t0 = time.perf_counter()
+        it_b = shortest_path_(A, srcs, Dm)
+        torch.cuda.synchronize()
+        t_b = time.perf_counter() - t0
+        same = all(torch.equal(Dm[:, i], plain[i]) for i in range(k))
+        exact = sssp_certificate(H.rowptr, H.colval, H.nzval, plain[0].cpu().numpy(), int(srcs[0]))
+        assert same and exact, f"SSSP {name}: batched equals plain {same}, certificate {exact}"
+        B_it = 8 * H.nnz + 4 * (m + 1) + 4 * m + 8 * m  # one full Jacobi iteration
+        out[name] = dict(n=m, nnz=H.nnz, sources=k, plain_qps=k / sum(lat),
+                         plain_p50_ms=pct(lat, 50) * 1e3, plain_p99_ms=pct(lat, 99) * 1e3,
+                         plain_iters_mean=float(np.mean(iters)), iter_kernel_us=kms / max(kn, 1) * 1e3,
+                         full_iter_bytes=B_it, batched_qps=k / t_b, batched_ms=t_b * 1e3,
+                         batched_iters=it_b, batched_vs_plain=(k / t_b) / (k / sum(lat)),
+                         exact_certificate=exact)
+        log(f"SSSP {name}: plain {k / sum(lat):.1f} q/s (p99 {pct(lat, 99) * 1e3:.1f} ms, "
+            f"{np.mean(iters):.0f} iters, {kms / max(kn, 1) * 1e3:.0f} us/iter); batched k={k} "
+            f"{k / t_b:.1f} q/s ({it_b} iters) = {(k / t_b) / (k / sum(lat)):.1f}x")
+        del plain, Dm
+    return out
+
+
+def copy_calibration(torch, dev, nbytes=1 << 30, reps=10):
+    """Plain device copy of a known size: FETCH_SIZE / WRITE_SIZE per dispatch of
+    this kernel divided by nbytes is the counters' scale factor."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    log(f"copy {nbytes} B: {ms * 1e3:.1f} us, {2 * nbytes / ms / 1e6:.0f} GB/s (read + write)")
+    print(json.dumps({"copy_bytes": nbytes, "us": round(ms * 1e3, 1),
+                      "gbs_read_plus_write": round(2 * nbytes / ms / 1e6, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,7 +153,12 @@ def main():
     ap.add_argument("--plain-roots", type=int, default=64)
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", default="", help="profile one part: spmv | msbfs | bfs")
+    ap.add_argument("--grid", type=int, default=4096, help="Bellman-Ford grid side")
+    ap.add_argument("--sssp-sources", type=int, default=16, help="grid Bellman-Ford sources")
+    ap.add_argument("--sssp-k", type=int, default=32, help="batched SSSP sources on R-MAT")
+    ap.add_argument("--only", default="",
+                    help="one part: spmv | msbfs | bfs | sssp, or copy (a 1 GiB device copy that "
+                         "calibrates the HBM byte counters)")
     args = ap.parse_args()
     relabel = not args.no_relabel
     sigma = 1 if relabel else 4096
@@ -70,6 +179,9 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.only == "copy":
+        copy_calibration(torch, dev)
+        return
     R0 = gg.ROCBackend(0)
     seed = 20261203
 
@@ -213,6 +325,11 @@ def main():
                               spmv_gbs=(nnz * 8 + 12 * n) / cpu_spmv_s / 1e9)
             log(f"CPU oracle ({cores} threads): BFS {cpu_bfs_s:.2f} s, SpMV {cpu_spmv_s * 1e3:.0f} ms")
 
+    # ----------------------------------------------------------------- SSSP
+    if args.only in ("", "sssp"):
+        res["sssp"] = sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed,
+                               sigma, relabel)
+
     # ---------------------------------------------------------------- report
     ms = res.get("msbfs")
     bfs = res.get("bfs", {})
@@ -269,6 +386,8 @@ def main():
         "spmv_us": round(spmv["ms"] * 1e3, 1) if spmv else None,
         "spmv_roofline": spmv_roof,
         "roofline": roofline,
+        "sssp": {name: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in c.items()}
+                 for name, c in res.get("sssp", {}).items()} or None,
         "cpu_baseline": None if not cpu else {
             "value": round(cpu["qps"], 3), "unit": "queries/s", "cores": cpu["cores"],
             "kind": "port",

@@ -98,7 +98,28 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
       for (int w = 0; w < W; ++w) { sn[w] = a.seen[rid * W + w]; acc[w] = 0; }
       if (!saturated<W>(acc, sn, a)) {
         const TA* val = (const TA*)g.val;
-        for (int64_t kk = 0; kk < len; ++kk) {
+        int64_t kk = 0;
+        if (has_len) {  // batches of 4: index loads, bitmap tests, then mask loads
+          bool full = false;
+          for (; kk + 4 <= len && !full; kk += 4) {
+            int32_t c[4];
+            bool on[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) c[u] = g.colval[base + (kk + u) * g.C];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              on[u] = bit_get(a.front_any, c[u] - 1) &&
+                      (UNIT || edge_on(val, base + (kk + u) * g.C));
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (on[u])
+#pragma unroll
+                for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(c[u] - 1) * W + w];
+            full = saturated<W>(acc, sn, a);
+          }
+          if (full) kk = len;
+        }
+        for (; kk < len; ++kk) {
           const int64_t i = base + kk * g.C;
           const int32_t col = g.colval[i];
           if (!has_len && col == -1) break;
@@ -210,6 +231,26 @@ __global__ void msbfs_seed_kernel(const int32_t* sources, int64_t k, int64_t m, 
   }
 }
 
+// Row-major m x k -> column-major m x k through a 64 x 64 LDS tile, so both
+// the reads and the writes are whole lines.
+__global__ __launch_bounds__(256) void transpose_rm_cm_kernel(const int32_t* __restrict__ src,
+                                                              int64_t m, int64_t k,
+                                                              int32_t* __restrict__ dst) {
+  __shared__ int32_t tile[64][65];
+  const int64_t tiles_k = (k + 63) / 64;
+  const int64_t r0 = ((int64_t)blockIdx.x / tiles_k) * 64;
+  const int64_t c0 = ((int64_t)blockIdx.x % tiles_k) * 64;
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int r = i >> 6, c = i & 63;
+    if (r0 + r < m && c0 + c < k) tile[r][c] = src[(r0 + r) * k + c0 + c];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int c = i >> 6, r = i & 63;
+    if (r0 + r < m && c0 + c < k) dst[(r0 + r) + (c0 + c) * m] = tile[r][c];
+  }
+}
+
 __global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -253,10 +294,18 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc)
   });
 }
 
-// Runs one batch of k <= 512 sources.
+// Runs one batch of k <= 512 sources.  The levels write each new (vertex,
+// source) distance into a row-major scratch, where a vertex's k entries share
+// lines; a column-major result is transposed from it at the end.
 int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64_t k,
-              int32_t* dist, int layout, int64_t* levels_out) {
+              int32_t* dist_out, int layout, int64_t* levels_out) {
   const int64_t m = A->m;
+  int32_t* dist = dist_out;
+  if (layout == GG_COL_MAJOR) {
+    void* p;
+    GG_TRY(ws_get(ctx, WS_MSDIST, sizeof(int32_t) * m * k, &p));
+    dist = (int32_t*)p;
+  }
   const int64_t ns = state_size(A);  // rows of the traversal state
   int W = (int)((k + 63) / 64);
   int Wc = W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : 8;
@@ -278,12 +327,12 @@ int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64
                      dist, m * k, (int32_t)2147483647);
   const int32_t* vinv = (A->flags & GG_FLAG_INTERNAL_IDS) ? A->vinv : nullptr;
   hipLaunchKernelGGL(msbfs_seed_kernel, dim3(1), dim3(1), 0, ctx->stream, sources_dev, k, m, Wc,
-                     layout, vinv, (uint64_t*)f, (uint32_t*)fa, (uint64_t*)sn, dist);
+                     GG_ROW_MAJOR, vinv, (uint64_t*)f, (uint32_t*)fa, (uint64_t*)sn, dist);
   GG_HIP(hipGetLastError());
   MsArgs a{};
   a.m = m;
   a.k = k;
-  a.layout = layout;
+  a.layout = GG_ROW_MAJOR;
   a.seen = (uint64_t*)sn;
   a.dist = dist;
   a.found = (unsigned*)cnt;
@@ -314,6 +363,12 @@ int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64
     if (*(unsigned*)ctx->host_pinned == 0) break;
     std::swap(fr, nxp);
     std::swap(fany, nany);
+  }
+  if (layout == GG_COL_MAJOR && m > 0) {
+    const int64_t tiles = ((m + 63) / 64) * ((k + 63) / 64);
+    hipLaunchKernelGGL(transpose_rm_cm_kernel, dim3((unsigned)tiles), dim3(256), 0, ctx->stream,
+                       dist, m, k, dist_out);
+    GG_HIP(hipGetLastError());
   }
   if (levels_out) *levels_out = level - 1;
   return GG_OK;

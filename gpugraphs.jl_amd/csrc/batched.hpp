@@ -27,12 +27,14 @@ struct MsArgs {
 // One or-and level over A_T with W = Wc words per vertex (1, 2, 4 or 8).
 int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc);
 
-// One Jacobi min-plus step of k sources over a row-major working set:
-// dout[v, j] = min(din[v, j], min_u safe_add(w(u->v), din[u, j])).  Rows
-// whose value fell are counted in *changed; when changed_mask is given, bit j
-// is set for every column j that changed (k <= 64).
-int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k,
-                  const void* din, void* dout, unsigned int* changed,
+// One in-place min-plus step of k sources over a row-major working set d
+// (state rows x k): d[v, j] = min(d[v, j], min_u safe_add(w(u->v), d[u, j]))
+// over the in-neighbours u whose bit is set in chp (rows that fell in the
+// previous step, or were seeded).  Rows that fall set their bit in chn and
+// are counted in *changed (one per (row, column)); when changed_mask is
+// given, bit j is set for every column j that fell (k <= 64).
+int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k, void* d,
+                  const uint32_t* chp, uint32_t* chn, unsigned int* changed,
                   unsigned long long* changed_mask);
 
 }  // namespace gg

@@ -41,6 +41,7 @@ enum WsSlot {
   WS_SPMSPV0, WS_SPMSPV1, WS_SPMSPV2,
   WS_MS0, WS_MS1, WS_MS2, WS_MS3, WS_MS4,
   WS_XPERM,         // input vector / matrix gathered into internal order
+  WS_MSDIST,        // row-major distances of a multi-source traversal
   WS_NSLOTS
 };
 

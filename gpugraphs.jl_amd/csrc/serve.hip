@@ -7,9 +7,10 @@
 //    source bit starts at the current level and its distances are counted from
 //    there, so queries admitted at different times share every level kernel.
 //    A slot is finished when its bit is absent from the whole next frontier.
-//  - SSSP: 32 columns of a row-major min-plus working set (the Jacobi step of
-//    gg_shortest_path_multi).  A query resets its column between two steps;
-//    it is finished after a step that changed nothing in its column.
+//  - SSSP: 32 columns of a row-major min-plus working set (the in-place step
+//    of gg_shortest_path_multi).  A query resets its column between two steps
+//    and marks its source as changed; it is finished after a step that
+//    lowered nothing in its column.
 // Every query is exactly the single-source BFS / Bellman-Ford from its source:
 // slots never interact (bit-parallel or column-parallel), and each slot's
 // traversal is the level-synchronous one.
@@ -86,12 +87,13 @@ __global__ void serve_zero_kernel(unsigned int* found, unsigned long long* mask)
 // Working set rows are state indices (ns of them).
 template <class TD>
 __global__ void serve_sssp_seed_kernel(TD* d, int64_t ns, int k, SeedList sl,
-                                       const int32_t* vinv) {
+                                       const int32_t* vinv, uint32_t* chp) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns * sl.n) return;
   const int64_t q = i / ns, r = i - q * ns;
   const int64_t rs = vinv ? (int64_t)vinv[sl.src[q]] - 1 : sl.src[q];
   d[r * k + sl.slot[q]] = r == rs ? TD(0) : Lim<TD>::max();
+  if (r == rs) atomicOr(&chp[r >> 5], 1u << (r & 31));
 }
 
 template <class TD>
@@ -195,8 +197,8 @@ struct Engine {
   // SSSP
   int dist_dtype = GG_INT32;
   size_t esz = 4;
-  void* buf[2] = {nullptr, nullptr};
-  int parity = 0;
+  void* buf = nullptr;                      // ns x 32 row-major distances
+  uint32_t *chp = nullptr, *chn = nullptr;  // rows that fell in the last / this step
   std::vector<void*> allocs;
   int64_t steps = 0;
 };
@@ -268,16 +270,19 @@ static int make_engine(gg_server* s, const gg_matrix* A, int kind, std::unique_p
     if (e->dist_dtype != GG_INT32 && e->dist_dtype != GG_FLOAT32 && e->dist_dtype != GG_INT64 &&
         e->dist_dtype != GG_FLOAT64)
       return set_error(GG_ERR_ARGUMENT, "unsupported SSSP value type");
-    for (int b = 0; b < 2; ++b) {
-      GG_TRY(engine_alloc(s, *e, e->esz * ns * 32, &p));
-      e->buf[b] = p;
-      GG_TRY(with_type(e->dist_dtype, [&](auto t) -> int {
-        typedef typename decltype(t)::type TD;
-        hipLaunchKernelGGL(serve_fill_kernel<TD>, dim3(blocks_for(ns * 32, 256)), dim3(256), 0,
-                           e->stream, (TD*)p, ns * 32, Lim<TD>::max());
-        return GG_OK;
-      }));
-    }
+    GG_TRY(engine_alloc(s, *e, e->esz * ns * 32, &p));
+    e->buf = p;
+    GG_TRY(with_type(e->dist_dtype, [&](auto t) -> int {
+      typedef typename decltype(t)::type TD;
+      hipLaunchKernelGGL(serve_fill_kernel<TD>, dim3(blocks_for(ns * 32, 256)), dim3(256), 0,
+                         e->stream, (TD*)p, ns * 32, Lim<TD>::max());
+      return GG_OK;
+    }));
+    const int64_t words = (ns + 31) / 32;
+    GG_TRY(engine_alloc(s, *e, 4 * words, &p)); e->chp = (uint32_t*)p;
+    GG_TRY(engine_alloc(s, *e, 4 * words, &p)); e->chn = (uint32_t*)p;
+    hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(words, 256)), dim3(256), 0,
+                       e->stream, e->chp, words, 0u);
   }
   GG_HIP(hipGetLastError());
   GG_HIP(hipStreamSynchronize(e->stream));
@@ -323,7 +328,7 @@ static int retire(gg_server* s, Engine& e) {
       GG_TRY(with_type(e.dist_dtype, [&](auto tt) -> int {
         typedef typename decltype(tt)::type TD;
         hipLaunchKernelGGL(serve_extract_kernel<TD>, dim3(blocks_for(e.m, 256)), dim3(256), 0,
-                           e.stream, (const TD*)e.buf[e.parity], e.m, 32, sl, e.vinv, (TD*)dst);
+                           e.stream, (const TD*)e.buf, e.m, 32, sl, e.vinv, (TD*)dst);
         return GG_OK;
       }));
     }
@@ -409,13 +414,16 @@ static int launch_step(gg_server* s, Engine& e) {
       GG_TRY(with_type(e.dist_dtype, [&](auto tt) -> int {
         typedef typename decltype(tt)::type TD;
         hipLaunchKernelGGL(serve_sssp_seed_kernel<TD>, dim3(blocks_for(ns * sl.n, 256)), dim3(256),
-                           0, e.stream, (TD*)e.buf[e.parity], ns, 32, sl, e.vinv);
+                           0, e.stream, (TD*)e.buf, ns, 32, sl, e.vinv, e.chp);
         return GG_OK;
       }));
     }
-    GG_TRY(bf_multi_step(e.stream, &e.A, e.dist_dtype, 32, e.buf[e.parity], e.buf[1 - e.parity],
-                         e.cnt_d, e.mask_d));
-    e.parity = 1 - e.parity;
+    const int64_t words = (ns + 31) / 32;
+    hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(words, 256)), dim3(256), 0,
+                       e.stream, e.chn, words, 0u);
+    GG_TRY(bf_multi_step(e.stream, &e.A, e.dist_dtype, 32, e.buf, e.chp, e.chn, e.cnt_d,
+                         e.mask_d));
+    std::swap(e.chp, e.chn);
   }
   GG_HIP(hipGetLastError());
   GG_HIP(hipMemcpyAsync(e.mask_h, e.mask_d, 8, hipMemcpyDeviceToHost, e.stream));

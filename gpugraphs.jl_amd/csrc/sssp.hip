@@ -1,18 +1,23 @@
-// sssp.hip -- Jacobi Bellman-Ford (min, safe_add) from one or many sources.
+// sssp.hip -- Bellman-Ford (min, safe_add) from one or many sources.
 //
 // src/algorithms/shortest_path.jl:34-83 iterates next = min(next, A_T
-// (min.safe_add) dist) and then dist .= next; since next == dist after every
-// copy, one iteration maps dist_in to dist_out[v] = min(dist_in[v],
-// min_u safe_add(w(u->v), dist_in[u])).  That is what one kernel does here,
-// reading one buffer and writing the other and counting the rows whose
-// distance fell; the loop stops at the first iteration without a change (the
-// reference checks every 8 iterations and stops at the same fixpoint, which
-// is unique: every relaxation order converges to the minimum over paths of the
-// left-fold sums, and min is exact).
+// (min.safe_add) dist), dist .= next, until an iteration changes nothing.  Its
+// fixpoint is unique: safe_add is monotone in the distance, so every fair
+// relaxation order reaches the least solution of d[v] = min(d[v], min_u
+// safe_add(w(u->v), d[u])), the minimum over paths of the left-fold sums from
+// the source (exact for the integer-valued weights of the tests).
 //
-// k sources (src/algorithms/shortest_path.jl:112-161) use a row-major m x k
-// working set: a group of lanes owns one row, one lane per source, so the k
-// distances of a neighbour are one contiguous read; the reference's
+// Here one kernel per iteration relaxes every row in place and relaxes a row
+// only from the in-neighbours whose distance changed in the previous iteration
+// (a bitmap, L2-resident): an unchanged u cannot lower d[v] again, because v
+// already read w + d[u] after u's last change.  Rows that fall set their bit
+// for the next iteration; an iteration in which nothing falls is the fixpoint.
+// The distances equal the reference's; the iteration count can be lower (in
+// place, a change is seen within the same iteration when its row runs later).
+//
+// k sources (src/algorithms/shortest_path.jl:112-161) use a row-major working
+// set (state rows x k): a group of lanes owns one row, one lane per source, so
+// the k distances of a neighbour are one contiguous read; the reference's
 // column-major result is written at the end.
 #include "batched.hpp"
 
@@ -36,8 +41,10 @@ template <class TA, class TD>
 __global__ __launch_bounds__(kT) void bf_csr_kernel(const int32_t* __restrict__ rowptr,
                                                     const int32_t* __restrict__ colval,
                                                     const TA* __restrict__ val, int64_t m,
-                                                    int logL, const TD* __restrict__ din,
-                                                    TD* __restrict__ dout, unsigned int* changed) {
+                                                    int logL, TD* d,
+                                                    const uint32_t* __restrict__ chp,
+                                                    uint32_t* __restrict__ chn,
+                                                    unsigned int* changed) {
   const int L = 1 << logL;
   const int wl = threadIdx.x & 63;
   const int lane = wl & (L - 1);
@@ -49,26 +56,33 @@ __global__ __launch_bounds__(kT) void bf_csr_kernel(const int32_t* __restrict__ 
     TD acc = Lim<TD>::max();
     if (row < m) {
       for (int32_t i = rowptr[row] - 1 + lane; i < rowptr[row + 1] - 1; i += L) {
-        acc = jmin(acc, sadd(TD(val[i]), din[colval[i] - 1]));
+        const int32_t c = colval[i] - 1;
+        if (bit_get(chp, c)) acc = jmin(acc, sadd(TD(val[i]), d[c]));
       }
     }
     for (int o = L >> 1; o > 0; o >>= 1) acc = jmin(acc, shfl_xor(acc, o));
     bool ch = false;
     if (row < m && lane == 0) {
-      const TD dv = din[row];
+      const TD dv = d[row];
       const TD nv = jmin(dv, acc);
-      dout[row] = nv;
-      ch = nv < dv;
+      if (nv < dv) {
+        d[row] = nv;
+        bit_claim(chn, row);
+        ch = true;
+      }
     }
     count_changes(ch, changed);
   }
 }
 
 // ---------------------------------------------------------- one source, SELL
+// Light slots: one lane per row, entries in batches of 4 (index loads, bitmap
+// tests, then the value / distance loads of the changed neighbours).  Heavy
+// rows: one block per row.
 template <class TA, class TD>
-__global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_blocks,
-                                                     const TD* __restrict__ din,
-                                                     TD* __restrict__ dout,
+__global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_blocks, TD* d,
+                                                     const uint32_t* __restrict__ chp,
+                                                     uint32_t* __restrict__ chn,
                                                      unsigned int* changed) {
   if ((int64_t)blockIdx.x < n_sell_blocks) {
     int64_t row, base, len;
@@ -78,56 +92,79 @@ __global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_
     if (sell_slot(g, slot, row, base, len, has_len)) {
       const TA* val = (const TA*)g.val;
       TD acc = Lim<TD>::max();
-      for (int64_t k = 0; k < len; ++k) {
+      int64_t k = 0;
+      if (has_len) {
+        for (; k + 4 <= len; k += 4) {
+          int32_t c[4];
+          bool on[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) c[u] = g.colval[base + (k + u) * g.C] - 1;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) on[u] = bit_get(chp, c[u]);
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (on[u]) acc = jmin(acc, sadd(TD(val[base + (k + u) * g.C]), d[c[u]]));
+        }
+      }
+      for (; k < len; ++k) {
         const int64_t i = base + k * g.C;
         const int32_t col = g.colval[i];
         if (!has_len && col == -1) break;
-        acc = jmin(acc, sadd(TD(val[i]), din[col - 1]));
+        if (bit_get(chp, col - 1)) acc = jmin(acc, sadd(TD(val[i]), d[col - 1]));
       }
       const int64_t rid = g.internal ? slot : row;
-      const TD dv = din[rid];
+      const TD dv = d[rid];
       const TD nv = jmin(dv, acc);
-      dout[rid] = nv;
-      ch = nv < dv;
+      if (nv < dv) {
+        d[rid] = nv;
+        bit_claim(chn, rid);
+        ch = true;
+      }
     }
     count_changes(ch, changed);
     return;
   }
-  // one block per heavy row
   __shared__ TD red[kT / 64];
   const int64_t h = (int64_t)blockIdx.x - n_sell_blocks;
   const TA* hv = (const TA*)g.heavy_val;
   TD acc = Lim<TD>::max();
-  for (int64_t i = g.heavy_ptr[h] + threadIdx.x; i < g.heavy_ptr[h + 1]; i += kT)
-    acc = jmin(acc, sadd(TD(hv[i]), din[g.heavy_col[i] - 1]));
+  for (int64_t i = g.heavy_ptr[h] + threadIdx.x; i < g.heavy_ptr[h + 1]; i += kT) {
+    const int32_t c = g.heavy_col[i] - 1;
+    if (bit_get(chp, c)) acc = jmin(acc, sadd(TD(hv[i]), d[c]));
+  }
   for (int o = 32; o > 0; o >>= 1) acc = jmin(acc, shfl_xor(acc, o));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < kT / 64; ++w) acc = jmin(acc, red[w]);
     const int64_t rid = heavy_rid(g, h);
-    const TD dv = din[rid];
+    const TD dv = d[rid];
     const TD nv = jmin(dv, acc);
-    dout[rid] = nv;
-    if (nv < dv) atomicAdd(changed, 1u);
+    if (nv < dv) {
+      d[rid] = nv;
+      bit_claim(chn, rid);
+      atomicAdd(changed, 1u);
+    }
   }
 }
 
 // -------------------------------------------------- k sources, row-major set
 // Rows of A_T as (entry list) for either format: CSR row, SELL slot, or a
 // heavy row.  A group of Lk lanes (one per source, Lk >= k when k <= 64)
-// walks the row; heavy rows get a whole block whose groups split the entries.
+// walks the row; the bitmap test is uniform across a group (one column).
+// Heavy rows get a whole block whose groups split the entries.
 template <class TA, class TD>
 __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t* __restrict__ rowptr,
                                                       SellView g, int64_t n_light_groups,
-                                                      int64_t m, int64_t k, int logLk,
-                                                      const TD* __restrict__ din,
-                                                      TD* __restrict__ dout,
+                                                      int64_t k, int logLk, TD* d,
+                                                      const uint32_t* __restrict__ chp,
+                                                      uint32_t* __restrict__ chn,
                                                       unsigned int* changed,
                                                       unsigned long long* changed_mask) {
   const int Lk = 1 << logLk;
   const int wl = threadIdx.x & 63;
   const int lane = wl & (Lk - 1);
+  const int gbase = wl & ~(Lk - 1);
   const int64_t jblocks = (k + Lk - 1) >> logLk;
   const int64_t gpb = kT >> logLk;  // groups per block
   const int64_t light_blocks = (n_light_groups * jblocks + gpb - 1) / gpb;
@@ -136,42 +173,45 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
     const int64_t unit = q / jblocks;
     const int64_t j = (q - unit * jblocks) * Lk + lane;
     bool ch = false;
-    if (unit < n_light_groups && j < k) {
+    int64_t rid = -1;
+    if (unit < n_light_groups) {
       TD acc = Lim<TD>::max();
-      int64_t row = -1;
+      const TA* val = (const TA*)g.val;
       if (format == GG_CSR) {
-        row = unit;
-        const TA* val = (const TA*)g.val;
-        for (int32_t i = rowptr[row] - 1; i < rowptr[row + 1] - 1; ++i)
-          acc = jmin(acc, sadd(TD(val[i]), din[(int64_t)(g.colval[i] - 1) * k + j]));
+        rid = unit;
+        for (int32_t i = rowptr[rid] - 1; i < rowptr[rid + 1] - 1; ++i) {
+          const int32_t c = g.colval[i] - 1;
+          if (bit_get(chp, c) && j < k) acc = jmin(acc, sadd(TD(val[i]), d[(int64_t)c * k + j]));
+        }
       } else {
-        int64_t base, len;
+        int64_t row, base, len;
         bool has_len;
         if (sell_slot(g, unit, row, base, len, has_len)) {
-          if (g.internal) row = unit;  // working set in internal order
-          const TA* val = (const TA*)g.val;
+          rid = g.internal ? unit : row;
           for (int64_t kk = 0; kk < len; ++kk) {
             const int64_t i = base + kk * g.C;
             const int32_t col = g.colval[i];
             if (!has_len && col == -1) break;
-            acc = jmin(acc, sadd(TD(val[i]), din[(int64_t)(col - 1) * k + j]));
+            if (bit_get(chp, col - 1) && j < k)
+              acc = jmin(acc, sadd(TD(val[i]), d[(int64_t)(col - 1) * k + j]));
           }
-        } else {
-          row = -1;
         }
       }
-      if (row >= 0) {
-        const TD dv = din[row * k + j];
+      if (rid >= 0 && j < k) {
+        const TD dv = d[rid * k + j];
         const TD nv = jmin(dv, acc);
-        dout[row * k + j] = nv;
-        ch = nv < dv;
+        if (nv < dv) {
+          d[rid * k + j] = nv;
+          ch = true;
+        }
       }
     }
     count_changes(ch, changed);
+    const unsigned long long b = __ballot(ch);
+    const unsigned long long low = Lk == 64 ? ~0ull : ((1ull << Lk) - 1ull);
+    if (lane == 0 && ((b >> gbase) & low)) bit_claim(chn, rid);  // this group's row fell
     if (changed_mask) {  // k <= 64: lane l of a group is column l
-      const unsigned long long b = __ballot(ch);
       unsigned long long cm = 0;
-      const unsigned long long low = Lk == 64 ? ~0ull : ((1ull << Lk) - 1ull);
       for (int o = 0; o < 64; o += Lk) cm |= (b >> o) & low;
       if (wl == 0 && cm) atomicOr(changed_mask, cm);
     }
@@ -180,39 +220,48 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
   // heavy rows: block (h, jblock); the block's groups split the row's entries
   extern __shared__ unsigned char smem[];
   TD* red = (TD*)smem;  // [gpb][Lk]
+  __shared__ int s_any;
   const int64_t hb = (int64_t)blockIdx.x - light_blocks;
   const int64_t h = hb / jblocks;
   const int64_t j = (hb - h * jblocks) * Lk + lane;
   const int64_t grp = threadIdx.x >> logLk;
   const TA* hv = (const TA*)g.heavy_val;
   TD acc = Lim<TD>::max();
+  if (threadIdx.x == 0) s_any = 0;
   if (j < k)
-    for (int64_t i = g.heavy_ptr[h] + grp; i < g.heavy_ptr[h + 1]; i += gpb)
-      acc = jmin(acc, sadd(TD(hv[i]), din[(int64_t)(g.heavy_col[i] - 1) * k + j]));
+    for (int64_t i = g.heavy_ptr[h] + grp; i < g.heavy_ptr[h + 1]; i += gpb) {
+      const int32_t c = g.heavy_col[i] - 1;
+      if (bit_get(chp, c)) acc = jmin(acc, sadd(TD(hv[i]), d[(int64_t)c * k + j]));
+    }
   red[grp * Lk + lane] = acc;
   __syncthreads();
   if (grp == 0) {
     for (int64_t o = 1; o < gpb; ++o) acc = jmin(acc, red[o * Lk + lane]);
     if (j < k) {
-      const int64_t row = heavy_rid(g, h);
-      const TD dv = din[row * k + j];
+      const int64_t rid = heavy_rid(g, h);
+      const TD dv = d[rid * k + j];
       const TD nv = jmin(dv, acc);
-      dout[row * k + j] = nv;
       if (nv < dv) {
+        d[rid * k + j] = nv;
         atomicAdd(changed, 1u);
         if (changed_mask) atomicOr(changed_mask, 1ull << j);
+        s_any = 1;
       }
     }
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_any) bit_claim(chn, heavy_rid(g, h));
 }
 
 template <class TD>
 __global__ void bf_multi_seed_kernel(TD* d, const int32_t* sources, int64_t k,
-                                     const int32_t* vinv) {
+                                     const int32_t* vinv, uint32_t* chp) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= k) return;
   const int64_t v = sources[i] - 1;
-  d[(vinv ? (int64_t)vinv[v] - 1 : v) * k + i] = TD(0);
+  const int64_t r = vinv ? (int64_t)vinv[v] - 1 : v;
+  d[r * k + i] = TD(0);
+  atomicOr(&chp[r >> 5], 1u << (r & 31));
 }
 
 template <class TD>
@@ -222,8 +271,10 @@ __global__ void fill_td_kernel(TD* p, int64_t n, TD v) {
 }
 
 template <class TD>
-__global__ void seed_one_kernel(TD* p, int64_t s, const int32_t* vinv) {
-  p[vinv ? (int64_t)vinv[s] - 1 : s] = TD(0);
+__global__ void seed_one_kernel(TD* p, int64_t s, const int32_t* vinv, uint32_t* chp) {
+  const int64_t r = vinv ? (int64_t)vinv[s] - 1 : s;
+  p[r] = TD(0);
+  chp[r >> 5] |= 1u << (r & 31);
 }
 
 // Row-major working set (state rows, k columns) -> result at original rows,
@@ -269,16 +320,39 @@ static int read_u32(gg_ctx* ctx, const unsigned int* d, unsigned int* h) {
   return GG_OK;
 }
 
-// Iterate until no row changes; the check reads a device counter every
-// iteration for the first 16 and then every 8 (extra iterations are no-ops).
+// Changed-row bitmaps of an iteration: prev (read), next (written, cleared
+// before the step).  Both live in one workspace slot.
+struct Changed {
+  uint32_t* prev;
+  uint32_t* next;
+  int64_t words;
+};
+
+static int changed_bitmaps(gg_ctx* ctx, int64_t ns, Changed& c) {
+  c.words = (ns + 31) / 32;
+  void* p;
+  GG_TRY(ws_get(ctx, WS_SSSP2, sizeof(uint32_t) * 2 * (c.words + 64), &p));
+  c.prev = (uint32_t*)p;
+  c.next = c.prev + c.words + 64;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(c.words, kT)), dim3(kT), 0, ctx->stream,
+                     c.prev, c.words, 0u);
+  return GG_OK;
+}
+
+// Iterate until an iteration lowers nothing; the check reads a device counter
+// every iteration for the first 16 and then every 8 (an iteration after the
+// fixpoint reads no neighbour and changes nothing).
 template <class Step>
-static int iterate(gg_ctx* ctx, unsigned int* changed, int64_t max_iter, Step&& step,
-                   int64_t* iters, int64_t* parity) {
+static int iterate(gg_ctx* ctx, unsigned int* changed, Changed& c, int64_t max_iter, Step&& step,
+                   int64_t* iters) {
   int64_t it = 0;
   while (true) {
     ++it;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, ctx->stream, changed);
-    GG_TRY(step(it));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(c.words, kT)), dim3(kT), 0, ctx->stream,
+                       c.next, c.words, 0u);
+    GG_TRY(step());
+    std::swap(c.prev, c.next);
     if (it <= 16 || it % 8 == 0) {
       unsigned h = 0;
       GG_TRY(read_u32(ctx, changed, &h));
@@ -286,76 +360,70 @@ static int iterate(gg_ctx* ctx, unsigned int* changed, int64_t max_iter, Step&& 
     }
     if (max_iter > 0 && it >= max_iter) {
       if (iters) *iters = it;
-      *parity = it;
       return set_error(GG_ERR_NOT_CONVERGED, "no fixpoint after %lld iterations (negative cycle?)",
                        (long long)it);
     }
   }
   if (iters) *iters = it;
-  *parity = it;
   return GG_OK;
 }
 
 int sssp_one(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist, int dist_dtype,
              int64_t* iterations) {
-  return with_ad(A->val_dtype, dist_dtype, [&](auto a, auto d) -> int {
+  return with_ad(A->val_dtype, dist_dtype, [&](auto a, auto dt) -> int {
     typedef typename decltype(a)::type TA;
-    typedef typename decltype(d)::type TD;
+    typedef typename decltype(dt)::type TD;
     const int64_t m = A->m;
-    // A relabelled SELL iterates in internal order in two workspace buffers
-    // and scatters the fixpoint to dist; otherwise dist is one of the two.
+    // a relabelled SELL iterates in internal order in a workspace and
+    // scatters the fixpoint to dist; otherwise dist itself is the working set
     const int32_t* vinv = (A->flags & GG_FLAG_INTERNAL_IDS) ? A->vinv : nullptr;
     const int64_t ns = state_size(A);
-    void *tmp, *tmp2 = nullptr, *cnt;
-    GG_TRY(ws_get(ctx, WS_SSSP0, sizeof(TD) * ns, &tmp));
-    if (vinv) GG_TRY(ws_get(ctx, WS_SSSP1, sizeof(TD) * ns, &tmp2));
+    void *tmp = nullptr, *cnt;
+    if (vinv) GG_TRY(ws_get(ctx, WS_SSSP0, sizeof(TD) * ns, &tmp));
     GG_TRY(ws_get(ctx, WS_FLAG, 64, &cnt));
-    TD* buf[2] = {vinv ? (TD*)tmp2 : (TD*)dist, (TD*)tmp};
-    hipLaunchKernelGGL(fill_td_kernel<TD>, dim3(blocks_for(ns, kT)), dim3(kT), 0, ctx->stream,
-                       buf[0], ns, Lim<TD>::max());
-    hipLaunchKernelGGL(seed_one_kernel<TD>, dim3(1), dim3(1), 0, ctx->stream, buf[0], source - 1,
-                       vinv);
+    Changed c;
+    GG_TRY(changed_bitmaps(ctx, ns, c));
+    TD* d = vinv ? (TD*)tmp : (TD*)dist;
+    hipLaunchKernelGGL(fill_td_kernel<TD>, dim3(blocks_for(ns, kT)), dim3(kT), 0, ctx->stream, d,
+                       ns, Lim<TD>::max());
+    hipLaunchKernelGGL(seed_one_kernel<TD>, dim3(1), dim3(1), 0, ctx->stream, d, source - 1, vinv,
+                       c.prev);
     const int lg = log_lanes_for(m ? (double)A->nnz / m : 1.0);
     SellView g = sell_view(A);
     const int64_t nsb = (g.n_slots + kT - 1) / kT;
-    auto step = [&](int64_t it) -> int {
-      const TD* din = buf[(it - 1) & 1];
-      TD* dout = buf[it & 1];
+    auto step = [&]() -> int {
       HotTimer ht(ctx);
       if (A->format == GG_CSR) {
         unsigned nb = blocks_for(m << lg, kT);
         const unsigned cap = (unsigned)ctx->num_cus * 32;
         if (nb > cap) nb = cap;
         hipLaunchKernelGGL((bf_csr_kernel<TA, TD>), dim3(nb), dim3(kT), 0, ctx->stream, A->ptr,
-                           A->idx, (const TA*)A->val, m, lg, din, dout, (unsigned*)cnt);
-      } else {
+                           A->idx, (const TA*)A->val, m, lg, d, c.prev, c.next, (unsigned*)cnt);
+      } else if (nsb + g.n_heavy > 0) {
         hipLaunchKernelGGL((bf_sell_kernel<TA, TD>), dim3((unsigned)(nsb + g.n_heavy)), dim3(kT),
-                           0, ctx->stream, g, nsb, din, dout, (unsigned*)cnt);
+                           0, ctx->stream, g, nsb, d, c.prev, c.next, (unsigned*)cnt);
       }
       GG_HIP(hipGetLastError());
       return GG_OK;
     };
-    int64_t last = 0;
-    int st = iterate(ctx, (unsigned*)cnt, m + 16, step, iterations, &last);
+    int st = iterate(ctx, (unsigned*)cnt, c, m + 16, step, iterations);
     if (st == GG_OK && vinv) {
       hipLaunchKernelGGL(state_to_result_kernel<TD>, dim3(blocks_for(m, kT)), dim3(kT), 0,
-                         ctx->stream, buf[last & 1], vinv, m, (int64_t)1, 1, (TD*)dist);
+                         ctx->stream, d, vinv, m, (int64_t)1, 1, (TD*)dist);
       GG_HIP(hipGetLastError());
-    } else if (st == GG_OK && (last & 1)) {  // the final buffer is the workspace one
-      GG_HIP(hipMemcpyAsync(dist, tmp, sizeof(TD) * m, hipMemcpyDeviceToDevice, ctx->stream));
     }
     GG_HIP(hipStreamSynchronize(ctx->stream));
     return st;
   });
 }
 
-int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k,
-                  const void* din, void* dout, unsigned int* changed,
+int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k, void* d,
+                  const uint32_t* chp, uint32_t* chn, unsigned int* changed,
                   unsigned long long* changed_mask) {
   if (changed_mask && k > 64) return set_error(GG_ERR_ARGUMENT, "column masks need k <= 64");
-  return with_ad(A->val_dtype, dist_dtype, [&](auto a, auto d) -> int {
+  return with_ad(A->val_dtype, dist_dtype, [&](auto a, auto dt) -> int {
     typedef typename decltype(a)::type TA;
-    typedef typename decltype(d)::type TD;
+    typedef typename decltype(dt)::type TD;
     int lg = 0;
     while (lg < 6 && (1LL << lg) < k) ++lg;
     const int64_t Lk = 1LL << lg;
@@ -367,48 +435,47 @@ int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_
     const int64_t light_blocks = (n_light * jblocks + gpb - 1) / gpb;
     const int64_t nb = light_blocks + g.n_heavy * jblocks;
     if (nb > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
-    hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
-                       stream, A->format, A->ptr, g, n_light, A->m, k, lg, (const TD*)din, (TD*)dout,
-                       changed, changed_mask);
+    if (nb > 0)
+      hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
+                         stream, A->format, A->ptr, g, n_light, k, lg, (TD*)d, chp, chn, changed,
+                         changed_mask);
     GG_HIP(hipGetLastError());
     return GG_OK;
   });
 }
 
-// k-source Bellman-Ford into a row-major working set; result written
+// k-source Bellman-Ford in a row-major working set; result written
 // column-major into dist (or row-major when rowmajor_out).
 int sssp_multi(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_host, int64_t k,
                void* dist, int dist_dtype, bool rowmajor_out, int64_t* iterations) {
-  return with_ad(A->val_dtype, dist_dtype, [&](auto, auto d) -> int {
-    typedef typename decltype(d)::type TD;
+  return with_ad(A->val_dtype, dist_dtype, [&](auto, auto dt) -> int {
+    typedef typename decltype(dt)::type TD;
     const int64_t m = A->m;
     const int64_t ns = state_size(A);
     const int32_t* vinv = (A->flags & GG_FLAG_INTERNAL_IDS) ? A->vinv : nullptr;
-    void *b0, *b1, *cnt, *src;
+    void *b0, *cnt, *src;
     GG_TRY(ws_get(ctx, WS_MS0, sizeof(TD) * ns * k, &b0));
-    GG_TRY(ws_get(ctx, WS_MS1, sizeof(TD) * ns * k, &b1));
     GG_TRY(ws_get(ctx, WS_MS2, sizeof(int32_t) * k, &src));
     GG_TRY(ws_get(ctx, WS_FLAG, 64, &cnt));
+    Changed c;
+    GG_TRY(changed_bitmaps(ctx, ns, c));
     GG_HIP(hipMemcpyAsync(src, sources_host, sizeof(int32_t) * k, hipMemcpyHostToDevice, ctx->stream));
-    TD* buf[2] = {(TD*)b0, (TD*)b1};
     hipLaunchKernelGGL(fill_td_kernel<TD>, dim3(blocks_for(ns * k, kT)), dim3(kT), 0, ctx->stream,
-                       buf[0], ns * k, Lim<TD>::max());
+                       (TD*)b0, ns * k, Lim<TD>::max());
     hipLaunchKernelGGL(bf_multi_seed_kernel<TD>, dim3(blocks_for(k, kT)), dim3(kT), 0,
-                       ctx->stream, buf[0], (const int32_t*)src, k, vinv);
-    auto step = [&](int64_t it) -> int {
+                       ctx->stream, (TD*)b0, (const int32_t*)src, k, vinv, c.prev);
+    auto step = [&]() -> int {
       HotTimer ht(ctx);
-      return bf_multi_step(ctx->stream, A, dist_dtype, k, buf[(it - 1) & 1], buf[it & 1],
-                           (unsigned*)cnt, nullptr);
+      return bf_multi_step(ctx->stream, A, dist_dtype, k, b0, c.prev, c.next, (unsigned*)cnt,
+                           nullptr);
     };
-    int64_t last = 0;
-    int st = iterate(ctx, (unsigned*)cnt, m + 16, step, iterations, &last);
-    const TD* fin = buf[last & 1];
+    int st = iterate(ctx, (unsigned*)cnt, c, m + 16, step, iterations);
     if (st == GG_OK) {
       if (rowmajor_out && !vinv) {
-        GG_HIP(hipMemcpyAsync(dist, fin, sizeof(TD) * m * k, hipMemcpyDeviceToDevice, ctx->stream));
+        GG_HIP(hipMemcpyAsync(dist, b0, sizeof(TD) * m * k, hipMemcpyDeviceToDevice, ctx->stream));
       } else {
         hipLaunchKernelGGL(state_to_result_kernel<TD>, dim3(blocks_for(m * k, kT)), dim3(kT), 0,
-                           ctx->stream, fin, vinv, m, k, rowmajor_out ? 1 : 0, (TD*)dist);
+                           ctx->stream, (const TD*)b0, vinv, m, k, rowmajor_out ? 1 : 0, (TD*)dist);
         GG_HIP(hipGetLastError());
       }
     }
