@@ -75,7 +75,7 @@ def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma,
     Gd = G.grid(args.grid, args.grid, 0.01, 1, 1000, seed=seed, dtype=np.float32)
     w_r = rng.integers(1, 1001, Rg.nnz).astype(np.float32)
     Rw = G.HostCSR(Rg.m, Rg.n, Rg.rowptr, Rg.colval, w_r)
-    cases = [("grid", Gd, Gd.to_sell(R0, 64, 64, 0, args.seg),
+    cases = [("grid", Gd, Gd.to_sell(R0, 64, sigma, 0, args.seg, relabel),
               (rng.choice(Gd.m, args.sssp_sources, replace=False) + 1).astype(np.int32)),
              ("rmat", Rw, Rw.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel),
               roots[:args.sssp_k].astype(np.int32))]

@@ -32,7 +32,8 @@ inline uint64_t word_mask(int64_t k, int w) {
 
 template <int W>
 __device__ __forceinline__ bool finish_row(const MsArgs& a, int64_t rid, int64_t row,
-                                           const uint64_t* acc, const uint64_t* sn, uint64_t* nw) {
+                                           const uint64_t* acc, const uint64_t* sn, uint64_t* nw,
+                                           bool claim_any = true) {
   bool any = false;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
@@ -45,7 +46,7 @@ __device__ __forceinline__ bool finish_row(const MsArgs& a, int64_t rid, int64_t
     if (nw[w]) a.seen[rid * W + w] = sn[w] | nw[w];
   }
   if (any) {
-    bit_claim(a.next_any, rid);
+    if (claim_any) bit_claim(a.next_any, rid);
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       uint64_t b = nw[w];
@@ -77,7 +78,7 @@ template <int W> __device__ __forceinline__ void alive_wave(const MsArgs& a, con
   for (int w = 0; w < W; ++w) {
     uint64_t v = nw[w];
     for (int o = 32; o > 0; o >>= 1) v |= shfl_xor(v, o);
-    if ((threadIdx.x & 63) == 0 && v) atomicOr(&a.alive[w], (unsigned long long)v);
+    if ((threadIdx.x & 63) == 0 && v) or_new_bits(&a.alive[w], (unsigned long long)v);
   }
 }
 
@@ -128,32 +129,45 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
           for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
           if (saturated<W>(acc, sn, a)) break;
         }
-        newly = finish_row<W>(a, rid, row, acc, sn, nw);
+        newly = finish_row<W>(a, rid, row, acc, sn, nw, !g.internal);
       } else {
 #pragma unroll
         for (int w = 0; w < W; ++w) a.next[rid * W + w] = 0;
       }
     }
-    const unsigned long long b = __ballot(newly);
-    if (b && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)b) - 1)
-      atomicAdd(a.found, (unsigned)__popcll(b));
+    if (g.internal) wave_set_bits(a.next_any, slot & ~(int64_t)63, newly);
+    wave_flag(a.found, newly);
     alive_wave<W>(a, nw);
     return;
   }
-  // one block per heavy row: OR across the block, early exit once saturated
+  // one block per heavy segment: OR across the block, early exit once
+  // saturated.  A row of several segments ORs them into a.hacc; the last
+  // segment to arrive (a.hcnt) finishes the row and clears both, so they are
+  // zero again for the next level.  A row saturated before the level is
+  // settled by its first segment alone (every segment sees the same seen).
   __shared__ uint64_t red[kM / 64][W];
   __shared__ int s_full;
-  const int64_t h = (int64_t)blockIdx.x - n_sell_blocks;
+  const int64_t sg = (int64_t)blockIdx.x - n_sell_blocks;
+  int64_t h, s, e;
+  heavy_segment(g, sg, h, s, e);
+  const int64_t nseg = (int64_t)g.heavy_seg[h + 1] - g.heavy_seg[h];
   const int64_t row = g.heavy_row[h] - 1;
   const int64_t rid = heavy_rid(g, h);
   uint64_t sn[W], acc[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) { sn[w] = a.seen[rid * W + w]; acc[w] = 0; }
-  if (threadIdx.x == 0) s_full = saturated<W>(acc, sn, a);
+  const bool full0 = saturated<W>(acc, sn, a);
+  if (full0) {
+    if (sg == g.heavy_seg[h] && threadIdx.x == 0)
+#pragma unroll
+      for (int w = 0; w < W; ++w) a.next[rid * W + w] = 0;
+    return;
+  }
+  if (threadIdx.x == 0) s_full = 0;
   __syncthreads();
-  if (!s_full) {
+  {
     const TA* hv = (const TA*)g.heavy_val;
-    for (int64_t i = g.heavy_ptr[h] + threadIdx.x; i < g.heavy_ptr[h + 1]; i += kM) {
+    for (int64_t i = s + threadIdx.x; i < e; i += kM) {
       if (__atomic_load_n(&s_full, __ATOMIC_RELAXED)) break;
       const int32_t col = g.heavy_col[i];
       if (!bit_get(a.front_any, col - 1) || !(UNIT || edge_on(hv, i))) continue;
@@ -173,12 +187,23 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
 #pragma unroll
     for (int w = 0; w < W; ++w)
       for (int q = 1; q < kM / 64; ++q) acc[w] |= red[q][w];
+    if (nseg > 1) {
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        if (acc[w]) atomicOr(&a.hacc[h * W + w], (unsigned long long)acc[w]);
+      __threadfence();
+      if (atomicAdd(&a.hcnt[h], 1u) != (unsigned)(nseg - 1)) return;  // not the last segment
+      __threadfence();
+#pragma unroll
+      for (int w = 0; w < W; ++w) acc[w] = atomicExch(&a.hacc[h * W + w], 0ull);
+      atomicExch(&a.hcnt[h], 0u);
+    }
     uint64_t nw[W];
-    if (finish_row<W>(a, rid, row, acc, sn, nw)) atomicAdd(a.found, 1u);
+    if (finish_row<W>(a, rid, row, acc, sn, nw)) set_flag(a.found);
     if (a.alive)
 #pragma unroll
       for (int w = 0; w < W; ++w)
-        if (nw[w]) atomicOr(&a.alive[w], (unsigned long long)nw[w]);
+        if (nw[w]) or_new_bits(&a.alive[w], (unsigned long long)nw[w]);
   }
 }
 
@@ -209,9 +234,7 @@ __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict
       for (int w = 0; w < W; ++w) a.next[row * W + w] = 0;
     }
   }
-  const unsigned long long b = __ballot(newly);
-  if (b && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)b) - 1)
-    atomicAdd(a.found, (unsigned)__popcll(b));
+  wave_flag(a.found, newly);
   alive_wave<W>(a, nw);
 }
 
@@ -271,7 +294,9 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc)
       } else {
         SellView g = sell_view(A);
         const int64_t nsb = (g.n_slots + kM - 1) / kM;
-        hipLaunchKernelGGL((msbfs_sell_kernel<TA, U, WW>), dim3((unsigned)(nsb + g.n_heavy)),
+        if (g.n_heavy > 0 && (!a.hacc || !a.hcnt))
+          return set_error(GG_ERR_ARGUMENT, "heavy rows need segment accumulators");
+        hipLaunchKernelGGL((msbfs_sell_kernel<TA, U, WW>), dim3((unsigned)(nsb + g.n_seg)),
                            dim3(kM), 0, stream, g, nsb, a);
       }
       GG_HIP(hipGetLastError());
@@ -310,13 +335,14 @@ int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64
   int W = (int)((k + 63) / 64);
   int Wc = W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : 8;
   const int64_t words = (ns + 31) / 32;
-  void *f, *nx, *sn, *fa, *na, *cnt;
+  void *f, *nx, *sn, *fa, *na, *cnt, *hv;
   GG_TRY(ws_get(ctx, WS_MS0, sizeof(uint64_t) * ns * Wc, &f));
   GG_TRY(ws_get(ctx, WS_MS1, sizeof(uint64_t) * ns * Wc, &nx));
   GG_TRY(ws_get(ctx, WS_MS2, sizeof(uint64_t) * ns * Wc, &sn));
   GG_TRY(ws_get(ctx, WS_BFS1, sizeof(uint32_t) * words, &fa));
   GG_TRY(ws_get(ctx, WS_BFS2, sizeof(uint32_t) * words, &na));
   GG_TRY(ws_get(ctx, WS_FLAG, 64, &cnt));
+  GG_TRY(ws_get(ctx, WS_MSHEAVY, ms_heavy_bytes(A, Wc), &hv, true));  // kept zero by the kernel
   hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(ns * Wc, kM)), dim3(kM), 0, ctx->stream,
                      (uint64_t*)f, ns * Wc, 0ull);
   hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(ns * Wc, kM)), dim3(kM), 0, ctx->stream,
@@ -339,6 +365,8 @@ int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64
   for (int w = 0; w < 8; ++w) a.live[w] = w < Wc ? word_mask(k, w) : 0ull;
   a.start = nullptr;
   a.alive = nullptr;
+  a.hacc = (unsigned long long*)hv;
+  a.hcnt = (unsigned*)((unsigned long long*)hv + (A->format == GG_SELL ? A->n_heavy : 0) * Wc);
   uint64_t* fr = (uint64_t*)f;
   uint64_t* nxp = (uint64_t*)nx;
   uint32_t* fany = (uint32_t*)fa;

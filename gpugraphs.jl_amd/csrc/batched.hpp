@@ -18,11 +18,22 @@ struct MsArgs {
   uint64_t* seen;              // [ns*W]
   int32_t* dist;               // original rows
   int32_t level;
-  unsigned int* found;         // rows with a new bit this level
+  unsigned int* found;         // set to 1 when some row got a new bit this level
   uint64_t live[8];            // bits of the sources being traversed, per word
   const int32_t* start;        // per source: level at which it was seeded (NULL: 0)
   unsigned long long* alive;   // OR of every next word, per word (NULL: not kept)
+  // per heavy row of a SELL hybrid: OR of its segments' words [n_heavy*W] and
+  // arrival count [n_heavy]; zero before a level, left zero after it
+  unsigned long long* hacc;
+  unsigned int* hcnt;
 };
+
+// Zero-filled per-heavy-row accumulators for msbfs_level (MsArgs::hacc /
+// hcnt), W words per row, in one allocation of ms_heavy_bytes(A, W).
+inline size_t ms_heavy_bytes(const gg_matrix* A, int W) {
+  const int64_t nh = A->format == GG_SELL ? A->n_heavy : 0;
+  return (size_t)nh * (8 * (size_t)W + 4) + 64;
+}
 
 // One or-and level over A_T with W = Wc words per vertex (1, 2, 4 or 8).
 int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc);
@@ -31,8 +42,8 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc)
 // (state rows x k): d[v, j] = min(d[v, j], min_u safe_add(w(u->v), d[u, j]))
 // over the in-neighbours u whose bit is set in chp (rows that fell in the
 // previous step, or were seeded).  Rows that fall set their bit in chn and
-// are counted in *changed (one per (row, column)); when changed_mask is
-// given, bit j is set for every column j that fell (k <= 64).
+// set *changed to 1; when changed_mask is given, bit j is set for every
+// column j that fell (k <= 64).
 int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k, void* d,
                   const uint32_t* chp, uint32_t* chn, unsigned int* changed,
                   unsigned long long* changed_mask);

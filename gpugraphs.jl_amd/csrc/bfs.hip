@@ -37,9 +37,10 @@ __global__ void fill_i32v_kernel(int32_t* p, int64_t n, int32_t v) {
 }
 
 struct BfsCounters {
-  unsigned int found;        // vertices claimed by the level
+  unsigned int found;        // pull: 1 when the level claimed a vertex; push: claimed count
   unsigned int qlen;         // push: length of the next queue
   unsigned long long scout;  // push: out-degree sum of the claimed vertices
+  unsigned int awake;        // set bits of a bitmap (popcount_kernel)
 };
 
 // out at the original source s; the bitmaps at its state index.
@@ -53,11 +54,20 @@ __global__ void bfs_seed_kernel(int32_t* out, int32_t out_val, uint32_t* visited
   if (queue) queue[0] = (int32_t)s;
 }
 
-// All 64 lanes must be active.
-__device__ __forceinline__ void count_wave(bool newly, unsigned int* found) {
-  const unsigned long long b = __ballot(newly);
-  if (b && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)b) - 1)
-    atomicAdd(found, (unsigned)__popcll(b));
+// Number of set bits of a bitmap, one atomic per block.
+__global__ __launch_bounds__(kB) void popcount_kernel(const uint32_t* __restrict__ bm,
+                                                      int64_t words, unsigned int* out) {
+  __shared__ unsigned int part[kB / 64];
+  unsigned int c = 0;
+  for (int64_t w = (int64_t)blockIdx.x * kB + threadIdx.x; w < words; w += (int64_t)gridDim.x * kB)
+    c += __popc(bm[w]);
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kB / 64; ++i) c += part[i];
+    if (c) atomicAdd(out, c);
+  }
 }
 
 // ------------------------------------------------------------ pull, SELL
@@ -75,7 +85,9 @@ __global__ __launch_bounds__(kB) void bfs_pull_sell_kernel(
     bool has_len;
     const int64_t slot = (int64_t)blockIdx.x * kB + threadIdx.x;
     if (sell_slot(g, slot, row, base, len, has_len)) {
-      const int64_t rid = g.internal ? slot : row;  // index of the row's traversal state
+      // state index of the row; with internal ids the wavefront's 64 slots own
+      // two whole bitmap words, set below with one OR each
+      const int64_t rid = g.internal ? slot : row;
       if (!bit_get(visited, rid)) {
         const TA* val = (const TA*)g.val;
         int32_t hit = 0;
@@ -103,12 +115,20 @@ __global__ __launch_bounds__(kB) void bfs_pull_sell_kernel(
         }
         if (hit) {
           out[row] = PARENTS ? hit : level;
-          bit_claim(next, rid);
-          newly = bit_claim(visited, rid);
+          if (g.internal) {
+            newly = true;  // only this lane sets this bit
+          } else {
+            bit_claim(next, rid);
+            newly = bit_claim(visited, rid);
+          }
         }
       }
     }
-    count_wave(newly, &cnt->found);
+    if (g.internal) {
+      wave_set_bits(next, slot & ~(int64_t)63, newly);
+      wave_set_bits(visited, slot & ~(int64_t)63, newly);
+    }
+    wave_flag(&cnt->found, newly);
     return;
   }
   __shared__ int s_skip;
@@ -166,7 +186,7 @@ __global__ __launch_bounds__(kB) void bfs_pull_sell_kernel(
     if (hit) {
       out[row] = PARENTS ? hit : level;
       bit_claim(next, rid);
-      if (bit_claim(visited, rid)) atomicAdd(&cnt->found, 1u);
+      if (bit_claim(visited, rid)) set_flag(&cnt->found);
     }
   }
 }
@@ -216,7 +236,7 @@ __global__ __launch_bounds__(kB) void bfs_pull_csr_kernel(
       bit_claim(next, row);
       newly = bit_claim(visited, row);
     }
-    count_wave(newly, &cnt->found);
+    wave_flag(&cnt->found, newly);
   }
 }
 
@@ -307,6 +327,7 @@ __global__ void zero_counters_kernel(BfsCounters* c) {
   c->found = 0;
   c->qlen = 0;
   c->scout = 0;
+  c->awake = 0;
 }
 
 __global__ void degree_kernel(const int32_t* rowptr, int64_t v, BfsCounters* c) {
@@ -522,8 +543,11 @@ int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_ou
         fill_words(ctx, w.next, w.words, 0u);
         hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
         GG_TRY(pull_level(ctx, A_T, false, w, dist, level));
+        hipLaunchKernelGGL(popcount_kernel, dim3(std::min<int64_t>(blocks_for(w.words, kB), 1024)),
+                           dim3(kB), 0, ctx->stream, w.next, w.words, &w.cnt->awake);
+        GG_HIP(hipGetLastError());
         GG_TRY(read_counters(ctx, w, &hc));
-        awake = hc.found;
+        awake = hc.awake;
         std::swap(w.front, w.next);
       } while (awake > 0 && (awake >= old || (double)awake > (double)n / beta));
       if (awake == 0) break;

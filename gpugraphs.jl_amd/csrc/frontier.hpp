@@ -18,6 +18,35 @@ __device__ __forceinline__ bool bit_claim(uint32_t* bm, int64_t v) {
   return !(atomicOr(&bm[v >> 5], b) & b);
 }
 
+// Sets bit (base + lane) for every lane of the wavefront with `on`, where base
+// is the wavefront's first id, a multiple of 64 (the internal ids of one
+// wavefront's 64 slots): one OR per 32-bit word instead of one atomic per
+// lane on a shared word.  All 64 lanes must call it.
+__device__ __forceinline__ void wave_set_bits(uint32_t* bm, int64_t base, bool on) {
+  const unsigned long long b = __ballot(on);
+  if ((threadIdx.x & 63) == 0 && b) {
+    const uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
+    if (lo) atomicOr(&bm[base >> 5], lo);
+    if (hi) atomicOr(&bm[(base >> 5) + 1], hi);
+  }
+}
+
+// *f = 1 when any lane of the wavefront has v.  The flag is read first and
+// written only while it is still 0, so the wavefronts of a launch do not
+// queue on one address (an atomic per wavefront on one counter costs about
+// as much as the whole level).  All 64 lanes must call it.
+__device__ __forceinline__ void wave_flag(unsigned int* f, bool v) {
+  if (__ballot(v) && (threadIdx.x & 63) == 0 && __atomic_load_n(f, __ATOMIC_RELAXED) == 0u)
+    __atomic_store_n(f, 1u, __ATOMIC_RELAXED);
+}
+__device__ __forceinline__ void set_flag(unsigned int* f) {
+  if (__atomic_load_n(f, __ATOMIC_RELAXED) == 0u) __atomic_store_n(f, 1u, __ATOMIC_RELAXED);
+}
+// *m |= bits, skipping the atomic when every bit is already set.
+__device__ __forceinline__ void or_new_bits(unsigned long long* m, unsigned long long bits) {
+  if (bits & ~__atomic_load_n(m, __ATOMIC_RELAXED)) atomicOr(m, bits);
+}
+
 // Value of a stored entry as an or-and edge: (value & 1) != 0, i.e. Bool true
 // or an odd integer; with curr in {0,1} this is exactly when
 // GPUGraphs_band(value, curr) == 1 (src/algebra.jl:98).
@@ -121,6 +150,36 @@ __device__ __forceinline__ bool sell_slot(const SellView& g, int64_t slot, int64
   has_len = g.slot_len != nullptr;
   len = has_len ? (int64_t)g.slot_len[slot] : ((int64_t)g.slice_ptr[sl + 1] - g.slice_ptr[sl]) / g.C;
   return true;
+}
+
+// Heavy segment sg -> (heavy row h, entries [s, e)): heavy_seg[h] <= sg <
+// heavy_seg[h + 1], segments of seg_len entries from heavy_ptr[h].
+__device__ __forceinline__ void heavy_segment(const SellView& g, int64_t sg, int64_t& h,
+                                              int64_t& s, int64_t& e) {
+  int64_t lo = 0, hi = g.n_heavy;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (g.heavy_seg[mid] <= sg) lo = mid; else hi = mid;
+  }
+  h = lo;
+  s = g.heavy_ptr[h] + (sg - g.heavy_seg[h]) * g.seg_len;
+  e = min(s + g.seg_len, g.heavy_ptr[h + 1]);
+}
+
+// *p = min(*p, v) as one atomic step on the value's bits; true when v lowered
+// *p (v < *p: a NaN never lowers, as in the non-atomic update).
+template <class T> __device__ __forceinline__ bool atomic_min_lowered(T* p, T v) {
+  typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned int>::type U;
+  static_assert(sizeof(T) == sizeof(U), "4- or 8-byte distances");
+  U* q = (U*)p;
+  U old = __atomic_load_n(q, __ATOMIC_RELAXED);
+  const U vb = __builtin_bit_cast(U, v);
+  while (v < __builtin_bit_cast(T, old)) {
+    const U prev = atomicCAS(q, old, vb);
+    if (prev == old) return true;
+    old = prev;
+  }
+  return false;
 }
 
 // x in internal order: xp[slot] = x[perm[slot]] (0 for an empty slot),

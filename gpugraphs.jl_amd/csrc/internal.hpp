@@ -42,6 +42,7 @@ enum WsSlot {
   WS_MS0, WS_MS1, WS_MS2, WS_MS3, WS_MS4,
   WS_XPERM,         // input vector / matrix gathered into internal order
   WS_MSDIST,        // row-major distances of a multi-source traversal
+  WS_MSHEAVY,       // per-heavy-row segment accumulators (kept zero between levels)
   WS_NSLOTS
 };
 

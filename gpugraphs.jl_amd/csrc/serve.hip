@@ -193,6 +193,7 @@ struct Engine {
   uint64_t *front = nullptr, *next = nullptr, *seen = nullptr;
   uint32_t *fany = nullptr, *nany = nullptr;
   int32_t *dist = nullptr, *start = nullptr;
+  void* heavy_acc = nullptr;  // segment accumulators of heavy rows (kept zero)
   int32_t level = 0;
   // SSSP
   int dist_dtype = GG_INT32;
@@ -260,6 +261,10 @@ static int make_engine(gg_server* s, const gg_matrix* A, int kind, std::unique_p
     GG_TRY(engine_alloc(s, *e, 4 * words, &p)); e->nany = (uint32_t*)p;
     GG_TRY(engine_alloc(s, *e, 4 * m * 64, &p)); e->dist = (int32_t*)p;
     GG_TRY(engine_alloc(s, *e, 4 * 64, &p)); e->start = (int32_t*)p;
+    const size_t hb = ms_heavy_bytes(A, 1);
+    GG_TRY(engine_alloc(s, *e, hb, &p)); e->heavy_acc = p;
+    hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(hb / 4, 256)), dim3(256), 0,
+                       e->stream, (uint32_t*)p, (int64_t)(hb / 4), 0u);
     hipLaunchKernelGGL(serve_fill_kernel<uint64_t>, dim3(nb), dim3(256), 0, e->stream, e->front, ns, 0ull);
     hipLaunchKernelGGL(serve_fill_kernel<uint64_t>, dim3(nb), dim3(256), 0, e->stream, e->seen, ns, 0ull);
     hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(words, 256)), dim3(256), 0,
@@ -405,6 +410,8 @@ static int launch_step(gg_server* s, Engine& e) {
     a.live[0] = e.active;
     a.start = e.start;
     a.alive = e.mask_d;
+    a.hacc = (unsigned long long*)e.heavy_acc;
+    a.hcnt = (unsigned*)(a.hacc + (e.A.format == GG_SELL ? e.A.n_heavy : 0));
     GG_TRY(msbfs_level(e.stream, &e.A, a, 1));
     e.level += 1;
     std::swap(e.front, e.next);

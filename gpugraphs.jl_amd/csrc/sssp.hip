@@ -30,11 +30,6 @@ template <class TD> __device__ __forceinline__ TD sadd(TD w, TD d) {
   return jmax(w, d) == sat ? sat : jadd(w, d);  // GPUGraphs_safe_add, src/algebra.jl:47-48
 }
 
-__device__ __forceinline__ void count_changes(bool ch, unsigned int* cnt) {
-  const unsigned long long b = __ballot(ch);
-  if (b && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)b) - 1)
-    atomicAdd(cnt, (unsigned)__popcll(b));
-}
 
 // ----------------------------------------------------------- one source, CSR
 template <class TA, class TD>
@@ -71,14 +66,14 @@ __global__ __launch_bounds__(kT) void bf_csr_kernel(const int32_t* __restrict__ 
         ch = true;
       }
     }
-    count_changes(ch, changed);
+    wave_flag(changed, ch);
   }
 }
 
 // ---------------------------------------------------------- one source, SELL
 // Light slots: one lane per row, entries in batches of 4 (index loads, bitmap
 // tests, then the value / distance loads of the changed neighbours).  Heavy
-// rows: one block per row.
+// rows: one block per segment, folded into the row with an atomic min.
 template <class TA, class TD>
 __global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_blocks, TD* d,
                                                      const uint32_t* __restrict__ chp,
@@ -117,18 +112,20 @@ __global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_
       const TD nv = jmin(dv, acc);
       if (nv < dv) {
         d[rid] = nv;
-        bit_claim(chn, rid);
+        if (!g.internal) bit_claim(chn, rid);
         ch = true;
       }
     }
-    count_changes(ch, changed);
+    if (g.internal) wave_set_bits(chn, slot & ~(int64_t)63, ch);  // the wavefront's two words
+    wave_flag(changed, ch);
     return;
   }
   __shared__ TD red[kT / 64];
-  const int64_t h = (int64_t)blockIdx.x - n_sell_blocks;
+  int64_t h, s, e;
+  heavy_segment(g, (int64_t)blockIdx.x - n_sell_blocks, h, s, e);
   const TA* hv = (const TA*)g.heavy_val;
   TD acc = Lim<TD>::max();
-  for (int64_t i = g.heavy_ptr[h] + threadIdx.x; i < g.heavy_ptr[h + 1]; i += kT) {
+  for (int64_t i = s + threadIdx.x; i < e; i += kT) {
     const int32_t c = g.heavy_col[i] - 1;
     if (bit_get(chp, c)) acc = jmin(acc, sadd(TD(hv[i]), d[c]));
   }
@@ -138,12 +135,9 @@ __global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_
   if (threadIdx.x == 0) {
     for (int w = 1; w < kT / 64; ++w) acc = jmin(acc, red[w]);
     const int64_t rid = heavy_rid(g, h);
-    const TD dv = d[rid];
-    const TD nv = jmin(dv, acc);
-    if (nv < dv) {
-      d[rid] = nv;
+    if (atomic_min_lowered(&d[rid], acc)) {
       bit_claim(chn, rid);
-      atomicAdd(changed, 1u);
+      set_flag(changed);
     }
   }
 }
@@ -152,11 +146,13 @@ __global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_
 // Rows of A_T as (entry list) for either format: CSR row, SELL slot, or a
 // heavy row.  A group of Lk lanes (one per source, Lk >= k when k <= 64)
 // walks the row; the bitmap test is uniform across a group (one column).
-// Heavy rows get a whole block whose groups split the entries.
+// Heavy rows are cut into parts of at most 256 entries (each heavy segment in
+// `parts` pieces); a block per (part, column block) splits its entries over
+// its groups and folds the result into the row with per-column atomic mins.
 template <class TA, class TD>
 __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t* __restrict__ rowptr,
                                                       SellView g, int64_t n_light_groups,
-                                                      int64_t k, int logLk, TD* d,
+                                                      int64_t parts, int64_t k, int logLk, TD* d,
                                                       const uint32_t* __restrict__ chp,
                                                       uint32_t* __restrict__ chn,
                                                       unsigned int* changed,
@@ -206,14 +202,14 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
         }
       }
     }
-    count_changes(ch, changed);
+    wave_flag(changed, ch);
     const unsigned long long b = __ballot(ch);
     const unsigned long long low = Lk == 64 ? ~0ull : ((1ull << Lk) - 1ull);
     if (lane == 0 && ((b >> gbase) & low)) bit_claim(chn, rid);  // this group's row fell
     if (changed_mask) {  // k <= 64: lane l of a group is column l
       unsigned long long cm = 0;
       for (int o = 0; o < 64; o += Lk) cm |= (b >> o) & low;
-      if (wl == 0 && cm) atomicOr(changed_mask, cm);
+      if (wl == 0 && cm) or_new_bits(changed_mask, cm);
     }
     return;
   }
@@ -222,14 +218,20 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
   TD* red = (TD*)smem;  // [gpb][Lk]
   __shared__ int s_any;
   const int64_t hb = (int64_t)blockIdx.x - light_blocks;
-  const int64_t h = hb / jblocks;
-  const int64_t j = (hb - h * jblocks) * Lk + lane;
+  const int64_t unit = hb / jblocks;
+  const int64_t j = (hb - unit * jblocks) * Lk + lane;
   const int64_t grp = threadIdx.x >> logLk;
+  int64_t h, s, e;
+  heavy_segment(g, unit / parts, h, s, e);
+  const int64_t plen = (g.seg_len + parts - 1) / parts;
+  s += (unit % parts) * plen;
+  e = min(e, s + plen);
+  if (s >= e) return;  // a part past the end of the row's last segment (uniform)
   const TA* hv = (const TA*)g.heavy_val;
   TD acc = Lim<TD>::max();
   if (threadIdx.x == 0) s_any = 0;
   if (j < k)
-    for (int64_t i = g.heavy_ptr[h] + grp; i < g.heavy_ptr[h + 1]; i += gpb) {
+    for (int64_t i = s + grp; i < e; i += gpb) {
       const int32_t c = g.heavy_col[i] - 1;
       if (bit_get(chp, c)) acc = jmin(acc, sadd(TD(hv[i]), d[(int64_t)c * k + j]));
     }
@@ -239,12 +241,9 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
     for (int64_t o = 1; o < gpb; ++o) acc = jmin(acc, red[o * Lk + lane]);
     if (j < k) {
       const int64_t rid = heavy_rid(g, h);
-      const TD dv = d[rid * k + j];
-      const TD nv = jmin(dv, acc);
-      if (nv < dv) {
-        d[rid * k + j] = nv;
-        atomicAdd(changed, 1u);
-        if (changed_mask) atomicOr(changed_mask, 1ull << j);
+      if (atomic_min_lowered(&d[rid * k + j], acc)) {
+        set_flag(changed);
+        if (changed_mask) or_new_bits(changed_mask, 1ull << j);
         s_any = 1;
       }
     }
@@ -399,8 +398,8 @@ int sssp_one(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist, int di
         if (nb > cap) nb = cap;
         hipLaunchKernelGGL((bf_csr_kernel<TA, TD>), dim3(nb), dim3(kT), 0, ctx->stream, A->ptr,
                            A->idx, (const TA*)A->val, m, lg, d, c.prev, c.next, (unsigned*)cnt);
-      } else if (nsb + g.n_heavy > 0) {
-        hipLaunchKernelGGL((bf_sell_kernel<TA, TD>), dim3((unsigned)(nsb + g.n_heavy)), dim3(kT),
+      } else if (nsb + g.n_seg > 0) {
+        hipLaunchKernelGGL((bf_sell_kernel<TA, TD>), dim3((unsigned)(nsb + g.n_seg)), dim3(kT),
                            0, ctx->stream, g, nsb, d, c.prev, c.next, (unsigned*)cnt);
       }
       GG_HIP(hipGetLastError());
@@ -430,15 +429,21 @@ int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_
     const int64_t jblocks = (k + Lk - 1) / Lk;
     const int64_t gpb = kT / Lk;
     SellView g = sell_view(A);
-    if (A->format == GG_CSR) { g.colval = A->idx; g.val = A->val; g.n_heavy = 0; }
+    if (A->format == GG_CSR) {
+      g.colval = A->idx;
+      g.val = A->val;
+      g.n_heavy = 0;
+      g.n_seg = 0;
+    }
     const int64_t n_light = A->format == GG_CSR ? A->m : g.n_slots;
     const int64_t light_blocks = (n_light * jblocks + gpb - 1) / gpb;
-    const int64_t nb = light_blocks + g.n_heavy * jblocks;
+    const int64_t parts = g.n_seg > 0 ? (g.seg_len + 255) / 256 : 1;
+    const int64_t nb = light_blocks + g.n_seg * parts * jblocks;
     if (nb > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
     if (nb > 0)
       hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
-                         stream, A->format, A->ptr, g, n_light, k, lg, (TD*)d, chp, chn, changed,
-                         changed_mask);
+                         stream, A->format, A->ptr, g, n_light, parts, k, lg, (TD*)d, chp, chn,
+                         changed, changed_mask);
     GG_HIP(hipGetLastError());
     return GG_OK;
   });

@@ -9,8 +9,11 @@
 // OpenMP splits rows across threads the way KernelAbstractions' CPU backend
 // splits an ndrange.  Built with -ffp-contract=off so a*b+c is never fused.
 //
-// Pinning (what checks each function independently of this restatement) is
-// listed in oracle/README.md and exercised by tests/test_oracle_*.py.
+// Every exported function is pinned independently of this restatement by
+// tests/test_oracle_pinning.py: the SELL known-answer test and CSR error
+// cases of the reference's test/structs.jl (tests/golden/), FP64 products,
+// closed forms of masks / ranges / saturation, scipy's BFS and Dijkstra,
+// BFS-tree validity and brute force on tiny graphs.
 
 #include <stdint.h>
 #include <string.h>
