@@ -149,7 +149,10 @@ def main():
                     help="SELL heavy-row threshold (default 1024 relabelled, 256 sigma windows)")
     ap.add_argument("--no-relabel", action="store_true",
                     help="sigma = 4096 windows in original ids instead of degree relabelling")
-    ap.add_argument("--seg", type=int, default=4096, help="heavy segment length")
+    ap.add_argument("--seg", type=int, default=4096,
+                    help="heavy segment length of the SpMV and Bellman-Ford matrices")
+    ap.add_argument("--bfs-seg", type=int, default=65536,
+                    help="heavy segment length of the BFS (Bool pattern) matrix")
     ap.add_argument("--plain-roots", type=int, default=64)
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -194,7 +197,7 @@ def main():
     t = time.time()
     A_f = Rg.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel)   # fp32 values for SpMV
     Rb = G.HostCSR(n, n, Rg.rowptr, Rg.colval, np.ones(nnz, np.bool_))
-    A_b = Rb.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel)   # Bool pattern for BFS
+    A_b = Rb.to_sell(R0, 64, sigma, args.heavy, args.bfs_seg, relabel)   # Bool pattern for BFS
     A_csr = Rb.to_csr(R0)                                          # out-edges for push (symmetric)
     torch.cuda.synchronize()
     log(f"SELL builds + upload {time.time() - t:.1f}s; stored/nnz {(A_f.n_stored + (int(A_f.heavy_ptr[-1].item()) if A_f.n_heavy else 0)) / nnz:.3f}")

@@ -54,7 +54,8 @@ __device__ __forceinline__ bool finish_row(const MsArgs& a, int64_t rid, int64_t
         const int s = w * 64 + __ffsll((unsigned long long)b) - 1;
         b &= b - 1;
         const int32_t d = a.start ? a.level - a.start[s] : a.level;
-        if (a.layout == GG_ROW_MAJOR) a.dist[row * a.k + s] = d;
+        if (a.dist8) a.dist8[row * a.k + s] = (uint8_t)d;
+        else if (a.layout == GG_ROW_MAJOR) a.dist[row * a.k + s] = d;
         else a.dist[row + (int64_t)s * a.m] = d;
       }
     }
@@ -238,19 +239,42 @@ __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict
   alive_wave<W>(a, nw);
 }
 
-__global__ void msbfs_seed_kernel(const int32_t* sources, int64_t k, int64_t m, int W, int layout,
-                                  const int32_t* vinv, uint64_t* front, uint32_t* front_any,
-                                  uint64_t* seen, int32_t* dist) {
-  // one thread: several sources may share a vertex
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  for (int64_t s = 0; s < k; ++s) {
-    const int64_t v = sources[s] - 1;
-    const int64_t r = vinv ? (int64_t)vinv[v] - 1 : v;  // state index
-    front[r * W + s / 64] |= 1ull << (s & 63);
-    seen[r * W + s / 64] |= 1ull << (s & 63);
-    front_any[r >> 5] |= 1u << (r & 31);
-    if (layout == GG_ROW_MAJOR) dist[v * k + s] = 0;
-    else dist[v + s * m] = 0;
+// One thread per source (several sources may share a vertex: atomic ORs);
+// distances row-major m x k, bytes when dist8 is given.
+__global__ void msbfs_seed_kernel(const int32_t* sources, int64_t k, int W, const int32_t* vinv,
+                                  uint64_t* front, uint32_t* front_any, uint64_t* seen,
+                                  int32_t* dist, uint8_t* dist8) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= k) return;
+  const int64_t v = sources[s] - 1;
+  const int64_t r = vinv ? (int64_t)vinv[v] - 1 : v;  // state index
+  atomicOr((unsigned long long*)&front[r * W + s / 64], 1ull << (s & 63));
+  atomicOr((unsigned long long*)&seen[r * W + s / 64], 1ull << (s & 63));
+  atomicOr(&front_any[r >> 5], 1u << (r & 31));
+  if (dist8) dist8[v * k + s] = 0;
+  else dist[v * k + s] = 0;
+}
+
+// Byte levels row-major m x k -> Int32 column-major m x k (255 = unreached ->
+// typemax(Int32)) through a 64 x 64 LDS tile.
+__global__ __launch_bounds__(256) void widen_transpose_kernel(const uint8_t* __restrict__ src,
+                                                              int64_t m, int64_t k,
+                                                              int32_t* __restrict__ dst) {
+  __shared__ uint8_t tile[64][65];
+  const int64_t tiles_k = (k + 63) / 64;
+  const int64_t r0 = ((int64_t)blockIdx.x / tiles_k) * 64;
+  const int64_t c0 = ((int64_t)blockIdx.x % tiles_k) * 64;
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int r = i >> 6, c = i & 63;
+    if (r0 + r < m && c0 + c < k) tile[r][c] = src[(r0 + r) * k + c0 + c];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int c = i >> 6, r = i & 63;
+    if (r0 + r < m && c0 + c < k) {
+      const uint8_t d = tile[r][c];
+      dst[(r0 + r) + (c0 + c) * m] = d == 255 ? 2147483647 : (int32_t)d;
+    }
   }
 }
 
@@ -321,15 +345,21 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc)
 
 // Runs one batch of k <= 512 sources.  The levels write each new (vertex,
 // source) distance into a row-major scratch, where a vertex's k entries share
-// lines; a column-major result is transposed from it at the end.
-int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64_t k,
-              int32_t* dist_out, int layout, int64_t* levels_out) {
+// lines: bytes (a quarter of the traffic) while the levels stay below 255,
+// then widened and transposed to the column-major result.  A traversal that
+// reaches level 255 is run again with Int32 levels.
+static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64_t k,
+                      int32_t* dist_out, int layout, bool narrow, bool* overflow,
+                      int64_t* levels_out) {
   const int64_t m = A->m;
   int32_t* dist = dist_out;
+  uint8_t* dist8 = nullptr;
+  *overflow = false;
   if (layout == GG_COL_MAJOR) {
     void* p;
-    GG_TRY(ws_get(ctx, WS_MSDIST, sizeof(int32_t) * m * k, &p));
-    dist = (int32_t*)p;
+    GG_TRY(ws_get(ctx, WS_MSDIST, (narrow ? 1 : sizeof(int32_t)) * m * k + 4, &p));
+    if (narrow) dist8 = (uint8_t*)p;
+    else dist = (int32_t*)p;
   }
   const int64_t ns = state_size(A);  // rows of the traversal state
   int W = (int)((k + 63) / 64);
@@ -349,11 +379,18 @@ int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64
                      (uint64_t*)sn, ns * Wc, 0ull);
   hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, ctx->stream,
                      (uint32_t*)fa, words, 0u);
-  hipLaunchKernelGGL(fill_i32v_kernel, dim3(blocks_for(m * k, kM)), dim3(kM), 0, ctx->stream,
-                     dist, m * k, (int32_t)2147483647);
+  if (dist8) {
+    const int64_t w4 = (m * k + 3) / 4;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(w4, kM)), dim3(kM), 0, ctx->stream,
+                       (uint32_t*)dist8, w4, 0xffffffffu);
+  } else {
+    hipLaunchKernelGGL(fill_i32v_kernel, dim3(blocks_for(m * k, kM)), dim3(kM), 0, ctx->stream,
+                       dist, m * k, (int32_t)2147483647);
+  }
   const int32_t* vinv = (A->flags & GG_FLAG_INTERNAL_IDS) ? A->vinv : nullptr;
-  hipLaunchKernelGGL(msbfs_seed_kernel, dim3(1), dim3(1), 0, ctx->stream, sources_dev, k, m, Wc,
-                     GG_ROW_MAJOR, vinv, (uint64_t*)f, (uint32_t*)fa, (uint64_t*)sn, dist);
+  hipLaunchKernelGGL(msbfs_seed_kernel, dim3(blocks_for(k, kM)), dim3(kM), 0, ctx->stream,
+                     sources_dev, k, Wc, vinv, (uint64_t*)f, (uint32_t*)fa, (uint64_t*)sn, dist,
+                     dist8);
   GG_HIP(hipGetLastError());
   MsArgs a{};
   a.m = m;
@@ -361,6 +398,7 @@ int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64
   a.layout = GG_ROW_MAJOR;
   a.seen = (uint64_t*)sn;
   a.dist = dist;
+  a.dist8 = dist8;
   a.found = (unsigned*)cnt;
   for (int w = 0; w < 8; ++w) a.live[w] = w < Wc ? word_mask(k, w) : 0ull;
   a.start = nullptr;
@@ -374,6 +412,10 @@ int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64
   int32_t level = 0;
   while (true) {
     ++level;
+    if (dist8 && level == 255) {  // byte levels are full
+      *overflow = true;
+      return GG_OK;
+    }
     a.front = fr;
     a.front_any = fany;
     a.next = nxp;
@@ -394,11 +436,24 @@ int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64
   }
   if (layout == GG_COL_MAJOR && m > 0) {
     const int64_t tiles = ((m + 63) / 64) * ((k + 63) / 64);
-    hipLaunchKernelGGL(transpose_rm_cm_kernel, dim3((unsigned)tiles), dim3(256), 0, ctx->stream,
-                       dist, m, k, dist_out);
+    if (dist8)
+      hipLaunchKernelGGL(widen_transpose_kernel, dim3((unsigned)tiles), dim3(256), 0,
+                         ctx->stream, dist8, m, k, dist_out);
+    else
+      hipLaunchKernelGGL(transpose_rm_cm_kernel, dim3((unsigned)tiles), dim3(256), 0,
+                         ctx->stream, dist, m, k, dist_out);
     GG_HIP(hipGetLastError());
   }
   if (levels_out) *levels_out = level - 1;
+  return GG_OK;
+}
+
+int msbfs_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64_t k,
+              int32_t* dist_out, int layout, int64_t* levels_out) {
+  bool overflow = false;
+  GG_TRY(msbfs_once(ctx, A, sources_dev, k, dist_out, layout, true, &overflow, levels_out));
+  if (overflow)
+    GG_TRY(msbfs_once(ctx, A, sources_dev, k, dist_out, layout, false, &overflow, levels_out));
   return GG_OK;
 }
 

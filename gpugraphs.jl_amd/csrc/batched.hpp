@@ -17,6 +17,7 @@ struct MsArgs {
   uint32_t* next_any;
   uint64_t* seen;              // [ns*W]
   int32_t* dist;               // original rows
+  uint8_t* dist8;              // when set: byte levels (< 255), row-major m x k, instead of dist
   int32_t level;
   unsigned int* found;         // set to 1 when some row got a new bit this level
   uint64_t live[8];            // bits of the sources being traversed, per word

@@ -41,6 +41,7 @@ struct BfsCounters {
   unsigned int qlen;         // push: length of the next queue
   unsigned long long scout;  // push: out-degree sum of the claimed vertices
   unsigned int awake;        // set bits of a bitmap (popcount_kernel)
+  unsigned int nbig;         // push: hub pieces queued for push_big_kernel
 };
 
 // out at the original source s; the bitmaps at its state index.
@@ -241,57 +242,118 @@ __global__ __launch_bounds__(kB) void bfs_pull_csr_kernel(
 }
 
 // -------------------------------------------------------------- push, CSR
-// Frontier queue over A_out (row u = out-neighbours of u), groups of L lanes
-// per frontier vertex; claimed vertices go to the next queue (one atomic per
-// wavefront) and to the next bitmap.
-__global__ __launch_bounds__(kB) void bfs_push_kernel(
-    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
-    const int32_t* __restrict__ queue, int64_t qlen, int logL, int32_t* __restrict__ next_queue,
-    uint32_t* __restrict__ next, uint32_t* __restrict__ visited, int32_t* __restrict__ dist,
-    int32_t level, const int32_t* __restrict__ vinv, BfsCounters* cnt) {
-  const int L = 1 << logL;
-  const int wl = threadIdx.x & 63;
-  const int lane = wl & (L - 1);
-  const int64_t groups_per_wave = 64 >> logL;
-  const int64_t wave = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * kB) >> 6;
-  for (int64_t w0 = wave * groups_per_wave; w0 < qlen; w0 += nwaves * groups_per_wave) {
-    const int64_t q = w0 + (wl >> logL);
-    int32_t s = 0, e = 0;
-    if (q < qlen) {
-      const int32_t u = queue[q];
-      s = rowptr[u] - 1;
-      e = rowptr[u + 1] - 1;
+// Frontier queue over A_out (row u = out-neighbours of u), expanded edge by
+// edge so a hub's edges spread over many lanes: a block takes 256 queue
+// entries, scans their out-degrees in LDS and walks the tile's edges 256 at
+// a time (each lane finds its vertex by binary search in the scan).  A vertex
+// with more than kBigDeg out-edges is cut into kBigDeg-edge pieces that a
+// second kernel spreads over blocks.  Claimed vertices go to the next queue
+// (one atomic per wavefront) and to the next bitmap.
+constexpr int kBigDeg = 4096;
+
+struct PushArgs {
+  const int32_t* rowptr;
+  const int32_t* colval;
+  const int32_t* queue;
+  int64_t qlen;
+  int32_t* next_queue;
+  uint32_t* next;
+  uint32_t* visited;
+  int32_t* dist;
+  int32_t level;
+  const int32_t* vinv;  // original -> internal id + 1 (bitmaps in state order), or NULL
+  BfsCounters* cnt;
+  int2* big;            // pieces (vertex, first edge) of the hubs in the queue
+};
+
+// Edge u -> v (0-based v) when valid.  All 64 lanes must call it.
+__device__ __forceinline__ void push_edge(const PushArgs& a, bool valid, int32_t v) {
+  bool newly = false;
+  if (valid) {
+    const int64_t r = a.vinv ? (int64_t)a.vinv[v] - 1 : v;
+    if (!bit_get(a.visited, r) && bit_claim(a.visited, r)) {
+      newly = true;
+      a.dist[v] = a.level;
+      bit_claim(a.next, r);
     }
-    int32_t trips = (e - s + L - 1) >> logL;
-    for (int o = 32; o > 0; o >>= 1) trips = max(trips, __shfl_xor(trips, o, 64));
-    for (int32_t t = 0; t < trips; ++t) {
-      const int32_t i = s + t * L + lane;
-      bool newly = false;
+  }
+  const unsigned long long b = __ballot(newly);
+  if (b) {
+    const int wl = threadIdx.x & 63;
+    const int leader = __ffsll(b) - 1;
+    unsigned long long deg = newly ? (unsigned long long)(a.rowptr[v + 1] - a.rowptr[v]) : 0ull;
+    for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o, 64);
+    unsigned base = 0;
+    if (wl == leader) {
+      base = atomicAdd(&a.cnt->qlen, (unsigned)__popcll(b));
+      atomicAdd(&a.cnt->scout, deg);
+    }
+    base = __shfl(base, leader, 64);
+    if (newly) a.next_queue[base + __popcll(b & ((1ull << wl) - 1))] = v;
+  }
+}
+
+__global__ __launch_bounds__(kB) void push_tile_kernel(PushArgs a) {
+  __shared__ int32_t s_beg[kB];
+  __shared__ int32_t s_scan[kB + 1];
+  __shared__ int32_t s_wsum[kB / 64];
+  const int wl = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int64_t t0 = (int64_t)blockIdx.x * kB; t0 < a.qlen; t0 += (int64_t)gridDim.x * kB) {
+    const int64_t qi = t0 + threadIdx.x;
+    int32_t beg = 0, deg = 0;
+    if (qi < a.qlen) {
+      const int32_t u = a.queue[qi];
+      beg = a.rowptr[u] - 1;
+      deg = a.rowptr[u + 1] - 1 - beg;
+      if (deg > kBigDeg) {  // hub: pieces for push_big_kernel
+        const int np = (deg + kBigDeg - 1) / kBigDeg;
+        const unsigned p0 = atomicAdd(&a.cnt->nbig, (unsigned)np);
+        for (int p = 0; p < np; ++p) a.big[p0 + p] = make_int2(u, beg + p * kBigDeg);
+        deg = 0;
+      }
+    }
+    // exclusive scan of the tile's degrees: wavefront scans, then wavefront sums
+    int32_t incl = deg;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(incl, o, 64);
+      if (wl >= o) incl += y;
+    }
+    if (wl == 63) s_wsum[wv] = incl;
+    __syncthreads();
+    int32_t off = 0;
+    for (int w = 0; w < wv; ++w) off += s_wsum[w];
+    s_beg[threadIdx.x] = beg;
+    s_scan[threadIdx.x] = off + incl - deg;
+    if (threadIdx.x == kB - 1) s_scan[kB] = off + incl;
+    __syncthreads();
+    const int32_t total = s_scan[kB];
+    for (int32_t e0 = 0; e0 < total; e0 += kB) {
+      const int32_t e = e0 + threadIdx.x;
+      const bool valid = e < total;
       int32_t v = 0;
-      if (i < e) {
-        v = colval[i] - 1;
-        const int64_t r = vinv ? (int64_t)vinv[v] - 1 : v;  // bitmaps in state order
-        if (!bit_get(visited, r) && bit_claim(visited, r)) {
-          newly = true;
-          dist[v] = level;
-          bit_claim(next, r);
+      if (valid) {
+        int lo = 0, hi = kB;  // s_scan[lo] <= e < s_scan[hi]; lo is never a 0-degree entry
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_scan[mid] <= e) lo = mid; else hi = mid;
         }
+        v = a.colval[s_beg[lo] + (e - s_scan[lo])] - 1;
       }
-      const unsigned long long b = __ballot(newly);
-      if (b) {
-        const int leader = __ffsll(b) - 1;
-        unsigned base = 0;
-        unsigned long long deg = newly ? (unsigned long long)(rowptr[v + 1] - rowptr[v]) : 0ull;
-        for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o, 64);
-        if (wl == leader) {
-          base = atomicAdd(&cnt->qlen, (unsigned)__popcll(b));
-          atomicAdd(&cnt->found, (unsigned)__popcll(b));
-          atomicAdd(&cnt->scout, deg);
-        }
-        base = __shfl(base, leader, 64);
-        if (newly) next_queue[base + __popcll(b & ((1ull << wl) - 1))] = v;
-      }
+      push_edge(a, valid, v);
+    }
+    __syncthreads();  // the next tile reuses the LDS arrays
+  }
+}
+
+__global__ __launch_bounds__(kB) void push_big_kernel(PushArgs a) {
+  const int64_t nbig = __atomic_load_n(&a.cnt->nbig, __ATOMIC_RELAXED);  // from push_tile_kernel
+  for (int64_t p = blockIdx.x; p < nbig; p += gridDim.x) {
+    const int2 pc = a.big[p];
+    const int32_t stop = min(pc.y + kBigDeg, a.rowptr[pc.x + 1] - 1);
+    for (int32_t i0 = pc.y; i0 < stop; i0 += kB) {
+      const int32_t i = i0 + (int32_t)threadIdx.x;
+      const bool valid = i < stop;
+      push_edge(a, valid, valid ? a.colval[i] - 1 : 0);
     }
   }
 }
@@ -328,6 +390,7 @@ __global__ void zero_counters_kernel(BfsCounters* c) {
   c->qlen = 0;
   c->scout = 0;
   c->awake = 0;
+  c->nbig = 0;
 }
 
 __global__ void degree_kernel(const int32_t* rowptr, int64_t v, BfsCounters* c) {
@@ -533,7 +596,10 @@ int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_ou
   double scout = (double)hc.scout;
   int64_t qlen = 1;
   int32_t level = 0;
-  const int lg_push = 4;
+  // hub pieces: at most one per kBigDeg edges plus one per hub
+  void* bp;
+  GG_TRY(ws_get(ctx, WS_BFS5, sizeof(int2) * (size_t)(2 * (A_out->nnz / kBigDeg) + 16), &bp));
+  int2* big = (int2*)bp;
   while (qlen > 0) {
     if (scout > edges_to_check / alpha) {
       int64_t awake = qlen, old;
@@ -563,14 +629,16 @@ int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_ou
       ++level;
       fill_words(ctx, w.next, w.words, 0u);
       hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
-      unsigned nb = blocks_for((qlen << lg_push), kB);
-      const unsigned cap = (unsigned)ctx->num_cus * 32;
+      PushArgs pa{A_out->ptr, A_out->idx, w.q0, qlen, w.q1, w.next, w.visited, dist, level,
+                  vinv, w.cnt, big};
+      unsigned nb = blocks_for(qlen, kB);
+      const unsigned cap = (unsigned)ctx->num_cus * 16;
       if (nb > cap) nb = cap;
       {
         HotTimer ht(ctx);
-        hipLaunchKernelGGL(bfs_push_kernel, dim3(nb), dim3(kB), 0, ctx->stream, A_out->ptr,
-                           A_out->idx, w.q0, qlen, lg_push, w.q1, w.next, w.visited, dist, level,
-                           vinv, w.cnt);
+        hipLaunchKernelGGL(push_tile_kernel, dim3(nb), dim3(kB), 0, ctx->stream, pa);
+        hipLaunchKernelGGL(push_big_kernel, dim3((unsigned)ctx->num_cus * 4), dim3(kB), 0,
+                           ctx->stream, pa);
       }
       GG_HIP(hipGetLastError());
       GG_TRY(read_counters(ctx, w, &hc));

@@ -450,3 +450,20 @@ def test_multi_source_bfs_equals_single(env, k):
         Ao = orc.from_scipy_csr(R.to_scipy())
         for i in rng.choice(k, min(k, 6), replace=False):
             np.testing.assert_array_equal(D[:, i], orc.bfs_distances(Ao, int(srcs[i])))
+
+
+@pytest.mark.parametrize("fmt", ["csr", "sell_relabel"])
+def test_multi_source_bfs_deep_levels(env, fmt):
+    """More than 254 levels (a path of 700 vertices): the byte-level scratch
+    overflows and the batch is traversed again with Int32 levels."""
+    gg, torch, orc = env
+    n = 700
+    S = sp.diags([np.ones(n - 1), np.ones(n - 1)], [-1, 1], format="csr").astype(np.int32)
+    A = make(gg, S, fmt)
+    srcs = np.array([1, n, n // 2, 3], np.int32)
+    print_case("msbfs deep", fmt)
+    D = host(gg.multi_source_bfs(A, srcs))
+    Ao = orc.from_scipy_csr(S)
+    for i, s in enumerate(srcs):
+        np.testing.assert_array_equal(D[:, i], orc.bfs_distances(Ao, int(s)))
+    assert D.max() == n - 1
