@@ -31,9 +31,9 @@ inline uint64_t word_mask(int64_t k, int w) {
 }
 
 template <int W>
-__device__ __forceinline__ bool finish_row(const MsArgs& a, int64_t rid, int64_t row,
-                                           const uint64_t* acc, const uint64_t* sn, uint64_t* nw,
-                                           bool claim_any = true) {
+__device__ __forceinline__ bool finish_row(const MsArgs& a, int32_t level, int64_t rid,
+                                           int64_t row, const uint64_t* acc, const uint64_t* sn,
+                                           uint64_t* nw, bool claim_any = true) {
   bool any = false;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
@@ -53,7 +53,7 @@ __device__ __forceinline__ bool finish_row(const MsArgs& a, int64_t rid, int64_t
       while (b) {
         const int s = w * 64 + __ffsll((unsigned long long)b) - 1;
         b &= b - 1;
-        const int32_t d = a.start ? a.level - a.start[s] : a.level;
+        const int32_t d = a.start ? level - a.start[s] : level;
         if (a.dist8) a.dist8[row * a.k + s] = (uint8_t)d;
         else if (a.layout == GG_ROW_MAJOR) a.dist[row * a.k + s] = d;
         else a.dist[row + (int64_t)s * a.m] = d;
@@ -85,6 +85,8 @@ template <int W> __device__ __forceinline__ void alive_wave(const MsArgs& a, con
 
 template <class TA, bool UNIT, int W>
 __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_sell_blocks, MsArgs a) {
+  if (a.ctl && a.ctl[1]) return;  // a captured step after the last level
+  const int32_t level = a.ctl ? a.ctl[0] : a.level;
   if ((int64_t)blockIdx.x < n_sell_blocks) {
     int64_t row, base, len;
     bool has_len;
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
           for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
           if (saturated<W>(acc, sn, a)) break;
         }
-        newly = finish_row<W>(a, rid, row, acc, sn, nw, !g.internal);
+        newly = finish_row<W>(a, level, rid, row, acc, sn, nw, !g.internal);
       } else {
 #pragma unroll
         for (int w = 0; w < W; ++w) a.next[rid * W + w] = 0;
@@ -200,7 +202,7 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
       atomicExch(&a.hcnt[h], 0u);
     }
     uint64_t nw[W];
-    if (finish_row<W>(a, rid, row, acc, sn, nw)) set_flag(a.found);
+    if (finish_row<W>(a, level, rid, row, acc, sn, nw)) set_flag(a.found);
     if (a.alive)
 #pragma unroll
       for (int w = 0; w < W; ++w)
@@ -212,6 +214,8 @@ template <class TA, bool UNIT, int W>
 __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict__ rowptr,
                                                        const int32_t* __restrict__ colval,
                                                        const TA* __restrict__ val, MsArgs a) {
+  if (a.ctl && a.ctl[1]) return;
+  const int32_t level = a.ctl ? a.ctl[0] : a.level;
   const int64_t row = (int64_t)blockIdx.x * kM + threadIdx.x;
   bool newly = false;
   uint64_t nw[W];
@@ -229,7 +233,7 @@ __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict
         for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
         if (saturated<W>(acc, sn, a)) break;
       }
-      newly = finish_row<W>(a, row, row, acc, sn, nw);
+      newly = finish_row<W>(a, level, row, row, acc, sn, nw);
     } else {
 #pragma unroll
       for (int w = 0; w < W; ++w) a.next[row * W + w] = 0;
@@ -303,6 +307,25 @@ __global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
   if (i < n) p[i] = v;
 }
 __global__ void ms_zero_kernel(unsigned int* p) { *p = 0u; }
+
+// Captured levels: ctl = {level, done}.  Begin: level 1, not done.  End of a
+// level: done (1) when it found nothing, else the next level; done = 2 when
+// the next level would reach `cap` (byte levels full; 0 = no cap).
+__global__ void ms_ctl_begin_kernel(unsigned int* found, int32_t* ctl) {
+  *found = 0u;
+  ctl[0] = 1;
+  ctl[1] = 0;
+}
+__global__ void ms_level_end_kernel(unsigned int* found, int32_t* ctl, int32_t cap) {
+  if (ctl[1]) return;
+  if (*found == 0u) {
+    ctl[1] = 1;
+    return;
+  }
+  *found = 0u;
+  ctl[0] += 1;
+  if (cap && ctl[0] >= cap) ctl[1] = 2;
+}
 
 int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc) {
   const bool unit = (A->flags & GG_FLAG_UNIT_PATTERN) != 0;
@@ -410,29 +433,67 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
   uint32_t* fany = (uint32_t*)fa;
   uint32_t* nany = (uint32_t*)na;
   int32_t level = 0;
-  while (true) {
-    ++level;
-    if (dist8 && level == 255) {  // byte levels are full
+  if (ctx->graph_steps > 0) {
+    // U levels per graph launch, front / next alternating (U even)
+    const int U = (ctx->graph_steps + 1) & ~1;
+    int32_t* ctl = (int32_t*)cnt + 4;
+    hipLaunchKernelGGL(ms_ctl_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, (unsigned*)cnt, ctl);
+    a.ctl = ctl;
+    MsArgs ab[2] = {a, a};
+    ab[0].front = fr; ab[0].front_any = fany; ab[0].next = nxp; ab[0].next_any = nany;
+    ab[1].front = nxp; ab[1].front_any = nany; ab[1].next = fr; ab[1].next_any = fany;
+    std::vector<uint64_t> key{2, (uint64_t)U, (uint64_t)Wc, (uint64_t)words};
+    sig_add(key, *A);
+    sig_add(key, ab[0]);
+    sig_add(key, ab[1]);
+    auto steps = [&](hipStream_t st) -> int {
+      for (int i = 0; i < U; ++i) {
+        const MsArgs& x = ab[i & 1];
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, st,
+                           x.next_any, words, 0u);
+        GG_TRY(msbfs_level(st, A, x, Wc));
+        hipLaunchKernelGGL(ms_level_end_kernel, dim3(1), dim3(1), 0, st, (unsigned*)cnt, ctl,
+                           dist8 ? 255 : 0);
+      }
+      GG_HIP(hipGetLastError());
+      return GG_OK;
+    };
+    GG_TRY(graph_loop(ctx, key, ctl + 1, m / U + 3, steps, nullptr));
+    GG_HIP(hipMemcpyAsync(ctx->host_pinned, ctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    GG_HIP(hipStreamSynchronize(ctx->stream));
+    const int32_t* hctl = (const int32_t*)ctx->host_pinned;
+    if (hctl[1] == 2) {
       *overflow = true;
       return GG_OK;
     }
-    a.front = fr;
-    a.front_any = fany;
-    a.next = nxp;
-    a.next_any = nany;
-    a.level = level;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, ctx->stream,
-                       nany, words, 0u);
-    hipLaunchKernelGGL(ms_zero_kernel, dim3(1), dim3(1), 0, ctx->stream, (unsigned*)cnt);
-    {
-      HotTimer ht(ctx);
-      GG_TRY(msbfs_level(ctx->stream, A, a, Wc));
+    level = hctl[0];
+  } else {
+    while (true) {
+      ++level;
+      if (dist8 && level == 255) {  // byte levels are full
+        *overflow = true;
+        return GG_OK;
+      }
+      a.front = fr;
+      a.front_any = fany;
+      a.next = nxp;
+      a.next_any = nany;
+      a.level = level;
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, ctx->stream,
+                         nany, words, 0u);
+      hipLaunchKernelGGL(ms_zero_kernel, dim3(1), dim3(1), 0, ctx->stream, (unsigned*)cnt);
+      {
+        HotTimer ht(ctx);
+        GG_TRY(msbfs_level(ctx->stream, A, a, Wc));
+      }
+      GG_HIP(hipMemcpyAsync(ctx->host_pinned, cnt, sizeof(unsigned), hipMemcpyDeviceToHost,
+                            ctx->stream));
+      GG_HIP(hipStreamSynchronize(ctx->stream));
+      if (*(unsigned*)ctx->host_pinned == 0) break;
+      std::swap(fr, nxp);
+      std::swap(fany, nany);
     }
-    GG_HIP(hipMemcpyAsync(ctx->host_pinned, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
-    GG_HIP(hipStreamSynchronize(ctx->stream));
-    if (*(unsigned*)ctx->host_pinned == 0) break;
-    std::swap(fr, nxp);
-    std::swap(fany, nany);
   }
   if (layout == GG_COL_MAJOR && m > 0) {
     const int64_t tiles = ((m + 63) / 64) * ((k + 63) / 64);

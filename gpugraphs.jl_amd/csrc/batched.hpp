@@ -19,6 +19,9 @@ struct MsArgs {
   int32_t* dist;               // original rows
   uint8_t* dist8;              // when set: byte levels (< 255), row-major m x k, instead of dist
   int32_t level;
+  // captured loops: ctl[0] is the level (used instead of `level`), and a
+  // nonzero ctl[1] (done) makes the kernel return at once; NULL otherwise
+  const int32_t* ctl;
   unsigned int* found;         // set to 1 when some row got a new bit this level
   uint64_t live[8];            // bits of the sources being traversed, per word
   const int32_t* start;        // per source: level at which it was seeded (NULL: 0)
@@ -47,6 +50,6 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc)
 // column j that fell (k <= 64).
 int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k, void* d,
                   const uint32_t* chp, uint32_t* chn, unsigned int* changed,
-                  unsigned long long* changed_mask);
+                  unsigned long long* changed_mask, const int32_t* done = nullptr);
 
 }  // namespace gg

@@ -42,6 +42,9 @@ struct BfsCounters {
   unsigned long long scout;  // push: out-degree sum of the claimed vertices
   unsigned int awake;        // set bits of a bitmap (popcount_kernel)
   unsigned int nbig;         // push: hub pieces queued for push_big_kernel
+  int32_t level;             // level the next pull step assigns
+  int32_t done;              // set by level_end_kernel after a level that found nothing
+  int32_t* out;              // dist or parents
 };
 
 // out at the original source s; the bitmaps at its state index.
@@ -78,8 +81,10 @@ __global__ __launch_bounds__(kB) void popcount_kernel(const uint32_t* __restrict
 template <class TA, bool UNIT, bool PARENTS>
 __global__ __launch_bounds__(kB) void bfs_pull_sell_kernel(
     SellView g, int64_t n_sell_blocks, const uint32_t* __restrict__ frontier,
-    uint32_t* __restrict__ next, uint32_t* __restrict__ visited, int32_t* __restrict__ out,
-    int32_t level, BfsCounters* cnt) {
+    uint32_t* __restrict__ next, uint32_t* __restrict__ visited, BfsCounters* cnt) {
+  if (cnt->done) return;  // a captured step after the last level
+  const int32_t level = cnt->level;
+  int32_t* __restrict__ out = cnt->out;
   if ((int64_t)blockIdx.x < n_sell_blocks) {
     bool newly = false;
     int64_t row, base, len;
@@ -199,8 +204,10 @@ template <class TA, bool UNIT, bool PARENTS>
 __global__ __launch_bounds__(kB) void bfs_pull_csr_kernel(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
     const TA* __restrict__ val, int64_t m, int logL, const uint32_t* __restrict__ frontier,
-    uint32_t* __restrict__ next, uint32_t* __restrict__ visited, int32_t* __restrict__ out,
-    int32_t level, BfsCounters* cnt) {
+    uint32_t* __restrict__ next, uint32_t* __restrict__ visited, BfsCounters* cnt) {
+  if (cnt->done) return;
+  const int32_t level = cnt->level;
+  int32_t* __restrict__ out = cnt->out;
   const int L = 1 << logL;
   const int wl = threadIdx.x & 63;
   const int lane = wl & (L - 1);
@@ -385,12 +392,26 @@ __global__ __launch_bounds__(kB) void bitmap_to_queue_kernel(const uint32_t* __r
   }
 }
 
-__global__ void zero_counters_kernel(BfsCounters* c) {
+__global__ void level_begin_kernel(BfsCounters* c, int32_t level, int32_t* out) {
   c->found = 0;
   c->qlen = 0;
   c->scout = 0;
   c->awake = 0;
   c->nbig = 0;
+  c->level = level;
+  c->done = 0;
+  c->out = out;
+}
+
+// After a pull step: done when it found nothing, else advance the level.
+__global__ void level_end_kernel(BfsCounters* c) {
+  if (c->done) return;
+  if (c->found == 0) {
+    c->done = 1;
+  } else {
+    c->level += 1;
+    c->found = 0;
+  }
 }
 
 __global__ void degree_kernel(const int32_t* rowptr, int64_t v, BfsCounters* c) {
@@ -423,8 +444,11 @@ static int bfs_work(gg_ctx* ctx, int64_t m, bool queues, BfsWork& w) {
   return GG_OK;
 }
 
+static void fill_words(hipStream_t st, uint32_t* p, int64_t n, uint32_t v) {
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(n, kB)), dim3(kB), 0, st, p, n, v);
+}
 static void fill_words(gg_ctx* ctx, uint32_t* p, int64_t n, uint32_t v) {
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(n, kB)), dim3(kB), 0, ctx->stream, p, n, v);
+  fill_words(ctx->stream, p, n, v);
 }
 
 static int read_counters(gg_ctx* ctx, const BfsWork& w, BfsCounters* host) {
@@ -443,11 +467,13 @@ static int csr_lanes(const gg_matrix* A) {
   return lg;
 }
 
-// One pull level (levels or parents) over A_T.
-static int pull_level(gg_ctx* ctx, const gg_matrix* A, bool parents, BfsWork& w, int32_t* out,
-                      int32_t level) {
+// One pull level (levels or parents) over A_T, front -> next, at the level
+// and into the output that cnt holds.  `st` is the context stream, or the
+// capture stream of a graph (then the launch is not timed).
+static int pull_step(gg_ctx* ctx, hipStream_t st, const gg_matrix* A, bool parents,
+                     const uint32_t* front, uint32_t* next, uint32_t* visited, BfsCounters* cnt) {
   const bool unit = (A->flags & GG_FLAG_UNIT_PATTERN) != 0;
-  HotTimer ht(ctx);
+  HotTimer ht(ctx, st == ctx->stream);
   return with_type(A->val_dtype, [&](auto ta) -> int {
     typedef typename decltype(ta)::type TA;
     auto go = [&](auto unit_tag, auto par_tag) -> int {
@@ -458,16 +484,15 @@ static int pull_level(gg_ctx* ctx, const gg_matrix* A, bool parents, BfsWork& w,
         unsigned nb = blocks_for(A->m << lg, kB);
         const unsigned cap = (unsigned)ctx->num_cus * 32;
         if (nb > cap) nb = cap;
-        hipLaunchKernelGGL((bfs_pull_csr_kernel<TA, U, P>), dim3(nb), dim3(kB), 0, ctx->stream,
-                           A->ptr, A->idx, (const TA*)A->val, A->m, lg, w.front, w.next,
-                           w.visited, out, level, w.cnt);
+        hipLaunchKernelGGL((bfs_pull_csr_kernel<TA, U, P>), dim3(nb), dim3(kB), 0, st, A->ptr,
+                           A->idx, (const TA*)A->val, A->m, lg, front, next, visited, cnt);
       } else {
         SellView g = sell_view(A);
         const int64_t nsb = (g.n_slots + kB - 1) / kB;
         const int64_t nb = nsb + (P ? g.n_heavy : g.n_seg);
         if (nb > 0)
-          hipLaunchKernelGGL((bfs_pull_sell_kernel<TA, U, P>), dim3((unsigned)nb), dim3(kB), 0,
-                             ctx->stream, g, nsb, w.front, w.next, w.visited, out, level, w.cnt);
+          hipLaunchKernelGGL((bfs_pull_sell_kernel<TA, U, P>), dim3((unsigned)nb), dim3(kB), 0, st,
+                             g, nsb, front, next, visited, cnt);
       }
       GG_HIP(hipGetLastError());
       return GG_OK;
@@ -509,17 +534,42 @@ static int check_square_source(const gg_matrix* A, int64_t source) {
   return GG_OK;
 }
 
+// Pull levels until one finds nothing.  With graph steps, U levels are
+// captured (front and next alternate, U even) and relaunched until the
+// device flag says done; otherwise one level per host round trip.
 int bfs_levels_pull(gg_ctx* ctx, const gg_matrix* A, int64_t source, int32_t* dist,
                     int64_t* nlevels) {
   BfsWork w;
   GG_TRY(bfs_begin(ctx, A, source, dist, 0, w, false));
-  int32_t level = 0;
   BfsCounters hc{};
+  if (ctx->graph_steps > 0) {
+    const int U = (ctx->graph_steps + 1) & ~1;
+    hipLaunchKernelGGL(level_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt, 1, dist);
+    std::vector<uint64_t> key{1, (uint64_t)U};
+    sig_add(key, *A);
+    sig_add(key, w);
+    auto steps = [&](hipStream_t st) -> int {
+      for (int i = 0; i < U; ++i) {
+        uint32_t* f = (i & 1) ? w.next : w.front;
+        uint32_t* nx = (i & 1) ? w.front : w.next;
+        fill_words(st, nx, w.words, 0u);
+        GG_TRY(pull_step(ctx, st, A, false, f, nx, w.visited, w.cnt));
+        hipLaunchKernelGGL(level_end_kernel, dim3(1), dim3(1), 0, st, w.cnt);
+      }
+      GG_HIP(hipGetLastError());
+      return GG_OK;
+    };
+    GG_TRY(graph_loop(ctx, key, &w.cnt->done, A->m / U + 2, steps, nullptr));
+    GG_TRY(read_counters(ctx, w, &hc));
+    if (nlevels) *nlevels = hc.level - 1;
+    return GG_OK;
+  }
+  int32_t level = 0;
   while (true) {
     ++level;
     fill_words(ctx, w.next, w.words, 0u);
-    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
-    GG_TRY(pull_level(ctx, A, false, w, dist, level));
+    hipLaunchKernelGGL(level_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt, level, dist);
+    GG_TRY(pull_step(ctx, ctx->stream, A, false, w.front, w.next, w.visited, w.cnt));
     GG_TRY(read_counters(ctx, w, &hc));
     if (hc.found == 0) break;
     std::swap(w.front, w.next);
@@ -560,8 +610,8 @@ int gg_bfs_parents(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, int32_t* p
   while (true) {
     ++level;
     fill_words(ctx, w.next, w.words, 0u);
-    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
-    GG_TRY(pull_level(ctx, A_T, true, w, parents, level));
+    hipLaunchKernelGGL(level_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt, level, parents);
+    GG_TRY(pull_step(ctx, ctx->stream, A_T, true, w.front, w.next, w.visited, w.cnt));
     GG_TRY(read_counters(ctx, w, &hc));
     if (hc.found == 0) break;
     std::swap(w.front, w.next);
@@ -607,8 +657,9 @@ int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_ou
         old = awake;
         ++level;
         fill_words(ctx, w.next, w.words, 0u);
-        hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
-        GG_TRY(pull_level(ctx, A_T, false, w, dist, level));
+        hipLaunchKernelGGL(level_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt, level,
+                           dist);
+        GG_TRY(pull_step(ctx, ctx->stream, A_T, false, w.front, w.next, w.visited, w.cnt));
         hipLaunchKernelGGL(popcount_kernel, dim3(std::min<int64_t>(blocks_for(w.words, kB), 1024)),
                            dim3(kB), 0, ctx->stream, w.next, w.words, &w.cnt->awake);
         GG_HIP(hipGetLastError());
@@ -617,7 +668,7 @@ int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_ou
         std::swap(w.front, w.next);
       } while (awake > 0 && (awake >= old || (double)awake > (double)n / beta));
       if (awake == 0) break;
-      hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
+      hipLaunchKernelGGL(level_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt, level, dist);
       hipLaunchKernelGGL(bitmap_to_queue_kernel, dim3(blocks_for(w.words, kB)), dim3(kB), 0,
                          ctx->stream, w.front, w.words, w.m, g, w.q0, w.cnt);
       GG_HIP(hipGetLastError());
@@ -628,7 +679,7 @@ int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_ou
       edges_to_check -= scout;
       ++level;
       fill_words(ctx, w.next, w.words, 0u);
-      hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt);
+      hipLaunchKernelGGL(level_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt, level, dist);
       PushArgs pa{A_out->ptr, A_out->idx, w.q0, qlen, w.q1, w.next, w.visited, dist, level,
                   vinv, w.cnt, big};
       unsigned nb = blocks_for(qlen, kB);

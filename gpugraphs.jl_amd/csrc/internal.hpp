@@ -4,7 +4,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
+
+#include <functional>
+#include <map>
 #include <string>
+#include <vector>
 
 #include "../../include/gpugraphs.h"
 #include "semiring.hpp"
@@ -61,6 +65,11 @@ struct gg_ctx {
   double hot_ms = 0.0;                       // summed hot-kernel launch durations
   int64_t hot_launches = 0;
   int num_cus = 256;
+  // captured loop steps (gg_ctx_set_graph_steps): steps per graph launch, the
+  // stream graphs are captured on, and the instantiated graphs by signature
+  int graph_steps = 0;
+  hipStream_t cap_stream = nullptr;
+  std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
 };
 
 namespace gg {
@@ -77,11 +86,12 @@ void timer_end(gg_ctx* ctx);
 // entry point) with events on the context stream when timing is enabled.
 struct HotTimer {
   gg_ctx* c;
-  explicit HotTimer(gg_ctx* ctx) : c(ctx) {
-    if (c->timing) (void)hipEventRecord(c->ev2, c->stream);
+  bool on;
+  explicit HotTimer(gg_ctx* ctx, bool active = true) : c(ctx), on(active && ctx->timing) {
+    if (on) (void)hipEventRecord(c->ev2, c->stream);
   }
   ~HotTimer() {
-    if (!c->timing) return;
+    if (!on) return;
     (void)hipEventRecord(c->ev3, c->stream);
     if (hipEventSynchronize(c->ev3) == hipSuccess) {
       float ms = 0.f;
@@ -119,6 +129,23 @@ inline int64_t sell_slots(const gg_matrix* A) {
   const int64_t all = A->nslices * A->slice_size;
   return A->perm ? all : (A->m < all ? A->m : all);
 }
+
+// Appends the bytes of a value to a graph signature.
+template <class T> inline void sig_add(std::vector<uint64_t>& key, const T& v) {
+  const size_t n = (sizeof(T) + 7) / 8;
+  const size_t at = key.size();
+  key.resize(at + n, 0);
+  memcpy(key.data() + at, &v, sizeof(T));
+}
+
+// Runs a loop as a captured graph: `enqueue(stream)` enqueues U steps of the
+// loop on `stream` (captured once per signature `key` and cached in the
+// context); the graph is launched on the context stream until the device
+// flag *done becomes nonzero, at most max_launches times.  The steps must be
+// kernels only, and each must do nothing once *done is set.
+int graph_loop(gg_ctx* ctx, const std::vector<uint64_t>& key, const int32_t* done,
+               int64_t max_launches, const std::function<int(hipStream_t)>& enqueue,
+               int64_t* launches);
 
 // Internal launchers (used by the algorithms; no argument checks).
 int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const void* B,

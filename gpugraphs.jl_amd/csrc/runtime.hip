@@ -52,6 +52,48 @@ void timer_end(gg_ctx* ctx) {
   }
 }
 
+int graph_loop(gg_ctx* ctx, const std::vector<uint64_t>& key, const int32_t* done,
+               int64_t max_launches, const std::function<int(hipStream_t)>& enqueue,
+               int64_t* launches) {
+  hipGraphExec_t exec = nullptr;
+  auto it = ctx->graphs.find(key);
+  if (it != ctx->graphs.end()) {
+    exec = it->second;
+  } else {
+    if (!ctx->cap_stream) GG_HIP(hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    GG_HIP(hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeRelaxed));
+    const int st = enqueue(ctx->cap_stream);
+    const hipError_t e = hipStreamEndCapture(ctx->cap_stream, &graph);
+    if (st != GG_OK) {
+      if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+      return st;
+    }
+    GG_HIP(e);
+    const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    GG_HIP(e2);
+    if (ctx->graphs.size() >= 64) {  // bound the cache
+      for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+      ctx->graphs.clear();
+    }
+    ctx->graphs[key] = exec;
+  }
+  for (int64_t l = 1;; ++l) {
+    GG_HIP(hipGraphLaunch(exec, ctx->stream));
+    GG_HIP(hipMemcpyAsync(ctx->host_pinned, done, sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    GG_HIP(hipStreamSynchronize(ctx->stream));
+    if (*(const int32_t*)ctx->host_pinned) {
+      if (launches) *launches = l;
+      return GG_OK;
+    }
+    if (l >= max_launches)
+      return set_error(GG_ERR_NOT_CONVERGED, "captured loop did not finish in %lld launches",
+                       (long long)l);
+  }
+}
+
 int check_matrix(const gg_matrix* A) {
   if (!A) return set_error(GG_ERR_ARGUMENT, "matrix descriptor is NULL");
   if (A->m < 0 || A->n < 0 || A->nnz < 0)
@@ -161,6 +203,8 @@ int gg_ctx_destroy(gg_ctx* ctx) {
   if (!ctx) return GG_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
   for (int i = 0; i < WS_NSLOTS; ++i)
     if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
@@ -220,6 +264,13 @@ int gg_memcpy_dtoh(gg_ctx* ctx, void* dst, const void* src, size_t bytes) {
 int gg_ctx_enable_timing(gg_ctx* ctx, int on) {
   if (!ctx) return set_error(GG_ERR_ARGUMENT, "ctx is NULL");
   ctx->timing = on != 0;
+  return GG_OK;
+}
+
+int gg_ctx_set_graph_steps(gg_ctx* ctx, int steps) {
+  if (!ctx) return set_error(GG_ERR_ARGUMENT, "ctx is NULL");
+  if (steps < 0 || steps > 256) return set_error(GG_ERR_ARGUMENT, "graph steps must be in 0:256");
+  ctx->graph_steps = steps;
   return GG_OK;
 }
 

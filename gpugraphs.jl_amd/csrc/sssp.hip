@@ -39,7 +39,8 @@ __global__ __launch_bounds__(kT) void bf_csr_kernel(const int32_t* __restrict__ 
                                                     int logL, TD* d,
                                                     const uint32_t* __restrict__ chp,
                                                     uint32_t* __restrict__ chn,
-                                                    unsigned int* changed) {
+                                                    unsigned int* changed, const int32_t* done) {
+  if (done && *done) return;  // a captured step after the fixpoint
   const int L = 1 << logL;
   const int wl = threadIdx.x & 63;
   const int lane = wl & (L - 1);
@@ -78,7 +79,8 @@ template <class TA, class TD>
 __global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_blocks, TD* d,
                                                      const uint32_t* __restrict__ chp,
                                                      uint32_t* __restrict__ chn,
-                                                     unsigned int* changed) {
+                                                     unsigned int* changed, const int32_t* done) {
+  if (done && *done) return;
   if ((int64_t)blockIdx.x < n_sell_blocks) {
     int64_t row, base, len;
     bool has_len;
@@ -156,7 +158,9 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
                                                       const uint32_t* __restrict__ chp,
                                                       uint32_t* __restrict__ chn,
                                                       unsigned int* changed,
-                                                      unsigned long long* changed_mask) {
+                                                      unsigned long long* changed_mask,
+                                                      const int32_t* done) {
+  if (done && *done) return;
   const int Lk = 1 << logLk;
   const int wl = threadIdx.x & 63;
   const int lane = wl & (Lk - 1);
@@ -291,6 +295,20 @@ __global__ void state_to_result_kernel(const TD* __restrict__ src, const int32_t
 
 __global__ void zero_u32_kernel(unsigned int* p) { *p = 0u; }
 
+// Captured iterations: ctl = {iterations, done}.  After an iteration: done
+// when nothing fell, else clear the flag for the next one.
+__global__ void iter_begin_kernel(unsigned int* changed, int32_t* ctl) {
+  *changed = 0u;
+  ctl[0] = 0;
+  ctl[1] = 0;
+}
+__global__ void iter_end_kernel(unsigned int* changed, int32_t* ctl) {
+  if (ctl[1]) return;
+  ctl[0] += 1;
+  if (*changed == 0u) ctl[1] = 1;
+  else *changed = 0u;
+}
+
 static int log_lanes_for(double avg) {
   int lg = 0;
   while (lg < 6 && (double)(1 << lg) < avg) ++lg;
@@ -338,19 +356,52 @@ static int changed_bitmaps(gg_ctx* ctx, int64_t ns, Changed& c) {
   return GG_OK;
 }
 
-// Iterate until an iteration lowers nothing; the check reads a device counter
+// Iterate until an iteration lowers nothing; the check reads a device flag
 // every iteration for the first 16 and then every 8 (an iteration after the
-// fixpoint reads no neighbour and changes nothing).
+// fixpoint reads no neighbour and changes nothing).  With graph steps, U
+// iterations are captured (the bitmaps alternate, U even) and relaunched
+// until the device-side flag says done.  step(stream, prev, next, done)
+// enqueues one iteration; `key` identifies everything its kernels read.
 template <class Step>
 static int iterate(gg_ctx* ctx, unsigned int* changed, Changed& c, int64_t max_iter, Step&& step,
-                   int64_t* iters) {
+                   std::vector<uint64_t> key, int64_t* iters) {
+  if (ctx->graph_steps > 0) {
+    const int U = (ctx->graph_steps + 1) & ~1;
+    int32_t* ctl = (int32_t*)changed + 4;
+    hipLaunchKernelGGL(iter_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, changed, ctl);
+    key.push_back((uint64_t)U);
+    sig_add(key, c);
+    sig_add(key, changed);
+    auto steps = [&](hipStream_t st) -> int {
+      for (int i = 0; i < U; ++i) {
+        uint32_t* p = (i & 1) ? c.next : c.prev;
+        uint32_t* nx = (i & 1) ? c.prev : c.next;
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(c.words, kT)), dim3(kT), 0, st, nx,
+                           c.words, 0u);
+        GG_TRY(step(st, p, nx, ctl + 1));
+        hipLaunchKernelGGL(iter_end_kernel, dim3(1), dim3(1), 0, st, changed, ctl);
+      }
+      GG_HIP(hipGetLastError());
+      return GG_OK;
+    };
+    const int st = graph_loop(ctx, key, ctl + 1, max_iter / U + 2, steps, nullptr);
+    if (st == GG_ERR_NOT_CONVERGED)
+      return set_error(st, "no fixpoint after %lld iterations (negative cycle?)",
+                       (long long)(max_iter / U + 2) * U);
+    GG_TRY(st);
+    GG_HIP(hipMemcpyAsync(ctx->host_pinned, ctl, sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    GG_HIP(hipStreamSynchronize(ctx->stream));
+    if (iters) *iters = *(const int32_t*)ctx->host_pinned;
+    return GG_OK;
+  }
   int64_t it = 0;
   while (true) {
     ++it;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, ctx->stream, changed);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(c.words, kT)), dim3(kT), 0, ctx->stream,
                        c.next, c.words, 0u);
-    GG_TRY(step());
+    GG_TRY(step(ctx->stream, c.prev, c.next, nullptr));
     std::swap(c.prev, c.next);
     if (it <= 16 || it % 8 == 0) {
       unsigned h = 0;
@@ -374,15 +425,17 @@ int sssp_one(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist, int di
     typedef typename decltype(dt)::type TD;
     const int64_t m = A->m;
     // a relabelled SELL iterates in internal order in a workspace and
-    // scatters the fixpoint to dist; otherwise dist itself is the working set
+    // scatters the fixpoint to dist; captured iterations use the workspace
+    // too (a graph keeps its pointers); otherwise dist is the working set
     const int32_t* vinv = (A->flags & GG_FLAG_INTERNAL_IDS) ? A->vinv : nullptr;
+    const bool in_ws = vinv || ctx->graph_steps > 0;
     const int64_t ns = state_size(A);
     void *tmp = nullptr, *cnt;
-    if (vinv) GG_TRY(ws_get(ctx, WS_SSSP0, sizeof(TD) * ns, &tmp));
+    if (in_ws) GG_TRY(ws_get(ctx, WS_SSSP0, sizeof(TD) * ns, &tmp));
     GG_TRY(ws_get(ctx, WS_FLAG, 64, &cnt));
     Changed c;
     GG_TRY(changed_bitmaps(ctx, ns, c));
-    TD* d = vinv ? (TD*)tmp : (TD*)dist;
+    TD* d = in_ws ? (TD*)tmp : (TD*)dist;
     hipLaunchKernelGGL(fill_td_kernel<TD>, dim3(blocks_for(ns, kT)), dim3(kT), 0, ctx->stream, d,
                        ns, Lim<TD>::max());
     hipLaunchKernelGGL(seed_one_kernel<TD>, dim3(1), dim3(1), 0, ctx->stream, d, source - 1, vinv,
@@ -390,23 +443,27 @@ int sssp_one(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist, int di
     const int lg = log_lanes_for(m ? (double)A->nnz / m : 1.0);
     SellView g = sell_view(A);
     const int64_t nsb = (g.n_slots + kT - 1) / kT;
-    auto step = [&]() -> int {
-      HotTimer ht(ctx);
+    auto step = [&](hipStream_t st, const uint32_t* chp, uint32_t* chn,
+                    const int32_t* done) -> int {
+      HotTimer ht(ctx, st == ctx->stream);
       if (A->format == GG_CSR) {
         unsigned nb = blocks_for(m << lg, kT);
         const unsigned cap = (unsigned)ctx->num_cus * 32;
         if (nb > cap) nb = cap;
-        hipLaunchKernelGGL((bf_csr_kernel<TA, TD>), dim3(nb), dim3(kT), 0, ctx->stream, A->ptr,
-                           A->idx, (const TA*)A->val, m, lg, d, c.prev, c.next, (unsigned*)cnt);
+        hipLaunchKernelGGL((bf_csr_kernel<TA, TD>), dim3(nb), dim3(kT), 0, st, A->ptr, A->idx,
+                           (const TA*)A->val, m, lg, d, chp, chn, (unsigned*)cnt, done);
       } else if (nsb + g.n_seg > 0) {
         hipLaunchKernelGGL((bf_sell_kernel<TA, TD>), dim3((unsigned)(nsb + g.n_seg)), dim3(kT),
-                           0, ctx->stream, g, nsb, d, c.prev, c.next, (unsigned*)cnt);
+                           0, st, g, nsb, d, chp, chn, (unsigned*)cnt, done);
       }
       GG_HIP(hipGetLastError());
       return GG_OK;
     };
-    int st = iterate(ctx, (unsigned*)cnt, c, m + 16, step, iterations);
-    if (st == GG_OK && vinv) {
+    std::vector<uint64_t> key{3, (uint64_t)dist_dtype};
+    sig_add(key, *A);
+    sig_add(key, d);
+    int st = iterate(ctx, (unsigned*)cnt, c, m + 16, step, key, iterations);
+    if (st == GG_OK && in_ws) {
       hipLaunchKernelGGL(state_to_result_kernel<TD>, dim3(blocks_for(m, kT)), dim3(kT), 0,
                          ctx->stream, d, vinv, m, (int64_t)1, 1, (TD*)dist);
       GG_HIP(hipGetLastError());
@@ -418,7 +475,7 @@ int sssp_one(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist, int di
 
 int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k, void* d,
                   const uint32_t* chp, uint32_t* chn, unsigned int* changed,
-                  unsigned long long* changed_mask) {
+                  unsigned long long* changed_mask, const int32_t* done) {
   if (changed_mask && k > 64) return set_error(GG_ERR_ARGUMENT, "column masks need k <= 64");
   return with_ad(A->val_dtype, dist_dtype, [&](auto a, auto dt) -> int {
     typedef typename decltype(a)::type TA;
@@ -443,7 +500,7 @@ int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_
     if (nb > 0)
       hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
                          stream, A->format, A->ptr, g, n_light, parts, k, lg, (TD*)d, chp, chn,
-                         changed, changed_mask);
+                         changed, changed_mask, done);
     GG_HIP(hipGetLastError());
     return GG_OK;
   });
@@ -469,12 +526,15 @@ int sssp_multi(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_host, int
                        (TD*)b0, ns * k, Lim<TD>::max());
     hipLaunchKernelGGL(bf_multi_seed_kernel<TD>, dim3(blocks_for(k, kT)), dim3(kT), 0,
                        ctx->stream, (TD*)b0, (const int32_t*)src, k, vinv, c.prev);
-    auto step = [&]() -> int {
-      HotTimer ht(ctx);
-      return bf_multi_step(ctx->stream, A, dist_dtype, k, b0, c.prev, c.next, (unsigned*)cnt,
-                           nullptr);
+    auto step = [&](hipStream_t st, const uint32_t* chp, uint32_t* chn,
+                    const int32_t* done) -> int {
+      HotTimer ht(ctx, st == ctx->stream);
+      return bf_multi_step(st, A, dist_dtype, k, b0, chp, chn, (unsigned*)cnt, nullptr, done);
     };
-    int st = iterate(ctx, (unsigned*)cnt, c, m + 16, step, iterations);
+    std::vector<uint64_t> key{4, (uint64_t)dist_dtype, (uint64_t)k};
+    sig_add(key, *A);
+    sig_add(key, b0);
+    int st = iterate(ctx, (unsigned*)cnt, c, m + 16, step, key, iterations);
     if (st == GG_OK) {
       if (rowmajor_out && !vinv) {
         GG_HIP(hipMemcpyAsync(dist, b0, sizeof(TD) * m * k, hipMemcpyDeviceToDevice, ctx->stream));

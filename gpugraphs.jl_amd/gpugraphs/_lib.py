@@ -120,6 +120,7 @@ SIGNATURES = {
     "gg_memcpy_htod": ([_vp, _vp, _vp, C.c_size_t], _i32),
     "gg_memcpy_dtoh": ([_vp, _vp, _vp, C.c_size_t], _i32),
     "gg_ctx_enable_timing": ([_vp, _i32], _i32),
+    "gg_ctx_set_graph_steps": ([_vp, _i32], _i32),
     "gg_ctx_last_kernel_ms": ([_vp, C.POINTER(_d)], _i32),
     "gg_ctx_hot_kernel_stats": ([_vp, C.POINTER(_d), C.POINTER(_i64), _i32], _i32),
     "gg_matrix_check": ([_PM], _i32),
@@ -224,6 +225,12 @@ def last_kernel_ms(ctx) -> float:
 
 def enable_timing(ctx, on: bool = True):
     check(lib().gg_ctx_enable_timing(ctx, int(on)))
+
+
+def set_graph_steps(ctx, steps: int):
+    """Run level / iteration loops as HIP graphs of `steps` captured steps
+    (0: one host-launched step at a time)."""
+    check(lib().gg_ctx_set_graph_steps(ctx, int(steps)))
 
 
 def hot_kernel_stats(ctx, reset: bool = True):

@@ -192,6 +192,15 @@ int gg_memcpy_dtoh(gg_ctx* ctx, void* dst, const void* src, size_t bytes);
 int gg_ctx_enable_timing(gg_ctx* ctx, int on);
 int gg_ctx_last_kernel_ms(gg_ctx* ctx, double* ms);
 int gg_ctx_hot_kernel_stats(gg_ctx* ctx, double* total_ms, int64_t* launches, int reset);
+/* Level / iteration loops (gg_bfs_distances, gg_msbfs_distances,
+ * gg_shortest_path[_multi]) run as HIP graphs of `steps` captured steps
+ * (rounded up to even), relaunched until a device-side convergence flag is
+ * set: one host round trip per `steps` levels instead of one per level.  The
+ * graphs are cached in the context per matrix and workspace.  0 (the
+ * default) launches every step from the host.  The reference's loops sync
+ * on the host every level (bfs.jl:60) or every 8 iterations
+ * (shortest_path.jl:71-76). */
+int gg_ctx_set_graph_steps(gg_ctx* ctx, int steps);
 
 /* Host-side consistency check of a descriptor (no device access): the checks
  * every entry point runs before launching. */
