@@ -53,6 +53,20 @@ def sssp_certificate(rowptr, colval, w, d, src):
     return bool(d64[src - 1] == 0 and np.all(d64 <= best) and np.all(best[reached] == d64[reached]))
 
 
+def kernel_pass(_lib, ctx, fn, graph_steps):
+    """(summed ms, launches) of the hot kernel over one host-stepped run of fn:
+    HIP events on the context stream around each launch (single launches
+    inside a captured graph cannot be timed; the kernels are the same)."""
+    _lib.set_graph_steps(ctx, 0)
+    _lib.enable_timing(ctx, True)
+    _lib.hot_kernel_stats(ctx, reset=True)
+    fn()
+    kms, kn = _lib.hot_kernel_stats(ctx, reset=True)
+    _lib.enable_timing(ctx, False)
+    _lib.set_graph_steps(ctx, graph_steps)
+    return kms, kn
+
+
 def _time_queries(torch, fn, items):
     lat, outs = [], []
     for it in items:
@@ -92,11 +106,8 @@ def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma,
             iters.append(shortest_path_(A, int(s), dist))
             return dist.clone()
         lat, plain = _time_queries(torch, one, srcs)
-        _lib.enable_timing(ctx, True)
-        _lib.hot_kernel_stats(ctx, reset=True)
-        shortest_path_(A, int(srcs[0]), dist)
-        kms, kn = _lib.hot_kernel_stats(ctx, reset=True)
-        _lib.enable_timing(ctx, False)
+        kms, kn = kernel_pass(_lib, ctx, lambda: shortest_path_(A, int(srcs[0]), dist),
+                              args.graph_steps)
         Dm = torch.empty(k * m, dtype=torch.float32, device=dev).view(k, m).T
         t0 = time.perf_counter()
         it_b = shortest_path_(A, srcs, Dm)
@@ -154,6 +165,8 @@ def main():
     ap.add_argument("--bfs-seg", type=int, default=65536,
                     help="heavy segment length of the BFS (Bool pattern) matrix")
     ap.add_argument("--plain-roots", type=int, default=64)
+    ap.add_argument("--graph-steps", type=int, default=8,
+                    help="levels / iterations per captured graph launch (0: host-stepped loops)")
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--grid", type=int, default=4096, help="Bellman-Ford grid side")
@@ -205,6 +218,7 @@ def main():
     rng = np.random.default_rng(seed)
     roots = rng.choice(np.nonzero(deg)[0], args.roots, replace=False).astype(np.int32) + 1
     ctx = _lib.context(0)
+    _lib.set_graph_steps(ctx, args.graph_steps)
     res = {}
 
     # ---------------------------------------------------------------- SpMV
@@ -221,12 +235,10 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
-        _lib.enable_timing(ctx, True)
-        _lib.hot_kernel_stats(ctx, reset=True)
-        for _ in range(args.spmv_reps):
-            gg.gpu_spmv_(y, A_f, x)
-        kms, kn = _lib.hot_kernel_stats(ctx, reset=True)
-        _lib.enable_timing(ctx, False)
+        def spmv_reps():
+            for _ in range(args.spmv_reps):
+                gg.gpu_spmv_(y, A_f, x)
+        kms, kn = kernel_pass(_lib, ctx, spmv_reps, args.graph_steps)
         spmv_kernel_ms = kms / max(kn, 1)
         B_spmv = nnz * 8 + 4 * (n + 1) + 4 * n + 4 * n  # algorithmic bytes per SpMV
         S64 = Rg.to_scipy().astype(np.float64)
@@ -254,11 +266,7 @@ def main():
             torch.cuda.synchronize()
             lat.append(time.perf_counter() - t0)
         step_s = float(np.mean(lat))
-        _lib.enable_timing(ctx, True)
-        _lib.hot_kernel_stats(ctx, reset=True)
-        D = gg.multi_source_bfs(A_b, roots)
-        kms, kn = _lib.hot_kernel_stats(ctx, reset=True)
-        _lib.enable_timing(ctx, False)
+        kms, kn = kernel_pass(_lib, ctx, lambda: gg.multi_source_bfs(A_b, roots), args.graph_steps)
         k = len(roots)
         B_ms = 4 * nnz + 2 * n * (k // 8)  # per level: pattern once + source masks in and out
         res["msbfs"] = dict(step_ms=step_s * 1e3, qps=k / step_s, p50_ms=pct(lat, 50) * 1e3,
@@ -289,11 +297,8 @@ def main():
             if D is not None and not torch.equal(d, D[:, i]):
                 mismatches += 1
         assert mismatches == 0, f"{mismatches} BFS results differ between plain, DO and batched"
-        _lib.enable_timing(ctx, True)
-        _lib.hot_kernel_stats(ctx, reset=True)
-        gg.bfs_distances(A_b, int(pr[0]))
-        kms, kn = _lib.hot_kernel_stats(ctx, reset=True)
-        _lib.enable_timing(ctx, False)
+        kms, kn = kernel_pass(_lib, ctx, lambda: gg.bfs_distances(A_b, int(pr[0])),
+                              args.graph_steps)
         d0 = d if len(pr) == 1 else gg.bfs_distances(A_b, int(pr[0]))
         d0h = d0.cpu().numpy()
         reach = d0h != np.iinfo(np.int32).max
@@ -374,6 +379,7 @@ def main():
                                + ("degree-relabelled" if relabel else "sigma4096"),
                    "n": n, "nnz": nnz, "roots": args.roots, "slice_size": 64, "sigma": sigma,
                    "relabel": relabel, "heavy_threshold": args.heavy,
+                   "graph_steps": args.graph_steps,
                    "parallelism": "single GPU"},
         "p50_ms": None if not ms else round(ms["p50_ms"], 3),
         "p99_ms": None if not ms else round(ms["p99_ms"], 3),

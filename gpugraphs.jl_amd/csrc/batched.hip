@@ -47,16 +47,33 @@ __device__ __forceinline__ bool finish_row(const MsArgs& a, int32_t level, int64
   }
   if (any) {
     if (claim_any) bit_claim(a.next_any, rid);
+    if (a.dist8) {
+      // byte levels: one 8-byte read-modify-write per group of 8 sources with
+      // a new bit (this thread is the row's only writer in the level)
+      const uint64_t lv = (uint64_t)(uint8_t)level;
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      uint64_t b = nw[w];
-      while (b) {
-        const int s = w * 64 + __ffsll((unsigned long long)b) - 1;
-        b &= b - 1;
-        const int32_t d = a.start ? level - a.start[s] : level;
-        if (a.dist8) a.dist8[row * a.k + s] = (uint8_t)d;
-        else if (a.layout == GG_ROW_MAJOR) a.dist[row * a.k + s] = d;
-        else a.dist[row + (int64_t)s * a.m] = d;
+      for (int w = 0; w < W; ++w) {
+        for (int gq = 0; gq < 8; ++gq) {
+          const uint32_t bits = (uint32_t)(nw[w] >> (8 * gq)) & 0xffu;
+          if (!bits) continue;
+          uint64_t* p = (uint64_t*)(a.dist8 + row * a.k8 + w * 64 + gq * 8);
+          uint64_t v = *p;
+          for (int t = 0; t < 8; ++t)
+            if ((bits >> t) & 1u) v = (v & ~(0xffull << (8 * t))) | (lv << (8 * t));
+          *p = v;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        uint64_t b = nw[w];
+        while (b) {
+          const int s = w * 64 + __ffsll((unsigned long long)b) - 1;
+          b &= b - 1;
+          const int32_t d = a.start ? level - a.start[s] : level;
+          if (a.layout == GG_ROW_MAJOR) a.dist[row * a.k + s] = d;
+          else a.dist[row + (int64_t)s * a.m] = d;
+        }
       }
     }
   }
@@ -166,7 +183,18 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
       for (int w = 0; w < W; ++w) a.next[rid * W + w] = 0;
     return;
   }
-  if (threadIdx.x == 0) s_full = 0;
+  if (threadIdx.x == 0) {
+    // segments that start after the row's earlier segments saturated it
+    // (their OR is in hacc) skip the scan and only report arrival
+    bool full = false;
+    if (nseg > 1) {
+      uint64_t cur[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) cur[w] = __atomic_load_n(&a.hacc[h * W + w], __ATOMIC_RELAXED);
+      full = saturated<W>(cur, sn, a);
+    }
+    s_full = full;
+  }
   __syncthreads();
   {
     const TA* hv = (const TA*)g.heavy_val;
@@ -245,9 +273,9 @@ __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict
 
 // One thread per source (several sources may share a vertex: atomic ORs);
 // distances row-major m x k, bytes when dist8 is given.
-__global__ void msbfs_seed_kernel(const int32_t* sources, int64_t k, int W, const int32_t* vinv,
-                                  uint64_t* front, uint32_t* front_any, uint64_t* seen,
-                                  int32_t* dist, uint8_t* dist8) {
+__global__ void msbfs_seed_kernel(const int32_t* sources, int64_t k, int64_t k8, int W,
+                                  const int32_t* vinv, uint64_t* front, uint32_t* front_any,
+                                  uint64_t* seen, int32_t* dist, uint8_t* dist8) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= k) return;
   const int64_t v = sources[s] - 1;
@@ -255,14 +283,14 @@ __global__ void msbfs_seed_kernel(const int32_t* sources, int64_t k, int W, cons
   atomicOr((unsigned long long*)&front[r * W + s / 64], 1ull << (s & 63));
   atomicOr((unsigned long long*)&seen[r * W + s / 64], 1ull << (s & 63));
   atomicOr(&front_any[r >> 5], 1u << (r & 31));
-  if (dist8) dist8[v * k + s] = 0;
+  if (dist8) dist8[v * k8 + s] = 0;
   else dist[v * k + s] = 0;
 }
 
 // Byte levels row-major m x k -> Int32 column-major m x k (255 = unreached ->
 // typemax(Int32)) through a 64 x 64 LDS tile.
 __global__ __launch_bounds__(256) void widen_transpose_kernel(const uint8_t* __restrict__ src,
-                                                              int64_t m, int64_t k,
+                                                              int64_t m, int64_t k, int64_t k8,
                                                               int32_t* __restrict__ dst) {
   __shared__ uint8_t tile[64][65];
   const int64_t tiles_k = (k + 63) / 64;
@@ -270,7 +298,7 @@ __global__ __launch_bounds__(256) void widen_transpose_kernel(const uint8_t* __r
   const int64_t c0 = ((int64_t)blockIdx.x % tiles_k) * 64;
   for (int i = threadIdx.x; i < 64 * 64; i += 256) {
     const int r = i >> 6, c = i & 63;
-    if (r0 + r < m && c0 + c < k) tile[r][c] = src[(r0 + r) * k + c0 + c];
+    if (r0 + r < m && c0 + c < k) tile[r][c] = src[(r0 + r) * k8 + c0 + c];
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 64 * 64; i += 256) {
@@ -375,12 +403,13 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
                       int32_t* dist_out, int layout, bool narrow, bool* overflow,
                       int64_t* levels_out) {
   const int64_t m = A->m;
+  const int64_t k8 = (k + 7) & ~(int64_t)7;
   int32_t* dist = dist_out;
   uint8_t* dist8 = nullptr;
   *overflow = false;
   if (layout == GG_COL_MAJOR) {
     void* p;
-    GG_TRY(ws_get(ctx, WS_MSDIST, (narrow ? 1 : sizeof(int32_t)) * m * k + 4, &p));
+    GG_TRY(ws_get(ctx, WS_MSDIST, narrow ? m * k8 + 8 : sizeof(int32_t) * m * k, &p));
     if (narrow) dist8 = (uint8_t*)p;
     else dist = (int32_t*)p;
   }
@@ -403,7 +432,7 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
   hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, ctx->stream,
                      (uint32_t*)fa, words, 0u);
   if (dist8) {
-    const int64_t w4 = (m * k + 3) / 4;
+    const int64_t w4 = (m * k8 + 3) / 4;
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(w4, kM)), dim3(kM), 0, ctx->stream,
                        (uint32_t*)dist8, w4, 0xffffffffu);
   } else {
@@ -412,8 +441,8 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
   }
   const int32_t* vinv = (A->flags & GG_FLAG_INTERNAL_IDS) ? A->vinv : nullptr;
   hipLaunchKernelGGL(msbfs_seed_kernel, dim3(blocks_for(k, kM)), dim3(kM), 0, ctx->stream,
-                     sources_dev, k, Wc, vinv, (uint64_t*)f, (uint32_t*)fa, (uint64_t*)sn, dist,
-                     dist8);
+                     sources_dev, k, k8, Wc, vinv, (uint64_t*)f, (uint32_t*)fa, (uint64_t*)sn,
+                     dist, dist8);
   GG_HIP(hipGetLastError());
   MsArgs a{};
   a.m = m;
@@ -422,6 +451,7 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
   a.seen = (uint64_t*)sn;
   a.dist = dist;
   a.dist8 = dist8;
+  a.k8 = k8;
   a.found = (unsigned*)cnt;
   for (int w = 0; w < 8; ++w) a.live[w] = w < Wc ? word_mask(k, w) : 0ull;
   a.start = nullptr;
@@ -499,7 +529,7 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
     const int64_t tiles = ((m + 63) / 64) * ((k + 63) / 64);
     if (dist8)
       hipLaunchKernelGGL(widen_transpose_kernel, dim3((unsigned)tiles), dim3(256), 0,
-                         ctx->stream, dist8, m, k, dist_out);
+                         ctx->stream, dist8, m, k, k8, dist_out);
     else
       hipLaunchKernelGGL(transpose_rm_cm_kernel, dim3((unsigned)tiles), dim3(256), 0,
                          ctx->stream, dist, m, k, dist_out);

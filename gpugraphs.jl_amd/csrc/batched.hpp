@@ -17,7 +17,8 @@ struct MsArgs {
   uint32_t* next_any;
   uint64_t* seen;              // [ns*W]
   int32_t* dist;               // original rows
-  uint8_t* dist8;              // when set: byte levels (< 255), row-major m x k, instead of dist
+  uint8_t* dist8;              // when set: byte levels (< 255), row-major m x k8, instead of dist
+  int64_t k8;                  // row stride of dist8: k rounded up to a multiple of 8
   int32_t level;
   // captured loops: ctl[0] is the level (used instead of `level`), and a
   // nonzero ctl[1] (done) makes the kernel return at once; NULL otherwise
