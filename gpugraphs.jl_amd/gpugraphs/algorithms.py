@@ -48,6 +48,26 @@ def bfs_distances(A_T, source, use_mask=True):
     return dist
 
 
+def bfs_packed(A_T, offsets, sources):
+    """One BFS per graph of a block-diagonal pack (generators.pack) in one
+    persistent launch: (dist over all packed rows, levels per graph);
+    dist[offsets[g]:offsets[g+1]][:n_g] equals bfs_distances(graph g,
+    sources[g])."""
+    if not isinstance(A_T, SparseGPUMatrixCSR):
+        raise ArgumentError("a packed graph is a SparseGPUMatrixCSR")
+    dev = _dev(A_T)
+    off = np.ascontiguousarray(offsets, np.int64)
+    src = np.ascontiguousarray(sources, np.int32)
+    if len(off) != len(src) + 1:
+        raise ArgumentError("offsets needs one entry more than sources")
+    dist = _empty(A_T.m, np.int32, dev)
+    levels = np.zeros(len(src), np.int32)
+    d = A_T.descriptor()
+    check(lib().gg_bfs_packed(context(dev), C.byref(d), off.ctypes.data, len(src),
+                              src.ctypes.data, ptr(dist), levels.ctypes.data), "bfs_packed")
+    return dist, levels
+
+
 def bfs_distances_do(A_T, source, A_out=None, alpha=-1.0, beta=-1.0):
     """Direction-optimising BFS levels (push over A_out's rows, pull over A_T).
     A_out = CSR of A; for a symmetric graph it is A_T itself (the default)."""

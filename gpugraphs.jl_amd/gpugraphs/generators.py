@@ -99,3 +99,27 @@ def random_graph(n, degree=16, seed=1, dtype=np.bool_, values=UNIT, wmin=1, wmax
     check(lib().gg_gen_random_graph(n, degree, seed, dtype_code(dtype), values, wmin, wmax,
                                     C.byref(out)), "random_graph")
     return _take(out)
+
+
+def pack(graphs):
+    """Block-diagonal packing of square HostCSRs (A_T each, one value type):
+    (packed HostCSR, offsets) where graph g owns rows / columns
+    offsets[g] .. offsets[g+1]-1 (0-based), each block padded with isolated
+    vertices to a multiple of 256 rows (gg_pack_graphs)."""
+    hs = []
+    for g in graphs:
+        nz = np.ascontiguousarray(g.nzval)
+        if nz.dtype == np.bool_:
+            nz = nz.view(np.uint8)
+        h = GGCsrHost()
+        h.m, h.n, h.nnz = g.m, g.n, g.nnz
+        h.val_dtype = dtype_code(g.nzval.dtype)
+        rp = np.ascontiguousarray(g.rowptr, np.int32)
+        cv = np.ascontiguousarray(g.colval, np.int32)
+        h.rowptr, h.colval, h.nzval = rp.ctypes.data, cv.ctypes.data, nz.ctypes.data
+        hs.append((h, rp, cv, nz))   # the arrays stay alive through the call
+    arr = (C.POINTER(GGCsrHost) * len(hs))(*[C.pointer(h[0]) for h in hs])
+    offsets = np.zeros(len(hs) + 1, np.int64)
+    out = C.POINTER(GGCsrHost)()
+    check(lib().gg_pack_graphs(len(hs), arr, C.byref(out), offsets.ctypes.data), "pack")
+    return _take(out), offsets

@@ -260,6 +260,13 @@ int gg_gen_uniform_rows(int64_t n, int64_t per_row, uint64_t seed, int val_dtype
  * values: 0 unit, 1 U[0,1), 2 integer weights in [wmin, wmax]. */
 int gg_gen_random_graph(int64_t n, int64_t degree, uint64_t seed, int val_dtype, int values,
                         int32_t wmin, int32_t wmax, gg_csr_host** out);
+/* Block-diagonal packing of n_graphs square CSRs (A_T each, one value type):
+ * graph g owns rows and columns offsets[g] .. offsets[g+1]-1 (0-based;
+ * offsets has n_graphs+1 entries), each block padded with isolated vertices to
+ * a multiple of 256 rows.  No counterpart in the reference, which runs one
+ * graph per call (bfs.jl:1-111); this is the input of gg_bfs_packed. */
+int gg_pack_graphs(int n_graphs, const gg_csr_host* const* graphs, gg_csr_host** out,
+                   int64_t* offsets);
 
 /* ---------------------------------------------------------------- kernels */
 /* C = accum(C, A (add.mul) B) on rows selected by mask (mask[r] != 0) and the
@@ -326,6 +333,14 @@ int gg_msbfs_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources
  * m x k column-major like gg_shortest_path_multi. */
 int gg_mssp_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources, int64_t k,
                       void* dist, int dist_dtype, int64_t* iterations);
+/* One BFS per packed graph (gg_pack_graphs layout; A_T the block-diagonal
+ * CSR, offsets and sources host arrays, sources 1-based within each graph) in
+ * ONE cooperative launch of a persistent kernel fed by an atomic chunk queue:
+ * dist[offsets[g] + v] equals gg_bfs_distances(graph g, sources[g])[v];
+ * levels (host, optional) receives each graph's level count.  The
+ * reference runs bfs_distances (src/algorithms/bfs.jl:1-111) once per graph. */
+int gg_bfs_packed(gg_ctx* ctx, const gg_matrix* A_T, const int64_t* offsets, int n_graphs,
+                  const int32_t* sources, int32_t* dist, int32_t* levels);
 
 /* ---------------------------------------------------------------- serving */
 /* Continuous-batching query server over HBM-resident graphs (no counterpart

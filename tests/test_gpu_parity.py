@@ -467,3 +467,57 @@ def test_multi_source_bfs_deep_levels(env, fmt):
     for i, s in enumerate(srcs):
         np.testing.assert_array_equal(D[:, i], orc.bfs_distances(Ao, int(s)))
     assert D.max() == n - 1
+
+
+def _host_csr(S, dtype):
+    from gpugraphs import generators as GEN
+    S = sp.csr_matrix(S)
+    S.sort_indices()
+    return GEN.HostCSR(S.shape[0], S.shape[1], (S.indptr + 1).astype(np.int32),
+                       (S.indices + 1).astype(np.int32), S.data.astype(dtype))
+
+
+@pytest.mark.parametrize("dtype", [np.bool_, np.int32])
+def test_bfs_packed_equals_per_graph(env, dtype):
+    """One persistent launch over 40 block-diagonally packed graphs of uneven
+    sizes (1 to ~2K vertices) equals bfs_distances per graph; Int32 values
+    that are even are not edges, as in the single-graph BFS."""
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    rng = np.random.default_rng(77 if dtype == np.bool_ else 78)
+    mats = [sp.csr_matrix(np.ones((1, 1)) - np.eye(1))]            # one isolated vertex
+    for i in range(39):
+        kind = i % 4
+        if kind == 0:
+            n = int(rng.integers(10, 600))
+            M = G.undirected_adj(n, G.barabasi_albert(n, 3, int(rng.integers(1 << 30))), np.int32)
+        elif kind == 1:
+            M = G.random_digraph(rng, int(rng.integers(5, 400)), 0.02, wmax=4, dtype=np.int32)
+        elif kind == 2:
+            M = GEN.rmat(int(rng.integers(4, 12)), seed=int(rng.integers(1 << 30))).to_scipy()
+        else:
+            n = int(rng.integers(2, 300))
+            M = G.undirected_adj(n, G.dorogovtsev_mendes(n, int(rng.integers(1 << 30))), np.int32)
+        if dtype == np.bool_:
+            M = (M != 0).astype(np.int32)
+        mats.append(sp.csr_matrix(M.T))
+    hosts = [_host_csr(M, dtype) for M in mats]
+    P, off = GEN.pack(hosts)
+    A = P.to_csr(gg.ROCBackend(0))
+    srcs = np.array([int(rng.integers(M.shape[0])) + 1 for M in mats], np.int32)
+    print_case("bfs packed", np.dtype(dtype).name, P.m, P.nnz)
+    D, levels = gg.bfs_packed(A, off, srcs)
+    D = host(D)
+    for g, M in enumerate(mats):
+        Ao = orc.from_scipy_csr(sp.csr_matrix(M.astype(np.int32)) if dtype == np.int32 else
+                                sp.csr_matrix(M.astype(np.bool_)))
+        ref = orc.bfs_distances(Ao, int(srcs[g]))
+        blk = D[off[g]:off[g] + M.shape[0]]
+        np.testing.assert_array_equal(blk, ref)
+        assert np.all(D[off[g] + M.shape[0]:off[g + 1]] == INT32_MAX)   # padding unreached
+        fin = ref[ref != INT32_MAX]
+        assert levels[g] == fin.max()
+    # again, after other work on the context
+    gg.bfs_distances(A, 1)
+    D2, _ = gg.bfs_packed(A, off, srcs)
+    np.testing.assert_array_equal(host(D2), D)

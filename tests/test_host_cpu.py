@@ -343,3 +343,23 @@ def test_kernels_refuse_host_arrays():
     A = gg.SparseGPUMatrixCSR(np.eye(4, dtype=np.float32), CPU)
     with pytest.raises(RuntimeError):
         gg.gpu_spmv_(np.zeros(4, np.float32), A, np.ones(4, np.float32))
+
+
+def test_pack_graphs_block_diagonal():
+    """gg_pack_graphs: block g holds graph g's rows with columns shifted by its
+    offset; blocks start at multiples of 256; padding rows are empty."""
+    gs = [G.rmat(8, seed=1), G.random_graph(300, 4, seed=2),
+          G.grid(5, 7, 0.0, 1, 1, seed=3, dtype=np.bool_), G.rmat(9, seed=4)]
+    P, off = G.pack(gs)
+    assert len(off) == len(gs) + 1 and off[0] == 0 and off[-1] == P.m == P.n
+    assert all(o % 256 == 0 for o in off)
+    assert P.nnz == sum(g.nnz for g in gs)
+    S = P.to_scipy()
+    for g, H in enumerate(gs):
+        a, b = off[g], off[g + 1]
+        blk = S[a:a + H.m, a:a + H.m]
+        assert (blk != H.to_scipy()).nnz == 0
+        assert S[a:b].nnz == H.nnz                      # nothing outside the block
+        assert S[a + H.m:b].nnz == 0                    # padding rows are empty
+    with pytest.raises(gg.ArgumentError):
+        G.pack([G.rmat(6, seed=1), G.rmat(6, seed=1, dtype=np.int32)])
