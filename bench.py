@@ -130,6 +130,122 @@ def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma,
     return out
 
 
+def packed_leg(args, np, torch, gg, G, R0, seed):
+    """512 independent R-MAT graphs, scale uniform in 10..18 (edge factor 16,
+    same generator), packed block-diagonally: one BFS per graph from a random
+    non-isolated root, all in one persistent launch, against the same BFS
+    run one graph per call on a sample of the graphs."""
+    rng = np.random.default_rng(seed + 2)
+    t = time.time()
+    scales = rng.integers(10, 19, args.packed_graphs)
+    graphs = [G.rmat(int(s), 16, 0.57, 0.19, 0.19, seed=seed + 100 + i)
+              for i, s in enumerate(scales)]
+    roots = np.array([int(rng.choice(np.nonzero(g.degrees())[0])) + 1 for g in graphs], np.int32)
+    P, off = G.pack(graphs)
+    A = P.to_csr(R0)
+    torch.cuda.synchronize()
+    log(f"packed {len(graphs)} graphs: {P.m} rows, {P.nnz} entries; {time.time() - t:.1f}s")
+    gg.bfs_packed(A, off, roots)
+    torch.cuda.synchronize()
+    lat = []
+    for _ in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        D, levels = gg.bfs_packed(A, off, roots)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    t_pack = float(np.mean(lat))
+    sample = rng.choice(len(graphs), min(args.packed_plain, len(graphs)), replace=False)
+    mats = [(i, graphs[i].to_csr(R0)) for i in sample]
+    gg.bfs_distances(mats[0][1], int(roots[mats[0][0]]))
+    torch.cuda.synchronize()
+    plain_lat, same = [], True
+    for i, Ai in mats:
+        t0 = time.perf_counter()
+        d = gg.bfs_distances(Ai, int(roots[i]))
+        torch.cuda.synchronize()
+        plain_lat.append(time.perf_counter() - t0)
+        same &= torch.equal(d, D[int(off[i]):int(off[i]) + graphs[i].m])
+    assert same, "packed BFS differs from the per-graph BFS"
+    q_pack = len(graphs) / t_pack
+    q_plain = len(mats) / sum(plain_lat)
+    log(f"packed BFS: {len(graphs)} graphs in {t_pack * 1e3:.2f} ms = {q_pack:.0f} q/s; "
+        f"one graph per call {q_plain:.0f} q/s (p99 {pct(plain_lat, 99) * 1e3:.2f} ms); "
+        f"{q_pack / q_plain:.1f}x; max levels {int(levels.max())}")
+    del A, D
+    return dict(graphs=len(graphs), rows=int(P.m), entries=int(P.nnz), launch_ms=t_pack * 1e3,
+                qps=q_pack, plain_qps=q_plain, plain_p99_ms=pct(plain_lat, 99) * 1e3,
+                batched_vs_plain=q_pack / q_plain, max_levels=int(levels.max()),
+                plain_sample=len(mats))
+
+
+def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
+    """Continuous batching over 8 HBM-resident graphs: R-MAT scales 16, 18, 20,
+    22 (edge factor 16), 4-neighbour grids 1024^2, 2048^2, 4096^2 (+1%
+    long-range edges, integer weights 1..1000) and a 2^16-vertex uniform
+    random graph of degree 16.  Poisson arrivals; each query is a BFS or an
+    SSSP from a uniform random source on a uniform random graph (BFS on a grid
+    follows its pattern, SSSP its weights).  Reports q/s and p50/p99 latency
+    from arrival to completion, and checks a sample of results against the
+    same query run alone."""
+    from gpugraphs import serving as S
+    t = time.time()
+    specs = []   # (name, bfs matrix, sssp matrix)
+    for sc in (16, 18, 20, 22):
+        H = G.rmat(sc, 16, 0.57, 0.19, 0.19, seed=seed + sc)
+        M = H.to_sell(R0, 64, sigma, args.heavy, args.bfs_seg, relabel)
+        specs.append((f"rmat{sc}", M, M))
+    for side in (1024, 2048, 4096):
+        H = G.grid(side, side, 0.01, 1, 1000, seed=seed + side, dtype=np.int32)
+        Hb = G.HostCSR(H.m, H.n, H.rowptr, H.colval, np.ones(H.nnz, np.bool_))
+        specs.append((f"grid{side}", Hb.to_sell(R0, 64, sigma, 0, args.seg, relabel),
+                      H.to_sell(R0, 64, sigma, 0, args.seg, relabel)))
+    H = G.random_graph(1 << 16, 16, seed=seed + 7)
+    M = H.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel)
+    specs.append(("uniform16", M, M))
+    torch.cuda.synchronize()
+    log(f"serving graphs built in {time.time() - t:.1f}s")
+    budget = int(args.serve_budget_gib * (1 << 30))
+    srv = S.GraphServer(budget_bytes=budget)
+    table, route = [], []
+    for name, Mb, Ms in specs:
+        gb = srv.add_graph(Mb)
+        gs = gb if Ms is Mb else srv.add_graph(Ms)
+        route.append((gb, gs, Mb.m, None))
+        table.append((name, gb, gs, Mb, Ms))
+    srv.start()
+    done, wall, results, submitted = S.run_poisson(srv, route, args.serve_rate, args.serve_seconds,
+                                                   seed=seed, keep_every=97)
+    st = srv.stats()
+    srv.stop()
+    ok = [c for c in done if c.status == 0]
+    lat = [c.latency_s for c in ok]
+    by_id = {r: (g, k, s) for r, g, k, s in submitted}
+    mats = {}
+    for name, gb, gs, Mb, Ms in table:
+        mats[gb], mats[gs] = Mb, Ms
+    checked = 0
+    for rid, out in list(results.items())[:24]:
+        g, k, src = by_id[rid]
+        ref = gg.bfs_distances(mats[g], src) if k == S.BFS else gg.shortest_path(mats[g], src)
+        assert np.array_equal(out, ref.cpu().numpy()), f"served query {rid} differs from alone"
+        checked += 1
+    srv.close()
+    per_kind = {}
+    for kind, label in ((S.BFS, "bfs"), (S.SSSP, "sssp")):
+        kl = [c.latency_s for c in ok if c.kind == kind]
+        if kl:
+            per_kind[label] = dict(n=len(kl), p50_ms=pct(kl, 50) * 1e3, p99_ms=pct(kl, 99) * 1e3)
+    out = dict(graphs=[s[0] for s in specs], rate_qps=args.serve_rate,
+               arrival_s=args.serve_seconds, submitted=len(submitted), completed=len(ok),
+               failed=len(done) - len(ok), wall_s=wall, qps=len(ok) / wall,
+               p50_ms=pct(lat, 50) * 1e3 if lat else None,
+               p99_ms=pct(lat, 99) * 1e3 if lat else None, by_kind=per_kind,
+               budget_bytes=budget, steps=st["steps"], checked_alone=checked)
+    log(f"serving: {len(ok)}/{len(submitted)} queries in {wall:.1f}s = {len(ok) / wall:.0f} q/s, "
+        f"p50 {out['p50_ms']:.1f} ms p99 {out['p99_ms']:.1f} ms, {checked} checked against alone")
+    return out
+
+
 def copy_calibration(torch, dev, nbytes=1 << 30, reps=10):
     """Plain device copy of a known size: FETCH_SIZE / WRITE_SIZE per dispatch of
     this kernel divided by nbytes is the counters' scale factor."""
@@ -172,8 +288,16 @@ def main():
     ap.add_argument("--grid", type=int, default=4096, help="Bellman-Ford grid side")
     ap.add_argument("--sssp-sources", type=int, default=16, help="grid Bellman-Ford sources")
     ap.add_argument("--sssp-k", type=int, default=32, help="batched SSSP sources on R-MAT")
+    ap.add_argument("--packed-graphs", type=int, default=512,
+                    help="R-MAT graphs (scale 10..18) packed into one persistent BFS launch")
+    ap.add_argument("--packed-plain", type=int, default=64,
+                    help="graphs of the pack also run one BFS call at a time (plain baseline)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the serving leg")
+    ap.add_argument("--serve-budget-gib", type=float, default=8.0,
+                    help="HBM result-arena budget of the server (GiB)")
     ap.add_argument("--only", default="",
-                    help="one part: spmv | msbfs | bfs | sssp, or copy (a 1 GiB device copy that "
+                    help="one part: spmv | msbfs | bfs | sssp | packed | serve, or copy (a 1 GiB device copy that "
                          "calibrates the HBM byte counters)")
     args = ap.parse_args()
     relabel = not args.no_relabel
@@ -333,6 +457,14 @@ def main():
                               spmv_gbs=(nnz * 8 + 12 * n) / cpu_spmv_s / 1e9)
             log(f"CPU oracle ({cores} threads): BFS {cpu_bfs_s:.2f} s, SpMV {cpu_spmv_s * 1e3:.0f} ms")
 
+    # ------------------------------------------------------- packed graphs
+    if args.only in ("", "packed") and args.packed_graphs > 0:
+        res["packed"] = packed_leg(args, np, torch, gg, G, R0, seed)
+
+    # ------------------------------------------------------------- serving
+    if args.only in ("", "serve") and args.serve_seconds > 0:
+        res["serve"] = serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel)
+
     # ----------------------------------------------------------------- SSSP
     if args.only in ("", "sssp"):
         res["sssp"] = sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed,
@@ -397,6 +529,9 @@ def main():
         "roofline": roofline,
         "sssp": {name: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in c.items()}
                  for name, c in res.get("sssp", {}).items()} or None,
+        "packed": {k: (round(v, 3) if isinstance(v, float) else v)
+                   for k, v in res["packed"].items()} if "packed" in res else None,
+        "serve": res.get("serve"),
         "cpu_baseline": None if not cpu else {
             "value": round(cpu["qps"], 3), "unit": "queries/s", "cores": cpu["cores"],
             "kind": "port",

@@ -459,14 +459,6 @@ static int read_counters(gg_ctx* ctx, const BfsWork& w, BfsCounters* host) {
   return GG_OK;
 }
 
-static int csr_lanes(const gg_matrix* A) {
-  if (A->m == 0) return 0;
-  const double avg = (double)A->nnz / (double)A->m;
-  int lg = 0;
-  while (lg < 6 && (double)(1 << lg) < avg) ++lg;
-  return lg;
-}
-
 // One pull level (levels or parents) over A_T, front -> next, at the level
 // and into the output that cnt holds.  `st` is the context stream, or the
 // capture stream of a graph (then the launch is not timed).
@@ -480,7 +472,7 @@ static int pull_step(gg_ctx* ctx, hipStream_t st, const gg_matrix* A, bool paren
       constexpr bool U = decltype(unit_tag)::value;
       constexpr bool P = decltype(par_tag)::value;
       if (A->format == GG_CSR) {
-        const int lg = csr_lanes(A);
+        const int lg = csr_log_lanes(ctx, A);
         unsigned nb = blocks_for(A->m << lg, kB);
         const unsigned cap = (unsigned)ctx->num_cus * 32;
         if (nb > cap) nb = cap;

@@ -68,6 +68,7 @@ struct gg_ctx {
   // captured loop steps (gg_ctx_set_graph_steps): steps per graph launch, the
   // stream graphs are captured on, and the instantiated graphs by signature
   int graph_steps = 0;
+  int csr_log_lanes = -1;  // lanes per CSR row as log2 (gg_ctx_set_csr_lanes), -1 = by row length
   hipStream_t cap_stream = nullptr;
   std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
 };
@@ -128,6 +129,17 @@ int check_matrix(const gg_matrix* A);
 inline int64_t sell_slots(const gg_matrix* A) {
   const int64_t all = A->nslices * A->slice_size;
   return A->perm ? all : (A->m < all ? A->m : all);
+}
+
+// Lanes per row of the CSR kernels (log2): the context's setting, else the
+// next power of two >= the average row length, at most a wavefront.
+inline int csr_log_lanes(const gg_ctx* ctx, const gg_matrix* A) {
+  if (ctx->csr_log_lanes >= 0) return ctx->csr_log_lanes;
+  if (A->m == 0) return 0;
+  const double avg = (double)A->nnz / (double)A->m;
+  int lg = 0;
+  while (lg < 6 && (double)(1 << lg) < avg) ++lg;
+  return lg;
 }
 
 // Appends the bytes of a value to a graph signature.

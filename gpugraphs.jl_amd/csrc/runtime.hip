@@ -274,6 +274,20 @@ int gg_ctx_set_graph_steps(gg_ctx* ctx, int steps) {
   return GG_OK;
 }
 
+int gg_ctx_set_csr_lanes(gg_ctx* ctx, int lanes) {
+  if (!ctx) return set_error(GG_ERR_ARGUMENT, "ctx is NULL");
+  if (lanes == 0) {
+    ctx->csr_log_lanes = -1;
+    return GG_OK;
+  }
+  if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1)))
+    return set_error(GG_ERR_ARGUMENT, "lanes per CSR row must be 0 or a power of two in 1:64");
+  int l = 0;
+  while ((1 << l) < lanes) ++l;
+  ctx->csr_log_lanes = l;
+  return GG_OK;
+}
+
 int gg_ctx_last_kernel_ms(gg_ctx* ctx, double* ms) {
   if (!ctx || !ms) return set_error(GG_ERR_ARGUMENT, "NULL argument");
   *ms = ctx->last_ms;

@@ -273,14 +273,6 @@ __global__ __launch_bounds__(kBlock) void csc_spmv_kernel(T* __restrict__ c,
   }
 }
 
-static int csr_log_lanes(const gg_matrix* A) {
-  if (A->m == 0) return 0;
-  const double avg = (double)A->nnz / (double)A->m;
-  int lg = 0;
-  while (lg < 6 && (double)(1 << lg) < avg) ++lg;
-  return lg;
-}
-
 static int ilog2_exact(int64_t v) {
   if (v <= 0 || (v & (v - 1))) return -1;
   int l = 0;
@@ -303,7 +295,7 @@ int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const voi
     if (!accum_ok(accum, std::is_floating_point<TC>::value))
       return set_error(GG_ERR_UNSUPPORTED, "accum operator %d not supported for this type", accum);
     if (A->format == GG_CSR) {
-      int lg = csr_log_lanes(A);
+      int lg = csr_log_lanes(ctx, A);
       if (S::ADD == GG_OP_ADD && is_int<typename T::CT>()) lg = 0;  // exact absorb position
       // mask without range only (_dense_mask_gpu_spmv!, src/spmv/CSR.jl:191-205);
       // range + mask runs the generic kernel, src/spmv/CSR.jl:338-353

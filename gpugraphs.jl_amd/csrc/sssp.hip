@@ -309,12 +309,6 @@ __global__ void iter_end_kernel(unsigned int* changed, int32_t* ctl) {
   else *changed = 0u;
 }
 
-static int log_lanes_for(double avg) {
-  int lg = 0;
-  while (lg < 6 && (double)(1 << lg) < avg) ++lg;
-  return lg;
-}
-
 // Dist type of shortest_path: Int32 for Bool values, else the value type
 // (src/algorithms/shortest_path.jl:18).
 static int dist_type_for(int val_dtype) { return val_dtype == GG_BOOL ? GG_INT32 : val_dtype; }
@@ -440,7 +434,7 @@ int sssp_one(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist, int di
                        ns, Lim<TD>::max());
     hipLaunchKernelGGL(seed_one_kernel<TD>, dim3(1), dim3(1), 0, ctx->stream, d, source - 1, vinv,
                        c.prev);
-    const int lg = log_lanes_for(m ? (double)A->nnz / m : 1.0);
+    const int lg = csr_log_lanes(ctx, A);
     SellView g = sell_view(A);
     const int64_t nsb = (g.n_slots + kT - 1) / kT;
     auto step = [&](hipStream_t st, const uint32_t* chp, uint32_t* chn,

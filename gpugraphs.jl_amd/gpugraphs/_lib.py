@@ -121,6 +121,7 @@ SIGNATURES = {
     "gg_memcpy_dtoh": ([_vp, _vp, _vp, C.c_size_t], _i32),
     "gg_ctx_enable_timing": ([_vp, _i32], _i32),
     "gg_ctx_set_graph_steps": ([_vp, _i32], _i32),
+    "gg_ctx_set_csr_lanes": ([_vp, _i32], _i32),
     "gg_ctx_last_kernel_ms": ([_vp, C.POINTER(_d)], _i32),
     "gg_ctx_hot_kernel_stats": ([_vp, C.POINTER(_d), C.POINTER(_i64), _i32], _i32),
     "gg_matrix_check": ([_PM], _i32),
@@ -233,6 +234,11 @@ def set_graph_steps(ctx, steps: int):
     """Run level / iteration loops as HIP graphs of `steps` captured steps
     (0: one host-launched step at a time)."""
     check(lib().gg_ctx_set_graph_steps(ctx, int(steps)))
+
+
+def set_csr_lanes(ctx, lanes: int):
+    """Lanes per CSR row (power of two in 1:64; 0 = by average row length)."""
+    check(lib().gg_ctx_set_csr_lanes(ctx, int(lanes)))
 
 
 def hot_kernel_stats(ctx, reset: bool = True):

@@ -113,13 +113,15 @@ class GraphServer:
 
 
 def run_poisson(server: GraphServer, graphs, rate_qps: float, duration_s: float, seed: int = 0,
-                sssp_fraction: float = 0.5, keep_results: bool = False):
+                sssp_fraction: float = 0.5, keep_every: int = 0):
     """Submit Poisson arrivals (rate_qps) for duration_s: each query is BFS or
     SSSP (sssp_fraction) from a uniform random source on a uniform random
-    graph of ``graphs`` (list of (graph_id, n, sources_pool or None)).
-    A collector thread fetches results as they finish.  Returns
-    (completions, wall_s, results) with results keyed by request id when
-    keep_results."""
+    graph of ``graphs``, a list of (graph_id, n, sources_pool or None) or of
+    (bfs_graph_id, sssp_graph_id, n, sources_pool or None) when BFS and SSSP
+    queries of one graph go to two registrations (pattern and weights).  A
+    collector thread polls completions and fetches or releases each result;
+    every keep_every-th result is kept.  Returns (completions, wall_s,
+    results by request id, submitted (request id, graph id, kind, source))."""
     rng = np.random.default_rng(seed)
     done, results = [], {}
     submitted = []
@@ -131,7 +133,7 @@ def run_poisson(server: GraphServer, graphs, rate_qps: float, duration_s: float,
             batch = server.poll(wait_s=0.01)
             for c in batch:
                 if c.status == 0:
-                    if keep_results:
+                    if keep_every and c.request_id % keep_every == 0:
                         results[c.request_id] = server.fetch(c.request_id, c.graph, c.kind)
                     else:
                         server.release(c.request_id)
@@ -151,9 +153,15 @@ def run_poisson(server: GraphServer, graphs, rate_qps: float, duration_s: float,
             break
         while time.perf_counter() < t_next:
             pass
-        gid, n, pool = graphs[rng.integers(len(graphs))]
+        spec = graphs[rng.integers(len(graphs))]
+        if len(spec) == 4:
+            gb, gs, n, pool = spec
+        else:
+            gb, n, pool = spec
+            gs = gb
         src = int(pool[rng.integers(len(pool))]) if pool is not None else int(rng.integers(n)) + 1
         kind = SSSP if rng.random() < sssp_fraction else BFS
+        gid = gs if kind == SSSP else gb
         rid = server.submit(gid, kind, src)
         with lock:
             submitted.append((rid, gid, kind, src))

@@ -201,6 +201,10 @@ int gg_ctx_hot_kernel_stats(gg_ctx* ctx, double* total_ms, int64_t* launches, in
  * on the host every level (bfs.jl:60) or every 8 iterations
  * (shortest_path.jl:71-76). */
 int gg_ctx_set_graph_steps(gg_ctx* ctx, int steps);
+/* Lanes per row of the CSR SpMV, BFS and Bellman-Ford kernels (a power of
+ * two in 1:64; 0 = the next power of two >= the average row length).  The
+ * reference runs one work-item per row (src/spmv/CSR.jl:31-52). */
+int gg_ctx_set_csr_lanes(gg_ctx* ctx, int lanes);
 
 /* Host-side consistency check of a descriptor (no device access): the checks
  * every entry point runs before launching. */

@@ -1,0 +1,69 @@
+"""The autotuner times every candidate layout / switch on the device, keeps
+the fastest, caches it per graph signature, and the tuned choice computes
+the same results as any other."""
+import numpy as np
+import pytest
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env(oracle):
+    import torch
+    import gpugraphs as gg
+    assert torch.cuda.is_available()
+    return gg, torch, oracle
+
+
+@pytest.mark.parametrize("workload", ["spmv", "bfs", "msbfs", "sssp"])
+def test_tune_layout_picks_a_timed_candidate(env, workload, tmp_path):
+    gg, torch, orc = env
+    from gpugraphs import autotune as AT, generators as GEN
+    R = GEN.rmat(12, seed=5, dtype=np.float32, values=GEN.UNIFORM01) if workload == "spmv" else \
+        GEN.random_graph(3000, 8, seed=6, dtype=np.int32, values=GEN.INTEGER, wmin=1, wmax=99) \
+        if workload == "sssp" else GEN.rmat(12, seed=5)
+    roots = (np.nonzero(R.degrees())[0][:8] + 1).astype(np.int32)
+    cache = str(tmp_path / "tune.json")
+    tuner = AT.Autotuner(gg.ROCBackend(0), cache_path=cache, reps=2)
+    print_case("autotune", workload)
+    res = tuner.tune_layout(R, workload, roots=roots)
+    assert res.best in [c for c, _ in res.table]
+    assert res.best_ms == min(ms for _, ms in res.table) > 0
+    again = AT.Autotuner(gg.ROCBackend(0), cache_path=cache).tune_layout(R, workload, roots=roots)
+    assert again.best == res.best                      # served from the cache file
+    A = AT.build(R, gg.ROCBackend(0), res.best)
+    S = R.to_scipy()
+    if workload == "spmv":
+        x = np.random.default_rng(1).random(R.n, dtype=np.float32)
+        y = torch.zeros(R.m, dtype=torch.float32, device="cuda:0")
+        gg.gpu_spmv_(y, A, torch.from_numpy(x).cuda())
+        np.testing.assert_allclose(y.cpu().numpy(), S.astype(np.float64) @ x, rtol=1e-5, atol=1e-7)
+    elif workload == "sssp":
+        np.testing.assert_array_equal(gg.shortest_path(A, int(roots[0])).cpu().numpy(),
+                                      orc.shortest_path(orc.from_scipy_csr(S), int(roots[0])))
+    else:
+        np.testing.assert_array_equal(gg.bfs_distances(A, int(roots[0])).cpu().numpy(),
+                                      orc.bfs_distances(orc.from_scipy_csr(S), int(roots[0])))
+
+
+def test_tune_direction_and_graph_steps(env):
+    gg, torch, orc = env
+    from gpugraphs import autotune as AT, generators as GEN
+    R = GEN.rmat(13, seed=8)
+    A = R.to_csr(gg.ROCBackend(0))
+    roots = (np.nonzero(R.degrees())[0][:4] + 1).astype(np.int32)
+    tuner = AT.Autotuner(gg.ROCBackend(0), reps=1)
+    res = tuner.tune_direction(A, A, roots, alphas=(4.0, 15.0), betas=(18.0,))
+    assert set(res.best) == {"alpha", "beta"} and len(res.table) == 2
+    ref = gg.bfs_distances(A, int(roots[0])).cpu().numpy()
+    got = gg.bfs_distances_do(A, int(roots[0]), A_out=A, **res.best).cpu().numpy()
+    np.testing.assert_array_equal(got, ref)
+    from gpugraphs import _lib
+    try:
+        gs = tuner.tune_graph_steps(lambda: gg.bfs_distances(A, int(roots[1])), steps=(0, 4))
+        assert gs.best["steps"] in (0, 4)
+        np.testing.assert_array_equal(gg.bfs_distances(A, int(roots[0])).cpu().numpy(), ref)
+    finally:
+        _lib.set_graph_steps(_lib.context(0), 0)

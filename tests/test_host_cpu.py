@@ -363,3 +363,20 @@ def test_pack_graphs_block_diagonal():
         assert S[a + H.m:b].nnz == 0                    # padding rows are empty
     with pytest.raises(gg.ArgumentError):
         G.pack([G.rmat(6, seed=1), G.rmat(6, seed=1, dtype=np.int32)])
+
+
+def test_autotune_candidates_and_signature():
+    from gpugraphs import autotune as AT
+    R = G.rmat(10, seed=2)
+    cands = AT.default_candidates(R)
+    assert any(c["layout"] == "csr" for c in cands) and any(c.get("relabel") for c in cands)
+    mx = int(R.degrees().max())
+    assert all(c.get("heavy", 0) < mx for c in cands)
+    assert AT.signature(R, "bfs") != AT.signature(R, "spmv")
+    assert AT.signature(R, "bfs") == AT.signature(G.rmat(10, seed=2), "bfs")
+    rect = G.HostCSR(3, 5, np.array([1, 2, 3, 4], np.int32), np.array([1, 2, 5], np.int32),
+                     np.ones(3, np.float32))
+    assert not any(c.get("relabel") for c in AT.default_candidates(rect))
+    with pytest.raises(gg.ArgumentError):
+        gg.lib()  # library present
+        AT.Autotuner(CPU).tune_layout(R, "nope")
