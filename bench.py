@@ -196,7 +196,7 @@ def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
         specs.append((f"rmat{sc}", M, M))
     for side in (1024, 2048, 4096):
         H = G.grid(side, side, 0.01, 1, 1000, seed=seed + side, dtype=np.int32)
-        Hb = G.HostCSR(H.m, H.n, H.rowptr, H.colval, np.ones(H.nnz, np.bool_))
+        Hb = G.HostCSR(H.m, H.n, H.rowptr, H.colval, np.ones(H.nnz, np.bool_), H.symmetric)
         specs.append((f"grid{side}", Hb.to_sell(R0, 64, sigma, 0, args.seg, relabel),
                       H.to_sell(R0, 64, sigma, 0, args.seg, relabel)))
     H = G.random_graph(1 << 16, 16, seed=seed + 7)
@@ -333,7 +333,7 @@ def main():
 
     t = time.time()
     A_f = Rg.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel)   # fp32 values for SpMV
-    Rb = G.HostCSR(n, n, Rg.rowptr, Rg.colval, np.ones(nnz, np.bool_))
+    Rb = G.HostCSR(n, n, Rg.rowptr, Rg.colval, np.ones(nnz, np.bool_), Rg.symmetric)
     A_b = Rb.to_sell(R0, 64, sigma, args.heavy, args.bfs_seg, relabel)   # Bool pattern for BFS
     A_csr = Rb.to_csr(R0)                                          # out-edges for push (symmetric)
     torch.cuda.synchronize()

@@ -103,6 +103,7 @@ template <int W> __device__ __forceinline__ void alive_wave(const MsArgs& a, con
 template <class TA, bool UNIT, int W>
 __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_sell_blocks, MsArgs a) {
   if (a.ctl && a.ctl[1]) return;  // a captured step after the last level
+  if (a.pull_off && *a.pull_off) return;  // this level runs top-down
   const int32_t level = a.ctl ? a.ctl[0] : a.level;
   if ((int64_t)blockIdx.x < n_sell_blocks) {
     int64_t row, base, len;
@@ -243,6 +244,7 @@ __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict
                                                        const int32_t* __restrict__ colval,
                                                        const TA* __restrict__ val, MsArgs a) {
   if (a.ctl && a.ctl[1]) return;
+  if (a.pull_off && *a.pull_off) return;
   const int32_t level = a.ctl ? a.ctl[0] : a.level;
   const int64_t row = (int64_t)blockIdx.x * kM + threadIdx.x;
   bool newly = false;
@@ -355,10 +357,243 @@ __global__ void ms_level_end_kernel(unsigned int* found, int32_t* ctl, int32_t c
   if (cap && ctl[0] >= cap) ctl[1] = 2;
 }
 
-int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc) {
+// ------------------------------------------------------- top-down step
+// On a symmetric pattern, state row u of A_T lists u's out-neighbours.
+struct StateRow {
+  const int32_t* col;
+  int64_t base, stride, len;
+};
+__device__ __forceinline__ StateRow state_row(const SellView& g, bool csr,
+                                              const int32_t* __restrict__ rowptr, int64_t u) {
+  StateRow r{g.colval, 0, 1, 0};
+  if (csr) {
+    r.base = rowptr[u] - 1;
+    r.len = rowptr[u + 1] - rowptr[u];
+  } else if (u < g.slot_base) {
+    if (g.perm[u] == 0) return r;  // empty slot
+    const int64_t sl = u / g.C;
+    r.base = (int64_t)g.slice_ptr[sl] - 1 + (u - sl * g.C);
+    r.stride = g.C;
+    r.len = g.slot_len[u];
+  } else {
+    const int64_t h = u - g.slot_base;
+    r.col = g.heavy_col;
+    r.base = g.heavy_ptr[h];
+    r.len = g.heavy_ptr[h + 1] - r.base;
+  }
+  return r;
+}
+
+constexpr int kPushBig = 4096;  // out-edges from which a frontier row is cut into pieces
+
+__global__ void ms_push_start_kernel(MsArgs a, MsPush p) {
+  if (a.ctl && a.ctl[1]) return;
+  p.cnt[0] = p.cnt[1] = p.cnt[2] = 0u;
+  *p.scout = 0ull;
+}
+
+// front_any -> queue of frontier state ids, and the frontier's out-edges.
+__global__ __launch_bounds__(256) void ms_frontier_kernel(MsArgs a, MsPush p, SellView g, int csr,
+                                                          const int32_t* __restrict__ rowptr) {
+  if (a.ctl && a.ctl[1]) return;
+  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int wl = threadIdx.x & 63;
+  uint32_t bits = w < p.words ? a.front_any[w] : 0u;
+  const int c = __popc(bits);
+  int incl = c;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (wl >= o) incl += y;
+  }
+  unsigned base = 0;
+  if (wl == 63 && incl) base = atomicAdd(&p.cnt[0], (unsigned)incl);
+  base = __shfl(base, 63, 64);
+  unsigned pos = base + incl - c;
+  unsigned long long deg = 0;
+  while (bits) {
+    const int b = __ffs(bits) - 1;
+    bits &= bits - 1;
+    const int64_t u = w * 32 + b;
+    p.queue[pos++] = (int32_t)u;
+    deg += (unsigned long long)state_row(g, csr, rowptr, u).len;
+  }
+  for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o, 64);
+  if (wl == 0 && deg) atomicAdd(p.scout, deg);
+}
+
+// Push when the frontier's out-edges are below 1/16 of the matrix.
+__global__ void ms_decide_kernel(MsArgs a, MsPush p) {
+  if (a.ctl && a.ctl[1]) return;
+  const unsigned push = (*p.scout) * 16ull < (unsigned long long)p.nnz ? 1u : 0u;
+  p.cnt[3] = push;
+  p.cnt[4] = push;  // read by the pull kernels as a.pull_off
+}
+
+// Edge u -> v: OR u's frontier bits that v has not seen into acc[v]; the
+// first lane to touch v queues it.  All 64 lanes must call it.
+template <int W>
+__device__ __forceinline__ void push_pair(const MsArgs& a, const MsPush& p, int64_t u, int64_t v,
+                                          bool valid) {
+  bool newly = false;
+  if (valid) {
+    uint64_t add[W];
+    bool any = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      add[w] = a.front[u * W + w] & ~a.seen[v * W + w] & a.live[w];
+      any |= add[w] != 0;
+    }
+    if (any) {
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        if (add[w]) atomicOr(&p.acc[v * W + w], (unsigned long long)add[w]);
+      newly = bit_claim(p.touched, v);
+    }
+  }
+  const unsigned long long b = __ballot(newly);
+  if (b) {
+    const int wl = threadIdx.x & 63;
+    const int leader = __ffsll(b) - 1;
+    unsigned base = 0;
+    if (wl == leader) base = atomicAdd(&p.cnt[1], (unsigned)__popcll(b));
+    base = __shfl(base, leader, 64);
+    if (newly) p.tqueue[base + __popcll(b & ((1ull << wl) - 1))] = (int32_t)v;
+  }
+}
+
+// Frontier rows 256 per block: their out-degrees are scanned in LDS and the
+// tile's edges walked 256 at a time; rows longer than kPushBig become pieces.
+template <int W>
+__global__ __launch_bounds__(256) void ms_push_tile_kernel(MsArgs a, MsPush p, SellView g, int csr,
+                                                           const int32_t* __restrict__ rowptr) {
+  if (a.ctl && a.ctl[1]) return;
+  if (!p.cnt[3]) return;
+  __shared__ int64_t s_base[256];
+  __shared__ int32_t s_stride[256];
+  __shared__ int32_t s_u[256];
+  __shared__ int32_t s_heavy[256];
+  __shared__ int32_t s_scan[257];
+  __shared__ int32_t s_wsum[4];
+  const int wl = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t qlen = p.cnt[0];
+  for (int64_t t0 = (int64_t)blockIdx.x * 256; t0 < qlen; t0 += (int64_t)gridDim.x * 256) {
+    const int64_t qi = t0 + threadIdx.x;
+    int32_t deg = 0, u = 0;
+    StateRow r{g.colval, 0, 1, 0};
+    if (qi < qlen) {
+      u = p.queue[qi];
+      r = state_row(g, csr, rowptr, u);
+      deg = (int32_t)r.len;
+      if (deg > kPushBig) {
+        const int np = (deg + kPushBig - 1) / kPushBig;
+        const unsigned p0 = atomicAdd(&p.cnt[2], (unsigned)np);
+        for (int j = 0; j < np; ++j)
+          if (p0 + j < p.big_cap) p.big[p0 + j] = make_int2(u, j * kPushBig);
+        deg = 0;
+      }
+    }
+    int32_t incl = deg;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(incl, o, 64);
+      if (wl >= o) incl += y;
+    }
+    if (wl == 63) s_wsum[wv] = incl;
+    __syncthreads();
+    int32_t off = 0;
+    for (int q = 0; q < wv; ++q) off += s_wsum[q];
+    s_u[threadIdx.x] = u;
+    s_base[threadIdx.x] = r.base;
+    s_stride[threadIdx.x] = (int32_t)r.stride;
+    s_heavy[threadIdx.x] = r.col != g.colval;
+    s_scan[threadIdx.x] = off + incl - deg;
+    if (threadIdx.x == 255) s_scan[256] = off + incl;
+    __syncthreads();
+    const int32_t total = s_scan[256];
+    for (int32_t e0 = 0; e0 < total; e0 += 256) {
+      const int32_t e = e0 + threadIdx.x;
+      const bool valid = e < total;
+      int64_t uu = 0, v = 0;
+      if (valid) {
+        int lo = 0, hi = 256;  // s_scan[lo] <= e < s_scan[hi]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_scan[mid] <= e) lo = mid; else hi = mid;
+        }
+        const int32_t* col = s_heavy[lo] ? g.heavy_col : g.colval;
+        uu = s_u[lo];
+        v = col[s_base[lo] + (int64_t)(e - s_scan[lo]) * s_stride[lo]] - 1;
+      }
+      push_pair<W>(a, p, uu, v, valid);
+    }
+    __syncthreads();
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void ms_push_big_kernel(MsArgs a, MsPush p, SellView g, int csr,
+                                                          const int32_t* __restrict__ rowptr) {
+  if (a.ctl && a.ctl[1]) return;
+  if (!p.cnt[3]) return;
+  const int64_t n = min((int64_t)p.cnt[2], p.big_cap);
+  for (int64_t pc = blockIdx.x; pc < n; pc += gridDim.x) {
+    const int2 x = p.big[pc];
+    const StateRow r = state_row(g, csr, rowptr, x.x);
+    const int64_t hi = min((int64_t)x.y + kPushBig, r.len);
+    for (int64_t k0 = x.y; k0 < hi; k0 += 256) {
+      const int64_t k = k0 + threadIdx.x;
+      const bool valid = k < hi;
+      push_pair<W>(a, p, x.x, valid ? (int64_t)r.col[r.base + k * r.stride] - 1 : 0, valid);
+    }
+  }
+}
+
+// Touched rows: new bits = pushed bits not yet seen; acc is cleared for the
+// next push step.
+template <int W>
+__global__ __launch_bounds__(256) void ms_push_finish_kernel(MsArgs a, MsPush p, SellView g,
+                                                             int csr) {
+  if (a.ctl && a.ctl[1]) return;
+  if (!p.cnt[3]) return;
+  const int32_t level = a.ctl ? a.ctl[0] : a.level;
+  const int64_t n = p.cnt[1];
+  for (int64_t i0 = (int64_t)blockIdx.x * 256; i0 < n; i0 += (int64_t)gridDim.x * 256) {
+    const int64_t i = i0 + threadIdx.x;
+    bool newly = false;
+    uint64_t nw[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) nw[w] = 0;
+    if (i < n) {
+      const int64_t v = p.tqueue[i];
+      uint64_t acc[W], sn[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        acc[w] = p.acc[v * W + w];
+        p.acc[v * W + w] = 0ull;
+        sn[w] = a.seen[v * W + w];
+      }
+      newly = finish_row<W>(a, level, v, csr ? v : orig_of(g, v), acc, sn, nw);
+    }
+    wave_flag(a.found, newly);
+    alive_wave<W>(a, nw);
+  }
+}
+
+int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a_in, int Wc,
+                const MsPush* push, int ctx_cus) {
   const bool unit = (A->flags & GG_FLAG_UNIT_PATTERN) != 0;
   const int64_t m = A->m;
-  return with_type(A->val_dtype, [&](auto ta) -> int {
+  MsArgs a = a_in;
+  if (push) {
+    if (!ms_push_ok(A)) return set_error(GG_ERR_ARGUMENT, "top-down steps need a symmetric unit pattern");
+    a.pull_off = (const int32_t*)&push->cnt[4];
+    const SellView g = sell_view(A);
+    const int csr = A->format == GG_CSR;
+    hipLaunchKernelGGL(ms_push_start_kernel, dim3(1), dim3(1), 0, stream, a, *push);
+    hipLaunchKernelGGL(ms_frontier_kernel, dim3(blocks_for(push->words, 256)), dim3(256), 0, stream,
+                       a, *push, g, csr, A->ptr);
+    hipLaunchKernelGGL(ms_decide_kernel, dim3(1), dim3(1), 0, stream, a, *push);
+  }
+  int st = with_type(A->val_dtype, [&](auto ta) -> int {
     typedef typename decltype(ta)::type TA;
     auto go = [&](auto ut, auto wt) -> int {
       constexpr bool U = decltype(ut)::value;
@@ -392,6 +627,32 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc)
       return pick_w(std::false_type{});
     }
   });
+  GG_TRY(st);
+  if (push) {
+    const SellView g = sell_view(A);
+    const int csr = A->format == GG_CSR;
+    const unsigned nb = (unsigned)std::max(ctx_cus, 1) * 8;
+    auto run = [&](auto wt) -> int {
+      constexpr int WW = decltype(wt)::value;
+      hipLaunchKernelGGL(ms_push_tile_kernel<WW>, dim3(nb), dim3(256), 0, stream, a, *push, g, csr,
+                         A->ptr);
+      hipLaunchKernelGGL(ms_push_big_kernel<WW>, dim3(nb), dim3(256), 0, stream, a, *push, g, csr,
+                         A->ptr);
+      hipLaunchKernelGGL(ms_push_finish_kernel<WW>, dim3(nb), dim3(256), 0, stream, a, *push, g,
+                         csr);
+      return GG_OK;
+    };
+    switch (Wc) {
+      case 1: GG_TRY(run(std::integral_constant<int, 1>{})); break;
+      case 2: GG_TRY(run(std::integral_constant<int, 2>{})); break;
+      case 4: GG_TRY(run(std::integral_constant<int, 4>{})); break;
+      default: GG_TRY(run(std::integral_constant<int, 8>{})); break;
+    }
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(push->words, 256)), dim3(256), 0, stream,
+                       push->touched, push->words, 0u);
+    GG_HIP(hipGetLastError());
+  }
+  return GG_OK;
 }
 
 // Runs one batch of k <= 512 sources.  The levels write each new (vertex,
@@ -424,7 +685,26 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
   GG_TRY(ws_get(ctx, WS_BFS1, sizeof(uint32_t) * words, &fa));
   GG_TRY(ws_get(ctx, WS_BFS2, sizeof(uint32_t) * words, &na));
   GG_TRY(ws_get(ctx, WS_FLAG, 64, &cnt));
-  GG_TRY(ws_get(ctx, WS_MSHEAVY, ms_heavy_bytes(A, Wc), &hv, true));  // kept zero by the kernel
+  // The heavy-row accumulators and the push accumulators / touched bitmap are
+  // left zero by every level; they are cleared here because their layout
+  // depends on the matrix and on W (a slot reused with another layout holds
+  // other data at those places).
+  const size_t hb = ms_heavy_bytes(A, Wc);
+  GG_TRY(ws_get(ctx, WS_MSHEAVY, hb, &hv));
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(hb / 4, kM)), dim3(kM), 0, ctx->stream,
+                     (uint32_t*)hv, (int64_t)(hb / 4), 0u);
+  MsPush push{};
+  const MsPush* pp = nullptr;
+  if (ms_push_ok(A)) {
+    void* pb;
+    GG_TRY(ws_get(ctx, WS_MSPUSH, ms_push_bytes(A, Wc), &pb));
+    push = ms_push_layout(pb, A, Wc);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(ns * Wc, kM)), dim3(kM), 0, ctx->stream,
+                       (uint64_t*)push.acc, ns * Wc, 0ull);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(push.words, kM)), dim3(kM), 0, ctx->stream,
+                       push.touched, push.words, 0u);
+    pp = &push;
+  }
   hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(ns * Wc, kM)), dim3(kM), 0, ctx->stream,
                      (uint64_t*)f, ns * Wc, 0ull);
   hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(ns * Wc, kM)), dim3(kM), 0, ctx->stream,
@@ -476,12 +756,13 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
     sig_add(key, *A);
     sig_add(key, ab[0]);
     sig_add(key, ab[1]);
+    sig_add(key, push);
     auto steps = [&](hipStream_t st) -> int {
       for (int i = 0; i < U; ++i) {
         const MsArgs& x = ab[i & 1];
         hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, st,
                            x.next_any, words, 0u);
-        GG_TRY(msbfs_level(st, A, x, Wc));
+        GG_TRY(msbfs_level(st, A, x, Wc, pp, ctx->num_cus));
         hipLaunchKernelGGL(ms_level_end_kernel, dim3(1), dim3(1), 0, st, (unsigned*)cnt, ctl,
                            dist8 ? 255 : 0);
       }
@@ -515,7 +796,7 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
       hipLaunchKernelGGL(ms_zero_kernel, dim3(1), dim3(1), 0, ctx->stream, (unsigned*)cnt);
       {
         HotTimer ht(ctx);
-        GG_TRY(msbfs_level(ctx->stream, A, a, Wc));
+        GG_TRY(msbfs_level(ctx->stream, A, a, Wc, pp, ctx->num_cus));
       }
       GG_HIP(hipMemcpyAsync(ctx->host_pinned, cnt, sizeof(unsigned), hipMemcpyDeviceToHost,
                             ctx->stream));

@@ -31,7 +31,53 @@ struct MsArgs {
   // arrival count [n_heavy]; zero before a level, left zero after it
   unsigned long long* hacc;
   unsigned int* hcnt;
+  // when set and nonzero, the pull kernels do nothing (a push step ran)
+  const int32_t* pull_off;
 };
+
+// Buffers of the top-down (push) step of a multi-source BFS level: used on a
+// symmetric unit-pattern A_T in state order (CSR, or a relabelled SELL).  A
+// device-side switch picks push when the frontier's edges are few; every
+// kernel of the other direction returns at once, so a level is the same
+// sequence of launches either way (and can be captured in a graph).
+struct MsPush {
+  unsigned long long* acc;  // [ns*W] OR of pushed masks, zero outside a push step
+  int32_t* queue;           // [ns] frontier state ids
+  int32_t* tqueue;          // [ns] rows touched by the push
+  uint32_t* touched;        // [words] bitmap of tqueue, zero outside a push step
+  int2* big;                // hub pieces (state id, first entry)
+  unsigned int* cnt;        // [0] qlen, [1] tlen, [2] nbig, [3] mode (1 push)
+  unsigned long long* scout;  // out-edges of the frontier
+  int64_t words, nnz, big_cap;
+};
+
+inline bool ms_push_ok(const gg_matrix* A) {
+  const int64_t f = A->flags;
+  return (f & GG_FLAG_SYMMETRIC) && (f & GG_FLAG_UNIT_PATTERN) &&
+         (A->format == GG_CSR || (A->format == GG_SELL && (f & GG_FLAG_INTERNAL_IDS)));
+}
+inline size_t ms_push_bytes(const gg_matrix* A, int W) {
+  const int64_t ns = state_size(A), words = (ns + 31) / 32;
+  const int64_t big = 2 * (A->nnz / 4096) + 16;
+  return 8 * (size_t)ns * W + 8 * (size_t)ns + 4 * (size_t)(words + 8) + 8 * (size_t)big + 64;
+}
+// Lays out an MsPush over one zero-filled allocation of ms_push_bytes(A, W).
+inline MsPush ms_push_layout(void* base, const gg_matrix* A, int W) {
+  MsPush p{};
+  const int64_t ns = state_size(A);
+  p.words = (ns + 31) / 32;
+  p.nnz = A->nnz;
+  p.big_cap = 2 * (A->nnz / 4096) + 16;
+  char* c = (char*)base;
+  p.acc = (unsigned long long*)c; c += 8 * (size_t)ns * W;
+  p.big = (int2*)c; c += 8 * (size_t)p.big_cap;
+  p.scout = (unsigned long long*)c; c += 8;
+  p.cnt = (unsigned int*)c; c += 4 * 8;
+  p.queue = (int32_t*)c; c += 4 * (size_t)ns;
+  p.tqueue = (int32_t*)c; c += 4 * (size_t)ns;
+  p.touched = (uint32_t*)c;
+  return p;
+}
 
 // Zero-filled per-heavy-row accumulators for msbfs_level (MsArgs::hacc /
 // hcnt), W words per row, in one allocation of ms_heavy_bytes(A, W).
@@ -41,7 +87,10 @@ inline size_t ms_heavy_bytes(const gg_matrix* A, int W) {
 }
 
 // One or-and level over A_T with W = Wc words per vertex (1, 2, 4 or 8).
-int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc);
+// With push (ms_push_ok(A)), the level first decides on the device between
+// the pull kernel and the top-down step; `ctx_cus` sizes the push grids.
+int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc,
+                const MsPush* push = nullptr, int ctx_cus = 256);
 
 // One in-place min-plus step of k sources over a row-major working set d
 // (state rows x k): d[v, j] = min(d[v, j], min_u safe_add(w(u->v), d[u, j]))

@@ -47,6 +47,7 @@ enum WsSlot {
   WS_XPERM,         // input vector / matrix gathered into internal order
   WS_MSDIST,        // row-major distances of a multi-source traversal
   WS_MSHEAVY,       // per-heavy-row segment accumulators (kept zero between levels)
+  WS_MSPUSH,        // top-down step buffers of multi-source BFS (accumulators kept zero)
   WS_NSLOTS
 };
 

@@ -194,6 +194,7 @@ struct Engine {
   uint32_t *fany = nullptr, *nany = nullptr;
   int32_t *dist = nullptr, *start = nullptr;
   void* heavy_acc = nullptr;  // segment accumulators of heavy rows (kept zero)
+
   int32_t level = 0;
   // SSSP
   int dist_dtype = GG_INT32;
@@ -265,6 +266,7 @@ static int make_engine(gg_server* s, const gg_matrix* A, int kind, std::unique_p
     GG_TRY(engine_alloc(s, *e, hb, &p)); e->heavy_acc = p;
     hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(hb / 4, 256)), dim3(256), 0,
                        e->stream, (uint32_t*)p, (int64_t)(hb / 4), 0u);
+
     hipLaunchKernelGGL(serve_fill_kernel<uint64_t>, dim3(nb), dim3(256), 0, e->stream, e->front, ns, 0ull);
     hipLaunchKernelGGL(serve_fill_kernel<uint64_t>, dim3(nb), dim3(256), 0, e->stream, e->seen, ns, 0ull);
     hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(words, 256)), dim3(256), 0,
@@ -412,6 +414,8 @@ static int launch_step(gg_server* s, Engine& e) {
     a.alive = e.mask_d;
     a.hacc = (unsigned long long*)e.heavy_acc;
     a.hcnt = (unsigned*)(a.hacc + (e.A.format == GG_SELL ? e.A.n_heavy : 0));
+    // pull steps only: the slots of a step sit at different levels, so the
+    // union of their frontiers is rarely small enough for top-down steps
     GG_TRY(msbfs_level(e.stream, &e.A, a, 1));
     e.level += 1;
     std::swap(e.front, e.next);
@@ -512,6 +516,8 @@ int gg_server_create(int device, size_t budget_bytes, gg_server** out) {
   GG_HIP(hipSetDevice(device));
   gg_server* s = new gg_server();
   s->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) s->num_cus = prop.multiProcessorCount;
   if (budget_bytes) {
     if (hipMalloc((void**)&s->arena_base, budget_bytes) != hipSuccess) {
       delete s;

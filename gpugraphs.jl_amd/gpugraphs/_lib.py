@@ -21,6 +21,7 @@ CSR, SELL, CSC = 1, 2, 3
 COL_MAJOR, ROW_MAJOR = 0, 1
 FLAG_UNIT_PATTERN = 1
 FLAG_INTERNAL_IDS = 2
+FLAG_SYMMETRIC = 4
 SELL_RELABEL = 1
 
 

@@ -23,6 +23,7 @@ class HostCSR:
     rowptr: np.ndarray  # int32 [m+1], 1-based
     colval: np.ndarray  # int32 [nnz], 1-based
     nzval: np.ndarray
+    symmetric: bool = False  # the pattern is symmetric (A_T rows are also out-neighbour lists)
 
     @property
     def nnz(self):
@@ -35,7 +36,9 @@ class HostCSR:
 
     def to_csr(self, backend):
         from .storage import SparseGPUMatrixCSR
-        return SparseGPUMatrixCSR(self.m, self.n, self.rowptr, self.colval, self.nzval, backend)
+        M = SparseGPUMatrixCSR(self.m, self.n, self.rowptr, self.colval, self.nzval, backend)
+        M.symmetric = self.symmetric
+        return M
 
     def to_sell(self, backend, slice_size=64, sigma=4096, heavy_threshold=0, heavy_seg_len=4096,
                 relabel=False):
@@ -43,6 +46,7 @@ class HostCSR:
         M = SparseGPUMatrixSELL.__new__(SparseGPUMatrixSELL)
         M._build(self.m, self.n, self.rowptr, self.colval, np.ascontiguousarray(self.nzval),
                  backend, slice_size, sigma, heavy_threshold, heavy_seg_len, relabel)
+        M.symmetric = self.symmetric
         return M
 
     def degrees(self):
@@ -73,7 +77,9 @@ def rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, permute=True, dt
     out = C.POINTER(GGCsrHost)()
     check(lib().gg_gen_rmat(scale, edge_factor, a, b, c, seed, int(permute), dtype_code(dtype),
                             values, C.byref(out)), "rmat")
-    return _take(out)
+    H = _take(out)
+    H.symmetric = True   # both directions of every edge
+    return H
 
 
 def grid(rows, cols, long_range_frac=0.01, wmin=1, wmax=1000, seed=1, dtype=np.float32) -> HostCSR:
@@ -82,7 +88,9 @@ def grid(rows, cols, long_range_frac=0.01, wmin=1, wmax=1000, seed=1, dtype=np.f
     out = C.POINTER(GGCsrHost)()
     check(lib().gg_gen_grid(rows, cols, long_range_frac, wmin, wmax, seed, dtype_code(dtype),
                             C.byref(out)), "grid")
-    return _take(out)
+    H = _take(out)
+    H.symmetric = True   # both directions of every edge
+    return H
 
 
 def uniform_rows(n, per_row=16, seed=1, dtype=np.float32) -> HostCSR:
@@ -98,7 +106,9 @@ def random_graph(n, degree=16, seed=1, dtype=np.bool_, values=UNIT, wmin=1, wmax
     out = C.POINTER(GGCsrHost)()
     check(lib().gg_gen_random_graph(n, degree, seed, dtype_code(dtype), values, wmin, wmax,
                                     C.byref(out)), "random_graph")
-    return _take(out)
+    H = _take(out)
+    H.symmetric = True   # both directions of every edge
+    return H
 
 
 def pack(graphs):

@@ -270,6 +270,8 @@ class SparseGPUMatrixCSR(AbstractSparseGPUMatrix):
         d.m, d.n, d.nnz = self.m, self.n, self.nnz()
         d.ptr, d.idx, d.val = P(self.rowptr), P(self.colval), P(self.nzval)
         d.flags = _lib.FLAG_UNIT_PATTERN if self.unit else 0
+        if getattr(self, "symmetric", False):
+            d.flags |= _lib.FLAG_SYMMETRIC
         return d
 
     def __getitem__(self, ij):
@@ -343,6 +345,8 @@ class SparseGPUMatrixCSC(AbstractSparseGPUMatrix):
         d.m, d.n, d.nnz = self.m, self.n, self.nnz()
         d.ptr, d.idx, d.val = P(self.colptr), P(self.rowval), P(self.nzval)
         d.flags = _lib.FLAG_UNIT_PATTERN if self.unit else 0
+        if getattr(self, "symmetric", False):
+            d.flags |= _lib.FLAG_SYMMETRIC
         return d
 
     def __getitem__(self, ij):
@@ -507,6 +511,8 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
             d.heavy_nseg = self.n_seg
             d.heavy_col, d.heavy_val = P(self.heavy_col), P(self.heavy_val)
         d.flags = _lib.FLAG_UNIT_PATTERN if self.unit else 0
+        if getattr(self, "symmetric", False):
+            d.flags |= _lib.FLAG_SYMMETRIC
         if self._vinv_dev is not None:
             d.flags |= _lib.FLAG_INTERNAL_IDS
             d.vinv = P(self._vinv_dev)

@@ -133,7 +133,11 @@ typedef enum gg_matrix_flags {
    * for x in {0,1}: or-and BFS may skip loading values. */
   GG_FLAG_UNIT_PATTERN = 1,
   /* A relabelled SELL (gg_sell_build with GG_SELL_RELABEL). */
-  GG_FLAG_INTERNAL_IDS = 2
+  GG_FLAG_INTERNAL_IDS = 2,
+  /* The pattern is symmetric: row v of A_T also lists the out-neighbours of
+   * v, so traversals may expand a small frontier top-down (push) over the
+   * same arrays; used with GG_FLAG_UNIT_PATTERN by multi-source BFS. */
+  GG_FLAG_SYMMETRIC = 4
 } gg_matrix_flags;
 
 /* Host-side SELL-C-sigma arrays produced by gg_sell_build (owned by the

@@ -521,3 +521,45 @@ def test_bfs_packed_equals_per_graph(env, dtype):
     gg.bfs_distances(A, 1)
     D2, _ = gg.bfs_packed(A, off, srcs)
     np.testing.assert_array_equal(host(D2), D)
+
+
+def _sym_host(S):
+    from gpugraphs import generators as GEN
+    S = sp.csr_matrix(S)
+    S.sort_indices()
+    assert (S != S.T).nnz == 0
+    return GEN.HostCSR(S.shape[0], S.shape[1], (S.indptr + 1).astype(np.int32),
+                       (S.indices + 1).astype(np.int32), np.ones(S.nnz, np.bool_), True)
+
+
+@pytest.mark.parametrize("steps", [0, 8])
+def test_multi_source_bfs_top_down(env, steps):
+    """Symmetric unit patterns switch levels with small frontiers to the
+    top-down step (hub rows longer than 4096 entries cut into pieces); the
+    levels equal the single-source BFS."""
+    gg, torch, orc = env
+    from gpugraphs import _lib, generators as GEN
+    n = 12001
+    star = sp.coo_matrix((np.ones(2 * (n - 1)), (np.r_[np.zeros(n - 1), np.arange(1, n)],
+                                                 np.r_[np.arange(1, n), np.zeros(n - 1)])),
+                         shape=(n, n)).tocsr()
+    path = sp.diags([np.ones(799), np.ones(799)], [-1, 1], format="csr")
+    R = GEN.rmat(13, seed=11)
+    cases = [("star", _sym_host(star)), ("path", _sym_host(path)), ("rmat13", R)]
+    ctx = _lib.context(0)
+    try:
+        _lib.set_graph_steps(ctx, steps)
+        for name, H in cases:
+            assert H.symmetric
+            Ao = orc.from_scipy_csr(H.to_scipy())
+            rng = np.random.default_rng(len(name))
+            srcs = rng.choice(np.nonzero(H.degrees())[0], 20, replace=False).astype(np.int32) + 1
+            srcs[0] = 1
+            for A in (H.to_csr(gg.ROCBackend(0)),
+                      H.to_sell(gg.ROCBackend(0), 64, 1, 256, 512, relabel=True)):
+                print_case("msbfs top-down", steps, name, type(A).__name__)
+                D = host(gg.multi_source_bfs(A, srcs))
+                for i in range(len(srcs)):
+                    np.testing.assert_array_equal(D[:, i], orc.bfs_distances(Ao, int(srcs[i])))
+    finally:
+        _lib.set_graph_steps(ctx, 0)
