@@ -281,6 +281,8 @@ def main():
     ap.add_argument("--bfs-seg", type=int, default=65536,
                     help="heavy segment length of the BFS (Bool pattern) matrix")
     ap.add_argument("--plain-roots", type=int, default=64)
+    ap.add_argument("--push-alpha", type=float, default=16.0,
+                    help="multi-source BFS levels run top-down when frontier edges * alpha < entries")
     ap.add_argument("--graph-steps", type=int, default=8,
                     help="levels / iterations per captured graph launch (0: host-stepped loops)")
     ap.add_argument("--spmv-reps", type=int, default=20)
@@ -343,6 +345,7 @@ def main():
     roots = rng.choice(np.nonzero(deg)[0], args.roots, replace=False).astype(np.int32) + 1
     ctx = _lib.context(0)
     _lib.set_graph_steps(ctx, args.graph_steps)
+    _lib.set_push_threshold(ctx, args.push_alpha)
     res = {}
 
     # ---------------------------------------------------------------- SpMV
@@ -511,7 +514,7 @@ def main():
                                + ("degree-relabelled" if relabel else "sigma4096"),
                    "n": n, "nnz": nnz, "roots": args.roots, "slice_size": 64, "sigma": sigma,
                    "relabel": relabel, "heavy_threshold": args.heavy,
-                   "graph_steps": args.graph_steps,
+                   "graph_steps": args.graph_steps, "push_alpha": args.push_alpha,
                    "parallelism": "single GPU"},
         "p50_ms": None if not ms else round(ms["p50_ms"], 3),
         "p99_ms": None if not ms else round(ms["p99_ms"], 3),

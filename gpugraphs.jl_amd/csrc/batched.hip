@@ -392,39 +392,48 @@ __global__ void ms_push_start_kernel(MsArgs a, MsPush p) {
   *p.scout = 0ull;
 }
 
-// front_any -> queue of frontier state ids, and the frontier's out-edges.
+// The frontier's out-edges (QUEUE = false), or, on a push level, the queue of
+// its state ids (QUEUE = true): one pass over front_any each.
+template <bool QUEUE>
 __global__ __launch_bounds__(256) void ms_frontier_kernel(MsArgs a, MsPush p, SellView g, int csr,
                                                           const int32_t* __restrict__ rowptr) {
   if (a.ctl && a.ctl[1]) return;
+  if (QUEUE && !p.cnt[3]) return;
   const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int wl = threadIdx.x & 63;
   uint32_t bits = w < p.words ? a.front_any[w] : 0u;
-  const int c = __popc(bits);
-  int incl = c;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(incl, o, 64);
-    if (wl >= o) incl += y;
+  if (QUEUE) {
+    const int c = __popc(bits);
+    int incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (wl >= o) incl += y;
+    }
+    unsigned base = 0;
+    if (wl == 63 && incl) base = atomicAdd(&p.cnt[0], (unsigned)incl);
+    base = __shfl(base, 63, 64);
+    unsigned pos = base + incl - c;
+    while (bits) {
+      const int b = __ffs(bits) - 1;
+      bits &= bits - 1;
+      p.queue[pos++] = (int32_t)(w * 32 + b);
+    }
+  } else {
+    unsigned long long deg = 0;
+    while (bits) {
+      const int b = __ffs(bits) - 1;
+      bits &= bits - 1;
+      deg += (unsigned long long)state_row(g, csr, rowptr, w * 32 + b).len;
+    }
+    for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o, 64);
+    if (wl == 0 && deg) atomicAdd(p.scout, deg);
   }
-  unsigned base = 0;
-  if (wl == 63 && incl) base = atomicAdd(&p.cnt[0], (unsigned)incl);
-  base = __shfl(base, 63, 64);
-  unsigned pos = base + incl - c;
-  unsigned long long deg = 0;
-  while (bits) {
-    const int b = __ffs(bits) - 1;
-    bits &= bits - 1;
-    const int64_t u = w * 32 + b;
-    p.queue[pos++] = (int32_t)u;
-    deg += (unsigned long long)state_row(g, csr, rowptr, u).len;
-  }
-  for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o, 64);
-  if (wl == 0 && deg) atomicAdd(p.scout, deg);
 }
 
-// Push when the frontier's out-edges are below 1/16 of the matrix.
+// Push when the frontier's out-edges are below 1/alpha of the matrix.
 __global__ void ms_decide_kernel(MsArgs a, MsPush p) {
   if (a.ctl && a.ctl[1]) return;
-  const unsigned push = (*p.scout) * 16ull < (unsigned long long)p.nnz ? 1u : 0u;
+  const unsigned push = p.alpha > 0.0 && (double)(*p.scout) * p.alpha < (double)p.nnz ? 1u : 0u;
   p.cnt[3] = push;
   p.cnt[4] = push;  // read by the pull kernels as a.pull_off
 }
@@ -589,9 +598,11 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a_in, int 
     const SellView g = sell_view(A);
     const int csr = A->format == GG_CSR;
     hipLaunchKernelGGL(ms_push_start_kernel, dim3(1), dim3(1), 0, stream, a, *push);
-    hipLaunchKernelGGL(ms_frontier_kernel, dim3(blocks_for(push->words, 256)), dim3(256), 0, stream,
-                       a, *push, g, csr, A->ptr);
+    hipLaunchKernelGGL(ms_frontier_kernel<false>, dim3(blocks_for(push->words, 256)), dim3(256), 0,
+                       stream, a, *push, g, csr, A->ptr);
     hipLaunchKernelGGL(ms_decide_kernel, dim3(1), dim3(1), 0, stream, a, *push);
+    hipLaunchKernelGGL(ms_frontier_kernel<true>, dim3(blocks_for(push->words, 256)), dim3(256), 0,
+                       stream, a, *push, g, csr, A->ptr);
   }
   int st = with_type(A->val_dtype, [&](auto ta) -> int {
     typedef typename decltype(ta)::type TA;
@@ -699,6 +710,7 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
     void* pb;
     GG_TRY(ws_get(ctx, WS_MSPUSH, ms_push_bytes(A, Wc), &pb));
     push = ms_push_layout(pb, A, Wc);
+    push.alpha = ctx->push_alpha;
     hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(ns * Wc, kM)), dim3(kM), 0, ctx->stream,
                        (uint64_t*)push.acc, ns * Wc, 0ull);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(push.words, kM)), dim3(kM), 0, ctx->stream,

@@ -49,6 +49,7 @@ struct MsPush {
   unsigned int* cnt;        // [0] qlen, [1] tlen, [2] nbig, [3] mode (1 push)
   unsigned long long* scout;  // out-edges of the frontier
   int64_t words, nnz, big_cap;
+  double alpha;             // push when frontier out-edges * alpha < nnz
 };
 
 inline bool ms_push_ok(const gg_matrix* A) {
@@ -76,6 +77,7 @@ inline MsPush ms_push_layout(void* base, const gg_matrix* A, int W) {
   p.queue = (int32_t*)c; c += 4 * (size_t)ns;
   p.tqueue = (int32_t*)c; c += 4 * (size_t)ns;
   p.touched = (uint32_t*)c;
+  p.alpha = 16.0;
   return p;
 }
 

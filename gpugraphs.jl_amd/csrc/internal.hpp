@@ -70,6 +70,7 @@ struct gg_ctx {
   // stream graphs are captured on, and the instantiated graphs by signature
   int graph_steps = 0;
   int csr_log_lanes = -1;  // lanes per CSR row as log2 (gg_ctx_set_csr_lanes), -1 = by row length
+  double push_alpha = 16.0;  // top-down when frontier out-edges * alpha < entries
   hipStream_t cap_stream = nullptr;
   std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
 };

@@ -123,6 +123,7 @@ SIGNATURES = {
     "gg_ctx_enable_timing": ([_vp, _i32], _i32),
     "gg_ctx_set_graph_steps": ([_vp, _i32], _i32),
     "gg_ctx_set_csr_lanes": ([_vp, _i32], _i32),
+    "gg_ctx_set_push_threshold": ([_vp, _d], _i32),
     "gg_ctx_last_kernel_ms": ([_vp, C.POINTER(_d)], _i32),
     "gg_ctx_hot_kernel_stats": ([_vp, C.POINTER(_d), C.POINTER(_i64), _i32], _i32),
     "gg_matrix_check": ([_PM], _i32),
@@ -240,6 +241,12 @@ def set_graph_steps(ctx, steps: int):
 def set_csr_lanes(ctx, lanes: int):
     """Lanes per CSR row (power of two in 1:64; 0 = by average row length)."""
     check(lib().gg_ctx_set_csr_lanes(ctx, int(lanes)))
+
+
+def set_push_threshold(ctx, alpha: float):
+    """Multi-source BFS levels run top-down when frontier out-edges * alpha <
+    entries (0: never)."""
+    check(lib().gg_ctx_set_push_threshold(ctx, float(alpha)))
 
 
 def hot_kernel_stats(ctx, reset: bool = True):

@@ -155,6 +155,19 @@ class Autotuner:
         (a, b), ms = min(table, key=lambda t: t[1])
         return TuneResult("bfs_do", dict(alpha=a, beta=b), ms, table)
 
+    def tune_push_threshold(self, fn, alphas=(0.0, 4.0, 16.0, 64.0)):
+        """Direction switch of the multi-source BFS (gg_ctx_set_push_threshold)
+        with the least time for fn(); 0 keeps every level bottom-up."""
+        import torch
+        ctx = _lib.context(self.backend.device)
+        table = []
+        for a in alphas:
+            _lib.set_push_threshold(ctx, a)
+            table.append((a, self._time(fn, torch)))
+        a, ms = min(table, key=lambda t: t[1])
+        _lib.set_push_threshold(ctx, a)
+        return TuneResult("push_threshold", dict(alpha=a), ms, table)
+
     def tune_graph_steps(self, fn, steps=(0, 2, 8, 32)):
         """Steps per captured graph launch with the least time for fn()."""
         import torch

@@ -62,8 +62,15 @@ def test_tune_direction_and_graph_steps(env):
     np.testing.assert_array_equal(got, ref)
     from gpugraphs import _lib
     try:
+        S = R.to_sell(gg.ROCBackend(0), 64, 1, 256, 1024, relabel=True)
+        srcs = (np.nonzero(R.degrees())[0][:40] + 1).astype(np.int32)
+        pa = tuner.tune_push_threshold(lambda: gg.multi_source_bfs(S, srcs), alphas=(0.0, 16.0))
+        assert pa.best["alpha"] in (0.0, 16.0)
+        D = gg.multi_source_bfs(S, srcs).cpu().numpy()
+        np.testing.assert_array_equal(D[:, 0], gg.bfs_distances(A, int(srcs[0])).cpu().numpy())
         gs = tuner.tune_graph_steps(lambda: gg.bfs_distances(A, int(roots[1])), steps=(0, 4))
         assert gs.best["steps"] in (0, 4)
         np.testing.assert_array_equal(gg.bfs_distances(A, int(roots[0])).cpu().numpy(), ref)
     finally:
         _lib.set_graph_steps(_lib.context(0), 0)
+        _lib.set_push_threshold(_lib.context(0), 16.0)
