@@ -53,6 +53,19 @@ def sssp_certificate(rowptr, colval, w, d, src):
     return bool(d64[src - 1] == 0 and np.all(d64 <= best) and np.all(best[reached] == d64[reached]))
 
 
+def measured_traffic(args, relabel, kernel):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    runs (profiles/hbm_traffic.json) when this run is the workload they
+    measured (the default R-MAT 22 configuration), else None."""
+    default = (args.scale == 22 and args.roots == 64 and relabel and args.heavy == 1024 and
+               args.bfs_seg == 65536 and args.seg == 4096 and args.push_alpha == 16.0)
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not default or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)[kernel]["traffic_bytes"]
+
+
 def kernel_pass(_lib, ctx, fn, graph_steps):
     """(summed ms, launches) of the hot kernel over one host-stepped run of fn:
     HIP events on the context stream around each launch (single launches
@@ -482,8 +495,10 @@ def main():
         kernel_ms = ms["kernel_ms"]
         achieved = ms["bytes_per_launch"] / kernel_ms / 1e6
         roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                        kernel="msbfs_sell_kernel", bytes_per_launch=ms["bytes_per_launch"],
+                        frac=round(achieved / HBM_PEAK_GBS, 4),
+                        traffic=measured_traffic(args, relabel, "msbfs_level"),
+                        kernel="multi-source BFS level (msbfs_sell_kernel + direction switch + "
+                               "top-down step)", bytes_per_launch=ms["bytes_per_launch"],
                         avg_launch_us=round(kernel_ms * 1e3, 2))
     else:
         value, roofline = None, None
@@ -492,6 +507,7 @@ def main():
         spmv_roof = dict(bound="hbm", achieved=round(spmv["kernel_gbs"], 1), peak=HBM_PEAK_GBS,
                          unit="GB/s", frac=round(spmv["kernel_gbs"] / HBM_PEAK_GBS, 4),
                          kernel="sell_spmv_kernel", bytes_per_launch=spmv["bytes"],
+                         traffic=measured_traffic(args, relabel, "sell_spmv"),
                          avg_launch_us=round(spmv["kernel_ms"] * 1e3, 2),
                          spmv_call_gbs=round(spmv["gbs"], 1))
     cpu = res.get("cpu")
