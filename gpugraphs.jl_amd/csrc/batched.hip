@@ -358,32 +358,6 @@ __global__ void ms_level_end_kernel(unsigned int* found, int32_t* ctl, int32_t c
 }
 
 // ------------------------------------------------------- top-down step
-// On a symmetric pattern, state row u of A_T lists u's out-neighbours.
-struct StateRow {
-  const int32_t* col;
-  int64_t base, stride, len;
-};
-__device__ __forceinline__ StateRow state_row(const SellView& g, bool csr,
-                                              const int32_t* __restrict__ rowptr, int64_t u) {
-  StateRow r{g.colval, 0, 1, 0};
-  if (csr) {
-    r.base = rowptr[u] - 1;
-    r.len = rowptr[u + 1] - rowptr[u];
-  } else if (u < g.slot_base) {
-    if (g.perm[u] == 0) return r;  // empty slot
-    const int64_t sl = u / g.C;
-    r.base = (int64_t)g.slice_ptr[sl] - 1 + (u - sl * g.C);
-    r.stride = g.C;
-    r.len = g.slot_len[u];
-  } else {
-    const int64_t h = u - g.slot_base;
-    r.col = g.heavy_col;
-    r.base = g.heavy_ptr[h];
-    r.len = g.heavy_ptr[h + 1] - r.base;
-  }
-  return r;
-}
-
 constexpr int kPushBig = 4096;  // out-edges from which a frontier row is cut into pieces
 
 __global__ void ms_push_start_kernel(MsArgs a, MsPush p) {
@@ -470,72 +444,14 @@ __device__ __forceinline__ void push_pair(const MsArgs& a, const MsPush& p, int6
   }
 }
 
-// Frontier rows 256 per block: their out-degrees are scanned in LDS and the
-// tile's edges walked 256 at a time; rows longer than kPushBig become pieces.
+// Frontier rows expanded edge by edge (expand_tiles / expand_pieces).
 template <int W>
 __global__ __launch_bounds__(256) void ms_push_tile_kernel(MsArgs a, MsPush p, SellView g, int csr,
                                                            const int32_t* __restrict__ rowptr) {
   if (a.ctl && a.ctl[1]) return;
   if (!p.cnt[3]) return;
-  __shared__ int64_t s_base[256];
-  __shared__ int32_t s_stride[256];
-  __shared__ int32_t s_u[256];
-  __shared__ int32_t s_heavy[256];
-  __shared__ int32_t s_scan[257];
-  __shared__ int32_t s_wsum[4];
-  const int wl = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t qlen = p.cnt[0];
-  for (int64_t t0 = (int64_t)blockIdx.x * 256; t0 < qlen; t0 += (int64_t)gridDim.x * 256) {
-    const int64_t qi = t0 + threadIdx.x;
-    int32_t deg = 0, u = 0;
-    StateRow r{g.colval, 0, 1, 0};
-    if (qi < qlen) {
-      u = p.queue[qi];
-      r = state_row(g, csr, rowptr, u);
-      deg = (int32_t)r.len;
-      if (deg > kPushBig) {
-        const int np = (deg + kPushBig - 1) / kPushBig;
-        const unsigned p0 = atomicAdd(&p.cnt[2], (unsigned)np);
-        for (int j = 0; j < np; ++j)
-          if (p0 + j < p.big_cap) p.big[p0 + j] = make_int2(u, j * kPushBig);
-        deg = 0;
-      }
-    }
-    int32_t incl = deg;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int32_t y = __shfl_up(incl, o, 64);
-      if (wl >= o) incl += y;
-    }
-    if (wl == 63) s_wsum[wv] = incl;
-    __syncthreads();
-    int32_t off = 0;
-    for (int q = 0; q < wv; ++q) off += s_wsum[q];
-    s_u[threadIdx.x] = u;
-    s_base[threadIdx.x] = r.base;
-    s_stride[threadIdx.x] = (int32_t)r.stride;
-    s_heavy[threadIdx.x] = r.col != g.colval;
-    s_scan[threadIdx.x] = off + incl - deg;
-    if (threadIdx.x == 255) s_scan[256] = off + incl;
-    __syncthreads();
-    const int32_t total = s_scan[256];
-    for (int32_t e0 = 0; e0 < total; e0 += 256) {
-      const int32_t e = e0 + threadIdx.x;
-      const bool valid = e < total;
-      int64_t uu = 0, v = 0;
-      if (valid) {
-        int lo = 0, hi = 256;  // s_scan[lo] <= e < s_scan[hi]
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (s_scan[mid] <= e) lo = mid; else hi = mid;
-        }
-        const int32_t* col = s_heavy[lo] ? g.heavy_col : g.colval;
-        uu = s_u[lo];
-        v = col[s_base[lo] + (int64_t)(e - s_scan[lo]) * s_stride[lo]] - 1;
-      }
-      push_pair<W>(a, p, uu, v, valid);
-    }
-    __syncthreads();
-  }
+  expand_tiles(p.queue, (int64_t)p.cnt[0], g, csr, rowptr, kPushBig, p.big, &p.cnt[2], p.big_cap,
+               [&](int64_t u, int64_t v, bool valid) { push_pair<W>(a, p, u, v, valid); });
 }
 
 template <int W>
@@ -543,17 +459,8 @@ __global__ __launch_bounds__(256) void ms_push_big_kernel(MsArgs a, MsPush p, Se
                                                           const int32_t* __restrict__ rowptr) {
   if (a.ctl && a.ctl[1]) return;
   if (!p.cnt[3]) return;
-  const int64_t n = min((int64_t)p.cnt[2], p.big_cap);
-  for (int64_t pc = blockIdx.x; pc < n; pc += gridDim.x) {
-    const int2 x = p.big[pc];
-    const StateRow r = state_row(g, csr, rowptr, x.x);
-    const int64_t hi = min((int64_t)x.y + kPushBig, r.len);
-    for (int64_t k0 = x.y; k0 < hi; k0 += 256) {
-      const int64_t k = k0 + threadIdx.x;
-      const bool valid = k < hi;
-      push_pair<W>(a, p, x.x, valid ? (int64_t)r.col[r.base + k * r.stride] - 1 : 0, valid);
-    }
-  }
+  expand_pieces(p.big, min((int64_t)p.cnt[2], p.big_cap), g, csr, rowptr, kPushBig,
+                [&](int64_t u, int64_t v, bool valid) { push_pair<W>(a, p, u, v, valid); });
 }
 
 // Touched rows: new bits = pushed bits not yet seen; acc is cleared for the

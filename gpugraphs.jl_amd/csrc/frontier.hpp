@@ -182,6 +182,132 @@ template <class T> __device__ __forceinline__ bool atomic_min_lowered(T* p, T v)
   return false;
 }
 
+// On a symmetric pattern, state row u of A_T lists u's out-neighbours.
+struct StateRow {
+  const int32_t* col;
+  int64_t base, stride, len;
+};
+__device__ __forceinline__ StateRow state_row(const SellView& g, bool csr,
+                                              const int32_t* __restrict__ rowptr, int64_t u) {
+  StateRow r{g.colval, 0, 1, 0};
+  if (csr) {
+    r.base = rowptr[u] - 1;
+    r.len = rowptr[u + 1] - rowptr[u];
+  } else if (u < g.slot_base) {
+    if (g.perm[u] == 0) return r;  // empty slot
+    const int64_t sl = u / g.C;
+    r.base = (int64_t)g.slice_ptr[sl] - 1 + (u - sl * g.C);
+    r.stride = g.C;
+    r.len = g.slot_len[u];
+  } else {
+    const int64_t h = u - g.slot_base;
+    r.col = g.heavy_col;
+    r.base = g.heavy_ptr[h];
+    r.len = g.heavy_ptr[h + 1] - r.base;
+  }
+  return r;
+}
+
+// Out-edges of queue[0..qlen) (state ids of a symmetric pattern), 256 queue
+// entries per tile: their degrees are scanned in LDS and each lane walks one
+// edge per step, finding its row by binary search in the scan.  Rows with more
+// than big_deg edges are appended to big[] as pieces (row, first entry) for
+// expand_pieces instead.  on_edge(u, v, valid) is called by all threads of
+// the block together (v 0-based; valid false on padding lanes).
+template <class F>
+__device__ void expand_tiles(const int32_t* __restrict__ queue, int64_t qlen, const SellView& g,
+                             bool csr, const int32_t* __restrict__ rowptr, int big_deg, int2* big,
+                             unsigned int* nbig, int64_t big_cap, F&& on_edge) {
+  __shared__ int64_t s_base[256];
+  __shared__ int32_t s_stride[256];
+  __shared__ int32_t s_u[256];
+  __shared__ int32_t s_heavy[256];
+  __shared__ int32_t s_scan[257];
+  __shared__ int32_t s_wsum[4];
+  const int wl = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int64_t t0 = (int64_t)blockIdx.x * 256; t0 < qlen; t0 += (int64_t)gridDim.x * 256) {
+    const int64_t qi = t0 + threadIdx.x;
+    int32_t deg = 0, u = 0;
+    StateRow r{g.colval, 0, 1, 0};
+    if (qi < qlen) {
+      u = queue[qi];
+      r = state_row(g, csr, rowptr, u);
+      deg = (int32_t)r.len;
+      if (deg > big_deg) {
+        const int np = (deg + big_deg - 1) / big_deg;
+        const unsigned p0 = atomicAdd(nbig, (unsigned)np);
+        for (int j = 0; j < np; ++j)
+          if (p0 + j < big_cap) big[p0 + j] = make_int2(u, j * big_deg);
+        deg = 0;
+      }
+    }
+    int32_t incl = deg;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(incl, o, 64);
+      if (wl >= o) incl += y;
+    }
+    if (wl == 63) s_wsum[wv] = incl;
+    __syncthreads();
+    int32_t off = 0;
+    for (int q = 0; q < wv; ++q) off += s_wsum[q];
+    s_u[threadIdx.x] = u;
+    s_base[threadIdx.x] = r.base;
+    s_stride[threadIdx.x] = (int32_t)r.stride;
+    s_heavy[threadIdx.x] = r.col != g.colval;
+    s_scan[threadIdx.x] = off + incl - deg;
+    if (threadIdx.x == 255) s_scan[256] = off + incl;
+    __syncthreads();
+    const int32_t total = s_scan[256];
+    for (int32_t e0 = 0; e0 < total; e0 += 256) {
+      const int32_t e = e0 + threadIdx.x;
+      const bool valid = e < total;
+      int64_t uu = 0, v = 0;
+      if (valid) {
+        int lo = 0, hi = 256;  // s_scan[lo] <= e < s_scan[hi]; lo is never a 0-degree entry
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_scan[mid] <= e) lo = mid; else hi = mid;
+        }
+        const int32_t* col = s_heavy[lo] ? g.heavy_col : g.colval;
+        uu = s_u[lo];
+        v = col[s_base[lo] + (int64_t)(e - s_scan[lo]) * s_stride[lo]] - 1;
+      }
+      on_edge(uu, v, valid);
+    }
+    __syncthreads();  // the next tile reuses the LDS arrays
+  }
+}
+
+// The pieces expand_tiles left: block-strided, 256 edges per step.
+template <class F>
+__device__ void expand_pieces(const int2* __restrict__ big, int64_t nbig, const SellView& g,
+                              bool csr, const int32_t* __restrict__ rowptr, int big_deg,
+                              F&& on_edge) {
+  for (int64_t pc = blockIdx.x; pc < nbig; pc += gridDim.x) {
+    const int2 x = big[pc];
+    const StateRow r = state_row(g, csr, rowptr, x.x);
+    const int64_t hi = min((int64_t)x.y + big_deg, r.len);
+    for (int64_t k0 = x.y; k0 < hi; k0 += 256) {
+      const int64_t k = k0 + threadIdx.x;
+      const bool valid = k < hi;
+      on_edge((int64_t)x.x, valid ? (int64_t)r.col[r.base + k * r.stride] - 1 : 0, valid);
+    }
+  }
+}
+
+// Appends v to q (length *qn) for every lane with `add`: one atomic per
+// wavefront.  All 64 lanes must call it.
+__device__ __forceinline__ void wave_append(int32_t* q, unsigned int* qn, int64_t v, bool add) {
+  const unsigned long long b = __ballot(add);
+  if (!b) return;
+  const int wl = threadIdx.x & 63;
+  const int leader = __ffsll(b) - 1;
+  unsigned base = 0;
+  if (wl == leader) base = atomicAdd(qn, (unsigned)__popcll(b));
+  base = __shfl(base, leader, 64);
+  if (add) q[base + __popcll(b & ((1ull << wl) - 1))] = (int32_t)v;
+}
+
 // x in internal order: xp[slot] = x[perm[slot]] (0 for an empty slot),
 // xp[slot_base + h] = x[heavy_row[h]]; rows of k values when k > 1 (B is
 // m x k in b_layout, xp is row-major).

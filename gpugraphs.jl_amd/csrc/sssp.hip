@@ -82,10 +82,10 @@ __global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_
                                                      unsigned int* changed, const int32_t* done) {
   if (done && *done) return;
   if ((int64_t)blockIdx.x < n_sell_blocks) {
+    const int64_t slot = (int64_t)blockIdx.x * kT + threadIdx.x;
     int64_t row, base, len;
     bool has_len;
     bool ch = false;
-    const int64_t slot = (int64_t)blockIdx.x * kT + threadIdx.x;
     if (sell_slot(g, slot, row, base, len, has_len)) {
       const TA* val = (const TA*)g.val;
       TD acc = Lim<TD>::max();
@@ -159,7 +159,8 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
                                                       uint32_t* __restrict__ chn,
                                                       unsigned int* changed,
                                                       unsigned long long* changed_mask,
-                                                      const int32_t* done) {
+                                                      const int32_t* done,
+                                                      const unsigned int* light_off) {
   if (done && *done) return;
   const int Lk = 1 << logLk;
   const int wl = threadIdx.x & 63;
@@ -169,6 +170,7 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
   const int64_t gpb = kT >> logLk;  // groups per block
   const int64_t light_blocks = (n_light_groups * jblocks + gpb - 1) / gpb;
   if ((int64_t)blockIdx.x < light_blocks) {
+    if (light_off && *light_off) return;
     const int64_t q = (int64_t)blockIdx.x * gpb + (threadIdx.x >> logLk);
     const int64_t unit = q / jblocks;
     const int64_t j = (q - unit * jblocks) * Lk + lane;
@@ -254,6 +256,192 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
   }
   __syncthreads();
   if (threadIdx.x == 0 && s_any) bit_claim(chn, heavy_rid(g, h));
+}
+
+// ----------------------------------------------------- sparse iterations
+// On the symmetric pattern of a relabelled SELL, row u of A_T also lists u's
+// out-neighbours.  A k-source iteration with few fallen rows (fallen * alpha <
+// rows) relaxes only the light rows their edges reach -- exactly the rows with
+// an in-neighbour that fell -- instead of every light slot, a group of lanes
+// per row over its k contiguous distances; the heavy rows keep their segment
+// blocks.  A device-side switch decides, so the iteration is the same launches
+// either way (and can be captured).  One source keeps the dense sweep: a lone
+// row of a SELL slice reads a cache line per entry, and the sweep's prologue
+// costs more than the rows it skips.
+struct BfSparse {
+  int32_t* queue;           // [ns] rows that fell in the previous iteration
+  int32_t* dq;              // [ns] light rows to relax
+  uint32_t* mark;           // [words] bitmap of dq, zero outside a sparse iteration
+  int2* big;                // pieces of long fallen rows
+  unsigned int* cnt;        // [0] fallen, [1] marked, [2] pieces, [3] sparse, [4] = [3]
+  unsigned long long* scout;  // number of fallen rows
+  int64_t words, ns, big_cap;
+  double alpha;
+};
+constexpr int kSparseBig = 4096;
+
+inline bool bf_sparse_ok(const gg_matrix* A) {
+  return A->format == GG_SELL && (A->flags & GG_FLAG_INTERNAL_IDS) &&
+         (A->flags & GG_FLAG_SYMMETRIC);
+}
+inline size_t bf_sparse_bytes(const gg_matrix* A) {
+  const int64_t ns = state_size(A), words = (ns + 31) / 32, big = 2 * (A->nnz / kSparseBig) + 16;
+  return 8 * (size_t)ns + 4 * (size_t)(words + 16) + 8 * (size_t)big + 64;
+}
+inline BfSparse bf_sparse_layout(void* base, const gg_matrix* A, double alpha) {
+  BfSparse s{};
+  const int64_t ns = state_size(A);
+  s.words = (ns + 31) / 32;
+  s.ns = ns;
+  s.big_cap = 2 * (A->nnz / kSparseBig) + 16;
+  s.alpha = alpha;
+  char* c = (char*)base;
+  s.big = (int2*)c; c += 8 * (size_t)s.big_cap;
+  s.scout = (unsigned long long*)c; c += 8;
+  s.cnt = (unsigned int*)c; c += 4 * 8;
+  s.queue = (int32_t*)c; c += 4 * (size_t)ns;
+  s.dq = (int32_t*)c; c += 4 * (size_t)ns;
+  s.mark = (uint32_t*)c;
+  return s;
+}
+
+__global__ void bf_sp_start_kernel(BfSparse s, const int32_t* done) {
+  if (done && *done) return;
+  s.cnt[0] = s.cnt[1] = s.cnt[2] = 0u;
+  *s.scout = 0ull;
+}
+
+// Number of fallen rows (QUEUE = false), or their queue (QUEUE = true,
+// sparse iterations only).
+template <bool QUEUE>
+__global__ __launch_bounds__(kT) void bf_sp_fallen_kernel(BfSparse s, const uint32_t* __restrict__ chp,
+                                                          SellView g, const int32_t* done) {
+  if (done && *done) return;
+  if (QUEUE && !s.cnt[3]) return;
+  const int64_t w = (int64_t)blockIdx.x * kT + threadIdx.x;
+  const int wl = threadIdx.x & 63;
+  uint32_t bits = w < s.words ? chp[w] : 0u;
+  if (QUEUE) {
+    const int c = __popc(bits);
+    int incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (wl >= o) incl += y;
+    }
+    unsigned base = 0;
+    if (wl == 63 && incl) base = atomicAdd(&s.cnt[0], (unsigned)incl);
+    base = __shfl(base, 63, 64);
+    unsigned pos = base + incl - c;
+    while (bits) {
+      const int b = __ffs(bits) - 1;
+      bits &= bits - 1;
+      s.queue[pos++] = (int32_t)(w * 32 + b);
+    }
+  } else {
+    unsigned long long c = (unsigned long long)__popc(bits);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (wl == 0 && c) atomicAdd(s.scout, c);
+  }
+}
+
+// Sparse when the fallen rows times alpha are below the rows (their out-edges
+// estimated at the average degree).
+__global__ void bf_sp_decide_kernel(BfSparse s, const int32_t* done) {
+  if (done && *done) return;
+  const unsigned sp = (double)(*s.scout) * s.alpha < (double)s.ns ? 1u : 0u;
+  s.cnt[3] = sp;
+  s.cnt[4] = sp;
+}
+
+__device__ __forceinline__ void bf_sp_mark(const BfSparse& s, const SellView& g, int64_t v,
+                                           bool valid) {
+  const bool add = valid && v < g.slot_base && bit_claim(s.mark, v);
+  wave_append(s.dq, &s.cnt[1], v, add);
+}
+
+__global__ __launch_bounds__(256) void bf_sp_mark_tile_kernel(BfSparse s, SellView g,
+                                                              const int32_t* done) {
+  if (done && *done) return;
+  if (!s.cnt[3]) return;
+  expand_tiles(s.queue, (int64_t)s.cnt[0], g, false, nullptr, kSparseBig, s.big, &s.cnt[2],
+               s.big_cap, [&](int64_t, int64_t v, bool valid) { bf_sp_mark(s, g, v, valid); });
+}
+
+__global__ __launch_bounds__(256) void bf_sp_mark_big_kernel(BfSparse s, SellView g,
+                                                             const int32_t* done) {
+  if (done && *done) return;
+  if (!s.cnt[3]) return;
+  expand_pieces(s.big, min((int64_t)s.cnt[2], s.big_cap), g, false, nullptr, kSparseBig,
+                [&](int64_t, int64_t v, bool valid) { bf_sp_mark(s, g, v, valid); });
+}
+
+// The marked light rows of a k-column working set: a group of Lk lanes per
+// row, one lane per column (as bf_multi_kernel's light part).
+template <class TA, class TD>
+__global__ __launch_bounds__(kT) void bf_sp_rows_multi_kernel(BfSparse s, SellView g, int64_t k,
+                                                              int logLk, TD* d,
+                                                              const uint32_t* __restrict__ chp,
+                                                              uint32_t* __restrict__ chn,
+                                                              unsigned int* changed,
+                                                              unsigned long long* changed_mask,
+                                                              const int32_t* done) {
+  if (done && *done) return;
+  if (!s.cnt[3]) return;
+  const int Lk = 1 << logLk;
+  const int wl = threadIdx.x & 63;
+  const int lane = wl & (Lk - 1);
+  const int gbase = wl & ~(Lk - 1);
+  const int64_t jblocks = (k + Lk - 1) >> logLk;
+  const int64_t gpb = kT >> logLk;
+  const int64_t ngroups = (int64_t)s.cnt[1] * jblocks;
+  const TA* val = (const TA*)g.val;
+  for (int64_t q0 = (int64_t)blockIdx.x * gpb; q0 < ngroups; q0 += (int64_t)gridDim.x * gpb) {
+    const int64_t q = q0 + (threadIdx.x >> logLk);
+    bool ch = false;
+    int64_t v = -1;
+    if (q < ngroups) {
+      const int64_t unit = q / jblocks;
+      const int64_t j = (q - unit * jblocks) * Lk + lane;
+      v = s.dq[unit];
+      if (j < k) {
+        const StateRow r = state_row(g, false, nullptr, v);
+        TD acc = Lim<TD>::max();
+        for (int64_t kk = 0; kk < r.len; ++kk) {
+          const int64_t e = r.base + kk * r.stride;
+          const int32_t c = r.col[e] - 1;
+          if (bit_get(chp, c)) acc = jmin(acc, sadd(TD(val[e]), d[(int64_t)c * k + j]));
+        }
+        const TD dv = d[v * k + j];
+        const TD nv = jmin(dv, acc);
+        if (nv < dv) {
+          d[v * k + j] = nv;
+          ch = true;
+        }
+      }
+    }
+    wave_flag(changed, ch);
+    const unsigned long long b = __ballot(ch);
+    const unsigned long long low = Lk == 64 ? ~0ull : ((1ull << Lk) - 1ull);
+    if (lane == 0 && ((b >> gbase) & low)) bit_claim(chn, v);
+    if (changed_mask) {
+      unsigned long long cm = 0;
+      for (int o = 0; o < 64; o += Lk) cm |= (b >> o) & low;
+      if (wl == 0 && cm) or_new_bits(changed_mask, cm);
+    }
+  }
+}
+
+// Enqueues the sparse-iteration prologue: fallen-row edges, the switch, the
+// fallen queue and the marking of the rows to relax.
+static void bf_sparse_prologue(hipStream_t st, const BfSparse& s, const SellView& g,
+                               const uint32_t* chp, const int32_t* done, int cus) {
+  const unsigned nbw = blocks_for(s.words, kT), nb = (unsigned)std::max(cus, 1) * 8;
+  hipLaunchKernelGGL(bf_sp_start_kernel, dim3(1), dim3(1), 0, st, s, done);
+  hipLaunchKernelGGL(bf_sp_fallen_kernel<false>, dim3(nbw), dim3(kT), 0, st, s, chp, g, done);
+  hipLaunchKernelGGL(bf_sp_decide_kernel, dim3(1), dim3(1), 0, st, s, done);
+  hipLaunchKernelGGL(bf_sp_fallen_kernel<true>, dim3(nbw), dim3(kT), 0, st, s, chp, g, done);
+  hipLaunchKernelGGL(bf_sp_mark_tile_kernel, dim3(nb), dim3(256), 0, st, s, g, done);
+  hipLaunchKernelGGL(bf_sp_mark_big_kernel, dim3(nb), dim3(256), 0, st, s, g, done);
 }
 
 template <class TD>
@@ -467,9 +655,22 @@ int sssp_one(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist, int di
   });
 }
 
+static int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k,
+                            void* d, const uint32_t* chp, uint32_t* chn, unsigned int* changed,
+                            unsigned long long* changed_mask, const int32_t* done,
+                            const BfSparse* sp, int cus);
+
 int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k, void* d,
                   const uint32_t* chp, uint32_t* chn, unsigned int* changed,
                   unsigned long long* changed_mask, const int32_t* done) {
+  return bf_multi_step_sp(stream, A, dist_dtype, k, d, chp, chn, changed, changed_mask, done,
+                          nullptr, 256);
+}
+
+static int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k,
+                            void* d, const uint32_t* chp, uint32_t* chn, unsigned int* changed,
+                            unsigned long long* changed_mask, const int32_t* done,
+                            const BfSparse* sp, int cus) {
   if (changed_mask && k > 64) return set_error(GG_ERR_ARGUMENT, "column masks need k <= 64");
   return with_ad(A->val_dtype, dist_dtype, [&](auto a, auto dt) -> int {
     typedef typename decltype(a)::type TA;
@@ -491,10 +692,18 @@ int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_
     const int64_t parts = g.n_seg > 0 ? (g.seg_len + 255) / 256 : 1;
     const int64_t nb = light_blocks + g.n_seg * parts * jblocks;
     if (nb > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
+    if (sp) bf_sparse_prologue(stream, *sp, g, chp, done, cus);
     if (nb > 0)
       hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
                          stream, A->format, A->ptr, g, n_light, parts, k, lg, (TD*)d, chp, chn,
-                         changed, changed_mask, done);
+                         changed, changed_mask, done, sp ? &sp->cnt[4] : nullptr);
+    if (sp) {
+      hipLaunchKernelGGL((bf_sp_rows_multi_kernel<TA, TD>), dim3((unsigned)std::max(cus, 1) * 8),
+                         dim3(kT), 0, stream, *sp, g, k, lg, (TD*)d, chp, chn, changed,
+                         changed_mask, done);
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(sp->words, kT)), dim3(kT), 0, stream,
+                         sp->mark, sp->words, 0u);
+    }
     GG_HIP(hipGetLastError());
     return GG_OK;
   });
@@ -520,14 +729,25 @@ int sssp_multi(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_host, int
                        (TD*)b0, ns * k, Lim<TD>::max());
     hipLaunchKernelGGL(bf_multi_seed_kernel<TD>, dim3(blocks_for(k, kT)), dim3(kT), 0,
                        ctx->stream, (TD*)b0, (const int32_t*)src, k, vinv, c.prev);
+    BfSparse sp{};
+    const bool sparse = bf_sparse_ok(A) && ctx->push_alpha > 0.0;
+    if (sparse) {
+      void* sb;
+      GG_TRY(ws_get(ctx, WS_SSSP1, bf_sparse_bytes(A), &sb));
+      sp = bf_sparse_layout(sb, A, ctx->push_alpha);
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(sp.words, kT)), dim3(kT), 0, ctx->stream,
+                         sp.mark, sp.words, 0u);
+    }
     auto step = [&](hipStream_t st, const uint32_t* chp, uint32_t* chn,
                     const int32_t* done) -> int {
       HotTimer ht(ctx, st == ctx->stream);
-      return bf_multi_step(st, A, dist_dtype, k, b0, chp, chn, (unsigned*)cnt, nullptr, done);
+      return bf_multi_step_sp(st, A, dist_dtype, k, b0, chp, chn, (unsigned*)cnt, nullptr, done,
+                              sparse ? &sp : nullptr, ctx->num_cus);
     };
     std::vector<uint64_t> key{4, (uint64_t)dist_dtype, (uint64_t)k};
     sig_add(key, *A);
     sig_add(key, b0);
+    sig_add(key, sp);
     int st = iterate(ctx, (unsigned*)cnt, c, m + 16, step, key, iterations);
     if (st == GG_OK) {
       if (rowmajor_out && !vinv) {

@@ -209,9 +209,12 @@ int gg_ctx_set_graph_steps(gg_ctx* ctx, int steps);
  * two in 1:64; 0 = the next power of two >= the average row length).  The
  * reference runs one work-item per row (src/spmv/CSR.jl:31-52). */
 int gg_ctx_set_csr_lanes(gg_ctx* ctx, int lanes);
-/* Direction switch of gg_msbfs_distances on a symmetric unit pattern
- * (GG_FLAG_SYMMETRIC): a level runs top-down when its frontier's out-edges
- * times alpha are below the matrix entries (default 16; 0 = never). */
+/* Direction switch on symmetric patterns (GG_FLAG_SYMMETRIC): a level of
+ * gg_msbfs_distances (unit pattern) runs top-down when its frontier's
+ * out-edges times alpha are below the matrix entries, and an iteration of
+ * gg_shortest_path_multi on a relabelled SELL relaxes only the rows next to
+ * the rows that fell when those rows times alpha are below the rows
+ * (default 16; 0 = never). */
 int gg_ctx_set_push_threshold(gg_ctx* ctx, double alpha);
 
 /* Host-side consistency check of a descriptor (no device access): the checks

@@ -563,3 +563,36 @@ def test_multi_source_bfs_top_down(env, steps):
                     np.testing.assert_array_equal(D[:, i], orc.bfs_distances(Ao, int(srcs[i])))
     finally:
         _lib.set_graph_steps(ctx, 0)
+
+
+@pytest.mark.parametrize("steps", [0, 8])
+def test_shortest_path_sparse_iterations(env, steps):
+    """Symmetric patterns (independent weights per direction) on a relabelled
+    SELL: k-source iterations relax only the rows next to the rows that fell
+    when those are few; distances equal the oracle's, one and many sources,
+    host-stepped and captured."""
+    gg, torch, orc = env
+    from gpugraphs import _lib, generators as GEN
+    rng = np.random.default_rng(5)
+    R = GEN.rmat(12, seed=4)
+    w = rng.integers(1, 100, R.nnz).astype(np.int32)
+    cases = [GEN.grid(60, 70, 0.01, 1, 1000, seed=9, dtype=np.float32),
+             GEN.grid(33, 41, 0.0, 1, 50, seed=2, dtype=np.int32),
+             GEN.HostCSR(R.m, R.n, R.rowptr, R.colval, w, True)]
+    ctx = _lib.context(0)
+    try:
+        _lib.set_graph_steps(ctx, steps)
+        for H in cases:
+            assert H.symmetric
+            S = H.to_scipy()
+            Ao = orc.from_scipy_csr(S)
+            A = H.to_sell(gg.ROCBackend(0), 64, 1, 256, 256, relabel=True)
+            srcs = (rng.choice(np.nonzero(H.degrees())[0], 6, replace=False) + 1).astype(np.int32)
+            print_case("sssp sparse", steps, H.m, H.nzval.dtype)
+            for s in srcs[:3]:
+                np.testing.assert_array_equal(host(gg.shortest_path(A, int(s))),
+                                              orc.shortest_path(Ao, int(s)))
+            np.testing.assert_array_equal(host(gg.shortest_path(A, srcs)),
+                                          orc.shortest_path_multi(Ao, srcs))
+    finally:
+        _lib.set_graph_steps(ctx, 0)
