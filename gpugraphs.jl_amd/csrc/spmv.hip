@@ -12,7 +12,7 @@
 //
 // SELL (src/spmv/SELL.jl): one lane per slot; the 64 lanes of a wavefront
 // cover one C=64 slice and read entry k of all rows as one contiguous 256-byte
-// line.  With slot lengths (sigma-sorted layout) a lane stops at its row's
+// line, with non-temporal loads (the entries are read once; x stays cached).  With slot lengths (sigma-sorted layout) a lane stops at its row's
 // end, so padding is never fetched; without them it stops at the -1 pad like
 // the reference.  Rows of the heavy partition are cut into segments of
 // heavy_seg_len entries, one 256-thread block per segment, in the same launch;
@@ -171,9 +171,9 @@ __global__ __launch_bounds__(kBlock) void sell_spmv_kernel(
         TA v[kU];
         TX xv[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) c[u] = g.colval[i + u * g.C];
+        for (int u = 0; u < kU; ++u) c[u] = __builtin_nontemporal_load(&g.colval[i + u * g.C]);
 #pragma unroll
-        for (int u = 0; u < kU; ++u) v[u] = val[i + u * g.C];
+        for (int u = 0; u < kU; ++u) v[u] = __builtin_nontemporal_load(&val[i + u * g.C]);
 #pragma unroll
         for (int u = 0; u < kU; ++u) xv[u] = x[c[u] - 1];
 #pragma unroll
@@ -217,8 +217,8 @@ __global__ __launch_bounds__(kBlock) void sell_spmv_kernel(
 #pragma unroll
     for (int u = 0; u < kH; ++u) {
       const int64_t i = i0 + (int64_t)u * kBlock;
-      c[u] = i < e ? g.heavy_col[i] : 1;  // column 1 is a valid, unused address
-      v[u] = i < e ? hv[i] : TA(0);
+      c[u] = i < e ? __builtin_nontemporal_load(&g.heavy_col[i]) : 1;  // column 1: valid, unused
+      v[u] = i < e ? __builtin_nontemporal_load(&hv[i]) : TA(0);
     }
 #pragma unroll
     for (int u = 0; u < kH; ++u) xv[u] = x[c[u] - 1];
