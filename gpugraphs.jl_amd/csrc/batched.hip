@@ -105,6 +105,7 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
   if (a.ctl && a.ctl[1]) return;  // a captured step after the last level
   if (a.pull_off && *a.pull_off) return;  // this level runs top-down
   const int32_t level = a.ctl ? a.ctl[0] : a.level;
+  const bool test_any = !(a.dense && *a.dense);
   if ((int64_t)blockIdx.x < n_sell_blocks) {
     int64_t row, base, len;
     bool has_len;
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
             for (int u = 0; u < 4; ++u) c[u] = g.colval[base + (kk + u) * g.C];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-              on[u] = bit_get(a.front_any, c[u] - 1) &&
+              on[u] = (!test_any || bit_get(a.front_any, c[u] - 1)) &&
                       (UNIT || edge_on(val, base + (kk + u) * g.C));
 #pragma unroll
             for (int u = 0; u < 4; ++u)
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
           const int64_t i = base + kk * g.C;
           const int32_t col = g.colval[i];
           if (!has_len && col == -1) break;
-          if (!bit_get(a.front_any, col - 1) || !(UNIT || edge_on(val, i))) continue;
+          if ((test_any && !bit_get(a.front_any, col - 1)) || !(UNIT || edge_on(val, i))) continue;
 #pragma unroll
           for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
           if (saturated<W>(acc, sn, a)) break;
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
     for (int64_t i = s + threadIdx.x; i < e; i += kM) {
       if (__atomic_load_n(&s_full, __ATOMIC_RELAXED)) break;
       const int32_t col = g.heavy_col[i];
-      if (!bit_get(a.front_any, col - 1) || !(UNIT || edge_on(hv, i))) continue;
+      if ((test_any && !bit_get(a.front_any, col - 1)) || !(UNIT || edge_on(hv, i))) continue;
 #pragma unroll
       for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
       if (saturated<W>(acc, sn, a)) __atomic_store_n(&s_full, 1, __ATOMIC_RELAXED);
@@ -246,6 +247,7 @@ __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict
   if (a.ctl && a.ctl[1]) return;
   if (a.pull_off && *a.pull_off) return;
   const int32_t level = a.ctl ? a.ctl[0] : a.level;
+  const bool test_any = !(a.dense && *a.dense);
   const int64_t row = (int64_t)blockIdx.x * kM + threadIdx.x;
   bool newly = false;
   uint64_t nw[W];
@@ -258,7 +260,7 @@ __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict
     if (!saturated<W>(acc, sn, a)) {
       for (int32_t i = rowptr[row] - 1; i < rowptr[row + 1] - 1; ++i) {
         const int32_t col = colval[i];
-        if (!bit_get(a.front_any, col - 1) || !(UNIT || edge_on(val, i))) continue;
+        if ((test_any && !bit_get(a.front_any, col - 1)) || !(UNIT || edge_on(val, i))) continue;
 #pragma unroll
         for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(col - 1) * W + w];
         if (saturated<W>(acc, sn, a)) break;
@@ -290,25 +292,47 @@ __global__ void msbfs_seed_kernel(const int32_t* sources, int64_t k, int64_t k8,
 }
 
 // Byte levels row-major m x k -> Int32 column-major m x k (255 = unreached ->
-// typemax(Int32)) through a 64 x 64 LDS tile.
+// typemax(Int32)) through a 64 x 64 LDS tile.  Reads are 8-byte words (k8 is
+// a multiple of 8); with vec (m % 4 == 0, dst 16-byte aligned) each thread
+// stores 4 rows of a column as one 16-byte non-temporal store, 16 threads per
+// column, so a wavefront writes four whole 256-byte column pieces.
+__device__ __forceinline__ int32_t widen_level(uint8_t d) {
+  return d == 255 ? 2147483647 : (int32_t)d;
+}
+
 __global__ __launch_bounds__(256) void widen_transpose_kernel(const uint8_t* __restrict__ src,
                                                               int64_t m, int64_t k, int64_t k8,
+                                                              int vec,
                                                               int32_t* __restrict__ dst) {
-  __shared__ uint8_t tile[64][65];
+  __shared__ uint8_t tile[64][72];
   const int64_t tiles_k = (k + 63) / 64;
   const int64_t r0 = ((int64_t)blockIdx.x / tiles_k) * 64;
   const int64_t c0 = ((int64_t)blockIdx.x % tiles_k) * 64;
-  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
-    const int r = i >> 6, c = i & 63;
-    if (r0 + r < m && c0 + c < k) tile[r][c] = src[(r0 + r) * k8 + c0 + c];
+  for (int i = threadIdx.x; i < 64 * 8; i += 256) {
+    const int r = i >> 3, q = i & 7;
+    if (r0 + r < m && c0 + q * 8 < k8)
+      *(uint64_t*)&tile[r][q * 8] = *(const uint64_t*)&src[(r0 + r) * k8 + c0 + q * 8];
   }
   __syncthreads();
+  if (vec && r0 + 64 <= m) {
+    const int rr = (threadIdx.x & 15) * 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = (threadIdx.x >> 4) + 16 * j;
+      if (c0 + c >= k) break;
+      typedef int32_t v4i __attribute__((ext_vector_type(4)));
+      v4i v;
+      v.x = widen_level(tile[rr][c]);
+      v.y = widen_level(tile[rr + 1][c]);
+      v.z = widen_level(tile[rr + 2][c]);
+      v.w = widen_level(tile[rr + 3][c]);
+      __builtin_nontemporal_store(v, (v4i*)&dst[(r0 + rr) + (c0 + c) * m]);
+    }
+    return;
+  }
   for (int i = threadIdx.x; i < 64 * 64; i += 256) {
     const int c = i >> 6, r = i & 63;
-    if (r0 + r < m && c0 + c < k) {
-      const uint8_t d = tile[r][c];
-      dst[(r0 + r) + (c0 + c) * m] = d == 255 ? 2147483647 : (int32_t)d;
-    }
+    if (r0 + r < m && c0 + c < k) dst[(r0 + r) + (c0 + c) * m] = widen_level(tile[r][c]);
   }
 }
 
@@ -405,11 +429,23 @@ __global__ __launch_bounds__(256) void ms_frontier_kernel(MsArgs a, MsPush p, Se
 }
 
 // Push when the frontier's out-edges are below 1/alpha of the matrix.
+// A pull level after a pull level (every front word written) whose frontier
+// has more than 1/kDenseFrac of the edges skips the front_any test.
+constexpr double kDenseFrac = 8.0;
+
 __global__ void ms_decide_kernel(MsArgs a, MsPush p) {
   if (a.ctl && a.ctl[1]) return;
   const unsigned push = p.alpha > 0.0 && (double)(*p.scout) * p.alpha < (double)p.nnz ? 1u : 0u;
   p.cnt[3] = push;
   p.cnt[4] = push;  // read by the pull kernels as a.pull_off
+  p.cnt[5] = !push && p.cnt[6] && (double)(*p.scout) * kDenseFrac > (double)p.nnz ? 1u : 0u;
+  p.cnt[6] = !push;
+}
+
+// Before a traversal's first level: the front is not known to be complete.
+__global__ void ms_push_reset_kernel(MsPush p) {
+  p.cnt[5] = 0u;
+  p.cnt[6] = 0u;
 }
 
 // Edge u -> v: OR u's frontier bits that v has not seen into acc[v]; the
@@ -502,6 +538,7 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a_in, int 
   if (push) {
     if (!ms_push_ok(A)) return set_error(GG_ERR_ARGUMENT, "top-down steps need a symmetric unit pattern");
     a.pull_off = (const int32_t*)&push->cnt[4];
+    a.dense = (const int32_t*)&push->cnt[5];
     const SellView g = sell_view(A);
     const int csr = A->format == GG_CSR;
     hipLaunchKernelGGL(ms_push_start_kernel, dim3(1), dim3(1), 0, stream, a, *push);
@@ -622,6 +659,7 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
                        (uint64_t*)push.acc, ns * Wc, 0ull);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(push.words, kM)), dim3(kM), 0, ctx->stream,
                        push.touched, push.words, 0u);
+    hipLaunchKernelGGL(ms_push_reset_kernel, dim3(1), dim3(1), 0, ctx->stream, push);
     pp = &push;
   }
   hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(ns * Wc, kM)), dim3(kM), 0, ctx->stream,
@@ -729,7 +767,8 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
     const int64_t tiles = ((m + 63) / 64) * ((k + 63) / 64);
     if (dist8)
       hipLaunchKernelGGL(widen_transpose_kernel, dim3((unsigned)tiles), dim3(256), 0,
-                         ctx->stream, dist8, m, k, k8, dist_out);
+                         ctx->stream, dist8, m, k, k8,
+                         (int)(m % 4 == 0 && ((uintptr_t)dist_out & 15) == 0), dist_out);
     else
       hipLaunchKernelGGL(transpose_rm_cm_kernel, dim3((unsigned)tiles), dim3(256), 0,
                          ctx->stream, dist, m, k, dist_out);

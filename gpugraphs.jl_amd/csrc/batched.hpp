@@ -33,6 +33,11 @@ struct MsArgs {
   unsigned int* hcnt;
   // when set and nonzero, the pull kernels do nothing (a push step ran)
   const int32_t* pull_off;
+  // when set and nonzero, the pull kernels read front without testing
+  // front_any first: every front word was written by the previous level (a
+  // pull level) and the frontier is dense, so the test filters little and
+  // costs a dependent gather per entry
+  const int32_t* dense;
 };
 
 // Buffers of the top-down (push) step of a multi-source BFS level: used on a
@@ -46,7 +51,8 @@ struct MsPush {
   int32_t* tqueue;          // [ns] rows touched by the push
   uint32_t* touched;        // [words] bitmap of tqueue, zero outside a push step
   int2* big;                // hub pieces (state id, first entry)
-  unsigned int* cnt;        // [0] qlen, [1] tlen, [2] nbig, [3] mode (1 push)
+  unsigned int* cnt;        // [0] qlen, [1] tlen, [2] nbig, [3] mode (1 push), [4] = [3],
+                            // [5] dense pull (MsArgs::dense), [6] this level pulled
   unsigned long long* scout;  // out-edges of the frontier
   int64_t words, nnz, big_cap;
   double alpha;             // push when frontier out-edges * alpha < nnz
