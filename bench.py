@@ -294,6 +294,8 @@ def main():
     ap.add_argument("--bfs-seg", type=int, default=65536,
                     help="heavy segment length of the BFS (Bool pattern) matrix")
     ap.add_argument("--plain-roots", type=int, default=64)
+    ap.add_argument("--wide-roots", type=int, default=256,
+                    help="sources of the second batched BFS measurement (0: skip)")
     ap.add_argument("--push-alpha", type=float, default=16.0,
                     help="multi-source BFS levels run top-down when frontier edges * alpha < entries")
     ap.add_argument("--graph-steps", type=int, default=8,
@@ -414,6 +416,26 @@ def main():
                             bytes_per_launch=B_ms)
         log(f"batched BFS k={k}: {step_s * 1e3:.2f} ms/step, {k / step_s:.0f} q/s, "
             f"{kn} levels, kernel avg {kms / max(kn, 1) * 1e3:.1f} us")
+        if args.wide_roots > 0:
+            # the same traversal with more sources per launch (4 mask words per vertex)
+            rw = np.random.default_rng(seed + args.wide_roots).choice(
+                np.nonzero(deg)[0], args.wide_roots, replace=False).astype(np.int32) + 1
+            Dw = gg.multi_source_bfs(A_b, rw)
+            torch.cuda.synchronize()
+            latw = []
+            for _ in range(max(args.steps // 2, 2)):
+                t0 = time.perf_counter()
+                Dw = gg.multi_source_bfs(A_b, rw)
+                torch.cuda.synchronize()
+                latw.append(time.perf_counter() - t0)
+            stepw = float(np.mean(latw))
+            d1 = gg.bfs_distances(A_b, int(rw[-1]))
+            assert torch.equal(d1, Dw[:, -1]), "wide batch differs from the plain BFS"
+            del Dw
+            res["msbfs_wide"] = dict(k=args.wide_roots, step_ms=stepw * 1e3,
+                                     qps=args.wide_roots / stepw, p99_ms=pct(latw, 99) * 1e3)
+            log(f"batched BFS k={args.wide_roots}: {stepw * 1e3:.2f} ms/step, "
+                f"{args.wide_roots / stepw:.0f} q/s")
 
     # ------------------------------------------------------------ plain BFS
     if args.only in ("", "bfs"):
@@ -540,6 +562,8 @@ def main():
         "do_qps": round(bfs["do_qps"], 1) if bfs else None,
         "do_p99_ms": round(bfs["do_p99_ms"], 3) if bfs else None,
         "batched_vs_plain": round(value / bfs["plain_qps"], 2) if (bfs and value) else None,
+        "wide_batch": {k: (round(v, 3) if isinstance(v, float) else v)
+                       for k, v in res["msbfs_wide"].items()} if "msbfs_wide" in res else None,
         "gteps_plain": round(bfs["gteps_plain"], 3) if bfs else None,
         "gteps_do": round(bfs["gteps_do"], 3) if bfs else None,
         "spmv_gbs": round(spmv["gbs"], 1) if spmv else None,
