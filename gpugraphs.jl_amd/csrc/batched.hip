@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
             int32_t c[4];
             bool on[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) c[u] = g.colval[base + (kk + u) * g.C];
+            for (int u = 0; u < 4; ++u) c[u] = __builtin_nontemporal_load(&g.colval[base + (kk + u) * g.C]);
 #pragma unroll
             for (int u = 0; u < 4; ++u)
               on[u] = (!test_any || bit_get(a.front_any, c[u] - 1)) &&
