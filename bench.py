@@ -218,45 +218,50 @@ def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
     torch.cuda.synchronize()
     log(f"serving graphs built in {time.time() - t:.1f}s")
     budget = int(args.serve_budget_gib * (1 << 30))
-    srv = S.GraphServer(budget_bytes=budget)
-    table, route = [], []
-    for name, Mb, Ms in specs:
-        gb = srv.add_graph(Mb)
-        gs = gb if Ms is Mb else srv.add_graph(Ms)
-        route.append((gb, gs, Mb.m, None))
-        table.append((name, gb, gs, Mb, Ms))
-    srv.start()
-    done, wall, results, submitted = S.run_poisson(srv, route, args.serve_rate, args.serve_seconds,
-                                                   seed=seed, keep_every=97)
-    st = srv.stats()
-    srv.stop()
-    ok = [c for c in done if c.status == 0]
-    lat = [c.latency_s for c in ok]
-    by_id = {r: (g, k, s) for r, g, k, s in submitted}
-    mats = {}
-    for name, gb, gs, Mb, Ms in table:
-        mats[gb], mats[gs] = Mb, Ms
-    checked = 0
-    for rid, out in list(results.items())[:24]:
-        g, k, src = by_id[rid]
-        ref = gg.bfs_distances(mats[g], src) if k == S.BFS else gg.shortest_path(mats[g], src)
-        assert np.array_equal(out, ref.cpu().numpy()), f"served query {rid} differs from alone"
-        checked += 1
-    srv.close()
-    per_kind = {}
-    for kind, label in ((S.BFS, "bfs"), (S.SSSP, "sssp")):
-        kl = [c.latency_s for c in ok if c.kind == kind]
-        if kl:
-            per_kind[label] = dict(n=len(kl), p50_ms=pct(kl, 50) * 1e3, p99_ms=pct(kl, 99) * 1e3)
-    out = dict(graphs=[s[0] for s in specs], rate_qps=args.serve_rate,
-               arrival_s=args.serve_seconds, submitted=len(submitted), completed=len(ok),
-               failed=len(done) - len(ok), wall_s=wall, qps=len(ok) / wall,
-               p50_ms=pct(lat, 50) * 1e3 if lat else None,
-               p99_ms=pct(lat, 99) * 1e3 if lat else None, by_kind=per_kind,
-               budget_bytes=budget, steps=st["steps"], checked_alone=checked)
-    log(f"serving: {len(ok)}/{len(submitted)} queries in {wall:.1f}s = {len(ok) / wall:.0f} q/s, "
-        f"p50 {out['p50_ms']:.1f} ms p99 {out['p99_ms']:.1f} ms, {checked} checked against alone")
-    return out
+    runs = []
+    for rate in [float(r) for r in str(args.serve_rate).split(",") if r]:
+        srv = S.GraphServer(budget_bytes=budget)
+        table, route = [], []
+        for name, Mb, Ms in specs:
+            gb = srv.add_graph(Mb)
+            gs = gb if Ms is Mb else srv.add_graph(Ms)
+            route.append((gb, gs, Mb.m, None))
+            table.append((name, gb, gs, Mb, Ms))
+        srv.start()
+        done, wall, results, submitted = S.run_poisson(srv, route, rate, args.serve_seconds,
+                                                       seed=seed, keep_every=97)
+        st = srv.stats()
+        srv.stop()
+        ok = [c for c in done if c.status == 0]
+        lat = [c.latency_s for c in ok]
+        by_id = {r: (g, k, s) for r, g, k, s in submitted}
+        mats = {}
+        for name, gb, gs, Mb, Ms in table:
+            mats[gb], mats[gs] = Mb, Ms
+        checked = 0
+        for rid, out in list(results.items())[:24]:
+            g, k, src = by_id[rid]
+            ref = gg.bfs_distances(mats[g], src) if k == S.BFS else gg.shortest_path(mats[g], src)
+            assert np.array_equal(out, ref.cpu().numpy()), f"served query {rid} differs from alone"
+            checked += 1
+        srv.close()
+        per_kind = {}
+        for kind, label in ((S.BFS, "bfs"), (S.SSSP, "sssp")):
+            kl = [c.latency_s for c in ok if c.kind == kind]
+            if kl:
+                per_kind[label] = dict(n=len(kl), p50_ms=round(pct(kl, 50) * 1e3, 2),
+                                       p99_ms=round(pct(kl, 99) * 1e3, 2))
+        run = dict(rate_qps=rate, arrival_s=args.serve_seconds, submitted=len(submitted),
+                   completed=len(ok), failed=len(done) - len(ok), wall_s=round(wall, 3),
+                   qps=round(len(ok) / wall, 1),
+                   p50_ms=round(pct(lat, 50) * 1e3, 2) if lat else None,
+                   p99_ms=round(pct(lat, 99) * 1e3, 2) if lat else None, by_kind=per_kind,
+                   steps=st["steps"], checked_alone=checked)
+        runs.append(run)
+        log(f"serving at {rate:.0f} q/s offered: {len(ok)}/{len(submitted)} queries in {wall:.1f}s "
+            f"= {len(ok) / wall:.0f} q/s, p50 {run['p50_ms']:.1f} ms p99 {run['p99_ms']:.1f} ms, "
+            f"{checked} checked against alone")
+    return dict(graphs=[s[0] for s in specs], budget_bytes=budget, runs=runs)
 
 
 def copy_calibration(torch, dev, nbytes=1 << 30, reps=10):
@@ -309,7 +314,8 @@ def main():
                     help="R-MAT graphs (scale 10..18) packed into one persistent BFS launch")
     ap.add_argument("--packed-plain", type=int, default=64,
                     help="graphs of the pack also run one BFS call at a time (plain baseline)")
-    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-rate", type=str, default="2000,250",
+                    help="Poisson arrivals per second, one serving run per comma-separated rate")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the serving leg")
     ap.add_argument("--serve-budget-gib", type=float, default=8.0,
                     help="HBM result-arena budget of the server (GiB)")

@@ -110,4 +110,50 @@ int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_
                   const uint32_t* chp, uint32_t* chn, unsigned int* changed,
                   unsigned long long* changed_mask, const int32_t* done = nullptr);
 
+// Sparse k-source iterations on the symmetric pattern of a relabelled SELL
+// (bf_sparse_ok): the buffers of the device-side switch and of the rows to
+// relax, in one allocation of bf_sparse_bytes(A) whose mark bitmap is zero.
+struct BfSparse {
+  int32_t* queue;           // [ns] rows that fell in the previous iteration
+  int32_t* dq;              // [ns] light rows to relax
+  uint32_t* mark;           // [words] bitmap of dq, zero outside a sparse iteration
+  int2* big;                // pieces of long fallen rows
+  unsigned int* cnt;        // [0] fallen, [1] marked, [2] pieces, [3] sparse, [4] = [3]
+  unsigned long long* scout;  // number of fallen rows
+  int64_t words, ns, big_cap;
+  double alpha;
+};
+constexpr int kSparseBig = 4096;
+
+inline bool bf_sparse_ok(const gg_matrix* A) {
+  return A->format == GG_SELL && (A->flags & GG_FLAG_INTERNAL_IDS) &&
+         (A->flags & GG_FLAG_SYMMETRIC);
+}
+inline size_t bf_sparse_bytes(const gg_matrix* A) {
+  const int64_t ns = state_size(A), words = (ns + 31) / 32, big = 2 * (A->nnz / kSparseBig) + 16;
+  return 8 * (size_t)ns + 4 * (size_t)(words + 16) + 8 * (size_t)big + 64;
+}
+inline BfSparse bf_sparse_layout(void* base, const gg_matrix* A, double alpha) {
+  BfSparse s{};
+  const int64_t ns = state_size(A);
+  s.words = (ns + 31) / 32;
+  s.ns = ns;
+  s.big_cap = 2 * (A->nnz / kSparseBig) + 16;
+  s.alpha = alpha;
+  char* c = (char*)base;
+  s.big = (int2*)c; c += 8 * (size_t)s.big_cap;
+  s.scout = (unsigned long long*)c; c += 8;
+  s.cnt = (unsigned int*)c; c += 4 * 8;
+  s.queue = (int32_t*)c; c += 4 * (size_t)ns;
+  s.dq = (int32_t*)c; c += 4 * (size_t)ns;
+  s.mark = (uint32_t*)c;
+  return s;
+}
+
+// bf_multi_step with sparse iterations when sp is set (cus sizes their grids).
+int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k, void* d,
+                     const uint32_t* chp, uint32_t* chn, unsigned int* changed,
+                     unsigned long long* changed_mask, const int32_t* done, const BfSparse* sp,
+                     int cus);
+
 }  // namespace gg

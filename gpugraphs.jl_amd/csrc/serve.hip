@@ -201,6 +201,8 @@ struct Engine {
   size_t esz = 4;
   void* buf = nullptr;                      // ns x 32 row-major distances
   uint32_t *chp = nullptr, *chn = nullptr;  // rows that fell in the last / this step
+  bool sparse = false;  // sparse steps (symmetric relabelled SELL): only rows next to fallen rows
+  BfSparse sp{};
   std::vector<void*> allocs;
   int64_t steps = 0;
 };
@@ -290,6 +292,14 @@ static int make_engine(gg_server* s, const gg_matrix* A, int kind, std::unique_p
     GG_TRY(engine_alloc(s, *e, 4 * words, &p)); e->chn = (uint32_t*)p;
     hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(words, 256)), dim3(256), 0,
                        e->stream, e->chp, words, 0u);
+    if (bf_sparse_ok(A)) {
+      const size_t sb = bf_sparse_bytes(A);
+      GG_TRY(engine_alloc(s, *e, sb, &p));
+      hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for((int64_t)(sb / 4), 256)),
+                         dim3(256), 0, e->stream, (uint32_t*)p, (int64_t)(sb / 4), 0u);
+      e->sp = bf_sparse_layout(p, A, 16.0);
+      e->sparse = true;
+    }
   }
   GG_HIP(hipGetLastError());
   GG_HIP(hipStreamSynchronize(e->stream));
@@ -432,8 +442,8 @@ static int launch_step(gg_server* s, Engine& e) {
     const int64_t words = (ns + 31) / 32;
     hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(words, 256)), dim3(256), 0,
                        e.stream, e.chn, words, 0u);
-    GG_TRY(bf_multi_step(e.stream, &e.A, e.dist_dtype, 32, e.buf, e.chp, e.chn, e.cnt_d,
-                         e.mask_d));
+    GG_TRY(bf_multi_step_sp(e.stream, &e.A, e.dist_dtype, 32, e.buf, e.chp, e.chn, e.cnt_d,
+                            e.mask_d, nullptr, e.sparse ? &e.sp : nullptr, s->num_cus));
     std::swap(e.chp, e.chn);
   }
   GG_HIP(hipGetLastError());

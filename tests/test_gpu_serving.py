@@ -19,9 +19,9 @@ def graphs():
     R = gg.ROCBackend(0)
     # weights 1..100: SSSP is min-plus, BFS follows the or-and edges (odd values)
     g0 = GEN.random_graph(2048, 8, seed=31, dtype=np.int32, values=GEN.INTEGER, wmin=1, wmax=100)
-    g1 = GEN.grid(30, 40, 0.01, 1, 50, seed=5, dtype=np.int32)
+    g1 = GEN.grid(30, 40, 0.01, 1, 50, seed=5, dtype=np.int32)   # relabelled: sparse SSSP steps
     g2 = GEN.rmat(13, seed=2)
-    g3 = GEN.rmat(12, seed=3)   # symmetric unit pattern, relabelled: top-down BFS steps
+    g3 = GEN.rmat(12, seed=3)   # symmetric pattern, relabelled: sparse SSSP steps
     mats = [g0.to_csr(R), g1.to_sell(R, 64, 1, 8, 64, relabel=True), g2.to_sell(R, 64, 1, 64, 256),
             g3.to_sell(R, 64, 1, 128, 256, relabel=True)]
     return gg, torch, mats
