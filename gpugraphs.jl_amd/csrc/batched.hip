@@ -48,8 +48,10 @@ __device__ __forceinline__ bool finish_row(const MsArgs& a, int32_t level, int64
   if (any) {
     if (claim_any) bit_claim(a.next_any, rid);
     if (a.dist8) {
-      // byte levels: one 8-byte read-modify-write per group of 8 sources with
-      // a new bit (this thread is the row's only writer in the level)
+      // byte levels: one 8-byte write per group of 8 sources with a new bit
+      // (this thread is the row's only writer in the level), read first only
+      // when a source of the group was reached before: the bytes of sources
+      // not reached yet still hold the 255 fill
       const uint64_t lv = (uint64_t)(uint8_t)level;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
@@ -57,7 +59,7 @@ __device__ __forceinline__ bool finish_row(const MsArgs& a, int32_t level, int64
           const uint32_t bits = (uint32_t)(nw[w] >> (8 * gq)) & 0xffu;
           if (!bits) continue;
           uint64_t* p = (uint64_t*)(a.dist8 + row * a.k8 + w * 64 + gq * 8);
-          uint64_t v = *p;
+          uint64_t v = ((sn[w] >> (8 * gq)) & 0xffu) ? *p : ~0ull;
           for (int t = 0; t < 8; ++t)
             if ((bits >> t) & 1u) v = (v & ~(0xffull << (8 * t))) | (lv << (8 * t));
           *p = v;
