@@ -1,7 +1,10 @@
 // serve.hip -- continuous-batching BFS / SSSP query server over HBM-resident
 // graphs (gg_server_*).
 //
-// Each registered graph owns two engines, each on its own HIP stream:
+// Each registered graph owns two engines.  Engines of large graphs (at least
+// kOwnStreamNnz entries) run on HIP streams of their own; those of small
+// graphs share one stream, so their short steps do not queue behind the long
+// steps of a large graph on a shared hardware queue (the device has few):
 //  - BFS: 64 slots of a bitmask traversal (the or-and SpMM step of
 //    gg_msbfs_distances).  A query joins a free slot between two levels; its
 //    source bit starts at the current level and its distances are counted from
@@ -172,6 +175,9 @@ struct Query {
   bool polled = false;
 };
 
+// Graphs with at least this many entries get streams of their own.
+constexpr int64_t kOwnStreamNnz = int64_t(1) << 24;
+
 struct Engine {
   int kind;  // GG_QUERY_BFS / GG_QUERY_SSSP
   int slots;
@@ -180,6 +186,7 @@ struct Engine {
   int64_t ns;                   // rows of the traversal state
   const int32_t* vinv = nullptr;  // original -> internal id + 1 (relabelled SELL)
   hipStream_t stream = nullptr;
+  bool own_stream = false;
   hipEvent_t step_done = nullptr;
   bool inflight = false;
   uint64_t active = 0, stepped = 0;
@@ -224,6 +231,7 @@ struct gg_server {
   int64_t next_id = 1;
   int64_t completed = 0, failed = 0;
   hipStream_t fetch_stream = nullptr;
+  hipStream_t small_stream = nullptr;  // shared by the engines of small graphs
   int num_cus = 256;
   std::string worker_error;
 };
@@ -248,7 +256,12 @@ static int make_engine(gg_server* s, const gg_matrix* A, int kind, std::unique_p
   e->slots = kind == GG_QUERY_BFS ? 64 : 32;
   e->slot_q.assign(e->slots, -1);
   const int64_t m = A->m, ns = e->ns;
-  GG_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  if (A->nnz >= kOwnStreamNnz) {
+    GG_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    e->own_stream = true;
+  } else {
+    e->stream = s->small_stream;
+  }
   GG_HIP(hipEventCreateWithFlags(&e->step_done, hipEventDisableTiming));
   GG_HIP(hipHostMalloc((void**)&e->mask_h, 64, hipHostMallocDefault));
   void* p;
@@ -313,7 +326,7 @@ static void free_engine(Engine& e) {
   e.allocs.clear();
   if (e.mask_h) (void)hipHostFree(e.mask_h);
   if (e.step_done) (void)hipEventDestroy(e.step_done);
-  if (e.stream) (void)hipStreamDestroy(e.stream);
+  if (e.stream && e.own_stream) (void)hipStreamDestroy(e.stream);
 }
 
 static hipEvent_t take_event(gg_server* s) {
@@ -535,7 +548,9 @@ int gg_server_create(int device, size_t budget_bytes, gg_server** out) {
     }
   }
   s->arena.init(budget_bytes);
-  if (hipStreamCreateWithFlags(&s->fetch_stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&s->fetch_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->small_stream, hipStreamNonBlocking) != hipSuccess) {
+    if (s->fetch_stream) (void)hipStreamDestroy(s->fetch_stream);
     if (s->arena_base) (void)hipFree(s->arena_base);
     delete s;
     return set_error(GG_ERR_HIP, "hipStreamCreate failed");
@@ -722,6 +737,7 @@ int gg_server_destroy(gg_server* s) {
     if (kv.second.ready) (void)hipEventDestroy(kv.second.ready);
   for (hipEvent_t ev : s->event_pool) (void)hipEventDestroy(ev);
   if (s->fetch_stream) (void)hipStreamDestroy(s->fetch_stream);
+  if (s->small_stream) (void)hipStreamDestroy(s->small_stream);
   if (s->arena_base) (void)hipFree(s->arena_base);
   delete s;
   return GG_OK;
