@@ -6,6 +6,7 @@ arrays, sorted, deduplicated) that converts to the storage types.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -108,6 +109,35 @@ def random_graph(n, degree=16, seed=1, dtype=np.bool_, values=UNIT, wmin=1, wmax
                                     C.byref(out)), "random_graph")
     H = _take(out)
     H.symmetric = True   # both directions of every edge
+    return H
+
+
+def read_matrix_market(path, dtype=None, transpose=False) -> HostCSR:
+    """MatrixMarket coordinate file (real / integer / pattern; general,
+    symmetric or skew-symmetric) as a HostCSR: the stored matrix, or its
+    transpose; duplicates add up; symmetric files come back with both
+    triangles and `symmetric` set.  dtype None keeps the file's type (pattern
+    -> bool, integer -> int64, real -> float64) (gg_read_matrix_market)."""
+    out = C.POINTER(GGCsrHost)()
+    sym = C.c_int(0)
+    code = 0 if dtype is None else dtype_code(dtype)
+    check(lib().gg_read_matrix_market(os.fsencode(path), code, int(bool(transpose)), C.byref(out),
+                                      C.byref(sym)), "read_matrix_market")
+    H = _take(out)
+    H.symmetric = bool(sym.value)
+    return H
+
+
+def read_edge_list(path, base=1, directed=False, dtype=np.bool_) -> HostCSR:
+    """Edge list ("u v" or "u v w" per line, '#' / '%' comments, vertices
+    counted from `base`) as the HostCSR of A_T (row v lists the tails of the
+    edges into v); undirected lists get both directions and `symmetric`; a
+    repeated edge keeps its smallest weight (gg_read_edge_list)."""
+    out = C.POINTER(GGCsrHost)()
+    check(lib().gg_read_edge_list(os.fsencode(path), int(base), int(bool(directed)),
+                                  dtype_code(dtype), C.byref(out)), "read_edge_list")
+    H = _take(out)
+    H.symmetric = not directed
     return H
 
 

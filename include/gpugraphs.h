@@ -275,6 +275,24 @@ int gg_gen_uniform_rows(int64_t n, int64_t per_row, uint64_t seed, int val_dtype
  * values: 0 unit, 1 U[0,1), 2 integer weights in [wmin, wmax]. */
 int gg_gen_random_graph(int64_t n, int64_t degree, uint64_t seed, int val_dtype, int values,
                         int32_t wmin, int32_t wmax, gg_csr_host** out);
+/* MatrixMarket coordinate file (real, integer or pattern; general, symmetric
+ * or skew-symmetric) -> CSR of the stored matrix, or of its transpose when
+ * transpose != 0; duplicates add up (pattern: kept once); a symmetric file is
+ * expanded to both triangles.  val_dtype 0 keeps the file's type (pattern ->
+ * Bool, integer -> Int64, real -> Float64); *symmetric (optional) is set to 1
+ * for a symmetric or skew-symmetric file.  The reference reads SuiteSparse
+ * matrices from the network (benchmark/benchmarks.jl:227-231,262-263); this
+ * reads the same matrices from local files. */
+int gg_read_matrix_market(const char* path, int val_dtype, int transpose, gg_csr_host** out,
+                          int* symmetric);
+/* Edge list: one "u v" or "u v w" per line ('#' or '%' starts a comment),
+ * vertices counted from `base` (0 or 1).  Output is A_T (row v lists the
+ * tails u of the edges u -> v, the generators' convention); directed == 0
+ * adds v -> u for every edge; a repeated edge keeps its smallest weight;
+ * unweighted edges have value 1 (Bool true).  No counterpart in the
+ * reference. */
+int gg_read_edge_list(const char* path, int base, int directed, int val_dtype,
+                      gg_csr_host** out);
 /* Block-diagonal packing of n_graphs square CSRs (A_T each, one value type):
  * graph g owns rows and columns offsets[g] .. offsets[g+1]-1 (0-based;
  * offsets has n_graphs+1 entries), each block padded with isolated vertices to

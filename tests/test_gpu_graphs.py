@@ -126,3 +126,32 @@ def test_graph_shortest_path_negative_cycle(env):
         L.set_graph_steps(ctx, steps)
         with pytest.raises(gg.NotConvergedError):
             gg.shortest_path(A, 1)
+
+
+def test_edge_list_and_matrix_market_inputs(env, tmp_path):
+    """Graphs read from files (an undirected weighted edge list, a symmetric
+    MatrixMarket pattern) go through the relabelled SELL: BFS levels and
+    Bellman-Ford distances equal the oracle's on the same CSR."""
+    import scipy.io
+    gg, torch, orc, ctx, _lib = env
+    from gpugraphs import generators as GEN
+    rng = np.random.default_rng(11)
+    n, e = 3000, 12000
+    u, v, w = rng.integers(1, n + 1, e), rng.integers(1, n + 1, e), rng.integers(1, 100, e)
+    p = tmp_path / "g.txt"
+    p.write_text("".join(f"{a} {b} {c}\n" for a, b, c in zip(u, v, w)))
+    H = GEN.read_edge_list(str(p), base=1, directed=False, dtype=np.int32)
+    S = sp.random(n, n, density=0.002, format="coo", random_state=rng)
+    q = str(tmp_path / "m.mtx")
+    scipy.io.mmwrite(q, sp.triu(S).tocoo(), field="pattern", symmetry="symmetric")
+    M = GEN.read_matrix_market(q)
+    for name, X in (("edge list", H), ("matrix market", M)):
+        assert X.symmetric
+        A = X.to_sell(gg.ROCBackend(0), 64, 1, 64, 128, relabel=True)
+        Ao = orc.from_scipy_csr(X.to_scipy())
+        src = int(np.nonzero(X.degrees())[0][0]) + 1
+        print_case("file input", name, X.m, X.nnz)
+        np.testing.assert_array_equal(host(gg.bfs_distances(A, src)), orc.bfs_distances(Ao, src))
+        if X.nzval.dtype != np.bool_:
+            np.testing.assert_array_equal(host(gg.shortest_path(A, src)),
+                                          orc.shortest_path(Ao, src))

@@ -380,3 +380,102 @@ def test_autotune_candidates_and_signature():
     with pytest.raises(gg.ArgumentError):
         gg.lib()  # library present
         AT.Autotuner(CPU).tune_layout(R, "nope")
+
+
+def _csr_of(H):
+    d = H.nzval.view(np.bool_) if H.nzval.dtype == np.uint8 else H.nzval
+    return sp.csr_matrix((d, H.colval - 1, H.rowptr - 1), shape=(H.m, H.n))
+
+
+@pytest.mark.parametrize("field,symmetry", [("real", "general"), ("integer", "general"),
+                                            ("pattern", "general"), ("real", "symmetric"),
+                                            ("integer", "skew-symmetric"),
+                                            ("pattern", "symmetric")])
+def test_read_matrix_market_matches_scipy(tmp_path, field, symmetry):
+    """The MatrixMarket reader against scipy.io.mmread on the same file
+    (scipy writes it): same pattern, values and type, both triangles for
+    symmetric files, transpose on request."""
+    import scipy.io
+    rng = np.random.default_rng(hash((field, symmetry)) % 2**32)
+    n = 40
+    S = sp.random(n, n, density=0.15, format="coo", random_state=rng)
+    if field == "integer":
+        S.data = rng.integers(-50, 50, S.nnz).astype(np.float64)
+        S.data[S.data == 0] = 7
+    if symmetry == "symmetric":
+        S = sp.triu(S) + sp.triu(S, 1).T
+    elif symmetry == "skew-symmetric":
+        S = sp.triu(S, 1) - sp.triu(S, 1).T
+    path = str(tmp_path / "m.mtx")
+    scipy.io.mmwrite(path, S.tocoo(), field=field, symmetry=symmetry)
+    ref = scipy.io.mmread(path).tocsr()
+    ref.sum_duplicates()
+    ref.sort_indices()
+    H = G.read_matrix_market(path)
+    assert H.symmetric == (symmetry != "general")
+    want = {"real": np.float64, "integer": np.int64, "pattern": np.bool_}[field]
+    got = _csr_of(H)
+    assert got.dtype == want
+    assert (got != ref.astype(want)).nnz == 0
+    np.testing.assert_array_equal(H.rowptr - 1, ref.indptr)
+    np.testing.assert_array_equal(H.colval - 1, ref.indices)
+    Ht = G.read_matrix_market(path, dtype=np.float32, transpose=True)
+    assert Ht.nzval.dtype == np.float32
+    assert abs(_csr_of(Ht) - ref.T.astype(np.float32)).max() == 0
+
+
+def test_read_matrix_market_duplicates_and_errors(tmp_path):
+    p = tmp_path / "d.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate real general\n% comment\n3 4 4\n"
+                 "1 2 1.5\n3 4 2\n1 2 0.25\n2 1 -1\n")
+    H = G.read_matrix_market(str(p), dtype=np.float32)
+    np.testing.assert_array_equal(_csr_of(H).toarray(),
+                                  [[0, 1.75, 0, 0], [-1, 0, 0, 0], [0, 0, 0, 2]])
+    bad = tmp_path / "b.mtx"
+    bad.write_text("%%MatrixMarket matrix coordinate real general\n2 2 1\n3 1 1.0\n")
+    with pytest.raises(gg.BoundsError):
+        G.read_matrix_market(str(bad))
+    short = tmp_path / "s.mtx"
+    short.write_text("%%MatrixMarket matrix coordinate pattern general\n2 2 3\n1 1\n2 2\n")
+    with pytest.raises(gg.ArgumentError):
+        G.read_matrix_market(str(short))
+    frac = tmp_path / "f.mtx"
+    frac.write_text("%%MatrixMarket matrix coordinate real general\n2 2 1\n1 1 0.5\n")
+    with pytest.raises(gg.ArgumentError):
+        G.read_matrix_market(str(frac), dtype=np.int32)
+    with pytest.raises(gg.ArgumentError):
+        G.read_matrix_market(str(tmp_path / "missing.mtx"))
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_read_edge_list_builds_transpose(tmp_path, directed):
+    """Edge list -> A_T (row v lists the tails of the edges into v): the same
+    matrix as a COO build of the edges, smallest weight kept on repeats."""
+    rng = np.random.default_rng(3 + directed)
+    n, e = 50, 300
+    u = rng.integers(0, n, e)
+    v = rng.integers(0, n, e)
+    w = rng.integers(1, 1000, e)
+    p = tmp_path / "g.txt"
+    p.write_text("# u v w\n" + "".join(f"{a} {b} {c}\n" for a, b, c in zip(u, v, w)))
+    H = G.read_edge_list(str(p), base=0, directed=directed, dtype=np.int32)
+    assert H.symmetric == (not directed)
+    best = {}
+    for a, b, c in zip(u, v, w):
+        pairs = [(b, a)] if directed or a == b else [(b, a), (a, b)]
+        for r, col in pairs:
+            best[(r, col)] = min(best.get((r, col), 1 << 30), c)
+    m = max(int(u.max()), int(v.max())) + 1
+    ref = np.zeros((m, m), np.int64)
+    for (r, col), c in best.items():
+        ref[r, col] = c
+    assert H.m == H.n == m
+    np.testing.assert_array_equal(_csr_of(H).toarray(), ref)
+    q = tmp_path / "g1.txt"
+    q.write_text("1 2\n2 3\n% comment\n3 1\n")
+    B = G.read_edge_list(str(q))
+    assert B.nzval.dtype == np.bool_ and B.m == 3 and B.nnz == 6
+    r = tmp_path / "g2.txt"
+    r.write_text("1 2 5\n2 3\n")
+    with pytest.raises(gg.ArgumentError):
+        G.read_edge_list(str(r), dtype=np.int32)
