@@ -145,6 +145,47 @@ __global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_
 }
 
 // -------------------------------------------------- k sources, row-major set
+// min over the changed in-neighbours c of a SELL slot of safe_add(w, d[c, j]):
+// entries in batches of 4 (index loads, bitmap tests, then the value and
+// distance loads), so a lane keeps four independent loads in flight.
+template <class TA, class TD>
+__device__ __forceinline__ TD multi_row_min(const SellView& g, const TA* __restrict__ val,
+                                            int64_t base, int64_t len, bool has_len,
+                                            const uint32_t* __restrict__ chp, const TD* d,
+                                            int64_t k, int64_t j) {
+  TD acc = Lim<TD>::max();
+  const bool jin = j < k;
+  int64_t kk = 0;
+  if (has_len) {
+    for (; kk + 4 <= len; kk += 4) {
+      int32_t c[4];
+      bool on[4];
+      TA w[4];
+      TD dn[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) c[u] = g.colval[base + (kk + u) * g.C] - 1;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) on[u] = jin && bit_get(chp, c[u]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (on[u]) {
+          w[u] = val[base + (kk + u) * g.C];
+          dn[u] = d[(int64_t)c[u] * k + j];
+        }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (on[u]) acc = jmin(acc, sadd(TD(w[u]), dn[u]));
+    }
+  }
+  for (; kk < len; ++kk) {
+    const int64_t i = base + kk * g.C;
+    const int32_t col = g.colval[i];
+    if (!has_len && col == -1) break;
+    if (jin && bit_get(chp, col - 1)) acc = jmin(acc, sadd(TD(val[i]), d[(int64_t)(col - 1) * k + j]));
+  }
+  return acc;
+}
+
 // Rows of A_T as (entry list) for either format: CSR row, SELL slot, or a
 // heavy row.  A group of Lk lanes (one per source, Lk >= k when k <= 64)
 // walks the row; the bitmap test is uniform across a group (one column).
@@ -190,13 +231,7 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
         bool has_len;
         if (sell_slot(g, unit, row, base, len, has_len)) {
           rid = g.internal ? unit : row;
-          for (int64_t kk = 0; kk < len; ++kk) {
-            const int64_t i = base + kk * g.C;
-            const int32_t col = g.colval[i];
-            if (!has_len && col == -1) break;
-            if (bit_get(chp, col - 1) && j < k)
-              acc = jmin(acc, sadd(TD(val[i]), d[(int64_t)(col - 1) * k + j]));
-          }
+          acc = multi_row_min<TA, TD>(g, val, base, len, has_len, chp, d, k, j);
         }
       }
       if (rid >= 0 && j < k) {
@@ -370,7 +405,27 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_multi_kernel(BfSparse s, SellVi
       if (j < k) {
         const StateRow r = state_row(g, false, nullptr, v);
         TD acc = Lim<TD>::max();
-        for (int64_t kk = 0; kk < r.len; ++kk) {
+        int64_t kk = 0;
+        for (; kk + 4 <= r.len; kk += 4) {  // batches of 4, as multi_row_min
+          int32_t c[4];
+          bool on[4];
+          TA w[4];
+          TD dn[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) c[u] = r.col[r.base + (kk + u) * r.stride] - 1;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) on[u] = bit_get(chp, c[u]);
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (on[u]) {
+              w[u] = val[r.base + (kk + u) * r.stride];
+              dn[u] = d[(int64_t)c[u] * k + j];
+            }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (on[u]) acc = jmin(acc, sadd(TD(w[u]), dn[u]));
+        }
+        for (; kk < r.len; ++kk) {
           const int64_t e = r.base + kk * r.stride;
           const int32_t c = r.col[e] - 1;
           if (bit_get(chp, c)) acc = jmin(acc, sadd(TD(val[e]), d[(int64_t)c * k + j]));
