@@ -293,6 +293,103 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
   if (threadIdx.x == 0 && s_any) bit_claim(chn, heavy_rid(g, h));
 }
 
+// Light SELL slots with V columns per lane (V = 16 / sizeof(TD), k a multiple
+// of V, d 16-byte aligned): a group of Lv lanes per row, each reading a
+// neighbour's V distances as one 16-byte load, so a wavefront covers 64 / Lv
+// rows.  Same relaxation as bf_multi_kernel's light part.
+template <class TA, class TD, int V>
+__global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_light, int64_t k,
+                                                          int logLv, TD* d,
+                                                          const uint32_t* __restrict__ chp,
+                                                          uint32_t* __restrict__ chn,
+                                                          unsigned int* changed,
+                                                          unsigned long long* changed_mask,
+                                                          const int32_t* done,
+                                                          const unsigned int* light_off) {
+  typedef TD VT __attribute__((ext_vector_type(V)));
+  if (done && *done) return;
+  if (light_off && *light_off) return;
+  const int Lv = 1 << logLv;
+  const int wl = threadIdx.x & 63;
+  const int lane = wl & (Lv - 1);
+  const int gbase = wl & ~(Lv - 1);
+  const int64_t kq = k / V;
+  const int64_t jblocks = (kq + Lv - 1) >> logLv;
+  const int64_t gpb = kT >> logLv;
+  const int64_t q = (int64_t)blockIdx.x * gpb + (threadIdx.x >> logLv);
+  const int64_t unit = q / jblocks;
+  const int64_t jq = (q - unit * jblocks) * Lv + lane;
+  const bool jin = jq < kq;
+  const int64_t j0 = jq * V;
+  unsigned chbits = 0;
+  int64_t rid = -1;
+  if (unit < n_light) {
+    int64_t row, base, len;
+    bool has_len;
+    if (sell_slot(g, unit, row, base, len, has_len)) {
+      rid = g.internal ? unit : row;
+      if (jin) {
+        const TA* val = (const TA*)g.val;
+        VT acc;
+#pragma unroll
+        for (int t = 0; t < V; ++t) acc[t] = Lim<TD>::max();
+        int64_t kk = 0;
+        if (has_len) {
+          for (; kk + 4 <= len; kk += 4) {
+            int32_t c[4];
+            bool on[4];
+            TA w[4];
+            VT dn[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) c[u] = g.colval[base + (kk + u) * g.C] - 1;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) on[u] = bit_get(chp, c[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (on[u]) {
+                w[u] = val[base + (kk + u) * g.C];
+                dn[u] = *(const VT*)&d[(int64_t)c[u] * k + j0];
+              }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (on[u])
+#pragma unroll
+                for (int t = 0; t < V; ++t) acc[t] = jmin(acc[t], sadd(TD(w[u]), dn[u][t]));
+          }
+        }
+        for (; kk < len; ++kk) {
+          const int64_t i = base + kk * g.C;
+          const int32_t col = g.colval[i];
+          if (!has_len && col == -1) break;
+          if (!bit_get(chp, col - 1)) continue;
+          const TD w = TD(val[i]);
+          const VT dn = *(const VT*)&d[(int64_t)(col - 1) * k + j0];
+#pragma unroll
+          for (int t = 0; t < V; ++t) acc[t] = jmin(acc[t], sadd(w, dn[t]));
+        }
+        VT own = *(const VT*)&d[rid * k + j0];
+#pragma unroll
+        for (int t = 0; t < V; ++t)
+          if (acc[t] < own[t]) {
+            own[t] = acc[t];
+            chbits |= 1u << t;
+          }
+        if (chbits) *(VT*)&d[rid * k + j0] = own;
+      }
+    }
+  }
+  const bool ch = chbits != 0;
+  wave_flag(changed, ch);
+  const unsigned long long b = __ballot(ch);
+  const unsigned long long low = Lv == 64 ? ~0ull : ((1ull << Lv) - 1ull);
+  if (lane == 0 && ((b >> gbase) & low)) bit_claim(chn, rid);  // this group's row fell
+  if (changed_mask) {  // k <= 64: bit j of the mask is column j
+    unsigned long long cm = ch ? (unsigned long long)chbits << j0 : 0ull;
+    for (int o = 32; o > 0; o >>= 1) cm |= __shfl_xor(cm, o, 64);
+    if (wl == 0 && cm) or_new_bits(changed_mask, cm);
+  }
+}
+
 // ----------------------------------------------------- sparse iterations
 // On the symmetric pattern of a relabelled SELL, row u of A_T also lists u's
 // out-neighbours.  A k-source iteration with few fallen rows (fallen * alpha <
@@ -707,10 +804,30 @@ int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int
     const int64_t nb = light_blocks + g.n_seg * parts * jblocks;
     if (nb > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
     if (sp) bf_sparse_prologue(stream, *sp, g, chp, done, cus);
-    if (nb > 0)
+    constexpr int V = 16 / (int)sizeof(TD);
+    if (A->format == GG_SELL && k % V == 0 && ((uintptr_t)d & 15) == 0) {
+      // light slots V columns per lane; bf_multi_kernel keeps the heavy rows
+      int lv = 0;
+      while (lv < 6 && (1LL << lv) < k / V) ++lv;
+      const int64_t jbv = (k / V + (1LL << lv) - 1) >> lv;
+      const int64_t gpbv = kT >> lv;
+      const int64_t lbv = (n_light * jbv + gpbv - 1) / gpbv;
+      if (lbv > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
+      if (lbv > 0)
+        hipLaunchKernelGGL((bf_multi_vec_kernel<TA, TD, V>), dim3((unsigned)lbv), dim3(kT), 0,
+                           stream, g, n_light, k, lv, (TD*)d, chp, chn, changed, changed_mask,
+                           done, sp ? &sp->cnt[4] : nullptr);
+      const int64_t nh = g.n_seg * parts * jblocks;
+      if (nh > 0)
+        hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nh), dim3(kT),
+                           sizeof(TD) * kT, stream, A->format, A->ptr, g, (int64_t)0, parts, k,
+                           lg, (TD*)d, chp, chn, changed, changed_mask, done,
+                           sp ? &sp->cnt[4] : nullptr);
+    } else if (nb > 0) {
       hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
                          stream, A->format, A->ptr, g, n_light, parts, k, lg, (TD*)d, chp, chn,
                          changed, changed_mask, done, sp ? &sp->cnt[4] : nullptr);
+    }
     if (sp) {
       hipLaunchKernelGGL((bf_sp_rows_multi_kernel<TA, TD>), dim3((unsigned)std::max(cus, 1) * 8),
                          dim3(kT), 0, stream, *sp, g, k, lg, (TD*)d, chp, chn, changed,

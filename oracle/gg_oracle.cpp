@@ -726,6 +726,7 @@ int orc_shortest_path_multi(const orc_matrix* A, const int32_t* sources, int64_t
     if (sources[i] < 1 || sources[i] > A->m) return S_BOUNDS;
   switch (dist_dtype) {
     case D_I32: return mssp_typed<int32_t>(*A, sources, k, (int32_t*)dist, nthreads, max_iter, iters);
+    case D_I64: return mssp_typed<int64_t>(*A, sources, k, (int64_t*)dist, nthreads, max_iter, iters);
     case D_F32: return mssp_typed<float>(*A, sources, k, (float*)dist, nthreads, max_iter, iters);
     case D_F64: return mssp_typed<double>(*A, sources, k, (double*)dist, nthreads, max_iter, iters);
   }

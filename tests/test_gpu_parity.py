@@ -596,3 +596,24 @@ def test_shortest_path_sparse_iterations(env, steps):
                                           orc.shortest_path_multi(Ao, srcs))
     finally:
         _lib.set_graph_steps(ctx, 0)
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.float32, np.int64, np.float64])
+@pytest.mark.parametrize("k", [2, 8, 12, 64, 72])
+def test_shortest_path_multi_column_vectors(env, dt, k):
+    """k sources with 16-byte column groups per lane (k a multiple of 4 for
+    32-bit distances, of 2 for 64-bit ones) on SELL layouts with heavy rows,
+    and the one-column-per-lane path otherwise: distances equal the oracle's."""
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    rng = np.random.default_rng(k * 7 + np.dtype(dt).itemsize)
+    R = GEN.rmat(11, seed=k)
+    w = rng.integers(1, 200, R.nnz).astype(dt)
+    H = GEN.HostCSR(R.m, R.n, R.rowptr, R.colval, w, True)
+    Ao = orc.from_scipy_csr(H.to_scipy())
+    srcs = (rng.choice(np.nonzero(H.degrees())[0], k, replace=True) + 1).astype(np.int32)
+    ref = orc.shortest_path_multi(Ao, srcs)
+    for A in (H.to_sell(gg.ROCBackend(0), 64, 1, 64, 128, relabel=True),
+              H.to_sell(gg.ROCBackend(0), 64, 256, 64, 128)):
+        print_case("sssp column vectors", np.dtype(dt).name, k, A.vinv is not None)
+        np.testing.assert_array_equal(host(gg.shortest_path(A, srcs)), ref)

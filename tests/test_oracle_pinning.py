@@ -298,18 +298,25 @@ def test_shortest_path_bool_is_hops(oracle):
         np.testing.assert_array_equal(d, oracle.bfs_distances(A, 1))
 
 
+@pytest.mark.parametrize("dt", [np.int32, np.int64, np.float32, np.float64])
 @pytest.mark.parametrize("fmt", ["csr", "sell"])
-def test_shortest_path_multi_all_sources(oracle, fmt):
-    """test/shortest_path.jl:82-126: all n sources vs per-source Dijkstra."""
+def test_shortest_path_multi_all_sources(oracle, fmt, dt):
+    """test/shortest_path.jl:82-126: all n sources vs per-source Dijkstra, for
+    each distance type (unreached: typemax for integers, Inf for floats)."""
     rng = np.random.default_rng(4)
-    Adj = G.random_digraph(rng, 40, 0.08, wmax=10, dtype=np.int32)
+    Adj = G.random_digraph(rng, 40, 0.08, wmax=10, dtype=np.int32).astype(dt)
     A = oracle.from_scipy_csr(Adj.T.tocsr())
     if fmt == "sell":
         A = oracle.sell_from_csr(A, 32)
     D = oracle.shortest_path_multi(A, np.arange(1, 41))
-    ref = csgraph.dijkstra(Adj.astype(np.float64), directed=True)
-    ref = np.where(np.isinf(ref), INT32_MAX, ref).astype(np.int32).T  # column i = source i
-    np.testing.assert_array_equal(D, ref)
+    assert D.dtype == dt
+    ref = csgraph.dijkstra(Adj.astype(np.float64), directed=True).T  # column i = source i
+    if np.issubdtype(dt, np.integer):
+        fin = ~np.isinf(ref)
+        want = np.full(ref.shape, np.iinfo(dt).max, dt)
+        want[fin] = ref[fin].astype(dt)
+        ref = want
+    np.testing.assert_array_equal(D, ref.astype(dt))
 
 
 def test_spmspv_vs_fp64_and_mask(oracle):
