@@ -390,6 +390,101 @@ __global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_
   }
 }
 
+// Heavy rows with V columns per lane: a block per (part of kHeavyVecPart
+// entries of a heavy segment, column block); its groups of Lv lanes split the
+// part's entries (batches of 4 per group), fold through LDS and lower the row
+// with per-column atomic mins.
+constexpr int64_t kHeavyVecPart = 1024;
+
+template <class TA, class TD, int V>
+__global__ __launch_bounds__(kT) void bf_multi_heavy_vec_kernel(SellView g, int64_t parts,
+                                                                int64_t k, int logLv, TD* d,
+                                                                const uint32_t* __restrict__ chp,
+                                                                uint32_t* __restrict__ chn,
+                                                                unsigned int* changed,
+                                                                unsigned long long* changed_mask,
+                                                                const int32_t* done) {
+  typedef TD VT __attribute__((ext_vector_type(V)));
+  __shared__ VT red[kT];
+  __shared__ int s_any;
+  if (done && *done) return;
+  const int Lv = 1 << logLv;
+  const int lane = threadIdx.x & (Lv - 1);
+  const int64_t grp = threadIdx.x >> logLv;
+  const int64_t gpb = kT >> logLv;
+  const int64_t kq = k / V;
+  const int64_t jblocks = (kq + Lv - 1) >> logLv;
+  const int64_t unit = (int64_t)blockIdx.x / jblocks;
+  const int64_t jq = ((int64_t)blockIdx.x - unit * jblocks) * Lv + lane;
+  const bool jin = jq < kq;
+  const int64_t j0 = jq * V;
+  int64_t h, s, e;
+  heavy_segment(g, unit / parts, h, s, e);
+  s += (unit % parts) * kHeavyVecPart;
+  e = min(e, s + kHeavyVecPart);
+  if (s >= e) return;  // a part past the end of the row's last segment (uniform)
+  const TA* hv = (const TA*)g.heavy_val;
+  VT acc;
+#pragma unroll
+  for (int t = 0; t < V; ++t) acc[t] = Lim<TD>::max();
+  if (threadIdx.x == 0) s_any = 0;
+  if (jin) {
+    int64_t i = s + grp;
+    for (; i + 3 * gpb < e; i += 4 * gpb) {
+      int32_t c[4];
+      bool on[4];
+      TA w[4];
+      VT dn[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) c[u] = g.heavy_col[i + u * gpb] - 1;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) on[u] = bit_get(chp, c[u]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (on[u]) {
+          w[u] = hv[i + u * gpb];
+          dn[u] = *(const VT*)&d[(int64_t)c[u] * k + j0];
+        }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (on[u])
+#pragma unroll
+          for (int t = 0; t < V; ++t) acc[t] = jmin(acc[t], sadd(TD(w[u]), dn[u][t]));
+    }
+    for (; i < e; i += gpb) {
+      const int32_t c = g.heavy_col[i] - 1;
+      if (!bit_get(chp, c)) continue;
+      const TD w = TD(hv[i]);
+      const VT dn = *(const VT*)&d[(int64_t)c * k + j0];
+#pragma unroll
+      for (int t = 0; t < V; ++t) acc[t] = jmin(acc[t], sadd(w, dn[t]));
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (grp == 0) {
+    for (int64_t o = 1; o < gpb; ++o) {
+      const VT r = red[o * Lv + lane];
+#pragma unroll
+      for (int t = 0; t < V; ++t) acc[t] = jmin(acc[t], r[t]);
+    }
+    if (jin) {
+      const int64_t rid = heavy_rid(g, h);
+      unsigned long long cm = 0;
+#pragma unroll
+      for (int t = 0; t < V; ++t)
+        if (atomic_min_lowered(&d[rid * k + j0 + t], acc[t])) cm |= 1ull << t;
+      if (cm) {
+        set_flag(changed);
+        if (changed_mask) or_new_bits(changed_mask, cm << j0);
+        s_any = 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_any) bit_claim(chn, heavy_rid(g, h));
+}
+
 // ----------------------------------------------------- sparse iterations
 // On the symmetric pattern of a relabelled SELL, row u of A_T also lists u's
 // out-neighbours.  A k-source iteration with few fallen rows (fallen * alpha <
@@ -817,12 +912,12 @@ int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int
         hipLaunchKernelGGL((bf_multi_vec_kernel<TA, TD, V>), dim3((unsigned)lbv), dim3(kT), 0,
                            stream, g, n_light, k, lv, (TD*)d, chp, chn, changed, changed_mask,
                            done, sp ? &sp->cnt[4] : nullptr);
-      const int64_t nh = g.n_seg * parts * jblocks;
+      const int64_t pv = g.n_seg > 0 ? (g.seg_len + kHeavyVecPart - 1) / kHeavyVecPart : 1;
+      const int64_t nh = g.n_seg * pv * jbv;
+      if (nh > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
       if (nh > 0)
-        hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nh), dim3(kT),
-                           sizeof(TD) * kT, stream, A->format, A->ptr, g, (int64_t)0, parts, k,
-                           lg, (TD*)d, chp, chn, changed, changed_mask, done,
-                           sp ? &sp->cnt[4] : nullptr);
+        hipLaunchKernelGGL((bf_multi_heavy_vec_kernel<TA, TD, V>), dim3((unsigned)nh), dim3(kT), 0,
+                           stream, g, pv, k, lv, (TD*)d, chp, chn, changed, changed_mask, done);
     } else if (nb > 0) {
       hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
                          stream, A->format, A->ptr, g, n_light, parts, k, lg, (TD*)d, chp, chn,

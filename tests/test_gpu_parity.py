@@ -614,6 +614,7 @@ def test_shortest_path_multi_column_vectors(env, dt, k):
     srcs = (rng.choice(np.nonzero(H.degrees())[0], k, replace=True) + 1).astype(np.int32)
     ref = orc.shortest_path_multi(Ao, srcs)
     for A in (H.to_sell(gg.ROCBackend(0), 64, 1, 64, 128, relabel=True),
+              H.to_sell(gg.ROCBackend(0), 64, 1, 32, 2048, relabel=True),
               H.to_sell(gg.ROCBackend(0), 64, 256, 64, 128)):
         print_case("sssp column vectors", np.dtype(dt).name, k, A.vinv is not None)
         np.testing.assert_array_equal(host(gg.shortest_path(A, srcs)), ref)
