@@ -644,6 +644,96 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_multi_kernel(BfSparse s, SellVi
   }
 }
 
+// The marked light rows with V columns per lane (as bf_multi_vec_kernel).
+template <class TA, class TD, int V>
+__global__ __launch_bounds__(kT) void bf_sp_rows_vec_kernel(BfSparse s, SellView g, int64_t k,
+                                                            int logLv, TD* d,
+                                                            const uint32_t* __restrict__ chp,
+                                                            uint32_t* __restrict__ chn,
+                                                            unsigned int* changed,
+                                                            unsigned long long* changed_mask,
+                                                            const int32_t* done) {
+  typedef TD VT __attribute__((ext_vector_type(V)));
+  if (done && *done) return;
+  if (!s.cnt[3]) return;
+  const int Lv = 1 << logLv;
+  const int wl = threadIdx.x & 63;
+  const int lane = wl & (Lv - 1);
+  const int gbase = wl & ~(Lv - 1);
+  const int64_t kq = k / V;
+  const int64_t jblocks = (kq + Lv - 1) >> logLv;
+  const int64_t gpb = kT >> logLv;
+  const int64_t ngroups = (int64_t)s.cnt[1] * jblocks;
+  const TA* val = (const TA*)g.val;
+  for (int64_t q0 = (int64_t)blockIdx.x * gpb; q0 < ngroups; q0 += (int64_t)gridDim.x * gpb) {
+    const int64_t q = q0 + (threadIdx.x >> logLv);
+    unsigned chbits = 0;
+    int64_t v = -1;
+    int64_t j0 = 0;
+    if (q < ngroups) {
+      const int64_t unit = q / jblocks;
+      const int64_t jq = (q - unit * jblocks) * Lv + lane;
+      j0 = jq * V;
+      v = s.dq[unit];
+      if (jq < kq) {
+        const StateRow r = state_row(g, false, nullptr, v);
+        VT acc;
+#pragma unroll
+        for (int t = 0; t < V; ++t) acc[t] = Lim<TD>::max();
+        int64_t kk = 0;
+        for (; kk + 4 <= r.len; kk += 4) {
+          int32_t c[4];
+          bool on[4];
+          TA w[4];
+          VT dn[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) c[u] = r.col[r.base + (kk + u) * r.stride] - 1;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) on[u] = bit_get(chp, c[u]);
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (on[u]) {
+              w[u] = val[r.base + (kk + u) * r.stride];
+              dn[u] = *(const VT*)&d[(int64_t)c[u] * k + j0];
+            }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (on[u])
+#pragma unroll
+              for (int t = 0; t < V; ++t) acc[t] = jmin(acc[t], sadd(TD(w[u]), dn[u][t]));
+        }
+        for (; kk < r.len; ++kk) {
+          const int64_t e = r.base + kk * r.stride;
+          const int32_t c = r.col[e] - 1;
+          if (!bit_get(chp, c)) continue;
+          const TD w = TD(val[e]);
+          const VT dn = *(const VT*)&d[(int64_t)c * k + j0];
+#pragma unroll
+          for (int t = 0; t < V; ++t) acc[t] = jmin(acc[t], sadd(w, dn[t]));
+        }
+        VT own = *(const VT*)&d[v * k + j0];
+#pragma unroll
+        for (int t = 0; t < V; ++t)
+          if (acc[t] < own[t]) {
+            own[t] = acc[t];
+            chbits |= 1u << t;
+          }
+        if (chbits) *(VT*)&d[v * k + j0] = own;
+      }
+    }
+    const bool ch = chbits != 0;
+    wave_flag(changed, ch);
+    const unsigned long long b = __ballot(ch);
+    const unsigned long long low = Lv == 64 ? ~0ull : ((1ull << Lv) - 1ull);
+    if (lane == 0 && ((b >> gbase) & low)) bit_claim(chn, v);
+    if (changed_mask) {  // k <= 64
+      unsigned long long cm = ch ? (unsigned long long)chbits << j0 : 0ull;
+      for (int o = 32; o > 0; o >>= 1) cm |= __shfl_xor(cm, o, 64);
+      if (wl == 0 && cm) or_new_bits(changed_mask, cm);
+    }
+  }
+}
+
 // Enqueues the sparse-iteration prologue: fallen-row edges, the switch, the
 // fallen queue and the marking of the rows to relax.
 static void bf_sparse_prologue(hipStream_t st, const BfSparse& s, const SellView& g,
@@ -902,10 +992,11 @@ int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int
     if (nb > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
     if (sp) bf_sparse_prologue(stream, *sp, g, chp, done, cus);
     constexpr int V = 16 / (int)sizeof(TD);
-    if (A->format == GG_SELL && k % V == 0 && ((uintptr_t)d & 15) == 0) {
-      // light slots V columns per lane; bf_multi_kernel keeps the heavy rows
-      int lv = 0;
-      while (lv < 6 && (1LL << lv) < k / V) ++lv;
+    const bool vec = A->format == GG_SELL && k % V == 0 && ((uintptr_t)d & 15) == 0;
+    int lv = 0;
+    while (lv < 6 && (1LL << lv) < k / V) ++lv;
+    if (vec) {
+      // light slots and heavy segments V columns per lane
       const int64_t jbv = (k / V + (1LL << lv) - 1) >> lv;
       const int64_t gpbv = kT >> lv;
       const int64_t lbv = (n_light * jbv + gpbv - 1) / gpbv;
@@ -925,10 +1016,16 @@ int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int
                          stream, A->format, A->ptr, g, n_light, parts, k, lg, (TD*)d, chp, chn,
                          changed, changed_mask, done, sp ? &sp->cnt[4] : nullptr);
     }
-    if (sp) {
+    if (sp && vec) {
+      hipLaunchKernelGGL((bf_sp_rows_vec_kernel<TA, TD, V>), dim3((unsigned)std::max(cus, 1) * 8),
+                         dim3(kT), 0, stream, *sp, g, k, lv, (TD*)d, chp, chn, changed,
+                         changed_mask, done);
+    } else if (sp) {
       hipLaunchKernelGGL((bf_sp_rows_multi_kernel<TA, TD>), dim3((unsigned)std::max(cus, 1) * 8),
                          dim3(kT), 0, stream, *sp, g, k, lg, (TD*)d, chp, chn, changed,
                          changed_mask, done);
+    }
+    if (sp) {
       hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(sp->words, kT)), dim3(kT), 0, stream,
                          sp->mark, sp->words, 0u);
     }
