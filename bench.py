@@ -360,14 +360,28 @@ def main():
     A_b = Rb.to_sell(R0, 64, sigma, args.heavy, args.bfs_seg, relabel)   # Bool pattern for BFS
     A_csr = Rb.to_csr(R0)                                          # out-edges for push (symmetric)
     torch.cuda.synchronize()
-    log(f"SELL builds + upload {time.time() - t:.1f}s; stored/nnz {(A_f.n_stored + (int(A_f.heavy_ptr[-1].item()) if A_f.n_heavy else 0)) / nnz:.3f}")
+    host_build_s = time.time() - t
+    log(f"SELL builds + upload {host_build_s:.1f}s; stored/nnz {(A_f.n_stored + (int(A_f.heavy_ptr[-1].item()) if A_f.n_heavy else 0)) / nnz:.3f}")
+    # the same BFS layout built on the device from the CSR already in HBM
+    gg.SparseGPUMatrixSELL.build_on_device(A_csr, 64, sigma, args.heavy, args.bfs_seg, relabel)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    A_d = gg.SparseGPUMatrixSELL.build_on_device(A_csr, 64, sigma, args.heavy, args.bfs_seg,
+                                                 relabel)
+    torch.cuda.synchronize()
+    dev_build_ms = (time.perf_counter() - t) * 1e3
+    assert torch.equal(A_d.colval, A_b.colval) and torch.equal(A_d.slice_ptr, A_b.slice_ptr), \
+        "device-built SELL differs from the host-built one"
+    del A_d
+    log(f"SELL build on the device from the resident CSR: {dev_build_ms:.1f} ms")
 
     rng = np.random.default_rng(seed)
     roots = rng.choice(np.nonzero(deg)[0], args.roots, replace=False).astype(np.int32) + 1
     ctx = _lib.context(0)
     _lib.set_graph_steps(ctx, args.graph_steps)
     _lib.set_push_threshold(ctx, args.push_alpha)
-    res = {}
+    res = {"sell_build": dict(host_two_layouts_s=round(host_build_s, 3),
+                              device_bfs_layout_ms=round(dev_build_ms, 2))}
 
     # ---------------------------------------------------------------- SpMV
     if args.only in ("", "spmv"):
@@ -568,6 +582,7 @@ def main():
         "do_qps": round(bfs["do_qps"], 1) if bfs else None,
         "do_p99_ms": round(bfs["do_p99_ms"], 3) if bfs else None,
         "batched_vs_plain": round(value / bfs["plain_qps"], 2) if (bfs and value) else None,
+        "sell_build": res["sell_build"],
         "wide_batch": {k: (round(v, 3) if isinstance(v, float) else v)
                        for k, v in res["msbfs_wide"].items()} if "msbfs_wide" in res else None,
         "gteps_plain": round(bfs["gteps_plain"], 3) if bfs else None,

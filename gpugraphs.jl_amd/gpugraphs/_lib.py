@@ -87,6 +87,13 @@ class GGSellHost(C.Structure):
     ]
 
 
+class GGSellSizes(C.Structure):
+    _fields_ = [("slice_size", C.c_int64), ("nslices", C.c_int64), ("n_stored", C.c_int64),
+                ("n_slots", C.c_int64), ("n_heavy", C.c_int64), ("heavy_nnz", C.c_int64),
+                ("n_seg", C.c_int64), ("n_internal", C.c_int64), ("has_perm", C.c_int32),
+                ("has_vinv", C.c_int32)]
+
+
 class GGCsrHost(C.Structure):
     _fields_ = [("m", C.c_int64), ("n", C.c_int64), ("nnz", C.c_int64),
                 ("val_dtype", C.c_int32), ("rowptr", C.c_void_p), ("colval", C.c_void_p),
@@ -133,6 +140,10 @@ SIGNATURES = {
     "gg_sell_build": ([_i64, _i64, _vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64, _i32,
                        C.POINTER(C.POINTER(GGSellHost))], _i32),
     "gg_sell_host_free": ([C.POINTER(GGSellHost)], None),
+    "gg_sell_plan_device": ([_vp, _i64, _i64, _vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64, _i32,
+                             C.POINTER(_vp), C.POINTER(GGSellSizes)], _i32),
+    "gg_sell_fill_device": ([_vp, _PM], _i32),
+    "gg_sell_plan_free": ([_vp], None),
     "gg_csr_host_free": ([C.POINTER(GGCsrHost)], None),
     "gg_gen_rmat": ([_i32, _i32, _d, _d, _d, _u64, _i32, _i32, _i32,
                      C.POINTER(C.POINTER(GGCsrHost))], _i32),

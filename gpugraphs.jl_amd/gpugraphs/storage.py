@@ -460,6 +460,74 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
             lib().gg_sell_host_free(out)
         self.unit = unit_pattern(nzval.view(np.bool_) if vdt == np.bool_ else nzval)
 
+    @classmethod
+    def build_on_device(cls, A, slice_size=32, sigma=1, heavy_threshold=0, heavy_seg_len=4096,
+                        relabel=False):
+        """The same layouts as the constructor, built in device memory from a
+        SparseGPUMatrixCSR on a ROCBackend (gg_sell_plan_device /
+        gg_sell_fill_device): the rows never visit the host."""
+        if not isinstance(A, SparseGPUMatrixCSR) or not isinstance(A.backend, ROCBackend):
+            raise ArgumentError("build_on_device takes a SparseGPUMatrixCSR on a ROCBackend")
+        torch = _torch()
+        backend = A.backend
+        ctx = _lib.context(backend.device)
+        vdt = np_dtype_of(A.nzval)
+        plan = C.c_void_p()
+        z = _lib.GGSellSizes()
+        check(lib().gg_sell_plan_device(ctx, A.m, A.n, ptr(A.rowptr), ptr(A.colval),
+                                        ptr(A.nzval), dtype_code(vdt), int(slice_size),
+                                        int(sigma), int(heavy_threshold), int(heavy_seg_len),
+                                        _lib.SELL_RELABEL if relabel else 0, C.byref(plan),
+                                        C.byref(z)), "SparseGPUMatrixSELL")
+        try:
+            dev = torch.device("cuda", backend.device)
+
+            def empty(count, dt):
+                return torch.empty(max(int(count), 1), dtype=_torch_dtype(dt), device=dev)[:int(count)]
+
+            M = cls.__new__(cls)
+            M.m, M.n, M.backend = int(A.m), int(A.n), backend
+            M.slice_size, M.nslices, M.n_stored = int(z.slice_size), int(z.nslices), int(z.n_stored)
+            M.sigma = int(sigma)
+            M.slice_ptr = empty(z.nslices + 1, np.int32)
+            M.colval = empty(z.n_stored, np.int32)
+            M.nzval = empty(z.n_stored, vdt)
+            perm_dev = empty(z.n_slots, np.int32) if z.has_perm else None
+            M.slot_len = empty(z.n_slots, np.int32) if z.has_perm else None
+            M.n_heavy, M.n_seg = int(z.n_heavy), int(z.n_seg)
+            M.heavy_seg_len = int(heavy_seg_len)
+            if z.n_heavy:
+                M.heavy_row = empty(z.n_heavy, np.int32)
+                M.heavy_ptr = empty(z.n_heavy + 1, np.int64)
+                M.heavy_seg = empty(z.n_heavy + 1, np.int32)
+                M.heavy_col = empty(z.heavy_nnz, np.int32)
+                M.heavy_val = empty(z.heavy_nnz, vdt)
+            else:
+                M.heavy_row = M.heavy_ptr = M.heavy_seg = M.heavy_col = M.heavy_val = None
+            vinv_dev = empty(A.m, np.int32) if z.has_vinv else None
+            d = GGMatrix()
+            d.ptr, d.idx, d.val = ptr(M.slice_ptr), ptr(M.colval), ptr(M.nzval)
+            if perm_dev is not None:
+                d.perm, d.slot_len = ptr(perm_dev), ptr(M.slot_len)
+            if z.n_heavy:
+                d.heavy_row, d.heavy_ptr, d.heavy_seg = (ptr(M.heavy_row), ptr(M.heavy_ptr),
+                                                         ptr(M.heavy_seg))
+                d.heavy_col, d.heavy_val = ptr(M.heavy_col), ptr(M.heavy_val)
+            if vinv_dev is not None:
+                d.vinv = ptr(vinv_dev)
+            check(lib().gg_sell_fill_device(plan, C.byref(d)), "SparseGPUMatrixSELL")
+        finally:
+            lib().gg_sell_plan_free(plan)
+        M._nnz = int(d.nnz)
+        M._perm_dev = perm_dev
+        M.perm = perm_dev.cpu().numpy() if perm_dev is not None else np.arange(1, A.m + 1,
+                                                                               dtype=np.int64)
+        M._vinv_dev = vinv_dev
+        M.vinv = vinv_dev.cpu().numpy() if vinv_dev is not None else None
+        M.unit = A.unit
+        M.symmetric = getattr(A, "symmetric", False)
+        return M
+
     def _init_raw(self, m, n, perm, slice_size, nslices, nnz, n_stored, slice_ptr, colval, nzval,
                   backend):
         # inner-constructor checks, src/storage/SELL.jl:42-52

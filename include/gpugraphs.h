@@ -172,6 +172,24 @@ typedef enum gg_sell_flags {
 
 typedef struct gg_ctx gg_ctx;
 
+/* The SELL builder on the device (no counterpart in the reference, which
+ * builds on the host, src/storage/SELL.jl:99-181): the layouts of
+ * gg_sell_build, array for array, from a CSR already in device memory.
+ * gg_sell_plan_device orders the rows and returns every output size; the
+ * caller allocates device arrays of those sizes in a gg_matrix -- ptr =
+ * slice_ptr[nslices+1], idx = colval[n_stored], val = nzval[n_stored];
+ * perm / slot_len [n_slots] when has_perm; heavy_row [n_heavy], heavy_ptr
+ * [n_heavy+1] (Int64), heavy_seg [n_heavy+1], heavy_col / heavy_val
+ * [heavy_nnz]; vinv [m] when has_vinv -- and gg_sell_fill_device writes them
+ * and the matrix's scalar fields.  The CSR arrays must stay alive until the
+ * fill; the plan is freed with gg_sell_plan_free. */
+typedef struct gg_sell_plan gg_sell_plan;
+typedef struct gg_sell_sizes {
+  int64_t slice_size, nslices, n_stored, n_slots;
+  int64_t n_heavy, heavy_nnz, n_seg, n_internal;
+  int32_t has_perm, has_vinv;
+} gg_sell_sizes;
+
 /* ---------------------------------------------------------------- runtime */
 int gg_abi_version(void);
 const char* gg_status_string(int status);
@@ -220,6 +238,13 @@ int gg_ctx_set_push_threshold(gg_ctx* ctx, double alpha);
 /* Host-side consistency check of a descriptor (no device access): the checks
  * every entry point runs before launching. */
 int gg_matrix_check(const gg_matrix* A);
+int gg_sell_plan_device(gg_ctx* ctx, int64_t m, int64_t n, const int32_t* rowptr,
+                        const int32_t* colval, const void* nzval, int val_dtype,
+                        int64_t slice_size, int64_t sigma, int64_t heavy_threshold,
+                        int64_t heavy_seg_len, int flags, gg_sell_plan** plan,
+                        gg_sell_sizes* sizes);
+int gg_sell_fill_device(gg_sell_plan* plan, gg_matrix* out);
+void gg_sell_plan_free(gg_sell_plan* plan);
 /* SELL slots the kernels walk: nslices*slice_size with a permutation, else
  * min(m, nslices*slice_size) (identity layout); 0 for CSR / CSC. */
 int gg_matrix_slots(const gg_matrix* A, int64_t* n_slots);

@@ -618,3 +618,63 @@ def test_shortest_path_multi_column_vectors(env, dt, k):
               H.to_sell(gg.ROCBackend(0), 64, 256, 64, 128)):
         print_case("sssp column vectors", np.dtype(dt).name, k, A.vinv is not None)
         np.testing.assert_array_equal(host(gg.shortest_path(A, srcs)), ref)
+
+
+def _sell_arrays(torch, M):
+    def h(x):
+        if x is None:
+            return None
+        return x.cpu().numpy() if torch.is_tensor(x) else np.asarray(x)
+    out = {k: h(getattr(M, k)) for k in ("slice_ptr", "colval", "nzval", "slot_len", "heavy_row",
+                                         "heavy_ptr", "heavy_seg", "heavy_col", "heavy_val")}
+    out["perm_dev"] = h(M._perm_dev)
+    out["vinv"] = h(M.vinv)
+    out["scalars"] = (M.m, M.n, M.nnz(), M.slice_size, M.nslices, M.n_stored, M.n_heavy, M.n_seg)
+    return out
+
+
+@pytest.mark.parametrize("layout", [(32, 1, 0, 4096, False), (64, 64, 8, 16, False),
+                                    (64, 1, 8, 16, True), (64, 4096, 256, 512, False),
+                                    (16, 1, 0, 4096, True), (64, 1, 0, 4096, False),
+                                    (8, 3, 5, 7, False)])
+def test_sell_device_builder_equals_host(env, layout):
+    """The device builder (gg_sell_plan_device / gg_sell_fill_device) gives
+    the host builder's arrays exactly, for the reference layout, sigma
+    windows, heavy partitions and relabelling, and the same SpMV."""
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    C_, sigma, heavy, seg, relabel = layout
+    rng = np.random.default_rng(sum(layout[:4]))
+    cases = [GEN.rmat(11, seed=2), GEN.grid(30, 40, 0.02, 1, 50, seed=3, dtype=np.float32)]
+    S = random_csr(rng, 300, 300, 0.03).astype(np.int32)
+    S.data[:] = rng.integers(1, 9, S.nnz)
+    S.data[S.indptr[5]:S.indptr[6]] = 0   # an empty row
+    S.eliminate_zeros()
+    cases.append(GEN.HostCSR(300, 300, (S.indptr + 1).astype(np.int32),
+                             (S.indices + 1).astype(np.int32), S.data.astype(np.int32)))
+    if not relabel:
+        R = random_csr(rng, 200, 500, 0.02).astype(np.float64)
+        cases.append(GEN.HostCSR(200, 500, (R.indptr + 1).astype(np.int32),
+                                 (R.indices + 1).astype(np.int32), R.data))
+    for H in cases:
+        print_case("sell device builder", layout, H.m, H.n, H.nzval.dtype)
+        A = H.to_csr(gg.ROCBackend(0))
+        ref = H.to_sell(gg.ROCBackend(0), C_, sigma, heavy, seg, relabel)
+        got = gg.SparseGPUMatrixSELL.build_on_device(A, C_, sigma, heavy, seg, relabel)
+        a, b = _sell_arrays(torch, ref), _sell_arrays(torch, got)
+        for k in a:
+            if a[k] is None or b[k] is None:
+                assert a[k] is None and b[k] is None or (a[k] is not None and len(a[k]) == 0) \
+                    or (b[k] is not None and len(b[k]) == 0), k
+            else:
+                np.testing.assert_array_equal(np.asarray(a[k]), np.asarray(b[k]), err_msg=k)
+        x = dev(torch, rng.random(H.n).astype(np.float32) if H.nzval.dtype == np.float32
+                else rng.integers(0, 5, H.n).astype(H.nzval.dtype if H.nzval.dtype != np.bool_
+                                                    else np.int32))
+        if H.nzval.dtype == np.bool_:
+            continue
+        y1 = dev(torch, np.zeros(H.m, H.nzval.dtype))
+        y2 = dev(torch, np.zeros(H.m, H.nzval.dtype))
+        gg.gpu_spmv_(y1, ref, x)
+        gg.gpu_spmv_(y2, got, x)
+        np.testing.assert_array_equal(host(y1), host(y2))
