@@ -384,6 +384,9 @@ __global__ void ms_level_end_kernel(unsigned int* found, int32_t* ctl, int32_t c
 }
 
 // ------------------------------------------------------- top-down step
+// Frontier words per thread of ms_frontier_kernel: one, since a dense
+// frontier's out-edge count loops over every set bit of a word.
+constexpr int kMsScanWords = 1;
 constexpr int kPushBig = 4096;  // out-edges from which a frontier row is cut into pieces
 
 __global__ void ms_push_start_kernel(MsArgs a, MsPush p) {
@@ -399,34 +402,35 @@ __global__ __launch_bounds__(256) void ms_frontier_kernel(MsArgs a, MsPush p, Se
                                                           const int32_t* __restrict__ rowptr) {
   if (a.ctl && a.ctl[1]) return;
   if (QUEUE && !p.cnt[3]) return;
-  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int wl = threadIdx.x & 63;
-  uint32_t bits = w < p.words ? a.front_any[w] : 0u;
+  int64_t w[kMsScanWords];
+  uint32_t bits[kMsScanWords];
+#pragma unroll
+  for (int i = 0; i < kMsScanWords; ++i) {
+    w[i] = (int64_t)blockIdx.x * (256 * kMsScanWords) + i * 256 + threadIdx.x;
+    bits[i] = w[i] < p.words ? a.front_any[w[i]] : 0u;
+  }
   if (QUEUE) {
-    const int c = __popc(bits);
-    int incl = c;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(incl, o, 64);
-      if (wl >= o) incl += y;
-    }
-    unsigned base = 0;
-    if (wl == 63 && incl) base = atomicAdd(&p.cnt[0], (unsigned)incl);
-    base = __shfl(base, 63, 64);
-    unsigned pos = base + incl - c;
-    while (bits) {
-      const int b = __ffs(bits) - 1;
-      bits &= bits - 1;
-      p.queue[pos++] = (int32_t)(w * 32 + b);
-    }
+    unsigned c = 0;
+#pragma unroll
+    for (int i = 0; i < kMsScanWords; ++i) c += __popc(bits[i]);
+    unsigned pos = block_claim(c, &p.cnt[0]);
+#pragma unroll
+    for (int i = 0; i < kMsScanWords; ++i)
+      while (bits[i]) {
+        const int b = __ffs(bits[i]) - 1;
+        bits[i] &= bits[i] - 1;
+        p.queue[pos++] = (int32_t)(w[i] * 32 + b);
+      }
   } else {
     unsigned long long deg = 0;
-    while (bits) {
-      const int b = __ffs(bits) - 1;
-      bits &= bits - 1;
-      deg += (unsigned long long)state_row(g, csr, rowptr, w * 32 + b).len;
-    }
-    for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o, 64);
-    if (wl == 0 && deg) atomicAdd(p.scout, deg);
+#pragma unroll
+    for (int i = 0; i < kMsScanWords; ++i)
+      while (bits[i]) {
+        const int b = __ffs(bits[i]) - 1;
+        bits[i] &= bits[i] - 1;
+        deg += (unsigned long long)state_row(g, csr, rowptr, w[i] * 32 + b).len;
+      }
+    block_add(deg, p.scout);
   }
 }
 
@@ -544,10 +548,10 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a_in, int 
     const SellView g = sell_view(A);
     const int csr = A->format == GG_CSR;
     hipLaunchKernelGGL(ms_push_start_kernel, dim3(1), dim3(1), 0, stream, a, *push);
-    hipLaunchKernelGGL(ms_frontier_kernel<false>, dim3(blocks_for(push->words, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(ms_frontier_kernel<false>, dim3(blocks_for(push->words, 256 * kMsScanWords)), dim3(256), 0,
                        stream, a, *push, g, csr, A->ptr);
     hipLaunchKernelGGL(ms_decide_kernel, dim3(1), dim3(1), 0, stream, a, *push);
-    hipLaunchKernelGGL(ms_frontier_kernel<true>, dim3(blocks_for(push->words, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(ms_frontier_kernel<true>, dim3(blocks_for(push->words, 256 * kMsScanWords)), dim3(256), 0,
                        stream, a, *push, g, csr, A->ptr);
   }
   int st = with_type(A->val_dtype, [&](auto ta) -> int {

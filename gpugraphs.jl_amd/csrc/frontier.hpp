@@ -18,6 +18,49 @@ __device__ __forceinline__ bool bit_claim(uint32_t* bm, int64_t v) {
   return !(atomicOr(&bm[v >> 5], b) & b);
 }
 
+// Block-wide (256 threads, all of them calling) counterparts of a per-lane
+// atomic on one counter: one atomic per block instead of one per lane or
+// wavefront (a single address serialises them).
+constexpr int kBlockClaimThreads = 256;
+// Exclusive position of this thread's c items in a queue of length *counter.
+__device__ __forceinline__ unsigned block_claim(unsigned c, unsigned* counter) {
+  __shared__ unsigned wsum[kBlockClaimThreads / 64], wbase[kBlockClaimThreads / 64];
+  const int wl = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  unsigned incl = c;
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned y = __shfl_up(incl, o, 64);
+    if (wl >= o) incl += y;
+  }
+  if (wl == 63) wsum[wv] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned t = 0;
+    for (int i = 0; i < kBlockClaimThreads / 64; ++i) {
+      wbase[i] = t;
+      t += wsum[i];
+    }
+    const unsigned b = t ? atomicAdd(counter, t) : 0u;
+    for (int i = 0; i < kBlockClaimThreads / 64; ++i) wbase[i] += b;
+  }
+  __syncthreads();
+  return wbase[wv] + incl - c;
+}
+// *target += the block's sum of v.
+__device__ __forceinline__ void block_add(unsigned long long v, unsigned long long* target) {
+  __shared__ unsigned long long ws[kBlockClaimThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kBlockClaimThreads / 64; ++i) t += ws[i];
+    if (t) atomicAdd(target, t);
+  }
+}
+// Words of a bitmap per thread in the frontier scans (4 per thread, a block
+// covers 1024 coalesced words).
+constexpr int kScanWords = 4;
+
 // Sets bit (base + lane) for every lane of the wavefront with `on`, where base
 // is the wavefront's first id, a multiple of 64 (the internal ids of one
 // wavefront's 64 slots): one OR per 32-bit word instead of one atomic per

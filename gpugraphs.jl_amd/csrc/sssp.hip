@@ -511,29 +511,26 @@ __global__ __launch_bounds__(kT) void bf_sp_fallen_kernel(BfSparse s, const uint
                                                           SellView g, const int32_t* done) {
   if (done && *done) return;
   if (QUEUE && !s.cnt[3]) return;
-  const int64_t w = (int64_t)blockIdx.x * kT + threadIdx.x;
-  const int wl = threadIdx.x & 63;
-  uint32_t bits = w < s.words ? chp[w] : 0u;
+  int64_t w[kScanWords];
+  uint32_t bits[kScanWords];
+  unsigned c = 0;
+#pragma unroll
+  for (int i = 0; i < kScanWords; ++i) {
+    w[i] = (int64_t)blockIdx.x * (kT * kScanWords) + i * kT + threadIdx.x;
+    bits[i] = w[i] < s.words ? chp[w[i]] : 0u;
+    c += __popc(bits[i]);
+  }
   if (QUEUE) {
-    const int c = __popc(bits);
-    int incl = c;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(incl, o, 64);
-      if (wl >= o) incl += y;
-    }
-    unsigned base = 0;
-    if (wl == 63 && incl) base = atomicAdd(&s.cnt[0], (unsigned)incl);
-    base = __shfl(base, 63, 64);
-    unsigned pos = base + incl - c;
-    while (bits) {
-      const int b = __ffs(bits) - 1;
-      bits &= bits - 1;
-      s.queue[pos++] = (int32_t)(w * 32 + b);
-    }
+    unsigned pos = block_claim(c, &s.cnt[0]);
+#pragma unroll
+    for (int i = 0; i < kScanWords; ++i)
+      while (bits[i]) {
+        const int b = __ffs(bits[i]) - 1;
+        bits[i] &= bits[i] - 1;
+        s.queue[pos++] = (int32_t)(w[i] * 32 + b);
+      }
   } else {
-    unsigned long long c = (unsigned long long)__popc(bits);
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-    if (wl == 0 && c) atomicAdd(s.scout, c);
+    block_add(c, s.scout);
   }
 }
 
@@ -738,7 +735,8 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_vec_kernel(BfSparse s, SellView
 // fallen queue and the marking of the rows to relax.
 static void bf_sparse_prologue(hipStream_t st, const BfSparse& s, const SellView& g,
                                const uint32_t* chp, const int32_t* done, int cus) {
-  const unsigned nbw = blocks_for(s.words, kT), nb = (unsigned)std::max(cus, 1) * 8;
+  static_assert(kT == kBlockClaimThreads, "block_claim / block_add assume 256 threads");
+  const unsigned nbw = blocks_for(s.words, kT * kScanWords), nb = (unsigned)std::max(cus, 1) * 8;
   hipLaunchKernelGGL(bf_sp_start_kernel, dim3(1), dim3(1), 0, st, s, done);
   hipLaunchKernelGGL(bf_sp_fallen_kernel<false>, dim3(nbw), dim3(kT), 0, st, s, chp, g, done);
   hipLaunchKernelGGL(bf_sp_decide_kernel, dim3(1), dim3(1), 0, st, s, done);
