@@ -455,7 +455,7 @@ __global__ void ms_push_reset_kernel(MsPush p) {
 }
 
 // Edge u -> v: OR u's frontier bits that v has not seen into acc[v]; the
-// first lane to touch v queues it.  All 64 lanes must call it.
+// first lane to touch v queues it.  All 256 threads of the block must call it.
 template <int W>
 __device__ __forceinline__ void push_pair(const MsArgs& a, const MsPush& p, int64_t u, int64_t v,
                                           bool valid) {
@@ -475,15 +475,7 @@ __device__ __forceinline__ void push_pair(const MsArgs& a, const MsPush& p, int6
       newly = bit_claim(p.touched, v);
     }
   }
-  const unsigned long long b = __ballot(newly);
-  if (b) {
-    const int wl = threadIdx.x & 63;
-    const int leader = __ffsll(b) - 1;
-    unsigned base = 0;
-    if (wl == leader) base = atomicAdd(&p.cnt[1], (unsigned)__popcll(b));
-    base = __shfl(base, leader, 64);
-    if (newly) p.tqueue[base + __popcll(b & ((1ull << wl) - 1))] = (int32_t)v;
-  }
+  block_append(p.tqueue, &p.cnt[1], v, newly);  // called block-synchronously by expand_*
 }
 
 // Frontier rows expanded edge by edge (expand_tiles / expand_pieces).

@@ -351,6 +351,13 @@ __device__ __forceinline__ void wave_append(int32_t* q, unsigned int* qn, int64_
   if (add) q[base + __popcll(b & ((1ull << wl) - 1))] = (int32_t)v;
 }
 
+// wave_append for the block-synchronous callbacks of expand_tiles /
+// expand_pieces: one atomic per block.  All 256 threads must call it.
+__device__ __forceinline__ void block_append(int32_t* q, unsigned int* qn, int64_t v, bool add) {
+  const unsigned pos = block_claim(add ? 1u : 0u, qn);
+  if (add) q[pos] = (int32_t)v;
+}
+
 // x in internal order: xp[slot] = x[perm[slot]] (0 for an empty slot),
 // xp[slot_base + h] = x[heavy_row[h]]; rows of k values when k > 1 (B is
 // m x k in b_layout, xp is row-major).

@@ -546,7 +546,7 @@ __global__ void bf_sp_decide_kernel(BfSparse s, const int32_t* done) {
 __device__ __forceinline__ void bf_sp_mark(const BfSparse& s, const SellView& g, int64_t v,
                                            bool valid) {
   const bool add = valid && v < g.slot_base && bit_claim(s.mark, v);
-  wave_append(s.dq, &s.cnt[1], v, add);
+  block_append(s.dq, &s.cnt[1], v, add);  // called block-synchronously by expand_*
 }
 
 __global__ __launch_bounds__(256) void bf_sp_mark_tile_kernel(BfSparse s, SellView g,
