@@ -306,6 +306,7 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=8,
                     help="levels / iterations per captured graph launch (0: host-stepped loops)")
     ap.add_argument("--spmv-reps", type=int, default=20)
+    ap.add_argument("--spmm-k", type=int, default=32, help="columns of the SpMM leg (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--grid", type=int, default=4096, help="Bellman-Ford grid side")
     ap.add_argument("--sssp-sources", type=int, default=16, help="grid Bellman-Ford sources")
@@ -320,7 +321,7 @@ def main():
     ap.add_argument("--serve-budget-gib", type=float, default=8.0,
                     help="HBM result-arena budget of the server (GiB)")
     ap.add_argument("--only", default="",
-                    help="one part: spmv | msbfs | bfs | sssp | packed | serve, or copy (a 1 GiB device copy that "
+                    help="one part: spmv | spmm | msbfs | bfs | sssp | packed | serve, or copy (a 1 GiB device copy that "
                          "calibrates the HBM byte counters)")
     args = ap.parse_args()
     relabel = not args.no_relabel
@@ -414,6 +415,36 @@ def main():
                            max_rel_err_vs_fp64=relerr)
         log(f"SELL SpMV: {spmv_ms * 1e3:.1f} us/SpMV ({B_spmv / spmv_ms / 1e6:.0f} GB/s), kernel "
             f"{spmv_kernel_ms * 1e3:.1f} us ({B_spmv / spmv_kernel_ms / 1e6:.0f} GB/s), rel err {relerr:.1e}")
+
+    # ---------------------------------------------------------------- SpMM
+    if args.only in ("", "spmm") and args.spmm_k > 0:
+        kk = args.spmm_k
+        Bm = torch.from_numpy(rng.random((n, kk), dtype=np.float32)).to(dev)   # row-major
+        Cm = torch.zeros((n, kk), dtype=torch.float32, device=dev)
+        gg.gpu_spmm_(Cm, A_f, Bm)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(3):
+            gg.gpu_spmm_(Cm, A_f, Bm)
+        e1.record()
+        torch.cuda.synchronize()
+        spmm_ms = e0.elapsed_time(e1) / 3
+        B_spmm = nnz * 8 + (n + n) * kk * 4   # pattern and values once, B read once, C written once
+        cols = np.random.default_rng(5).choice(kk, 3, replace=False)
+        S64 = Rg.to_scipy().astype(np.float64)
+        Bh = Bm.cpu().numpy().astype(np.float64)
+        Ch = Cm.cpu().numpy()
+        relerr = 0.0
+        for j in cols:
+            ref = S64 @ Bh[:, j]
+            relerr = max(relerr, float(np.max(np.abs(Ch[:, j] - ref) / np.maximum(np.abs(ref), 1e-30))))
+        assert relerr <= 1e-5, f"SpMM parity failed: max rel err {relerr}"
+        del S64, Bh, Ch, Bm, Cm
+        res["spmm"] = dict(k=kk, ms=spmm_ms, bytes=B_spmm, gbs=B_spmm / spmm_ms / 1e6,
+                           max_rel_err_vs_fp64=relerr)
+        log(f"SELL SpMM k={kk} (row-major B, C): {spmm_ms * 1e3:.1f} us ({B_spmm / spmm_ms / 1e6:.0f} GB/s "
+            f"algorithmic), rel err {relerr:.1e}")
 
     # ---------------------------------------------------------- batched BFS
     D = None
@@ -590,6 +621,8 @@ def main():
         "spmv_gbs": round(spmv["gbs"], 1) if spmv else None,
         "spmv_us": round(spmv["ms"] * 1e3, 1) if spmv else None,
         "spmv_roofline": spmv_roof,
+        "spmm": {k: (round(v, 4) if isinstance(v, float) else v)
+                 for k, v in res["spmm"].items()} if "spmm" in res else None,
         "roofline": roofline,
         "sssp": {name: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in c.items()}
                  for name, c in res.get("sssp", {}).items()} or None,

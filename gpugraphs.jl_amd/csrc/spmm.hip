@@ -118,6 +118,194 @@ __global__ __launch_bounds__(kBlk) void spmm_kernel(typename MmTypes<S, TA, TB>:
   }
 }
 
+// ------------------------------------------------ SELL, 16-byte column groups
+// SELL with a row-major B (k a multiple of VB = 16 / sizeof(TB), 16-byte
+// aligned): a group of Lv lanes per slot, each lane VB consecutive columns
+// read as one 16-byte load per entry, entries in batches of 4 (index / value
+// loads, then the B loads, then the folds in storage order).  Each column
+// folds its terms in the reference's order and stops at the absorbing value
+// (src/spmm/SELL.jl:221-242), so light rows give the one-lane-per-column
+// result exactly.  Heavy rows: one block per (segment, column block), its
+// groups splitting the segment's entries; a row of one segment is written
+// directly, the others fold their segment partials in segment order
+// (spmm_combine_kernel) -- exact for the idempotent monoids (min, max, or,
+// and, any) and a reassociation of the sum for plus on floats.
+template <class A, class MO>
+__device__ __forceinline__ void mm_fold(A& acc, A term) {
+  if (!(MO::has_absorb && acc == MO::absorb())) acc = MO::op(acc, term);
+}
+
+template <class TC, class CT>
+__device__ __forceinline__ void mm_store(TC* Cm, const MmArgs& g, int64_t row, int64_t j, int accum,
+                                         CT v) {
+  const int64_t ci = g.c_rowmajor ? row * g.k + j : row + j * g.m;
+  if (accum_reads_c(accum)) v = accum_apply<CT>(accum, CT(Cm[ci]), v);
+  Cm[ci] = TC(v);
+}
+
+template <class S, class TA, class TB, int VB>
+__global__ __launch_bounds__(kBlk) void spmm_sell_vec_kernel(
+    typename MmTypes<S, TA, TB>::TC* __restrict__ Cm, const TB* __restrict__ B, MmArgs g,
+    const int32_t* __restrict__ perm, int64_t n_slots, int accum, int logLv) {
+  typedef MmTypes<S, TA, TB> T;
+  typedef typename T::A A;
+  typedef typename T::CT CT;
+  typedef TB VT __attribute__((ext_vector_type(VB)));
+  const int Lv = 1 << logLv;
+  const int lane = threadIdx.x & (Lv - 1);
+  const int64_t kq = g.k / VB;
+  const int64_t jblocks = (kq + Lv - 1) >> logLv;
+  const int64_t q = ((int64_t)blockIdx.x * kBlk + threadIdx.x) >> logLv;
+  const int64_t slot = q / jblocks;
+  const int64_t jq = (q - slot * jblocks) * Lv + lane;
+  if (slot >= n_slots || jq >= kq) return;
+  int64_t row = slot;
+  if (perm) {
+    if (perm[slot] == 0) return;
+    row = perm[slot] - 1;
+  } else if (row >= g.m) {
+    return;
+  }
+  const int64_t j0 = jq * VB;
+  const int64_t sl = slot / g.C, r = slot - sl * g.C;
+  const int64_t base = (int64_t)g.ptr[sl] - 1 + r;
+  const int64_t len = g.slot_len ? (int64_t)g.slot_len[slot]
+                                 : ((int64_t)g.ptr[sl + 1] - g.ptr[sl]) / g.C;
+  const TA* __restrict__ val = (const TA*)g.val;
+  A acc[VB];
+#pragma unroll
+  for (int t = 0; t < VB; ++t) acc[t] = T::MO::neutral();
+  int64_t kk = 0;
+  bool end = false;
+  for (; kk + 4 <= len && !end; kk += 4) {
+    int32_t c[4];
+    TA v[4];
+    VT b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = g.idx[base + (kk + u) * g.C];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = val[base + (kk + u) * g.C];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (c[u] != -1) b[u] = *(const VT*)&B[(int64_t)(c[u] - 1) * g.k + j0];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (c[u] == -1) end = true;  // the reference's pad break
+      if (!end)
+#pragma unroll
+        for (int t = 0; t < VB; ++t)
+          mm_fold<A, typename T::MO>(acc[t],
+                                     A(T::MU::ap(CT(v[u]), CT(b[u][t]), mm_col<S>(g, c[u]))));
+    }
+  }
+  for (; kk < len && !end; ++kk) {
+    const int64_t i = base + kk * g.C;
+    const int32_t col = g.idx[i];
+    if (col == -1) break;
+    const VT b = *(const VT*)&B[(int64_t)(col - 1) * g.k + j0];
+#pragma unroll
+    for (int t = 0; t < VB; ++t)
+      mm_fold<A, typename T::MO>(acc[t], A(T::MU::ap(CT(val[i]), CT(b[t]), mm_col<S>(g, col))));
+  }
+#pragma unroll
+  for (int t = 0; t < VB; ++t) mm_store<typename T::TC, CT>(Cm, g, row, j0 + t, accum, CT(acc[t]));
+}
+
+template <class S, class TA, class TB, int VB>
+__global__ __launch_bounds__(kBlk) void spmm_heavy_vec_kernel(
+    typename MmTypes<S, TA, TB>::TC* __restrict__ Cm, const TB* __restrict__ B, MmArgs g,
+    const int32_t* __restrict__ heavy_row, const int32_t* __restrict__ heavy_seg,
+    int64_t n_heavy, int64_t seg_len, void* partial, int accum, int logLv) {
+  typedef MmTypes<S, TA, TB> T;
+  typedef typename T::A A;
+  typedef typename T::CT CT;
+  typedef TB VT __attribute__((ext_vector_type(VB)));
+  __shared__ A red[kBlk * VB];
+  const int Lv = 1 << logLv;
+  const int lane = threadIdx.x & (Lv - 1);
+  const int64_t grp = threadIdx.x >> logLv;
+  const int64_t gpb = kBlk >> logLv;
+  const int64_t kq = g.k / VB;
+  const int64_t jblocks = (kq + Lv - 1) >> logLv;
+  const int64_t sg = (int64_t)blockIdx.x / jblocks;
+  const int64_t jq = ((int64_t)blockIdx.x - sg * jblocks) * Lv + lane;
+  const bool jin = jq < kq;
+  const int64_t j0 = jq * VB;
+  int64_t lo = 0, hi = n_heavy;  // heavy_seg[lo] <= sg < heavy_seg[lo+1]
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (heavy_seg[mid] <= sg) lo = mid; else hi = mid;
+  }
+  const int64_t nseg = heavy_seg[lo + 1] - heavy_seg[lo];
+  const int64_t s = g.heavy_ptr[lo] + (sg - heavy_seg[lo]) * seg_len;
+  const int64_t e = min(s + seg_len, g.heavy_ptr[lo + 1]);
+  const TA* __restrict__ hv = (const TA*)g.heavy_val;
+  A acc[VB];
+#pragma unroll
+  for (int t = 0; t < VB; ++t) acc[t] = T::MO::neutral();
+  if (jin) {
+    int64_t i = s + grp;
+    for (; i + 3 * gpb < e; i += 4 * gpb) {
+      int32_t c[4];
+      TA v[4];
+      VT b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) c[u] = g.heavy_col[i + u * gpb];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = hv[i + u * gpb];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) b[u] = *(const VT*)&B[(int64_t)(c[u] - 1) * g.k + j0];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int t = 0; t < VB; ++t)
+          mm_fold<A, typename T::MO>(acc[t],
+                                     A(T::MU::ap(CT(v[u]), CT(b[u][t]), mm_col<S>(g, c[u]))));
+    }
+    for (; i < e; i += gpb) {
+      const int32_t c = g.heavy_col[i];
+      const VT b = *(const VT*)&B[(int64_t)(c - 1) * g.k + j0];
+#pragma unroll
+      for (int t = 0; t < VB; ++t)
+        mm_fold<A, typename T::MO>(acc[t], A(T::MU::ap(CT(hv[i]), CT(b[t]), mm_col<S>(g, c))));
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < VB; ++t) red[threadIdx.x * VB + t] = acc[t];
+  __syncthreads();
+  if (grp != 0 || !jin) return;
+  for (int64_t o = 1; o < gpb; ++o)
+#pragma unroll
+    for (int t = 0; t < VB; ++t) mm_fold<A, typename T::MO>(acc[t], red[(o * Lv + lane) * VB + t]);
+  if (nseg == 1) {
+    const int64_t row = heavy_row[lo] - 1;
+#pragma unroll
+    for (int t = 0; t < VB; ++t) mm_store<typename T::TC, CT>(Cm, g, row, j0 + t, accum, CT(acc[t]));
+  } else {
+#pragma unroll
+    for (int t = 0; t < VB; ++t) ((A*)partial)[sg * g.k + j0 + t] = acc[t];
+  }
+}
+
+// Rows of several heavy segments: their partials folded in segment order.
+template <class S, class TA, class TB>
+__global__ __launch_bounds__(kBlk) void spmm_combine_kernel(
+    typename MmTypes<S, TA, TB>::TC* __restrict__ Cm, MmArgs g,
+    const int32_t* __restrict__ heavy_row, const int32_t* __restrict__ heavy_seg,
+    int64_t n_heavy, const void* partial, int accum) {
+  typedef MmTypes<S, TA, TB> T;
+  typedef typename T::A A;
+  typedef typename T::CT CT;
+  const int64_t id = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+  const int64_t h = id / g.k, j = id - h * g.k;
+  if (h >= n_heavy) return;
+  const int64_t s0 = heavy_seg[h], s1 = heavy_seg[h + 1];
+  if (s1 - s0 < 2) return;
+  A acc = T::MO::neutral();
+  for (int64_t sg = s0; sg < s1; ++sg) mm_fold<A, typename T::MO>(acc, ((const A*)partial)[sg * g.k + j]);
+  mm_store<typename T::TC, CT>(Cm, g, heavy_row[h] - 1, j, accum, CT(acc));
+}
+
 // Inverse maps of a SELL hybrid (row -> slot, row -> heavy index), built once
 // per matrix view and cached in the context by device pointer of perm.
 __global__ void sell_row_maps_kernel(const int32_t* perm, int64_t n_slots, int32_t* slot_of_row) {
@@ -209,6 +397,40 @@ int launch_spmm(gg_ctx* ctx, void* C, int c_dtype, int c_layout, const gg_matrix
       g.slot_base = A->nslices * A->slice_size;
       g.perm = A->perm;
       g.heavy_row = A->heavy_row;
+    }
+    constexpr int VB = 16 / (int)sizeof(TB);
+    // SELL with a row-major B of 16-byte column groups; integer plus keeps the
+    // one-lane-per-column kernel (its absorbing exit depends on the order)
+    const bool int_plus = S::ADD == GG_OP_ADD && is_int<typename MmTypes<S, TA, TB>::CT>();
+    if (A->format == GG_SELL && g.b_rowmajor && k % VB == 0 && ((uintptr_t)B & 15) == 0 &&
+        !int_plus && !mask && nrows == A->m) {
+      int lv = 0;
+      while (lv < 6 && (1LL << lv) < k / VB) ++lv;
+      const int64_t jb = (k / VB + (1LL << lv) - 1) >> lv;
+      const int64_t n_slots = A->nslices * A->slice_size;
+      const int64_t nb_l = (((n_slots * jb) << lv) + kBlk - 1) / kBlk;
+      const int64_t nb_h = A->n_heavy > 0 ? A->heavy_nseg * jb : 0;
+      if (nb_l > 0x7fffffffLL || nb_h > 0x7fffffffLL)
+        return set_error(GG_ERR_ARGUMENT, "SpMM launch too large");
+      void* part = nullptr;
+      if (A->n_heavy > 0)
+        GG_TRY(ws_get(ctx, WS_PARTIAL, sizeof(typename MmTypes<S, TA, TB>::A) *
+                                           (size_t)(A->heavy_nseg * k), &part));
+      HotTimer ht(ctx);
+      if (n_slots > 0)
+        hipLaunchKernelGGL((spmm_sell_vec_kernel<S, TA, TB, VB>), dim3((unsigned)nb_l), dim3(kBlk),
+                           0, ctx->stream, (TC*)C, (const TB*)B, g, A->perm, n_slots, accum, lv);
+      if (nb_h > 0) {
+        hipLaunchKernelGGL((spmm_heavy_vec_kernel<S, TA, TB, VB>), dim3((unsigned)nb_h),
+                           dim3(kBlk), 0, ctx->stream, (TC*)C, (const TB*)B, g, A->heavy_row,
+                           A->heavy_seg, A->n_heavy, A->heavy_seg_len, part, accum, lv);
+        if (A->heavy_nseg > A->n_heavy)
+          hipLaunchKernelGGL((spmm_combine_kernel<S, TA, TB>),
+                             dim3(blocks_for(A->n_heavy * k, kBlk)), dim3(kBlk), 0, ctx->stream,
+                             (TC*)C, g, A->heavy_row, A->heavy_seg, A->n_heavy, part, accum);
+      }
+      GG_HIP(hipGetLastError());
+      return GG_OK;
     }
     const int64_t groups = nrows * ((k + (1 << lg) - 1) >> lg);
     unsigned nb = blocks_for(groups << lg, kBlk);

@@ -202,7 +202,7 @@ def _cm(torch, a):
 
 
 @pytest.mark.parametrize("fmt", FORMATS)
-@pytest.mark.parametrize("k", [1, 5, 20, 33])
+@pytest.mark.parametrize("k", [1, 5, 20, 32, 33, 64, 96])
 @pytest.mark.parametrize("layout", ["col", "row"])
 def test_spmm_plus_times_and_min_plus(env, fmt, k, layout):
     gg, torch, orc = env
@@ -678,3 +678,40 @@ def test_sell_device_builder_equals_host(env, layout):
         gg.gpu_spmv_(y1, ref, x)
         gg.gpu_spmv_(y2, got, x)
         np.testing.assert_array_equal(host(y1), host(y2))
+
+
+@pytest.mark.parametrize("relabel", [False, True])
+def test_spmm_power_law_heavy_rows(env, relabel):
+    """SpMM on a power-law matrix whose hubs span several heavy segments:
+    plus-times within the tolerance of an FP64 product, the min-plus forms
+    exact against the oracle (16-byte column groups, segment partials folded
+    in order)."""
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    rng = np.random.default_rng(31 + relabel)
+    R = GEN.rmat(13, seed=5, dtype=np.float32, values=GEN.UNIFORM01)
+    S = R.to_scipy()
+    A = R.to_sell(gg.ROCBackend(0), 64, 1, 64, 256, relabel=relabel)
+    k = 32
+    B = rng.random((R.n, k), dtype=np.float32)
+    C = dev(torch, np.zeros((R.m, k), np.float32))
+    print_case("spmm power law", relabel)
+    gg.gpu_spmm_(C, A, dev(torch, B))
+    np.testing.assert_allclose(host(C), S.astype(np.float64) @ B.astype(np.float64), rtol=RTOL,
+                               atol=1e-6)
+    W = R.to_scipy()
+    W.data = rng.integers(1, 500, W.nnz).astype(np.float32)
+    Hw = GEN.HostCSR(R.m, R.n, R.rowptr, R.colval, W.data.astype(np.float32), True)
+    Aw = Hw.to_sell(gg.ROCBackend(0), 64, 1, 64, 256, relabel=relabel)
+    Bi = rng.integers(0, 3000, (R.n, k)).astype(np.float32)
+    Bi[rng.random((R.n, k)) < 0.2] = np.inf
+    C0 = rng.integers(0, 9000, (R.m, k)).astype(np.float32)
+    for add, mul, acc in (("min", "safe_add", "min"), ("min", "add", "second")):
+        Cd = dev(torch, C0.copy())
+        gg.gpu_spmm_(Cd, Aw, dev(torch, Bi), mul=_ops(gg, mul), add=_ops(gg, add),
+                     accum=_ops(gg, acc))
+        Co = np.asfortranarray(C0.copy())
+        orc.spmm(Co, orc.from_scipy_csr(Hw.to_scipy()), np.asfortranarray(Bi),
+                 add=getattr(orc, "OP_" + add.upper()), mul=getattr(orc, "OP_" + mul.upper()),
+                 accum=getattr(orc, "OP_" + acc.upper()))
+        np.testing.assert_array_equal(host(Cd), Co)
