@@ -430,6 +430,18 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         spmm_ms = e0.elapsed_time(e1) / 3
+        # the reference's column-major (Julia) operand and result
+        Bc, Cc = Bm.t().contiguous().t(), torch.zeros((kk, n), dtype=torch.float32, device=dev).t()
+        gg.gpu_spmm_(Cc, A_f, Bc)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(3):
+            gg.gpu_spmm_(Cc, A_f, Bc)
+        e1.record()
+        torch.cuda.synchronize()
+        spmm_cm_ms = e0.elapsed_time(e1) / 3
+        assert torch.allclose(Cc, Cm, rtol=1e-5, atol=1e-6), "column-major SpMM differs"
+        del Bc, Cc
         B_spmm = nnz * 8 + (n + n) * kk * 4   # pattern and values once, B read once, C written once
         cols = np.random.default_rng(5).choice(kk, 3, replace=False)
         S64 = Rg.to_scipy().astype(np.float64)
@@ -442,9 +454,9 @@ def main():
         assert relerr <= 1e-5, f"SpMM parity failed: max rel err {relerr}"
         del S64, Bh, Ch, Bm, Cm
         res["spmm"] = dict(k=kk, ms=spmm_ms, bytes=B_spmm, gbs=B_spmm / spmm_ms / 1e6,
-                           max_rel_err_vs_fp64=relerr)
+                           colmajor_ms=spmm_cm_ms, max_rel_err_vs_fp64=relerr)
         log(f"SELL SpMM k={kk} (row-major B, C): {spmm_ms * 1e3:.1f} us ({B_spmm / spmm_ms / 1e6:.0f} GB/s "
-            f"algorithmic), rel err {relerr:.1e}")
+            f"algorithmic), column-major {spmm_cm_ms * 1e3:.1f} us, rel err {relerr:.1e}")
 
     # ---------------------------------------------------------- batched BFS
     D = None

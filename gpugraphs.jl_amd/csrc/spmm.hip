@@ -287,6 +287,26 @@ __global__ __launch_bounds__(kBlk) void spmm_heavy_vec_kernel(
   }
 }
 
+// Column-major n x k -> row-major through a 64 x 64 LDS tile (both sides
+// whole lines), for the SELL kernel's row-major operand.
+template <class T>
+__global__ __launch_bounds__(kBlk) void to_rowmajor_kernel(const T* __restrict__ src, int64_t n,
+                                                           int64_t k, T* __restrict__ dst) {
+  __shared__ T tile[64][65];
+  const int64_t tiles_n = (n + 63) / 64;
+  const int64_t r0 = ((int64_t)blockIdx.x % tiles_n) * 64;
+  const int64_t c0 = ((int64_t)blockIdx.x / tiles_n) * 64;
+  for (int i = threadIdx.x; i < 64 * 64; i += kBlk) {
+    const int c = i >> 6, r = i & 63;  // consecutive threads: consecutive rows of a column
+    if (r0 + r < n && c0 + c < k) tile[r][c] = src[(r0 + r) + (c0 + c) * n];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64 * 64; i += kBlk) {
+    const int r = i >> 6, c = i & 63;  // consecutive threads: consecutive columns of a row
+    if (r0 + r < n && c0 + c < k) dst[(r0 + r) * k + c0 + c] = tile[r][c];
+  }
+}
+
 // Rows of several heavy segments: their partials folded in segment order.
 template <class S, class TA, class TB>
 __global__ __launch_bounds__(kBlk) void spmm_combine_kernel(
@@ -402,8 +422,19 @@ int launch_spmm(gg_ctx* ctx, void* C, int c_dtype, int c_layout, const gg_matrix
     // SELL with a row-major B of 16-byte column groups; integer plus keeps the
     // one-lane-per-column kernel (its absorbing exit depends on the order)
     const bool int_plus = S::ADD == GG_OP_ADD && is_int<typename MmTypes<S, TA, TB>::CT>();
-    if (A->format == GG_SELL && g.b_rowmajor && k % VB == 0 && ((uintptr_t)B & 15) == 0 &&
-        !int_plus && !mask && nrows == A->m) {
+    const bool vec = A->format == GG_SELL && k % VB == 0 && !int_plus && !mask && nrows == A->m;
+    if (vec && !g.b_rowmajor) {  // the reference's column-major operand, made row-major once
+      void* bp;
+      GG_TRY(ws_get(ctx, WS_XPERM, sizeof(TB) * (size_t)(A->n * k), &bp));
+      const int64_t tiles = ((A->n + 63) / 64) * ((k + 63) / 64);
+      if (tiles > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "SpMM launch too large");
+      if (tiles > 0)
+        hipLaunchKernelGGL(to_rowmajor_kernel<TB>, dim3((unsigned)tiles), dim3(kBlk), 0,
+                           ctx->stream, (const TB*)B, A->n, k, (TB*)bp);
+      B = bp;
+      g.b_rowmajor = 1;
+    }
+    if (vec && ((uintptr_t)B & 15) == 0) {
       int lv = 0;
       while (lv < 6 && (1LL << lv) < k / VB) ++lv;
       const int64_t jb = (k / VB + (1LL << lv) - 1) >> lv;
