@@ -307,6 +307,32 @@ __global__ __launch_bounds__(kBlk) void to_rowmajor_kernel(const T* __restrict__
   }
 }
 
+// Row-major m x k result -> the caller's column-major C, applying accum
+// (the same accum_apply the kernels use on a direct store).
+template <class TC>
+__global__ __launch_bounds__(kBlk) void from_rowmajor_accum_kernel(const TC* __restrict__ src,
+                                                                   int64_t m, int64_t k,
+                                                                   int accum, TC* __restrict__ dst) {
+  __shared__ TC tile[64][65];
+  const int64_t tiles_m = (m + 63) / 64;
+  const int64_t r0 = ((int64_t)blockIdx.x % tiles_m) * 64;
+  const int64_t c0 = ((int64_t)blockIdx.x / tiles_m) * 64;
+  for (int i = threadIdx.x; i < 64 * 64; i += kBlk) {
+    const int r = i >> 6, c = i & 63;
+    if (r0 + r < m && c0 + c < k) tile[r][c] = src[(r0 + r) * k + c0 + c];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64 * 64; i += kBlk) {
+    const int c = i >> 6, r = i & 63;
+    if (r0 + r < m && c0 + c < k) {
+      const int64_t ci = (r0 + r) + (c0 + c) * m;
+      TC v = tile[r][c];
+      if (accum_reads_c(accum)) v = accum_apply<TC>(accum, dst[ci], v);
+      dst[ci] = v;
+    }
+  }
+}
+
 // Rows of several heavy segments: their partials folded in segment order.
 template <class S, class TA, class TB>
 __global__ __launch_bounds__(kBlk) void spmm_combine_kernel(
@@ -447,6 +473,18 @@ int launch_spmm(gg_ctx* ctx, void* C, int c_dtype, int c_layout, const gg_matrix
       if (A->n_heavy > 0)
         GG_TRY(ws_get(ctx, WS_PARTIAL, sizeof(typename MmTypes<S, TA, TB>::A) *
                                            (size_t)(A->heavy_nseg * k), &part));
+      // a column-major C is written row-major to a workspace first (the
+      // slots' rows are scattered, so direct column stores are short) and
+      // turned with accum applied by from_rowmajor_accum_kernel
+      void* const C_user = C;
+      const int accum_user = accum;
+      const bool via_rm = !g.c_rowmajor &&
+                          std::is_same<TC, typename MmTypes<S, TA, TB>::CT>::value;
+      if (via_rm) {
+        GG_TRY(ws_get(ctx, WS_MS1, sizeof(TC) * (size_t)(A->m * k), &C));
+        g.c_rowmajor = 1;
+        accum = GG_OP_SECOND;
+      }
       HotTimer ht(ctx);
       if (n_slots > 0)
         hipLaunchKernelGGL((spmm_sell_vec_kernel<S, TA, TB, VB>), dim3((unsigned)nb_l), dim3(kBlk),
@@ -459,6 +497,13 @@ int launch_spmm(gg_ctx* ctx, void* C, int c_dtype, int c_layout, const gg_matrix
           hipLaunchKernelGGL((spmm_combine_kernel<S, TA, TB>),
                              dim3(blocks_for(A->n_heavy * k, kBlk)), dim3(kBlk), 0, ctx->stream,
                              (TC*)C, g, A->heavy_row, A->heavy_seg, A->n_heavy, part, accum);
+      }
+      if (via_rm) {
+        const int64_t tiles = ((A->m + 63) / 64) * ((k + 63) / 64);
+        if (tiles > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "SpMM launch too large");
+        if (tiles > 0)
+          hipLaunchKernelGGL(from_rowmajor_accum_kernel<TC>, dim3((unsigned)tiles), dim3(kBlk), 0,
+                             ctx->stream, (const TC*)C, A->m, k, accum_user, (TC*)C_user);
       }
       GG_HIP(hipGetLastError());
       return GG_OK;
