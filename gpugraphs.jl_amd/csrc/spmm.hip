@@ -287,6 +287,63 @@ __global__ __launch_bounds__(kBlk) void spmm_heavy_vec_kernel(
   }
 }
 
+// CSR rows with VB columns per lane (the SELL kernel's column groups; the
+// group's lanes read the same entries of the row, a broadcast): rows
+// r0 .. r0 + nrows - 1, mask applied per row (src/spmm/CSR.jl:111-204).
+template <class S, class TA, class TB, int VB>
+__global__ __launch_bounds__(kBlk) void spmm_csr_vec_kernel(
+    typename MmTypes<S, TA, TB>::TC* __restrict__ Cm, const TB* __restrict__ B, MmArgs g,
+    int accum, const void* __restrict__ mask, int mdt, int logLv) {
+  typedef MmTypes<S, TA, TB> T;
+  typedef typename T::A A;
+  typedef typename T::CT CT;
+  typedef TB VT __attribute__((ext_vector_type(VB)));
+  const int Lv = 1 << logLv;
+  const int lane = threadIdx.x & (Lv - 1);
+  const int64_t kq = g.k / VB;
+  const int64_t jblocks = (kq + Lv - 1) >> logLv;
+  const int64_t G = ((int64_t)gridDim.x * kBlk) >> logLv;
+  const TA* __restrict__ val = (const TA*)g.val;
+  for (int64_t q = ((int64_t)blockIdx.x * kBlk + threadIdx.x) >> logLv; q < g.nrows * jblocks;
+       q += G) {
+    const int64_t row = g.r0 + q / jblocks;
+    const int64_t jq = (q % jblocks) * Lv + lane;
+    if (jq >= kq || !mask_on(mask, mdt, row)) continue;
+    const int64_t j0 = jq * VB;
+    A acc[VB];
+#pragma unroll
+    for (int t = 0; t < VB; ++t) acc[t] = T::MO::neutral();
+    int64_t i = (int64_t)g.ptr[row] - 1;
+    const int64_t e = (int64_t)g.ptr[row + 1] - 1;
+    for (; i + 4 <= e; i += 4) {
+      int32_t c[4];
+      TA v[4];
+      VT b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) c[u] = g.idx[i + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = val[i + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) b[u] = *(const VT*)&B[(int64_t)(c[u] - 1) * g.k + j0];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int t = 0; t < VB; ++t)
+          mm_fold<A, typename T::MO>(acc[t], A(T::MU::ap(CT(v[u]), CT(b[u][t]), c[u])));
+    }
+    for (; i < e; ++i) {
+      const int32_t col = g.idx[i];
+      const VT b = *(const VT*)&B[(int64_t)(col - 1) * g.k + j0];
+#pragma unroll
+      for (int t = 0; t < VB; ++t)
+        mm_fold<A, typename T::MO>(acc[t], A(T::MU::ap(CT(val[i]), CT(b[t]), col)));
+    }
+#pragma unroll
+    for (int t = 0; t < VB; ++t)
+      mm_store<typename T::TC, CT>(Cm, g, row, j0 + t, accum, CT(acc[t]));
+  }
+}
+
 // Column-major n x k -> row-major through a 64 x 64 LDS tile (both sides
 // whole lines), for the SELL kernel's row-major operand.
 template <class T>
@@ -448,7 +505,10 @@ int launch_spmm(gg_ctx* ctx, void* C, int c_dtype, int c_layout, const gg_matrix
     // SELL with a row-major B of 16-byte column groups; integer plus keeps the
     // one-lane-per-column kernel (its absorbing exit depends on the order)
     const bool int_plus = S::ADD == GG_OP_ADD && is_int<typename MmTypes<S, TA, TB>::CT>();
-    const bool vec = A->format == GG_SELL && k % VB == 0 && !int_plus && !mask && nrows == A->m;
+    const bool sell_vec = A->format == GG_SELL && k % VB == 0 && !int_plus && !mask &&
+                          nrows == A->m;
+    const bool csr_vec = A->format == GG_CSR && k % VB == 0 && !int_plus;
+    const bool vec = sell_vec || csr_vec;
     if (vec && !g.b_rowmajor) {  // the reference's column-major operand, made row-major once
       void* bp;
       GG_TRY(ws_get(ctx, WS_XPERM, sizeof(TB) * (size_t)(A->n * k), &bp));
@@ -460,7 +520,20 @@ int launch_spmm(gg_ctx* ctx, void* C, int c_dtype, int c_layout, const gg_matrix
       B = bp;
       g.b_rowmajor = 1;
     }
-    if (vec && ((uintptr_t)B & 15) == 0) {
+    if (csr_vec && ((uintptr_t)B & 15) == 0) {
+      int lv = 0;
+      while (lv < 6 && (1LL << lv) < k / VB) ++lv;
+      const int64_t jb = (k / VB + (1LL << lv) - 1) >> lv;
+      unsigned nb = blocks_for((nrows * jb) << lv, kBlk);
+      const unsigned cap = (unsigned)ctx->num_cus * 64;
+      if (nb > cap) nb = cap;
+      HotTimer ht(ctx);
+      hipLaunchKernelGGL((spmm_csr_vec_kernel<S, TA, TB, VB>), dim3(nb), dim3(kBlk), 0,
+                         ctx->stream, (TC*)C, (const TB*)B, g, accum, mask, mask_dtype, lv);
+      GG_HIP(hipGetLastError());
+      return GG_OK;
+    }
+    if (sell_vec && ((uintptr_t)B & 15) == 0) {
       int lv = 0;
       while (lv < 6 && (1LL << lv) < k / VB) ++lv;
       const int64_t jb = (k / VB + (1LL << lv) - 1) >> lv;

@@ -234,23 +234,30 @@ def test_spmm_plus_times_and_min_plus(env, fmt, k, layout):
     np.testing.assert_array_equal(host(Cd), Co)
 
 
-def test_spmm_csr_mask_range_and_asserts(env):
+@pytest.mark.parametrize("k", [6, 8, 32])
+@pytest.mark.parametrize("layout", ["col", "row"])
+def test_spmm_csr_mask_range_and_asserts(env, k, layout):
+    """CSR SpMM with a row mask or a row range (one lane per column for k = 6,
+    16-byte column groups for 8 and 32), exact against the oracle."""
     gg, torch, orc = env
-    rng = np.random.default_rng(7)
+    rng = np.random.default_rng(7 + k)
     S = random_csr(rng, 300, 300, 0.05)
     S.data = rng.integers(1, 100, S.nnz).astype(np.float32)
-    B = rng.integers(0, 100, (300, 6)).astype(np.float32)
+    B = rng.integers(0, 100, (300, k)).astype(np.float32)
     A = gg.SparseGPUMatrixCSR(S, gg.ROCBackend(0))
     mask = (rng.random(300) < 0.4).astype(np.int32)
+    mk = (lambda a: _cm(torch, a)) if layout == "col" else (lambda a: dev(torch, a))
+    print_case("spmm csr mask range", k, layout)
     for kw, okw in [({"mask": dev(torch, mask)}, {"mask": mask}),
-                    ({"range_": range(50, 151)}, {"range_": (50, 101)})]:
-        C0 = np.full((300, 6), -1.0, np.float32)
-        Cd = _cm(torch, C0)
-        gg.gpu_spmm_(Cd, A, _cm(torch, B), **kw)
+                    ({"range_": range(50, 151)}, {"range_": (50, 101)}), ({}, {})]:
+        C0 = np.full((300, k), -1.0, np.float32)
+        Cd = mk(C0)
+        gg.gpu_spmm_(Cd, A, mk(B), **kw)
         torch.cuda.synchronize()
         Co = np.asfortranarray(C0.copy())
         orc.spmm(Co, orc_mat(orc, S, "csr"), np.asfortranarray(B), **okw)
         np.testing.assert_array_equal(host(Cd), Co)
+    B = B[:, :6].copy()
     with pytest.raises(AssertionError):
         gg.gpu_spmm_(_cm(torch, np.zeros((300, 5), np.float32)), A, _cm(torch, B))
     with pytest.raises(AssertionError):
@@ -715,3 +722,28 @@ def test_spmm_power_law_heavy_rows(env, relabel):
                  add=getattr(orc, "OP_" + add.upper()), mul=getattr(orc, "OP_" + mul.upper()),
                  accum=getattr(orc, "OP_" + acc.upper()))
         np.testing.assert_array_equal(host(Cd), Co)
+
+
+@pytest.mark.parametrize("relabel", [False, True])
+def test_sell_device_builder_degenerate(env, relabel):
+    """Rows without entries and a single-row matrix: the device builder still
+    equals the host builder."""
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    cases = [GEN.HostCSR(7, 7, np.ones(8, np.int32), np.zeros(0, np.int32),
+                         np.zeros(0, np.float32)),
+             GEN.HostCSR(1, 1, np.array([1, 2], np.int32), np.array([1], np.int32),
+                         np.array([2.5], np.float32))]
+    for H in cases:
+        for layout in ((32, 1, 0, 4096), (64, 8, 1, 4)):
+            print_case("sell device builder degenerate", H.m, layout, relabel)
+            ref = H.to_sell(gg.ROCBackend(0), *layout, relabel=relabel)
+            got = gg.SparseGPUMatrixSELL.build_on_device(H.to_csr(gg.ROCBackend(0)), *layout,
+                                                         relabel=relabel)
+            a, b = _sell_arrays(torch, ref), _sell_arrays(torch, got)
+            for key in a:
+                if a[key] is None or b[key] is None:
+                    assert (a[key] is None or len(a[key]) == 0) and \
+                        (b[key] is None or len(b[key]) == 0), key
+                else:
+                    np.testing.assert_array_equal(a[key], b[key], err_msg=key)
