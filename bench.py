@@ -58,7 +58,8 @@ def measured_traffic(args, relabel, kernel):
     runs (profiles/hbm_traffic.json) when this run is the workload they
     measured (the default R-MAT 22 configuration), else None."""
     default = (args.scale == 22 and args.roots == 64 and relabel and args.heavy == 1024 and
-               args.bfs_seg == 65536 and args.seg == 4096 and args.push_alpha == 16.0)
+               args.bfs_seg == 65536 and args.seg == 4096 and args.push_alpha == 16.0 and
+               args.sort_entries)
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if not default or not os.path.exists(path):
         return None
@@ -102,9 +103,10 @@ def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma,
     Gd = G.grid(args.grid, args.grid, 0.01, 1, 1000, seed=seed, dtype=np.float32)
     w_r = rng.integers(1, 1001, Rg.nnz).astype(np.float32)
     Rw = G.HostCSR(Rg.m, Rg.n, Rg.rowptr, Rg.colval, w_r)
-    cases = [("grid", Gd, Gd.to_sell(R0, 64, sigma, 0, args.seg, relabel),
+    se = args.sort_entries
+    cases = [("grid", Gd, Gd.to_sell(R0, 64, sigma, 0, args.seg, relabel, se),
               (rng.choice(Gd.m, args.sssp_sources, replace=False) + 1).astype(np.int32)),
-             ("rmat", Rw, Rw.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel),
+             ("rmat", Rw, Rw.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel, se),
               roots[:args.sssp_k].astype(np.int32))]
     torch.cuda.synchronize()
     log(f"SSSP graphs: grid{args.grid} n={Gd.m} nnz={Gd.nnz}, weighted rmat; {time.time() - t:.1f}s")
@@ -205,15 +207,16 @@ def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
     specs = []   # (name, bfs matrix, sssp matrix)
     for sc in (16, 18, 20, 22):
         H = G.rmat(sc, 16, 0.57, 0.19, 0.19, seed=seed + sc)
-        M = H.to_sell(R0, 64, sigma, args.heavy, args.bfs_seg, relabel)
+        M = H.to_sell(R0, 64, sigma, args.heavy, args.bfs_seg, relabel, args.sort_entries)
         specs.append((f"rmat{sc}", M, M))
     for side in (1024, 2048, 4096):
         H = G.grid(side, side, 0.01, 1, 1000, seed=seed + side, dtype=np.int32)
         Hb = G.HostCSR(H.m, H.n, H.rowptr, H.colval, np.ones(H.nnz, np.bool_), H.symmetric)
-        specs.append((f"grid{side}", Hb.to_sell(R0, 64, sigma, 0, args.seg, relabel),
-                      H.to_sell(R0, 64, sigma, 0, args.seg, relabel)))
+        specs.append((f"grid{side}", Hb.to_sell(R0, 64, sigma, 0, args.seg, relabel,
+                                                args.sort_entries),
+                      H.to_sell(R0, 64, sigma, 0, args.seg, relabel, args.sort_entries)))
     H = G.random_graph(1 << 16, 16, seed=seed + 7)
-    M = H.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel)
+    M = H.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel, args.sort_entries)
     specs.append(("uniform16", M, M))
     torch.cuda.synchronize()
     log(f"serving graphs built in {time.time() - t:.1f}s")
@@ -307,6 +310,9 @@ def main():
                     help="levels / iterations per captured graph launch (0: host-stepped loops)")
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--spmm-k", type=int, default=32, help="columns of the SpMM leg (0: skip)")
+    ap.add_argument("--no-sort-entries", dest="sort_entries", action="store_false",
+                    help="keep each row's entries in input order (default: by increasing stored "
+                         "column, hubs first)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--grid", type=int, default=4096, help="Bellman-Ford grid side")
     ap.add_argument("--sssp-sources", type=int, default=16, help="grid Bellman-Ford sources")
@@ -356,22 +362,25 @@ def main():
     log(f"rmat{args.scale}: n={n} nnz={nnz} max_deg={deg.max()} gen {time.time() - t:.1f}s")
 
     t = time.time()
-    A_f = Rg.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel)   # fp32 values for SpMV
+    A_f = Rg.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel,   # fp32 values for SpMV
+                     sort_entries=args.sort_entries)
     Rb = G.HostCSR(n, n, Rg.rowptr, Rg.colval, np.ones(nnz, np.bool_), Rg.symmetric)
-    A_b = Rb.to_sell(R0, 64, sigma, args.heavy, args.bfs_seg, relabel)   # Bool pattern for BFS
+    A_b = Rb.to_sell(R0, 64, sigma, args.heavy, args.bfs_seg, relabel,   # Bool pattern for BFS
+                     sort_entries=args.sort_entries)
     A_csr = Rb.to_csr(R0)                                          # out-edges for push (symmetric)
     torch.cuda.synchronize()
     host_build_s = time.time() - t
     log(f"SELL builds + upload {host_build_s:.1f}s; stored/nnz {(A_f.n_stored + (int(A_f.heavy_ptr[-1].item()) if A_f.n_heavy else 0)) / nnz:.3f}")
     # the same BFS layout built on the device from the CSR already in HBM
-    gg.SparseGPUMatrixSELL.build_on_device(A_csr, 64, sigma, args.heavy, args.bfs_seg, relabel)
+    gg.SparseGPUMatrixSELL.build_on_device(A_csr, 64, sigma, args.heavy, args.bfs_seg, relabel,
+                                           args.sort_entries)
     torch.cuda.synchronize()
     t = time.perf_counter()
     A_d = gg.SparseGPUMatrixSELL.build_on_device(A_csr, 64, sigma, args.heavy, args.bfs_seg,
-                                                 relabel)
+                                                 relabel, args.sort_entries)
     torch.cuda.synchronize()
     dev_build_ms = (time.perf_counter() - t) * 1e3
-    assert torch.equal(A_d.colval, A_b.colval) and torch.equal(A_d.slice_ptr, A_b.slice_ptr), \
+    assert torch.equal(A_d.slice_ptr, A_b.slice_ptr) and torch.equal(A_d.colval, A_b.colval), \
         "device-built SELL differs from the host-built one"
     del A_d
     log(f"SELL build on the device from the resident CSR: {dev_build_ms:.1f} ms")
@@ -612,9 +621,10 @@ def main():
         "config": {"workload": f"rmat{args.scale} ef16 a.57 b.19 c.19 symmetrised permuted; "
                                f"batched BFS from {args.roots} random non-isolated roots "
                                "(one step = one k=64 or-and SpMM traversal); SELL-64 "
-                               + ("degree-relabelled" if relabel else "sigma4096"),
+                               + ("degree-relabelled" if relabel else "sigma4096")
+                               + (", row entries by column" if args.sort_entries else ""),
                    "n": n, "nnz": nnz, "roots": args.roots, "slice_size": 64, "sigma": sigma,
-                   "relabel": relabel, "heavy_threshold": args.heavy,
+                   "relabel": relabel, "sorted_entries": args.sort_entries, "heavy_threshold": args.heavy,
                    "graph_steps": args.graph_steps, "push_alpha": args.push_alpha,
                    "parallelism": "single GPU"},
         "p50_ms": None if not ms else round(ms["p50_ms"], 3),

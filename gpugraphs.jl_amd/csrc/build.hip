@@ -86,11 +86,45 @@ __global__ void vinv_kernel(const int32_t* __restrict__ slot_rows, int64_t nl,
   else if (i < nl + nh) vinv[heavy[i - nl] - 1] = (int32_t)(n_slots + (i - nl) + 1);
 }
 
+// Sort keys of the entries (the stored column) and their positions, for
+// GG_SELL_SORT_ENTRIES.
+__global__ void entry_keys_kernel(const int32_t* __restrict__ colval, int64_t nnz,
+                                  const int32_t* __restrict__ vinv, uint32_t* key, int32_t* pos) {
+  const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (e >= nnz) return;
+  const int32_t c = colval[e];
+  key[e] = (uint32_t)(vinv ? vinv[c - 1] : c);
+  pos[e] = (int32_t)e;
+}
+__global__ void zero_based_kernel(const int32_t* __restrict__ rowptr, int64_t m, int32_t* off) {
+  const int64_t r = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (r <= m) off[r] = rowptr[r] - 1;
+}
+
+// Entry t of a row starting at `start`: its stored column and its position
+// in the input (through the sorted order when sorted_col is set).
+struct EntryMap {
+  const int32_t* colval;
+  const int32_t* vinv;
+  const uint32_t* sorted_col;  // mapped columns, rows sorted; NULL: input order
+  const int32_t* sorted_pos;
+  __device__ __forceinline__ void get(int64_t e, int32_t& col, int64_t& src) const {
+    if (sorted_col) {
+      col = (int32_t)sorted_col[e];
+      src = sorted_pos[e];
+    } else {
+      const int32_t c = colval[e];
+      col = vinv ? vinv[c - 1] : c;
+      src = e;
+    }
+  }
+};
+
 template <class V>
-__global__ void slots_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
+__global__ void slots_kernel(const int32_t* __restrict__ rowptr, EntryMap em,
                              const V* __restrict__ val, const int32_t* __restrict__ slot_rows,
                              int64_t nl, int64_t n_slots, int64_t C,
-                             const int64_t* __restrict__ wscan, const int32_t* __restrict__ vinv,
+                             const int64_t* __restrict__ wscan,
                              int32_t* slice_ptr, int32_t* out_col, V* out_val, int32_t* perm,
                              int32_t* slot_len) {
   const int64_t q = (int64_t)blockIdx.x * kB + threadIdx.x;
@@ -111,9 +145,11 @@ __global__ void slots_kernel(const int32_t* __restrict__ rowptr, const int32_t* 
   for (int64_t k = 0; k < w; ++k) {
     const int64_t pos = base + k * C + r;  // column-major inside the slice
     if (k < len) {
-      const int32_t c = colval[start + k];
-      out_col[pos] = vinv ? vinv[c - 1] : c;
-      out_val[pos] = val[start + k];
+      int32_t c;
+      int64_t src;
+      em.get(start + k, c, src);
+      out_col[pos] = c;
+      out_val[pos] = val[src];
     } else {
       out_col[pos] = -1;
       out_val[pos] = V(0);
@@ -122,10 +158,10 @@ __global__ void slots_kernel(const int32_t* __restrict__ rowptr, const int32_t* 
 }
 
 template <class V>
-__global__ void heavy_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
+__global__ void heavy_kernel(const int32_t* __restrict__ rowptr, EntryMap em,
                              const V* __restrict__ val, const int32_t* __restrict__ heavy,
                              int64_t nh, const int64_t* __restrict__ escan,
-                             const int64_t* __restrict__ gscan, const int32_t* __restrict__ vinv,
+                             const int64_t* __restrict__ gscan,
                              int32_t* h_row, int64_t* h_ptr, int32_t* h_seg, int32_t* h_col,
                              V* h_val) {
   const int64_t i = blockIdx.x;
@@ -142,9 +178,11 @@ __global__ void heavy_kernel(const int32_t* __restrict__ rowptr, const int32_t* 
     }
   }
   for (int64_t t = threadIdx.x; t < len; t += kB) {
-    const int32_t c = colval[start + t];
-    h_col[o + t] = vinv ? vinv[c - 1] : c;
-    h_val[o + t] = val[start + t];
+    int32_t c;
+    int64_t src;
+    em.get(start + t, c, src);
+    h_col[o + t] = c;
+    h_val[o + t] = val[src];
   }
 }
 
@@ -168,6 +206,9 @@ struct gg_sell_plan {
   int64_t* escan = nullptr;      // [nh+1] heavy entry offsets
   int64_t* gscan = nullptr;      // [nh+1] heavy segment offsets
   int32_t* vinv = nullptr;       // [m] (relabel)
+  bool sort_entries = false;
+  uint32_t* sorted_col = nullptr;  // [nnz] rows by increasing stored column
+  int32_t* sorted_pos = nullptr;   // [nnz] their input positions
   gg_sell_sizes sizes{};
   std::vector<void*> allocs;
 };
@@ -319,6 +360,25 @@ int plan_build(gg_sell_plan* p, int64_t slice_size, int64_t sigma, int64_t thr) 
       hipLaunchKernelGGL(vinv_kernel, dim3(blocks_for(p->nl + p->nh, kB)), dim3(kB), 0, st,
                          p->slot_rows, p->nl, p->heavy, p->nh, p->ns * C, p->vinv);
   }
+  if (p->sort_entries && p->nnz > 0) {  // each row's entries by stored column, ties in input order
+    if (p->nnz > 2147483647LL) return set_error(GG_ERR_ARGUMENT, "too many entries to sort");
+    uint32_t* k0;
+    int32_t *p0, *off;
+    GG_TRY(plan_array(p, p->nnz, &k0));
+    GG_TRY(plan_array(p, p->nnz, &p0));
+    GG_TRY(plan_array(p, p->nnz, &p->sorted_col));
+    GG_TRY(plan_array(p, p->nnz, &p->sorted_pos));
+    GG_TRY(plan_array(p, m + 1, &off));
+    hipLaunchKernelGGL(entry_keys_kernel, dim3(blocks_for(p->nnz, kB)), dim3(kB), 0, st, p->colval,
+                       p->nnz, p->relabel ? p->vinv : nullptr, k0, p0);
+    hipLaunchKernelGGL(zero_based_kernel, dim3(blocks_for(m + 1, kB)), dim3(kB), 0, st, p->rowptr,
+                       m, off);
+    GG_TRY(cub_call(p, [&](void* t, size_t& tb) {
+      return hipcub::DeviceSegmentedRadixSort::SortPairs(t, tb, k0, p->sorted_col, p0,
+                                                         p->sorted_pos, (int)p->nnz, (int)m, off,
+                                                         off + 1, 0, 32, st);
+    }));
+  }
   GG_HIP(hipGetLastError());
   gg_sell_sizes& z = p->sizes;
   z.slice_size = C;
@@ -376,6 +436,7 @@ int gg_sell_plan_device(gg_ctx* ctx, int64_t m, int64_t n, const int32_t* rowptr
   p->nzval = nzval;
   p->seg_len = heavy_seg_len;
   p->relabel = relabel;
+  p->sort_entries = (flags & GG_SELL_SORT_ENTRIES) != 0;
   p->reference = sigma <= 1 && heavy_threshold <= 0 && !relabel;
   const int st = plan_build(p, slice_size, sigma, heavy_threshold);
   if (st != GG_OK) {
@@ -400,7 +461,7 @@ int gg_sell_fill_device(gg_sell_plan* p, gg_matrix* out) {
   gg_ctx* ctx = p->ctx;
   GG_HIP(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  const int32_t* vinv = p->relabel ? p->vinv : nullptr;
+  const EntryMap em{p->colval, p->relabel ? p->vinv : nullptr, p->sorted_col, p->sorted_pos};
   if (z.n_slots == 0) {
     const int32_t one = 1;
     GG_HIP(hipMemcpyAsync((int32_t*)out->ptr, &one, sizeof one, hipMemcpyHostToDevice, st));
@@ -410,14 +471,14 @@ int gg_sell_fill_device(gg_sell_plan* p, gg_matrix* out) {
     typedef typename decltype(t)::type T;
     if (z.n_slots > 0)
       hipLaunchKernelGGL(slots_kernel<T>, dim3(blocks_for(z.n_slots, kB)), dim3(kB), 0, st,
-                         p->rowptr, p->colval, (const T*)p->nzval, p->slot_rows, p->nl, z.n_slots,
-                         p->C, p->wscan, vinv, (int32_t*)out->ptr, (int32_t*)out->idx,
+                         p->rowptr, em, (const T*)p->nzval, p->slot_rows, p->nl, z.n_slots,
+                         p->C, p->wscan, (int32_t*)out->ptr, (int32_t*)out->idx,
                          (T*)out->val, z.has_perm ? (int32_t*)out->perm : nullptr,
                          z.has_perm ? (int32_t*)out->slot_len : nullptr);
     if (z.n_heavy > 0)
       hipLaunchKernelGGL(heavy_kernel<T>, dim3((unsigned)z.n_heavy), dim3(kB), 0, st, p->rowptr,
-                         p->colval, (const T*)p->nzval, p->heavy, z.n_heavy, p->escan, p->gscan,
-                         vinv, (int32_t*)out->heavy_row, (int64_t*)out->heavy_ptr,
+                         em, (const T*)p->nzval, p->heavy, z.n_heavy, p->escan, p->gscan,
+                         (int32_t*)out->heavy_row, (int64_t*)out->heavy_ptr,
                          (int32_t*)out->heavy_seg, (int32_t*)out->heavy_col,
                          (T*)out->heavy_val);
     GG_HIP(hipGetLastError());

@@ -192,6 +192,7 @@ int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* co
   // entries of a row in storage order: (stored column, source position); with
   // relabelled ids the columns are mapped but keep the row's input order, so
   // every fold visits a row's terms in the order the reference's SELL does
+  const bool sort_entries = (flags & GG_SELL_SORT_ENTRIES) != 0;
   auto row_entries = [&](int32_t row, std::vector<std::pair<int32_t, int64_t>>& ent) {
     const int64_t start = rowptr[row - 1] - 1, len = rowptr[row] - rowptr[row - 1];
     ent.resize(len);
@@ -199,6 +200,7 @@ int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* co
       const int32_t c = colval[start + t];
       ent[t] = {relabel ? vinv[c - 1] : c, start + t};
     }
+    if (sort_entries) std::stable_sort(ent.begin(), ent.end());
   };
 
   // slice widths and pointers (src/storage/SELL.jl:309-321)

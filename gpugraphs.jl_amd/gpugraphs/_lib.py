@@ -23,6 +23,7 @@ FLAG_UNIT_PATTERN = 1
 FLAG_INTERNAL_IDS = 2
 FLAG_SYMMETRIC = 4
 SELL_RELABEL = 1
+SELL_SORT_ENTRIES = 2
 
 
 class DimensionMismatch(ValueError):

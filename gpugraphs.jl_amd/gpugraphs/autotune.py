@@ -44,6 +44,9 @@ def default_candidates(H: HostCSR):
                 c.append(dict(layout="sell", slice_size=64, sigma=1, relabel=True, heavy=heavy,
                               seg=seg))
     c.append(dict(layout="sell", slice_size=64, sigma=1, relabel=True, heavy=0, seg=4096))
+    if 1024 < mx:  # each row's entries by increasing internal column (hubs first)
+        c.append(dict(layout="sell", slice_size=64, sigma=1, relabel=True, heavy=1024, seg=4096,
+                      sort=True))
     c.append(dict(layout="sell", slice_size=64, sigma=4096, relabel=False,
                   heavy=256 if 256 < mx else 0, seg=4096))
     c.append(dict(layout="sell", slice_size=32, sigma=1, relabel=True,
@@ -63,7 +66,7 @@ def build(H: HostCSR, backend, cand: dict):
     if cand["layout"] == "csr":
         return H.to_csr(backend)
     return H.to_sell(backend, cand["slice_size"], cand["sigma"], cand["heavy"], cand["seg"],
-                     cand["relabel"])
+                     cand["relabel"], cand.get("sort", False))
 
 
 def _runner(workload, A, H, roots, torch, gg):

@@ -42,11 +42,11 @@ class HostCSR:
         return M
 
     def to_sell(self, backend, slice_size=64, sigma=4096, heavy_threshold=0, heavy_seg_len=4096,
-                relabel=False):
+                relabel=False, sort_entries=False):
         from .storage import SparseGPUMatrixSELL
         M = SparseGPUMatrixSELL.__new__(SparseGPUMatrixSELL)
         M._build(self.m, self.n, self.rowptr, self.colval, np.ascontiguousarray(self.nzval),
-                 backend, slice_size, sigma, heavy_threshold, heavy_seg_len, relabel)
+                 backend, slice_size, sigma, heavy_threshold, heavy_seg_len, relabel, sort_entries)
         M.symmetric = self.symmetric
         return M
 

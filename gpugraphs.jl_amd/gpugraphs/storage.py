@@ -380,10 +380,12 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
     length and stores columns as internal ids (slot index, then heavy index),
     so a row's neighbours are gathered from the compact, degree-ordered state
     arrays; every operation still takes and returns vectors in original order.
+    sort_entries=True stores each row's entries by increasing stored column
+    (hubs first when relabelled): folds visit a row's terms in that order.
     """
 
     def __init__(self, *args, slice_size=None, sigma=1, heavy_threshold=0, heavy_seg_len=4096,
-                 relabel=False):
+                 relabel=False, sort_entries=False):
         if len(args) == 11:
             self._init_raw(*args)
             return
@@ -407,17 +409,19 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
             colval = (S.indices + 1).astype(np.int32)
             nzval = np.ascontiguousarray(S.data)
         self._build(m, n, rowptr, colval, nzval, backend, int(slice_size), int(sigma),
-                    int(heavy_threshold), int(heavy_seg_len), bool(relabel))
+                    int(heavy_threshold), int(heavy_seg_len), bool(relabel), bool(sort_entries))
 
     def _build(self, m, n, rowptr, colval, nzval, backend, slice_size, sigma, heavy_threshold,
-               heavy_seg_len, relabel=False):
+               heavy_seg_len, relabel=False, sort_entries=False):
         vdt = nzval.dtype
         if vdt == np.bool_:
             nzval = nzval.view(np.uint8)
         out = C.POINTER(GGSellHost)()
         check(lib().gg_sell_build(m, n, rowptr.ctypes.data, colval.ctypes.data, nzval.ctypes.data,
                                   dtype_code(vdt), slice_size, sigma, heavy_threshold,
-                                  heavy_seg_len, _lib.SELL_RELABEL if relabel else 0,
+                                  heavy_seg_len,
+                                  (_lib.SELL_RELABEL if relabel else 0)
+                                  | (_lib.SELL_SORT_ENTRIES if sort_entries else 0),
                                   C.byref(out)), "SparseGPUMatrixSELL")
         try:
             h = out.contents
@@ -462,7 +466,7 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
 
     @classmethod
     def build_on_device(cls, A, slice_size=32, sigma=1, heavy_threshold=0, heavy_seg_len=4096,
-                        relabel=False):
+                        relabel=False, sort_entries=False):
         """The same layouts as the constructor, built in device memory from a
         SparseGPUMatrixCSR on a ROCBackend (gg_sell_plan_device /
         gg_sell_fill_device): the rows never visit the host."""
@@ -477,7 +481,9 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
         check(lib().gg_sell_plan_device(ctx, A.m, A.n, ptr(A.rowptr), ptr(A.colval),
                                         ptr(A.nzval), dtype_code(vdt), int(slice_size),
                                         int(sigma), int(heavy_threshold), int(heavy_seg_len),
-                                        _lib.SELL_RELABEL if relabel else 0, C.byref(plan),
+                                        (_lib.SELL_RELABEL if relabel else 0)
+                                        | (_lib.SELL_SORT_ENTRIES if sort_entries else 0),
+                                        C.byref(plan),
                                         C.byref(z)), "SparseGPUMatrixSELL")
         try:
             dev = torch.device("cuda", backend.device)

@@ -167,7 +167,12 @@ typedef enum gg_sell_flags {
    * a heavy row's is n_slots + its heavy index, and stored columns use
    * internal ids.  Entry points gather x into internal order first; results
    * are written at the original row ids.  Needs a square matrix. */
-  GG_SELL_RELABEL = 1
+  GG_SELL_RELABEL = 1,
+  /* Store each row's entries by increasing stored column (with relabelling:
+   * hubs first) instead of the input order.  Folds then visit a row's terms
+   * in that order: same results for min / max / or / and, a reordered sum
+   * for plus on floats. */
+  GG_SELL_SORT_ENTRIES = 2
 } gg_sell_flags;
 
 typedef struct gg_ctx gg_ctx;

@@ -33,7 +33,8 @@ def host(t):
     return t.cpu().numpy()
 
 
-FORMATS = ["csr", "sell_ref", "sell_hybrid", "sell_relabel"]
+# sell_relabel_sorted: each row's entries by increasing internal column
+FORMATS = ["csr", "sell_ref", "sell_hybrid", "sell_relabel", "sell_relabel_sorted"]
 
 
 def make(gg, S, fmt):
@@ -42,11 +43,12 @@ def make(gg, S, fmt):
         return gg.SparseGPUMatrixCSR(S, R)
     if fmt == "sell_ref":
         return gg.SparseGPUMatrixSELL(S, R, 32)
-    if fmt == "sell_relabel":
+    if fmt.startswith("sell_relabel"):
         # degree-relabelled rows (internal column ids) with a heavy partition
         if S.shape[0] != S.shape[1]:
             pytest.skip("relabelling needs a square matrix")
-        return gg.SparseGPUMatrixSELL(S, R, 64, heavy_threshold=8, heavy_seg_len=16, relabel=True)
+        return gg.SparseGPUMatrixSELL(S, R, 64, heavy_threshold=8, heavy_seg_len=16, relabel=True,
+                                      sort_entries=fmt.endswith("_sorted"))
     # sigma-sorted C=64 slices with a heavy partition cut into 16-entry segments
     return gg.SparseGPUMatrixSELL(S, R, 64, sigma=64, heavy_threshold=8, heavy_seg_len=16)
 
@@ -88,8 +90,9 @@ def test_spmv_power_law_rows_f32(env, fmt):
     x = np.random.default_rng(1).random(R.n, dtype=np.float32)
     if fmt == "sell_hybrid":
         A = R.to_sell(gg.ROCBackend(0), 64, 4096, 256, 512)
-    elif fmt == "sell_relabel":
-        A = R.to_sell(gg.ROCBackend(0), 64, 1, 256, 512, relabel=True)
+    elif fmt.startswith("sell_relabel"):
+        A = R.to_sell(gg.ROCBackend(0), 64, 1, 256, 512, relabel=True,
+                      sort_entries=fmt.endswith("_sorted"))
     elif fmt == "sell_ref":
         A = R.to_sell(gg.ROCBackend(0), 64, 1, 0)
     else:
@@ -644,7 +647,8 @@ def _sell_arrays(torch, M):
                                     (64, 1, 8, 16, True), (64, 4096, 256, 512, False),
                                     (16, 1, 0, 4096, True), (64, 1, 0, 4096, False),
                                     (8, 3, 5, 7, False)])
-def test_sell_device_builder_equals_host(env, layout):
+@pytest.mark.parametrize("sort_entries", [False, True])
+def test_sell_device_builder_equals_host(env, layout, sort_entries):
     """The device builder (gg_sell_plan_device / gg_sell_fill_device) gives
     the host builder's arrays exactly, for the reference layout, sigma
     windows, heavy partitions and relabelling, and the same SpMV."""
@@ -666,8 +670,9 @@ def test_sell_device_builder_equals_host(env, layout):
     for H in cases:
         print_case("sell device builder", layout, H.m, H.n, H.nzval.dtype)
         A = H.to_csr(gg.ROCBackend(0))
-        ref = H.to_sell(gg.ROCBackend(0), C_, sigma, heavy, seg, relabel)
-        got = gg.SparseGPUMatrixSELL.build_on_device(A, C_, sigma, heavy, seg, relabel)
+        ref = H.to_sell(gg.ROCBackend(0), C_, sigma, heavy, seg, relabel, sort_entries)
+        got = gg.SparseGPUMatrixSELL.build_on_device(A, C_, sigma, heavy, seg, relabel,
+                                                     sort_entries)
         a, b = _sell_arrays(torch, ref), _sell_arrays(torch, got)
         for k in a:
             if a[k] is None or b[k] is None:
