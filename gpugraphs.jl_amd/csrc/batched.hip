@@ -387,7 +387,6 @@ __global__ void ms_level_end_kernel(unsigned int* found, int32_t* ctl, int32_t c
 // Frontier words per thread of ms_frontier_kernel: one, since a dense
 // frontier's out-edge count loops over every set bit of a word.
 constexpr int kMsScanWords = 1;
-constexpr int kPushBig = 4096;  // out-edges from which a frontier row is cut into pieces
 
 __global__ void ms_push_start_kernel(MsArgs a, MsPush p) {
   if (a.ctl && a.ctl[1]) return;
@@ -435,17 +434,31 @@ __global__ __launch_bounds__(256) void ms_frontier_kernel(MsArgs a, MsPush p, Se
 }
 
 // Push when the frontier's out-edges are below 1/alpha of the matrix.
-// A pull level after a pull level (every front word written) whose frontier
-// has more than 1/kDenseFrac of the edges skips the front_any test.
+// A pull level whose front is complete (every word written by the level
+// before) and whose frontier has more than 1/kDenseFrac of the edges skips
+// the front_any test.  A pull writes every word of its next; a push writes
+// the rows it touched, and a wide one (more than 1/kWidePush of the edges,
+// likely to be followed by a dense level) clears the rest first.
 constexpr double kDenseFrac = 8.0;
+constexpr double kWidePush = 64.0;
 
 __global__ void ms_decide_kernel(MsArgs a, MsPush p) {
   if (a.ctl && a.ctl[1]) return;
-  const unsigned push = p.alpha > 0.0 && (double)(*p.scout) * p.alpha < (double)p.nnz ? 1u : 0u;
+  const double e = (double)(*p.scout);
+  const unsigned push = p.alpha > 0.0 && e * p.alpha < (double)p.nnz ? 1u : 0u;
   p.cnt[3] = push;
   p.cnt[4] = push;  // read by the pull kernels as a.pull_off
-  p.cnt[5] = !push && p.cnt[6] && (double)(*p.scout) * kDenseFrac > (double)p.nnz ? 1u : 0u;
-  p.cnt[6] = !push;
+  p.cnt[5] = !push && p.cnt[6] && e * kDenseFrac > (double)p.nnz ? 1u : 0u;
+  p.cnt[6] = !push || e * kWidePush > (double)p.nnz;
+}
+
+// A wide push level clears next before the push writes the touched rows.
+__global__ __launch_bounds__(256) void ms_clear_next_kernel(MsArgs a, MsPush p, int64_t words) {
+  if (a.ctl && a.ctl[1]) return;
+  if (!p.cnt[3] || !p.cnt[6]) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < words;
+       i += (int64_t)gridDim.x * 256)
+    a.next[i] = 0ull;
 }
 
 // Before a traversal's first level: the front is not known to be complete.
@@ -455,7 +468,10 @@ __global__ void ms_push_reset_kernel(MsPush p) {
 }
 
 // Edge u -> v: OR u's frontier bits that v has not seen into acc[v]; the
-// first lane to touch v queues it.  All 256 threads of the block must call it.
+// first lane to touch v queues it.  With one word per vertex that lane is the
+// one whose OR found acc[v] zero (acc is zero outside a push step and every
+// OR adds a bit); with more words the touched bitmap decides.  All 256
+// threads of the block must call it.
 template <int W>
 __device__ __forceinline__ void push_pair(const MsArgs& a, const MsPush& p, int64_t u, int64_t v,
                                           bool valid) {
@@ -469,10 +485,14 @@ __device__ __forceinline__ void push_pair(const MsArgs& a, const MsPush& p, int6
       any |= add[w] != 0;
     }
     if (any) {
+      if constexpr (W == 1) {
+        newly = atomicOr(&p.acc[v], (unsigned long long)add[0]) == 0ull;
+      } else {
 #pragma unroll
-      for (int w = 0; w < W; ++w)
-        if (add[w]) atomicOr(&p.acc[v * W + w], (unsigned long long)add[w]);
-      newly = bit_claim(p.touched, v);
+        for (int w = 0; w < W; ++w)
+          if (add[w]) atomicOr(&p.acc[v * W + w], (unsigned long long)add[w]);
+        newly = bit_claim(p.touched, v);
+      }
     }
   }
   block_append(p.tqueue, &p.cnt[1], v, newly);  // called block-synchronously by expand_*
@@ -484,7 +504,9 @@ __global__ __launch_bounds__(256) void ms_push_tile_kernel(MsArgs a, MsPush p, S
                                                            const int32_t* __restrict__ rowptr) {
   if (a.ctl && a.ctl[1]) return;
   if (!p.cnt[3]) return;
-  expand_tiles(p.queue, (int64_t)p.cnt[0], g, csr, rowptr, kPushBig, p.big, &p.cnt[2], p.big_cap,
+  const int64_t qlen = p.cnt[0];
+  expand_tiles(p.queue, qlen, tile_for(qlen), g, csr, rowptr, kPushBig, p.big, &p.cnt[2],
+               p.big_cap,
                [&](int64_t u, int64_t v, bool valid) { push_pair<W>(a, p, u, v, valid); });
 }
 
@@ -545,6 +567,9 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a_in, int 
     hipLaunchKernelGGL(ms_decide_kernel, dim3(1), dim3(1), 0, stream, a, *push);
     hipLaunchKernelGGL(ms_frontier_kernel<true>, dim3(blocks_for(push->words, 256 * kMsScanWords)), dim3(256), 0,
                        stream, a, *push, g, csr, A->ptr);
+    const int64_t nw = state_size(A) * Wc;
+    hipLaunchKernelGGL(ms_clear_next_kernel, dim3((unsigned)std::min<int64_t>(blocks_for(nw, 256), 4096)),
+                       dim3(256), 0, stream, a, *push, nw);
   }
   int st = with_type(A->val_dtype, [&](auto ta) -> int {
     typedef typename decltype(ta)::type TA;
@@ -601,8 +626,9 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a_in, int 
       case 4: GG_TRY(run(std::integral_constant<int, 4>{})); break;
       default: GG_TRY(run(std::integral_constant<int, 8>{})); break;
     }
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(push->words, 256)), dim3(256), 0, stream,
-                       push->touched, push->words, 0u);
+    if (Wc > 1)  // one word per vertex claims rows without the touched bitmap
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(push->words, 256)), dim3(256), 0, stream,
+                         push->touched, push->words, 0u);
     GG_HIP(hipGetLastError());
   }
   return GG_OK;

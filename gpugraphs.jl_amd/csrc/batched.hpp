@@ -52,7 +52,8 @@ struct MsPush {
   uint32_t* touched;        // [words] bitmap of tqueue, zero outside a push step
   int2* big;                // hub pieces (state id, first entry)
   unsigned int* cnt;        // [0] qlen, [1] tlen, [2] nbig, [3] mode (1 push), [4] = [3],
-                            // [5] dense pull (MsArgs::dense), [6] this level pulled
+                            // [5] dense pull (MsArgs::dense), [6] every word of this
+                            // level's next will be written (a pull, or a wide push)
   unsigned long long* scout;  // out-edges of the frontier
   int64_t words, nnz, big_cap;
   double alpha;             // push when frontier out-edges * alpha < nnz
@@ -63,9 +64,12 @@ inline bool ms_push_ok(const gg_matrix* A) {
   return (f & GG_FLAG_SYMMETRIC) && (f & GG_FLAG_UNIT_PATTERN) &&
          (A->format == GG_CSR || (A->format == GG_SELL && (f & GG_FLAG_INTERNAL_IDS)));
 }
+// Out-edges from which a frontier row of a top-down step is cut into pieces
+// of this many edges (each piece one block's work).
+constexpr int kPushBig = 1024;
 inline size_t ms_push_bytes(const gg_matrix* A, int W) {
   const int64_t ns = state_size(A), words = (ns + 31) / 32;
-  const int64_t big = 2 * (A->nnz / 4096) + 16;
+  const int64_t big = 2 * (A->nnz / kPushBig) + 16;
   return 8 * (size_t)ns * W + 8 * (size_t)ns + 4 * (size_t)(words + 8) + 8 * (size_t)big + 64;
 }
 // Lays out an MsPush over one zero-filled allocation of ms_push_bytes(A, W).
@@ -74,7 +78,7 @@ inline MsPush ms_push_layout(void* base, const gg_matrix* A, int W) {
   const int64_t ns = state_size(A);
   p.words = (ns + 31) / 32;
   p.nnz = A->nnz;
-  p.big_cap = 2 * (A->nnz / 4096) + 16;
+  p.big_cap = 2 * (A->nnz / kPushBig) + 16;
   char* c = (char*)base;
   p.acc = (unsigned long long*)c; c += 8 * (size_t)ns * W;
   p.big = (int2*)c; c += 8 * (size_t)p.big_cap;

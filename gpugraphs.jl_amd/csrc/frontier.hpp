@@ -251,16 +251,18 @@ __device__ __forceinline__ StateRow state_row(const SellView& g, bool csr,
   return r;
 }
 
-// Out-edges of queue[0..qlen) (state ids of a symmetric pattern), 256 queue
-// entries per tile: their degrees are scanned in LDS and each lane walks one
-// edge per step, finding its row by binary search in the scan.  Rows with more
-// than big_deg edges are appended to big[] as pieces (row, first entry) for
-// expand_pieces instead.  on_edge(u, v, valid) is called by all threads of
-// the block together (v 0-based; valid false on padding lanes).
+// Out-edges of queue[0..qlen) (state ids of a symmetric pattern), `tile`
+// (1..256) queue entries per block step: their degrees are scanned in LDS and
+// each lane walks one edge per step, finding its row by binary search in the
+// scan.  Rows with more than big_deg edges are appended to big[] as pieces
+// (row, first entry) for expand_pieces instead.  on_edge(u, v, valid) is
+// called by all threads of the block together (v 0-based; valid false on
+// padding lanes).
 template <class F>
-__device__ void expand_tiles(const int32_t* __restrict__ queue, int64_t qlen, const SellView& g,
-                             bool csr, const int32_t* __restrict__ rowptr, int big_deg, int2* big,
-                             unsigned int* nbig, int64_t big_cap, F&& on_edge) {
+__device__ void expand_tiles(const int32_t* __restrict__ queue, int64_t qlen, int tile,
+                             const SellView& g, bool csr, const int32_t* __restrict__ rowptr,
+                             int big_deg, int2* big, unsigned int* nbig, int64_t big_cap,
+                             F&& on_edge) {
   __shared__ int64_t s_base[256];
   __shared__ int32_t s_stride[256];
   __shared__ int32_t s_u[256];
@@ -268,11 +270,11 @@ __device__ void expand_tiles(const int32_t* __restrict__ queue, int64_t qlen, co
   __shared__ int32_t s_scan[257];
   __shared__ int32_t s_wsum[4];
   const int wl = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int64_t t0 = (int64_t)blockIdx.x * 256; t0 < qlen; t0 += (int64_t)gridDim.x * 256) {
+  for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < qlen; t0 += (int64_t)gridDim.x * tile) {
     const int64_t qi = t0 + threadIdx.x;
     int32_t deg = 0, u = 0;
     StateRow r{g.colval, 0, 1, 0};
-    if (qi < qlen) {
+    if ((int)threadIdx.x < tile && qi < qlen) {
       u = queue[qi];
       r = state_row(g, csr, rowptr, u);
       deg = (int32_t)r.len;
@@ -319,6 +321,14 @@ __device__ void expand_tiles(const int32_t* __restrict__ queue, int64_t qlen, co
     }
     __syncthreads();  // the next tile reuses the LDS arrays
   }
+}
+
+// Queue entries per block step of expand_tiles: all 256 lanes' worth once the
+// queue fills the grid, fewer for a short queue, so that a few rows of many
+// edges spread over many blocks instead of one.
+__device__ __forceinline__ int tile_for(int64_t qlen) {
+  const int64_t t = (qlen + gridDim.x - 1) / gridDim.x;
+  return (int)(t < 1 ? 1 : t > 256 ? 256 : t);
 }
 
 // The pieces expand_tiles left: block-strided, 256 edges per step.

@@ -553,7 +553,8 @@ __global__ __launch_bounds__(256) void bf_sp_mark_tile_kernel(BfSparse s, SellVi
                                                               const int32_t* done) {
   if (done && *done) return;
   if (!s.cnt[3]) return;
-  expand_tiles(s.queue, (int64_t)s.cnt[0], g, false, nullptr, kSparseBig, s.big, &s.cnt[2],
+  const int64_t qlen = s.cnt[0];
+  expand_tiles(s.queue, qlen, tile_for(qlen), g, false, nullptr, kSparseBig, s.big, &s.cnt[2],
                s.big_cap, [&](int64_t, int64_t v, bool valid) { bf_sp_mark(s, g, v, valid); });
 }
 

@@ -542,11 +542,14 @@ def _sym_host(S):
                        (S.indices + 1).astype(np.int32), np.ones(S.nnz, np.bool_), True)
 
 
-@pytest.mark.parametrize("steps", [0, 8])
-def test_multi_source_bfs_top_down(env, steps):
+@pytest.mark.parametrize("steps,alpha,k", [(0, 16.0, 20), (8, 16.0, 20), (0, 2.0, 100),
+                                           (8, 2.0, 20), (0, 1e-9, 20), (8, 1e-9, 100)])
+def test_multi_source_bfs_top_down(env, steps, alpha, k):
     """Symmetric unit patterns switch levels with small frontiers to the
-    top-down step (hub rows longer than 4096 entries cut into pieces); the
-    levels equal the single-source BFS."""
+    top-down step (rows longer than 1024 entries cut into pieces; a short
+    queue spread over many blocks; a wide push clears the next front so that
+    a dense pull may follow; alpha 1e-9 pushes every level); one and two mask
+    words per vertex.  The levels equal the single-source BFS."""
     gg, torch, orc = env
     from gpugraphs import _lib, generators as GEN
     n = 12001
@@ -559,20 +562,25 @@ def test_multi_source_bfs_top_down(env, steps):
     ctx = _lib.context(0)
     try:
         _lib.set_graph_steps(ctx, steps)
+        _lib.set_push_threshold(ctx, alpha)
         for name, H in cases:
             assert H.symmetric
             Ao = orc.from_scipy_csr(H.to_scipy())
             rng = np.random.default_rng(len(name))
-            srcs = rng.choice(np.nonzero(H.degrees())[0], 20, replace=False).astype(np.int32) + 1
+            srcs = rng.choice(np.nonzero(H.degrees())[0], k, replace=False).astype(np.int32) + 1
             srcs[0] = 1
+            want = [orc.bfs_distances(Ao, int(s)) for s in srcs]
             for A in (H.to_csr(gg.ROCBackend(0)),
-                      H.to_sell(gg.ROCBackend(0), 64, 1, 256, 512, relabel=True)):
-                print_case("msbfs top-down", steps, name, type(A).__name__)
+                      H.to_sell(gg.ROCBackend(0), 64, 1, 256, 512, relabel=True),
+                      H.to_sell(gg.ROCBackend(0), 64, 1, 256, 512, relabel=True,
+                                sort_entries=True)):
+                print_case("msbfs top-down", steps, alpha, k, name, type(A).__name__)
                 D = host(gg.multi_source_bfs(A, srcs))
                 for i in range(len(srcs)):
-                    np.testing.assert_array_equal(D[:, i], orc.bfs_distances(Ao, int(srcs[i])))
+                    np.testing.assert_array_equal(D[:, i], want[i])
     finally:
         _lib.set_graph_steps(ctx, 0)
+        _lib.set_push_threshold(ctx, 16.0)
 
 
 @pytest.mark.parametrize("steps", [0, 8])
