@@ -65,7 +65,7 @@ __device__ __forceinline__ bool finish_row(const MsArgs& a, int32_t level, int64
           *p = v;
         }
       }
-    } else {
+    } else if (a.dist) {
 #pragma unroll
       for (int w = 0; w < W; ++w) {
         uint64_t b = nw[w];
@@ -364,13 +364,80 @@ __global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
 }
 __global__ void ms_zero_kernel(unsigned int* p) { *p = 0u; }
 
-// Captured levels: ctl = {level, done}.  Begin: level 1, not done.  End of a
-// level: done (1) when it found nothing, else the next level; done = 2 when
-// the next level would reach `cap` (byte levels full; 0 = no cap).
+// Captured levels: ctl = {level, done, window base}.  Begin: level 1, not
+// done.  End of a level: done (1) when it found nothing, else the next level;
+// done = 2 when the next level would reach `cap` (byte levels full; 0 = no
+// cap).
 __global__ void ms_ctl_begin_kernel(unsigned int* found, int32_t* ctl) {
   *found = 0u;
   ctl[0] = 1;
   ctl[1] = 0;
+  ctl[2] = 0;
+}
+
+// Byte levels from a ring of R per-level masks.  Level L's new bits are its
+// next words, kept in ring slot L % R (state order; slot L % R's any-bitmap
+// tells which rows the level wrote) until a flush after every R levels moves
+// the window base+1 .. base+R into the row-major byte levels.  One thread per
+// (state row, group of 8 sources): the 8 threads of a row write its 64-byte
+// piece of the original row together.  A group's earlier bytes (sources in
+// seen but not in the window) are read first; sources not reached get 255.
+// Intermediate flushes write only groups with bits in the window; the last
+// one writes every group of every row, so the byte levels need no fill.
+// Captured: base = ctl[2], the window ends at the level that found nothing
+// when done, and done = 2 (overflow) records nothing; host-stepped: the
+// arguments.
+__global__ __launch_bounds__(256) void ms_flush_kernel(SellView g, const uint64_t* __restrict__ ring,
+                                                       const uint32_t* __restrict__ ring_any,
+                                                       int R, int64_t ns, int64_t words, int W,
+                                                       int64_t k8, const uint64_t* __restrict__ seen,
+                                                       uint8_t* __restrict__ dist8,
+                                                       const int32_t* ctl, int32_t base_h,
+                                                       int32_t last_h, int final_h) {
+  int32_t base = base_h, last = last_h;
+  bool fin = final_h != 0;
+  if (ctl) {
+    if (ctl[1] == 2) return;
+    base = ctl[2];
+    fin = ctl[1] == 1;
+    last = fin ? ctl[0] : base + R;
+  }
+  const int64_t G = k8 / 8;
+  const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (gid >= ns * G) return;
+  const int64_t rid = gid / G;
+  const int grp = (int)(gid - rid * G);
+  const int w = grp >> 3, sh = (grp & 7) * 8;
+  const int64_t row = orig_of(g, rid);
+  if (row < 0) return;  // an empty slot
+  uint64_t lv = 0;
+  uint32_t wm = 0;
+  for (int j = 0; j < R; ++j) {
+    const int32_t level = base + (j == 0 ? R : j);
+    if (level > last) continue;
+    if (!((ring_any[(int64_t)j * words + (rid >> 5)] >> (rid & 31)) & 1u)) continue;
+    const uint32_t bits = (uint32_t)(ring[((int64_t)j * ns + rid) * W + w] >> sh) & 0xffu;
+    for (int t = 0; t < 8; ++t)
+      if ((bits >> t) & 1u) lv |= (uint64_t)(uint8_t)level << (8 * t);
+    wm |= bits;
+  }
+  if (!fin && !wm) return;
+  const uint32_t prev = ((uint32_t)(seen[rid * W + w] >> sh) & 0xffu) & ~wm;
+  uint64_t* p = (uint64_t*)(dist8 + row * k8 + grp * 8);
+  const uint64_t old = prev ? *p : 0ull;
+  uint64_t out = 0;
+  for (int t = 0; t < 8; ++t) {
+    const uint64_t b = ((wm >> t) & 1u)     ? (lv >> (8 * t)) & 0xffu
+                       : ((prev >> t) & 1u) ? (old >> (8 * t)) & 0xffu
+                                            : 0xffu;
+    out |= b << (8 * t);
+  }
+  *p = out;
+}
+
+__global__ void ms_window_next_kernel(int32_t* ctl, int R) {
+  if (ctl[1]) return;
+  ctl[2] += R;
 }
 __global__ void ms_level_end_kernel(unsigned int* found, int32_t* ctl, int32_t cap) {
   if (ctl[1]) return;
@@ -634,11 +701,24 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a_in, int 
   return GG_OK;
 }
 
-// Runs one batch of k <= 512 sources.  The levels write each new (vertex,
-// source) distance into a row-major scratch, where a vertex's k entries share
-// lines: bytes (a quarter of the traffic) while the levels stay below 255,
-// then widened and transposed to the column-major result.  A traversal that
-// reaches level 255 is run again with Int32 levels.
+// Runs one batch of k <= 512 sources.  For the column-major result the
+// distances go through a row-major byte scratch, where a vertex's k entries
+// share lines (a quarter of the traffic of Int32) while the levels stay below
+// 255, then are widened and transposed.  The levels do not write the bytes:
+// each level's next words stay in a ring of R slots and a flush after every R
+// levels (ms_flush_kernel) turns a whole window into bytes with 64-byte
+// writes, where per-level writes were scattered read-modify-writes of
+// 8-byte words.  A traversal that reaches level 255 is run again with Int32
+// levels written by the levels themselves.
+constexpr int kMsRingHost = 8;  // ring slots of a host-stepped traversal
+constexpr int kMsRingMax = 16;  // most ring slots of a captured one
+
+static int ms_ring_slots(int U) {
+  for (int r = std::min(U, kMsRingMax); r >= 2; --r)
+    if (U % r == 0) return r;
+  return 2;
+}
+
 static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_dev, int64_t k,
                       int32_t* dist_out, int layout, bool narrow, bool* overflow,
                       int64_t* levels_out) {
@@ -657,12 +737,15 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
   int W = (int)((k + 63) / 64);
   int Wc = W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : 8;
   const int64_t words = (ns + 31) / 32;
-  void *f, *nx, *sn, *fa, *na, *cnt, *hv;
-  GG_TRY(ws_get(ctx, WS_MS0, sizeof(uint64_t) * ns * Wc, &f));
-  GG_TRY(ws_get(ctx, WS_MS1, sizeof(uint64_t) * ns * Wc, &nx));
+  const int U = ctx->graph_steps > 0 ? (ctx->graph_steps + 1) & ~1 : 0;  // levels per launch
+  const int R = !dist8 ? 0 : U > 0 ? ms_ring_slots(U) : kMsRingHost;    // ring slots (0: none)
+  const int nbuf = R > 0 ? R : 1;
+  void *f, *nx = nullptr, *sn, *fa, *na = nullptr, *cnt, *hv;
+  GG_TRY(ws_get(ctx, WS_MS0, sizeof(uint64_t) * ns * Wc * nbuf, &f));
+  if (!R) GG_TRY(ws_get(ctx, WS_MS1, sizeof(uint64_t) * ns * Wc, &nx));
   GG_TRY(ws_get(ctx, WS_MS2, sizeof(uint64_t) * ns * Wc, &sn));
-  GG_TRY(ws_get(ctx, WS_BFS1, sizeof(uint32_t) * words, &fa));
-  GG_TRY(ws_get(ctx, WS_BFS2, sizeof(uint32_t) * words, &na));
+  GG_TRY(ws_get(ctx, WS_BFS1, sizeof(uint32_t) * words * nbuf, &fa));
+  if (!R) GG_TRY(ws_get(ctx, WS_BFS2, sizeof(uint32_t) * words, &na));
   GG_TRY(ws_get(ctx, WS_FLAG, 64, &cnt));
   // The heavy-row accumulators and the push accumulators / touched bitmap are
   // left zero by every level; they are cleared here because their layout
@@ -693,9 +776,7 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
   hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, ctx->stream,
                      (uint32_t*)fa, words, 0u);
   if (dist8) {
-    const int64_t w4 = (m * k8 + 3) / 4;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(w4, kM)), dim3(kM), 0, ctx->stream,
-                       (uint32_t*)dist8, w4, 0xffffffffu);
+    // no fill: the last flush writes every byte
   } else {
     hipLaunchKernelGGL(fill_i32v_kernel, dim3(blocks_for(m * k, kM)), dim3(kM), 0, ctx->stream,
                        dist, m * k, (int32_t)2147483647);
@@ -710,8 +791,8 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
   a.k = k;
   a.layout = GG_ROW_MAJOR;
   a.seen = (uint64_t*)sn;
-  a.dist = dist;
-  a.dist8 = dist8;
+  a.dist = R ? nullptr : dist;
+  a.dist8 = R ? nullptr : dist8;
   a.k8 = k8;
   a.found = (unsigned*)cnt;
   for (int w = 0; w < 8; ++w) a.live[w] = w < Wc ? word_mask(k, w) : 0ull;
@@ -719,33 +800,53 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
   a.alive = nullptr;
   a.hacc = (unsigned long long*)hv;
   a.hcnt = (unsigned*)((unsigned long long*)hv + (A->format == GG_SELL ? A->n_heavy : 0) * Wc);
-  uint64_t* fr = (uint64_t*)f;
-  uint64_t* nxp = (uint64_t*)nx;
-  uint32_t* fany = (uint32_t*)fa;
-  uint32_t* nany = (uint32_t*)na;
+  // front / next of level L: two buffers alternating, or ring slots
+  // (L - 1) % R and L % R
+  uint64_t* mk[2] = {(uint64_t*)f, (uint64_t*)nx};
+  uint32_t* ak[2] = {(uint32_t*)fa, (uint32_t*)na};
+  auto mask_of = [&](int32_t L) -> uint64_t* {
+    return R ? (uint64_t*)f + (int64_t)(L % R) * ns * Wc : mk[L & 1];
+  };
+  auto any_of = [&](int32_t L) -> uint32_t* {
+    return R ? (uint32_t*)fa + (int64_t)(L % R) * words : ak[L & 1];
+  };
+  const SellView gv = sell_view(A);
+  const unsigned flush_blocks = (unsigned)blocks_for(ns * (k8 / 8), 256);
+  auto flush = [&](hipStream_t st, const int32_t* ctl, int32_t base, int32_t last, int fin) {
+    hipLaunchKernelGGL(ms_flush_kernel, dim3(flush_blocks), dim3(256), 0, st, gv,
+                       (const uint64_t*)f, (const uint32_t*)fa, R, ns, words, Wc, k8,
+                       (const uint64_t*)sn, dist8, ctl, base, last, fin);
+  };
   int32_t level = 0;
-  if (ctx->graph_steps > 0) {
-    // U levels per graph launch, front / next alternating (U even)
-    const int U = (ctx->graph_steps + 1) & ~1;
+  if (U > 0) {
+    // U levels per graph launch; step i runs level (launch) * U + i + 1
     int32_t* ctl = (int32_t*)cnt + 4;
     hipLaunchKernelGGL(ms_ctl_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, (unsigned*)cnt, ctl);
     a.ctl = ctl;
-    MsArgs ab[2] = {a, a};
-    ab[0].front = fr; ab[0].front_any = fany; ab[0].next = nxp; ab[0].next_any = nany;
-    ab[1].front = nxp; ab[1].front_any = nany; ab[1].next = fr; ab[1].next_any = fany;
-    std::vector<uint64_t> key{2, (uint64_t)U, (uint64_t)Wc, (uint64_t)words};
+    std::vector<MsArgs> ab(U, a);
+    for (int i = 0; i < U; ++i) {
+      ab[i].front = mask_of(i);
+      ab[i].front_any = any_of(i);
+      ab[i].next = mask_of(i + 1);
+      ab[i].next_any = any_of(i + 1);
+    }
+    std::vector<uint64_t> key{2, (uint64_t)U, (uint64_t)Wc, (uint64_t)words, (uint64_t)R,
+                              (uint64_t)(uintptr_t)dist8, (uint64_t)k8};
     sig_add(key, *A);
-    sig_add(key, ab[0]);
-    sig_add(key, ab[1]);
+    for (int i = 0; i < U; ++i) sig_add(key, ab[i]);
     sig_add(key, push);
     auto steps = [&](hipStream_t st) -> int {
       for (int i = 0; i < U; ++i) {
-        const MsArgs& x = ab[i & 1];
+        const MsArgs& x = ab[i];
         hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, st,
                            x.next_any, words, 0u);
         GG_TRY(msbfs_level(st, A, x, Wc, pp, ctx->num_cus));
         hipLaunchKernelGGL(ms_level_end_kernel, dim3(1), dim3(1), 0, st, (unsigned*)cnt, ctl,
                            dist8 ? 255 : 0);
+        if (R && (i + 1) % R == 0) {
+          flush(st, ctl, 0, 0, 0);
+          hipLaunchKernelGGL(ms_window_next_kernel, dim3(1), dim3(1), 0, st, ctl, R);
+        }
       }
       GG_HIP(hipGetLastError());
       return GG_OK;
@@ -767,13 +868,13 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
         *overflow = true;
         return GG_OK;
       }
-      a.front = fr;
-      a.front_any = fany;
-      a.next = nxp;
-      a.next_any = nany;
+      a.front = mask_of(level - 1);
+      a.front_any = any_of(level - 1);
+      a.next = mask_of(level);
+      a.next_any = any_of(level);
       a.level = level;
       hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, ctx->stream,
-                         nany, words, 0u);
+                         a.next_any, words, 0u);
       hipLaunchKernelGGL(ms_zero_kernel, dim3(1), dim3(1), 0, ctx->stream, (unsigned*)cnt);
       {
         HotTimer ht(ctx);
@@ -783,8 +884,14 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
                             ctx->stream));
       GG_HIP(hipStreamSynchronize(ctx->stream));
       if (*(unsigned*)ctx->host_pinned == 0) break;
-      std::swap(fr, nxp);
-      std::swap(fany, nany);
+      if (R && level % R == 0) {
+        HotTimer ht(ctx, true, false);  // the levels' share of recording their distances
+        flush(ctx->stream, nullptr, level - R, level, 0);
+      }
+    }
+    if (R) {
+      HotTimer ht(ctx, true, false);
+      flush(ctx->stream, nullptr, (level - 1) / R * R, level, 1);
     }
   }
   if (layout == GG_COL_MAJOR && m > 0) {

@@ -16,7 +16,8 @@ struct MsArgs {
   uint64_t* next;              // [ns*W]
   uint32_t* next_any;
   uint64_t* seen;              // [ns*W]
-  int32_t* dist;               // original rows
+  int32_t* dist;               // original rows; NULL with dist8: no per-level distance writes
+                               // (the caller keeps each level's next and records them later)
   uint8_t* dist8;              // when set: byte levels (< 255), row-major m x k8, instead of dist
   int64_t k8;                  // row stride of dist8: k rounded up to a multiple of 8
   int32_t level;

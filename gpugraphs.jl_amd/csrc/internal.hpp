@@ -90,7 +90,9 @@ void timer_end(gg_ctx* ctx);
 struct HotTimer {
   gg_ctx* c;
   bool on;
-  explicit HotTimer(gg_ctx* ctx, bool active = true) : c(ctx), on(active && ctx->timing) {
+  bool count;  // false: the time joins the hot kernel's without counting a launch
+  explicit HotTimer(gg_ctx* ctx, bool active = true, bool counted = true)
+      : c(ctx), on(active && ctx->timing), count(counted) {
     if (on) (void)hipEventRecord(c->ev2, c->stream);
   }
   ~HotTimer() {
@@ -100,7 +102,7 @@ struct HotTimer {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, c->ev2, c->ev3) == hipSuccess) {
         c->hot_ms += ms;
-        c->hot_launches += 1;
+        if (count) c->hot_launches += 1;
       }
     }
   }

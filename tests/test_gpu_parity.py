@@ -479,6 +479,44 @@ def test_multi_source_bfs_deep_levels(env, fmt):
     assert D.max() == n - 1
 
 
+@pytest.mark.parametrize("steps", [0, 2, 6, 32])
+@pytest.mark.parametrize("fmt", ["csr", "sell_relabel"])
+def test_multi_source_bfs_level_windows(env, steps, fmt):
+    """Byte levels recorded a window of levels at a time (every 8 host-stepped
+    levels; captured: every 2, 6 or 16 levels): a path of 230 vertices, a
+    20 x 20 grid and an R-MAT graph as disjoint components, 100 sources, so
+    that one group of 8 sources is reached in many windows and most vertices
+    stay unreached by most sources (typemax(Int32))."""
+    gg, torch, orc = env
+    from gpugraphs import _lib, generators as GEN
+    n1 = 230
+    path = sp.diags([np.ones(n1 - 1), np.ones(n1 - 1)], [-1, 1], format="csr")
+    g = sp.diags([np.ones(399), np.ones(399)], [-1, 1], format="lil")
+    for i in range(19, 399, 20):
+        g[i, i + 1] = 0
+        g[i + 1, i] = 0
+    grid = (sp.csr_matrix(g) + sp.diags([np.ones(380), np.ones(380)], [-20, 20], format="csr"))
+    grid.eliminate_zeros()
+    R = GEN.rmat(10, seed=8).to_scipy()
+    S = sp.block_diag([path, grid, R], format="csr").astype(np.int32)
+    S.data[:] = 1
+    rng = np.random.default_rng(steps + 3)
+    srcs = rng.choice(S.shape[0], 100, replace=False).astype(np.int32) + 1
+    srcs[:4] = [1, n1, n1 // 2, n1 + 1]
+    A = make(gg, S, fmt)
+    Ao = orc.from_scipy_csr(S)
+    ctx = _lib.context(0)
+    try:
+        _lib.set_graph_steps(ctx, steps)
+        print_case("msbfs windows", steps, fmt)
+        D = host(gg.multi_source_bfs(A, srcs))
+    finally:
+        _lib.set_graph_steps(ctx, 0)
+    for i, s in enumerate(srcs):
+        np.testing.assert_array_equal(D[:, i], orc.bfs_distances(Ao, int(s)))
+    assert D[:n1, 0].max() == n1 - 1
+
+
 def _host_csr(S, dtype):
     from gpugraphs import generators as GEN
     S = sp.csr_matrix(S)
