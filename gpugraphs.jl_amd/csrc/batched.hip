@@ -375,18 +375,29 @@ __global__ void ms_ctl_begin_kernel(unsigned int* found, int32_t* ctl) {
   ctl[2] = 0;
 }
 
+constexpr int kMsRingHost = 8;  // ring slots of a host-stepped traversal
+constexpr int kMsRingMax = 16;  // most ring slots of a captured one
+
+// Byte t of the result is 0xff when bit t of b is set, else 0.
+__device__ __forceinline__ uint64_t byte_mask8(uint32_t b) {
+  uint64_t x = ((uint64_t)(b & 0xffu) * 0x0101010101010101ull) & 0x8040201008040201ull;
+  x = (((x & 0x7f7f7f7f7f7f7f7full) + 0x7f7f7f7f7f7f7f7full) | x) & 0x8080808080808080ull;
+  return (x >> 7) * 0xffull;
+}
+
 // Byte levels from a ring of R per-level masks.  Level L's new bits are its
 // next words, kept in ring slot L % R (state order; slot L % R's any-bitmap
 // tells which rows the level wrote) until a flush after every R levels moves
 // the window base+1 .. base+R into the row-major byte levels.  One thread per
-// (state row, group of 8 sources): the 8 threads of a row write its 64-byte
-// piece of the original row together.  A group's earlier bytes (sources in
-// seen but not in the window) are read first; sources not reached get 255.
-// Intermediate flushes write only groups with bits in the window; the last
-// one writes every group of every row, so the byte levels need no fill.
-// Captured: base = ctl[2], the window ends at the level that found nothing
-// when done, and done = 2 (overflow) records nothing; host-stepped: the
-// arguments.
+// (state row, mask word): it loads the R slots' words at once, composes the
+// word's 8 groups of 8 byte levels and leaves them in LDS; then 8 lanes write
+// each 64-byte piece of an original row together.  A group's earlier bytes
+// (sources in seen but not in the window) are read first; sources not
+// reached get 255.  Intermediate flushes write only groups with bits in the
+// window; the last one writes every group of every row, so the byte levels
+// need no fill.  Captured: base = ctl[2], the window ends at the level that
+// found nothing when done, and done = 2 (overflow) records nothing;
+// host-stepped: the arguments.
 __global__ __launch_bounds__(256) void ms_flush_kernel(SellView g, const uint64_t* __restrict__ ring,
                                                        const uint32_t* __restrict__ ring_any,
                                                        int R, int64_t ns, int64_t words, int W,
@@ -394,45 +405,79 @@ __global__ __launch_bounds__(256) void ms_flush_kernel(SellView g, const uint64_
                                                        uint8_t* __restrict__ dist8,
                                                        const int32_t* ctl, int32_t base_h,
                                                        int32_t last_h, int final_h) {
+  __shared__ uint64_t sb[256][9];
+  __shared__ int64_t sdst[256];
+  __shared__ uint32_t swm[256];
   int32_t base = base_h, last = last_h;
   bool fin = final_h != 0;
   if (ctl) {
-    if (ctl[1] == 2) return;
+    if (ctl[1] == 2) return;  // uniform: every thread returns
     base = ctl[2];
     fin = ctl[1] == 1;
     last = fin ? ctl[0] : base + R;
   }
-  const int64_t G = k8 / 8;
   const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (gid >= ns * G) return;
-  const int64_t rid = gid / G;
-  const int grp = (int)(gid - rid * G);
-  const int w = grp >> 3, sh = (grp & 7) * 8;
-  const int64_t row = orig_of(g, rid);
-  if (row < 0) return;  // an empty slot
-  uint64_t lv = 0;
-  uint32_t wm = 0;
-  for (int j = 0; j < R; ++j) {
-    const int32_t level = base + (j == 0 ? R : j);
-    if (level > last) continue;
-    if (!((ring_any[(int64_t)j * words + (rid >> 5)] >> (rid & 31)) & 1u)) continue;
-    const uint32_t bits = (uint32_t)(ring[((int64_t)j * ns + rid) * W + w] >> sh) & 0xffu;
-    for (int t = 0; t < 8; ++t)
-      if ((bits >> t) & 1u) lv |= (uint64_t)(uint8_t)level << (8 * t);
-    wm |= bits;
+  int64_t dst = -1;
+  uint32_t wmask = 0;  // groups of the piece to write
+  if (gid < ns * W) {
+    const int64_t rid = gid / W;
+    const int w = (int)(gid - rid * W);
+    const int gc = (int)min<int64_t>(8, k8 / 8 - 8 * w);
+    const int64_t row = gc > 0 ? orig_of(g, rid) : -1;
+    if (row >= 0) {  // not an empty slot, nor a word past k8
+      // every slot's any-word, then every wanted mask word: the loads of a
+      // phase are independent, so a wavefront waits twice, not 2R times
+      uint32_t on[kMsRingMax];
+#pragma unroll
+      for (int j = 0; j < kMsRingMax; ++j)
+        on[j] = j < R && base + (j == 0 ? R : j) <= last
+                    ? (ring_any[(int64_t)j * words + (rid >> 5)] >> (rid & 31)) & 1u
+                    : 0u;
+      uint64_t mw[kMsRingMax];
+      uint64_t win = 0;
+#pragma unroll
+      for (int j = 0; j < kMsRingMax; ++j) {
+        mw[j] = on[j] ? ring[((int64_t)j * ns + rid) * W + w] : 0ull;
+        win |= mw[j];
+      }
+      if (fin || win) {
+        const uint64_t prev = seen[rid * W + w] & ~win;
+        dst = row * k8 + 64 * w;
+        const uint64_t* old = (const uint64_t*)(dist8 + dst);
+        for (int q = 0; q < gc; ++q) {
+          const uint32_t wq = (uint32_t)(win >> (8 * q)) & 0xffu;
+          const uint32_t pq = (uint32_t)(prev >> (8 * q)) & 0xffu;
+          if (!fin && !wq) continue;
+          uint64_t out = 0, got = 0;
+#pragma unroll
+          for (int j = 0; j < kMsRingMax; ++j) {
+            const uint32_t bq = (uint32_t)(mw[j] >> (8 * q)) & 0xffu;
+            if (!bq) continue;
+            const uint64_t m = byte_mask8(bq);
+            out |= m & ((uint64_t)(uint8_t)(base + (j == 0 ? R : j)) * 0x0101010101010101ull);
+            got |= m;
+          }
+          if (pq) {
+            const uint64_t m = byte_mask8(pq);
+            out |= old[q] & m;
+            got |= m;
+          }
+          sb[threadIdx.x][q] = out | ~got;
+          wmask |= 1u << q;
+        }
+      }
+    }
   }
-  if (!fin && !wm) return;
-  const uint32_t prev = ((uint32_t)(seen[rid * W + w] >> sh) & 0xffu) & ~wm;
-  uint64_t* p = (uint64_t*)(dist8 + row * k8 + grp * 8);
-  const uint64_t old = prev ? *p : 0ull;
-  uint64_t out = 0;
-  for (int t = 0; t < 8; ++t) {
-    const uint64_t b = ((wm >> t) & 1u)     ? (lv >> (8 * t)) & 0xffu
-                       : ((prev >> t) & 1u) ? (old >> (8 * t)) & 0xffu
-                                            : 0xffu;
-    out |= b << (8 * t);
+  sdst[threadIdx.x] = dst;
+  swm[threadIdx.x] = wmask;
+  __syncthreads();
+  const int q = threadIdx.x & 7;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int u = i * 32 + (threadIdx.x >> 3);
+    if (sdst[u] >= 0 && ((swm[u] >> q) & 1u))
+      *(uint64_t*)(dist8 + sdst[u] + 8 * q) = sb[u][q];
   }
-  *p = out;
 }
 
 __global__ void ms_window_next_kernel(int32_t* ctl, int R) {
@@ -710,8 +755,6 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a_in, int 
 // writes, where per-level writes were scattered read-modify-writes of
 // 8-byte words.  A traversal that reaches level 255 is run again with Int32
 // levels written by the levels themselves.
-constexpr int kMsRingHost = 8;  // ring slots of a host-stepped traversal
-constexpr int kMsRingMax = 16;  // most ring slots of a captured one
 
 static int ms_ring_slots(int U) {
   for (int r = std::min(U, kMsRingMax); r >= 2; --r)
@@ -811,7 +854,7 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
     return R ? (uint32_t*)fa + (int64_t)(L % R) * words : ak[L & 1];
   };
   const SellView gv = sell_view(A);
-  const unsigned flush_blocks = (unsigned)blocks_for(ns * (k8 / 8), 256);
+  const unsigned flush_blocks = (unsigned)blocks_for(ns * Wc, 256);
   auto flush = [&](hipStream_t st, const int32_t* ctl, int32_t base, int32_t last, int fin) {
     hipLaunchKernelGGL(ms_flush_kernel, dim3(flush_blocks), dim3(256), 0, st, gv,
                        (const uint64_t*)f, (const uint32_t*)fa, R, ns, words, Wc, k8,
