@@ -592,7 +592,8 @@ def main():
                         frac=round(achieved / HBM_PEAK_GBS, 4),
                         traffic=measured_traffic(args, relabel, "msbfs_level"),
                         kernel="multi-source BFS level (msbfs_sell_kernel + direction switch + "
-                               "top-down step)", bytes_per_launch=ms["bytes_per_launch"],
+                               "top-down step + its share of the byte-level flush)",
+                        bytes_per_launch=ms["bytes_per_launch"],
                         avg_launch_us=round(kernel_ms * 1e3, 2))
     else:
         value, roofline = None, None
