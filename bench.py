@@ -254,12 +254,27 @@ def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
             if kl:
                 per_kind[label] = dict(n=len(kl), p50_ms=round(pct(kl, 50) * 1e3, 2),
                                        p99_ms=round(pct(kl, 99) * 1e3, 2))
+        # per graph and kind: count, p50 / p99 latency and mean time from admission to completion
+        gname = {}
+        for name, gb, gs, Mb, Ms in table:
+            gname[gb], gname[gs] = name, name
+        per_graph = {}
+        for name in dict.fromkeys(gname.values()):
+            for kind, label in ((S.BFS, "bfs"), (S.SSSP, "sssp")):
+                cs = [c for c in ok if gname[c.graph] == name and c.kind == kind]
+                if cs:
+                    kl = [c.latency_s for c in cs]
+                    per_graph[f"{name}/{label}"] = [len(cs), round(pct(kl, 50) * 1e3, 1),
+                                                   round(pct(kl, 99) * 1e3, 1),
+                                                   round(float(np.mean([c.complete_s - c.admit_s
+                                                                        for c in cs])) * 1e3, 1)]
         run = dict(rate_qps=rate, arrival_s=args.serve_seconds, submitted=len(submitted),
                    completed=len(ok), failed=len(done) - len(ok), wall_s=round(wall, 3),
                    qps=round(len(ok) / wall, 1),
                    p50_ms=round(pct(lat, 50) * 1e3, 2) if lat else None,
                    p99_ms=round(pct(lat, 99) * 1e3, 2) if lat else None, by_kind=per_kind,
-                   steps=st["steps"], checked_alone=checked)
+                   by_graph_n_p50_p99_service_ms=per_graph, steps=st["steps"],
+                   checked_alone=checked)
         runs.append(run)
         log(f"serving at {rate:.0f} q/s offered: {len(ok)}/{len(submitted)} queries in {wall:.1f}s "
             f"= {len(ok) / wall:.0f} q/s, p50 {run['p50_ms']:.1f} ms p99 {run['p99_ms']:.1f} ms, "

@@ -201,6 +201,10 @@ struct Engine {
   uint32_t *fany = nullptr, *nany = nullptr;
   int32_t *dist = nullptr, *start = nullptr;
   void* heavy_acc = nullptr;  // segment accumulators of heavy rows (kept zero)
+  // top-down steps (symmetric unit pattern): a step whose union frontier has
+  // few out-edges pushes them instead of pulling every row
+  bool has_push = false;
+  MsPush push{};
 
   int32_t level = 0;
   // SSSP
@@ -281,6 +285,15 @@ static int make_engine(gg_server* s, const gg_matrix* A, int kind, std::unique_p
     GG_TRY(engine_alloc(s, *e, hb, &p)); e->heavy_acc = p;
     hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(hb / 4, 256)), dim3(256), 0,
                        e->stream, (uint32_t*)p, (int64_t)(hb / 4), 0u);
+    if (ms_push_ok(A)) {
+      const size_t pb = ms_push_bytes(A, 1);
+      GG_TRY(engine_alloc(s, *e, pb, &p));
+      hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for((int64_t)(pb / 4), 256)),
+                         dim3(256), 0, e->stream, (uint32_t*)p, (int64_t)(pb / 4), 0u);
+      e->push = ms_push_layout(p, A, 1);
+      e->push.alpha = 16.0;
+      e->has_push = true;
+    }
 
     hipLaunchKernelGGL(serve_fill_kernel<uint64_t>, dim3(nb), dim3(256), 0, e->stream, e->front, ns, 0ull);
     hipLaunchKernelGGL(serve_fill_kernel<uint64_t>, dim3(nb), dim3(256), 0, e->stream, e->seen, ns, 0ull);
@@ -437,9 +450,9 @@ static int launch_step(gg_server* s, Engine& e) {
     a.alive = e.mask_d;
     a.hacc = (unsigned long long*)e.heavy_acc;
     a.hcnt = (unsigned*)(a.hacc + (e.A.format == GG_SELL ? e.A.n_heavy : 0));
-    // pull steps only: the slots of a step sit at different levels, so the
-    // union of their frontiers is rarely small enough for top-down steps
-    GG_TRY(msbfs_level(e.stream, &e.A, a, 1));
+    // the slots of a step sit at different levels; their union frontier
+    // decides the direction, as in a batched traversal
+    GG_TRY(msbfs_level(e.stream, &e.A, a, 1, e.has_push ? &e.push : nullptr, s->num_cus));
     e.level += 1;
     std::swap(e.front, e.next);
     std::swap(e.fany, e.nany);
