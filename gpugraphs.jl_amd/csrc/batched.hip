@@ -586,7 +586,7 @@ __global__ void ms_push_reset_kernel(MsPush p) {
 // threads of the block must call it.
 template <int W>
 __device__ __forceinline__ void push_pair(const MsArgs& a, const MsPush& p, int64_t u, int64_t v,
-                                          bool valid) {
+                                          bool valid, bool wide) {
   bool newly = false;
   if (valid) {
     uint64_t add[W];
@@ -596,7 +596,14 @@ __device__ __forceinline__ void push_pair(const MsArgs& a, const MsPush& p, int6
       add[w] = a.front[u * W + w] & ~a.seen[v * W + w] & a.live[w];
       any |= add[w] != 0;
     }
-    if (any) {
+    if (any && wide) {
+      // a wide push finishes every row afterwards: no claim, and ORs whose
+      // old value nobody reads
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        if (add[w]) __hip_atomic_fetch_or(&p.acc[v * W + w], (unsigned long long)add[w],
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (any) {
       if constexpr (W == 1) {
         newly = atomicOr(&p.acc[v], (unsigned long long)add[0]) == 0ull;
       } else {
@@ -617,9 +624,10 @@ __global__ __launch_bounds__(256) void ms_push_tile_kernel(MsArgs a, MsPush p, S
   if (a.ctl && a.ctl[1]) return;
   if (!p.cnt[3]) return;
   const int64_t qlen = p.cnt[0];
+  const bool wide = p.cnt[6] != 0;
   expand_tiles(p.queue, qlen, tile_for(qlen), g, csr, rowptr, kPushBig, p.big, &p.cnt[2],
                p.big_cap,
-               [&](int64_t u, int64_t v, bool valid) { push_pair<W>(a, p, u, v, valid); });
+               [&](int64_t u, int64_t v, bool valid) { push_pair<W>(a, p, u, v, valid, wide); });
 }
 
 template <int W>
@@ -627,19 +635,22 @@ __global__ __launch_bounds__(256) void ms_push_big_kernel(MsArgs a, MsPush p, Se
                                                           const int32_t* __restrict__ rowptr) {
   if (a.ctl && a.ctl[1]) return;
   if (!p.cnt[3]) return;
+  const bool wide = p.cnt[6] != 0;
   expand_pieces(p.big, min((int64_t)p.cnt[2], p.big_cap), g, csr, rowptr, kPushBig,
-                [&](int64_t u, int64_t v, bool valid) { push_pair<W>(a, p, u, v, valid); });
+                [&](int64_t u, int64_t v, bool valid) { push_pair<W>(a, p, u, v, valid, wide); });
 }
 
 // Touched rows: new bits = pushed bits not yet seen; acc is cleared for the
-// next push step.
+// next push step.  A wide push queued no rows: every state row is visited
+// and those with pushed bits finished (next was cleared before the push).
 template <int W>
 __global__ __launch_bounds__(256) void ms_push_finish_kernel(MsArgs a, MsPush p, SellView g,
-                                                             int csr) {
+                                                             int csr, int64_t ns) {
   if (a.ctl && a.ctl[1]) return;
   if (!p.cnt[3]) return;
   const int32_t level = a.ctl ? a.ctl[0] : a.level;
-  const int64_t n = p.cnt[1];
+  const bool wide = p.cnt[6] != 0;
+  const int64_t n = wide ? ns : (int64_t)p.cnt[1];
   for (int64_t i0 = (int64_t)blockIdx.x * 256; i0 < n; i0 += (int64_t)gridDim.x * 256) {
     const int64_t i = i0 + threadIdx.x;
     bool newly = false;
@@ -647,15 +658,22 @@ __global__ __launch_bounds__(256) void ms_push_finish_kernel(MsArgs a, MsPush p,
 #pragma unroll
     for (int w = 0; w < W; ++w) nw[w] = 0;
     if (i < n) {
-      const int64_t v = p.tqueue[i];
-      uint64_t acc[W], sn[W];
+      const int64_t v = wide ? i : p.tqueue[i];
+      uint64_t acc[W], any = 0;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
         acc[w] = p.acc[v * W + w];
-        p.acc[v * W + w] = 0ull;
-        sn[w] = a.seen[v * W + w];
+        any |= acc[w];
       }
-      newly = finish_row<W>(a, level, v, csr ? v : orig_of(g, v), acc, sn, nw);
+      if (any) {
+        uint64_t sn[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          p.acc[v * W + w] = 0ull;
+          sn[w] = a.seen[v * W + w];
+        }
+        newly = finish_row<W>(a, level, v, csr ? v : orig_of(g, v), acc, sn, nw);
+      }
     }
     wave_flag(a.found, newly);
     alive_wave<W>(a, nw);
@@ -729,7 +747,7 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a_in, int 
       hipLaunchKernelGGL(ms_push_big_kernel<WW>, dim3(nb), dim3(256), 0, stream, a, *push, g, csr,
                          A->ptr);
       hipLaunchKernelGGL(ms_push_finish_kernel<WW>, dim3(nb), dim3(256), 0, stream, a, *push, g,
-                         csr);
+                         csr, state_size(A));
       return GG_OK;
     };
     switch (Wc) {
