@@ -250,19 +250,22 @@ __global__ __launch_bounds__(kB) void bfs_pull_csr_kernel(
 
 // -------------------------------------------------------------- push, CSR
 // Frontier queue over A_out (row u = out-neighbours of u), expanded edge by
-// edge so a hub's edges spread over many lanes: a block takes 256 queue
-// entries, scans their out-degrees in LDS and walks the tile's edges 256 at
-// a time (each lane finds its vertex by binary search in the scan).  A vertex
-// with more than kBigDeg out-edges is cut into kBigDeg-edge pieces that a
-// second kernel spreads over blocks.  Claimed vertices go to the next queue
-// (one atomic per wavefront) and to the next bitmap.
-constexpr int kBigDeg = 4096;
+// edge so a hub's edges spread over many lanes: a block takes `tile` (at most
+// 256) queue entries, scans their out-degrees in LDS and walks the tile's
+// edges 256 at a time (each lane finds its vertex by binary search in the
+// scan); a short queue gets small tiles, so that its rows spread over many
+// blocks.  A vertex with more than kBigDeg out-edges is cut into
+// kBigDeg-edge pieces that a second kernel spreads over blocks.  Claimed
+// vertices go to the next queue (one atomic per wavefront) and to the next
+// bitmap.
+constexpr int kBigDeg = 1024;
 
 struct PushArgs {
   const int32_t* rowptr;
   const int32_t* colval;
   const int32_t* queue;
   int64_t qlen;
+  int tile;             // queue entries per block step (1..256)
   int32_t* next_queue;
   uint32_t* next;
   uint32_t* visited;
@@ -305,10 +308,11 @@ __global__ __launch_bounds__(kB) void push_tile_kernel(PushArgs a) {
   __shared__ int32_t s_scan[kB + 1];
   __shared__ int32_t s_wsum[kB / 64];
   const int wl = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int64_t t0 = (int64_t)blockIdx.x * kB; t0 < a.qlen; t0 += (int64_t)gridDim.x * kB) {
+  for (int64_t t0 = (int64_t)blockIdx.x * a.tile; t0 < a.qlen;
+       t0 += (int64_t)gridDim.x * a.tile) {
     const int64_t qi = t0 + threadIdx.x;
     int32_t beg = 0, deg = 0;
-    if (qi < a.qlen) {
+    if ((int)threadIdx.x < a.tile && qi < a.qlen) {
       const int32_t u = a.queue[qi];
       beg = a.rowptr[u] - 1;
       deg = a.rowptr[u + 1] - 1 - beg;
@@ -672,10 +676,11 @@ int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_ou
       ++level;
       fill_words(ctx, w.next, w.words, 0u);
       hipLaunchKernelGGL(level_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt, level, dist);
-      PushArgs pa{A_out->ptr, A_out->idx, w.q0, qlen, w.q1, w.next, w.visited, dist, level,
-                  vinv, w.cnt, big};
-      unsigned nb = blocks_for(qlen, kB);
       const unsigned cap = (unsigned)ctx->num_cus * 16;
+      const int tile = (int)std::min<int64_t>(kB, std::max<int64_t>(1, (qlen + cap - 1) / cap));
+      PushArgs pa{A_out->ptr, A_out->idx, w.q0, qlen, tile, w.q1, w.next, w.visited, dist,
+                  level, vinv, w.cnt, big};
+      unsigned nb = blocks_for(qlen, tile);
       if (nb > cap) nb = cap;
       {
         HotTimer ht(ctx);
