@@ -193,6 +193,94 @@ def packed_leg(args, np, torch, gg, G, R0, seed):
                 plain_sample=len(mats))
 
 
+def tuned_leg(args, np, torch, gg, G, R0, ctx, _lib, seed):
+    """The batched forms on R-MAT scale 20 (same generator) with the layout
+    and the push/pull threshold picked by the autotuner: its multi-source BFS
+    candidates timed on 64 roots, then the threshold; then batched BFS
+    (k = 64 and 256) against the plain BFS, and k = 32 Bellman-Ford
+    (integer weights 1..1000 as fp32) against one source at a time, on the
+    tuned layout."""
+    from gpugraphs import autotune as AT
+    from gpugraphs.algorithms import shortest_path_
+    sc = args.tuned_scale
+    t = time.time()
+    H = G.rmat(sc, 16, 0.57, 0.19, 0.19, seed=seed + 20)
+    Hb = G.HostCSR(H.m, H.n, H.rowptr, H.colval, np.ones(H.nnz, np.bool_), H.symmetric)
+    rng = np.random.default_rng(seed + 20)
+    nzr = np.nonzero(H.degrees())[0]
+    roots = rng.choice(nzr, 256, replace=False).astype(np.int32) + 1
+    tuner = AT.Autotuner(R0, reps=2)
+    cands = AT.default_candidates(Hb) + [dict(layout="sell", slice_size=64, sigma=1, relabel=True,
+                                               heavy=1024, seg=65536, sort=True)]
+    lay = tuner.tune_layout(Hb, "msbfs", roots=roots[:64], candidates=cands)
+    A = AT.build(Hb, R0, lay.best)
+    pa = tuner.tune_push_threshold(lambda: gg.multi_source_bfs(A, roots[:64]),
+                                   alphas=(0.0, 4.0, 16.0, 64.0))
+    tune_s = time.time() - t
+    out = dict(scale=sc, n=int(H.m), nnz=int(H.nnz), tune_s=round(tune_s, 1),
+               layout=lay.best, layout_ms=round(lay.best_ms, 3), push_alpha=pa.best["alpha"],
+               candidates=len(lay.table))
+    for k in (64, 256):
+        r = roots[:k]
+        gg.multi_source_bfs(A, r)
+        torch.cuda.synchronize()
+        lat = []
+        for _ in range(max(args.steps, 2)):
+            t0 = time.perf_counter()
+            D = gg.multi_source_bfs(A, r)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t0)
+        out[f"msbfs_k{k}_qps"] = round(k / float(np.mean(lat)), 1)
+        out[f"msbfs_k{k}_p99_ms"] = round(pct(lat, 99) * 1e3, 3)
+    plat, same = [], True
+    for i in range(16):
+        t0 = time.perf_counter()
+        d = gg.bfs_distances(A, int(roots[i]))
+        torch.cuda.synchronize()
+        plat.append(time.perf_counter() - t0)
+        same &= bool(torch.equal(d, D[:, i]))
+    assert same, "tuned batched BFS differs from the plain BFS"
+    out["plain_qps"] = round(16 / sum(plat), 1)
+    out["plain_p99_ms"] = round(pct(plat, 99) * 1e3, 3)
+    out["batched_vs_plain_k64"] = round(out["msbfs_k64_qps"] / out["plain_qps"], 2)
+    # k = 32 Bellman-Ford on the layout the tuner picks for Bellman-Ford
+    w = rng.integers(1, 1001, H.nnz).astype(np.float32)
+    Hw = G.HostCSR(H.m, H.n, H.rowptr, H.colval, w, H.symmetric)
+    lw = tuner.tune_layout(Hw, "sssp", roots=roots[:1])
+    out["sssp_layout"] = lw.best
+    Aw = AT.build(Hw, R0, lw.best)
+    srcs = roots[:args.sssp_k]
+    dist = torch.empty(H.m, dtype=torch.float32, device=f"cuda:{R0.device}")
+    shortest_path_(Aw, int(srcs[0]), dist)
+    torch.cuda.synchronize()
+    slat, plain = [], []
+    for s_ in srcs:
+        t0 = time.perf_counter()
+        shortest_path_(Aw, int(s_), dist)
+        torch.cuda.synchronize()
+        slat.append(time.perf_counter() - t0)
+        plain.append(dist.clone())
+    k = len(srcs)
+    Dm = torch.empty(k * H.m, dtype=torch.float32, device=dist.device).view(k, H.m).T
+    shortest_path_(Aw, srcs, Dm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    shortest_path_(Aw, srcs, Dm)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    assert all(torch.equal(Dm[:, i], plain[i]) for i in range(k)), "tuned batched SSSP differs"
+    out["sssp_k"] = k
+    out["sssp_plain_qps"] = round(k / sum(slat), 1)
+    out["sssp_batched_qps"] = round(k / tb, 1)
+    out["sssp_batched_vs_plain"] = round((k / tb) / (k / sum(slat)), 2)
+    log(f"tuned R-MAT {sc}: layout {lay.best} ({len(lay.table)} candidates, push alpha "
+        f"{pa.best['alpha']}, {tune_s:.1f}s); batched BFS k=64 {out['msbfs_k64_qps']:.0f} q/s, "
+        f"k=256 {out['msbfs_k256_qps']:.0f} q/s, plain {out['plain_qps']:.0f} q/s; SSSP k={k} "
+        f"{out['sssp_batched_qps']:.0f} q/s vs {out['sssp_plain_qps']:.0f} q/s")
+    _lib.set_push_threshold(ctx, args.push_alpha)
+    return out
+
+
 def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
     """Continuous batching over 8 HBM-resident graphs: R-MAT scales 16, 18, 20,
     22 (edge factor 16), 4-neighbour grids 1024^2, 2048^2, 4096^2 (+1%
@@ -336,13 +424,15 @@ def main():
                     help="R-MAT graphs (scale 10..18) packed into one persistent BFS launch")
     ap.add_argument("--packed-plain", type=int, default=64,
                     help="graphs of the pack also run one BFS call at a time (plain baseline)")
-    ap.add_argument("--serve-rate", type=str, default="2000,250",
+    ap.add_argument("--tuned-scale", type=int, default=20,
+                    help="R-MAT scale of the autotuned batched leg (0: skip)")
+    ap.add_argument("--serve-rate", type=str, default="2000,1000,250",
                     help="Poisson arrivals per second, one serving run per comma-separated rate")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the serving leg")
     ap.add_argument("--serve-budget-gib", type=float, default=8.0,
                     help="HBM result-arena budget of the server (GiB)")
     ap.add_argument("--only", default="",
-                    help="one part: spmv | spmm | msbfs | bfs | sssp | packed | serve, or copy (a 1 GiB device copy that "
+                    help="one part: spmv | spmm | msbfs | bfs | sssp | packed | tuned | serve, or copy (a 1 GiB device copy that "
                          "calibrates the HBM byte counters)")
     args = ap.parse_args()
     relabel = not args.no_relabel
@@ -586,6 +676,10 @@ def main():
     if args.only in ("", "packed") and args.packed_graphs > 0:
         res["packed"] = packed_leg(args, np, torch, gg, G, R0, seed)
 
+    # --------------------------------------------- autotuned, R-MAT scale 20
+    if args.only in ("", "tuned") and args.tuned_scale > 0:
+        res["tuned"] = tuned_leg(args, np, torch, gg, G, R0, ctx, _lib, seed)
+
     # ------------------------------------------------------------- serving
     if args.only in ("", "serve") and args.serve_seconds > 0:
         res["serve"] = serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel)
@@ -666,6 +760,7 @@ def main():
                  for name, c in res.get("sssp", {}).items()} or None,
         "packed": {k: (round(v, 3) if isinstance(v, float) else v)
                    for k, v in res["packed"].items()} if "packed" in res else None,
+        "tuned": res.get("tuned"),
         "serve": res.get("serve"),
         "cpu_baseline": None if not cpu else {
             "value": round(cpu["qps"], 3), "unit": "queries/s", "cores": cpu["cores"],
