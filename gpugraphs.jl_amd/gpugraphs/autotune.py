@@ -7,7 +7,8 @@ Times candidates on the device and keeps the fastest:
   - the direction-optimising BFS switch (alpha, beta) of bfs_distances_do;
   - the steps per captured graph launch of the level loops.
 The workload decides what is timed: one plus-times SpMV, BFS levels from the
-given roots, a multi-source BFS of up to 64 roots, or Bellman-Ford.  Results
+given roots, a multi-source BFS of up to 64 roots, Bellman-Ford from the first
+root, or Bellman-Ford from up to 32 roots at once.  Results
 are cached per graph signature (size, entries, degree profile) and workload,
 in memory and optionally in a JSON file.
 """
@@ -23,7 +24,7 @@ import numpy as np
 from . import _lib
 from .generators import HostCSR
 
-WORKLOADS = ("spmv", "bfs", "msbfs", "sssp")
+WORKLOADS = ("spmv", "bfs", "msbfs", "sssp", "msssp")
 
 
 @dataclass
@@ -81,8 +82,12 @@ def _runner(workload, A, H, roots, torch, gg):
     if workload == "msbfs":
         r = np.asarray(roots[:64], np.int32)
         return lambda: multi_source_bfs(A, r)
-    d = torch.empty(H.m, dtype=torch.float32 if H.nzval.dtype == np.float32 else torch.int32,
-                    device=dev)
+    dt = torch.float32 if H.nzval.dtype == np.float32 else torch.int32
+    if workload == "msssp":
+        r = np.asarray(roots[:32], np.int32)
+        D = torch.empty(len(r) * H.m, dtype=dt, device=dev).view(len(r), H.m).T
+        return lambda: shortest_path_(A, r, D)
+    d = torch.empty(H.m, dtype=dt, device=dev)
     return lambda: shortest_path_(A, int(roots[0]), d)
 
 

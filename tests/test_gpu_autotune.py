@@ -17,13 +17,13 @@ def env(oracle):
     return gg, torch, oracle
 
 
-@pytest.mark.parametrize("workload", ["spmv", "bfs", "msbfs", "sssp"])
+@pytest.mark.parametrize("workload", ["spmv", "bfs", "msbfs", "sssp", "msssp"])
 def test_tune_layout_picks_a_timed_candidate(env, workload, tmp_path):
     gg, torch, orc = env
     from gpugraphs import autotune as AT, generators as GEN
     R = GEN.rmat(12, seed=5, dtype=np.float32, values=GEN.UNIFORM01) if workload == "spmv" else \
         GEN.random_graph(3000, 8, seed=6, dtype=np.int32, values=GEN.INTEGER, wmin=1, wmax=99) \
-        if workload == "sssp" else GEN.rmat(12, seed=5)
+        if workload in ("sssp", "msssp") else GEN.rmat(12, seed=5)
     roots = (np.nonzero(R.degrees())[0][:8] + 1).astype(np.int32)
     cache = str(tmp_path / "tune.json")
     tuner = AT.Autotuner(gg.ROCBackend(0), cache_path=cache, reps=2)
@@ -43,6 +43,9 @@ def test_tune_layout_picks_a_timed_candidate(env, workload, tmp_path):
     elif workload == "sssp":
         np.testing.assert_array_equal(gg.shortest_path(A, int(roots[0])).cpu().numpy(),
                                       orc.shortest_path(orc.from_scipy_csr(S), int(roots[0])))
+    elif workload == "msssp":
+        np.testing.assert_array_equal(gg.shortest_path(A, roots).cpu().numpy(),
+                                      orc.shortest_path_multi(orc.from_scipy_csr(S), roots))
     else:
         np.testing.assert_array_equal(gg.bfs_distances(A, int(roots[0])).cpu().numpy(),
                                       orc.bfs_distances(orc.from_scipy_csr(S), int(roots[0])))
