@@ -417,7 +417,7 @@ def main():
     ap.add_argument("--spmm-k", type=int, default=32, help="columns of the SpMM leg (0: skip)")
     ap.add_argument("--no-sort-entries", dest="sort_entries", action="store_false",
                     help="keep each row's entries in input order (default: by increasing stored "
-                         "column, hubs first)")
+                         "column: light rows by degree, heavy rows last)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--grid", type=int, default=4096, help="Bellman-Ford grid side")
     ap.add_argument("--sssp-sources", type=int, default=16, help="grid Bellman-Ford sources")

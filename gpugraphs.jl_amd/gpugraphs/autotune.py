@@ -45,7 +45,7 @@ def default_candidates(H: HostCSR):
                 c.append(dict(layout="sell", slice_size=64, sigma=1, relabel=True, heavy=heavy,
                               seg=seg))
     c.append(dict(layout="sell", slice_size=64, sigma=1, relabel=True, heavy=0, seg=4096))
-    if 1024 < mx:  # each row's entries by increasing internal column (hubs first)
+    if 1024 < mx:  # each row's entries by increasing internal column (light hubs first, heavy rows last)
         c.append(dict(layout="sell", slice_size=64, sigma=1, relabel=True, heavy=1024, seg=4096,
                       sort=True))
     c.append(dict(layout="sell", slice_size=64, sigma=4096, relabel=False,

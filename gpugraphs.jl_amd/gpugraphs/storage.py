@@ -381,7 +381,7 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
     so a row's neighbours are gathered from the compact, degree-ordered state
     arrays; every operation still takes and returns vectors in original order.
     sort_entries=True stores each row's entries by increasing stored column
-    (hubs first when relabelled): folds visit a row's terms in that order.
+    (when relabelled: light rows by decreasing degree, then the heavy rows): folds visit a row's terms in that order.
     """
 
     def __init__(self, *args, slice_size=None, sigma=1, heavy_threshold=0, heavy_seg_len=4096,

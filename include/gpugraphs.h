@@ -169,9 +169,9 @@ typedef enum gg_sell_flags {
    * are written at the original row ids.  Needs a square matrix. */
   GG_SELL_RELABEL = 1,
   /* Store each row's entries by increasing stored column (with relabelling:
-   * hubs first) instead of the input order.  Folds then visit a row's terms
-   * in that order: same results for min / max / or / and, a reordered sum
-   * for plus on floats. */
+   * light rows by decreasing degree, heavy rows last) instead of the input
+   * order.  Folds then visit a row's terms in that order: same results for
+   * min / max / or / and, a reordered sum for plus on floats. */
   GG_SELL_SORT_ENTRIES = 2
 } gg_sell_flags;
 
