@@ -164,8 +164,9 @@ int graph_loop(gg_ctx* ctx, const std::vector<uint64_t>& key, const int32_t* don
                int64_t* launches);
 
 // Internal launchers (used by the algorithms; no argument checks).
+// rows (CSR only): a 1-based row list of nrows entries instead of r0 + q.
 int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const void* B,
                 int b_dtype, int add, int mul, int accum, const void* mask, int mask_dtype,
-                int64_t r0, int64_t nrows, bool has_range);
+                int64_t r0, int64_t nrows, bool has_range, const int32_t* rows = nullptr);
 
 }  // namespace gg

@@ -593,6 +593,29 @@ int launch_spmm(gg_ctx* ctx, void* C, int c_dtype, int c_layout, const gg_matrix
   });
 }
 
+// CSC SpMM by atomic scatter, the k-column form of csc_spmv_kernel!
+// (src/spmv/CSC.jl:2-21; src/spmm/CSC.jl is a placeholder): thread (column c
+// of A, column j of B) adds A[r,c] * B[c,j] into C[r,j] for every entry of
+// column c.  Element (i, j) of B / C sits at i * rs + j * cs.
+template <class T>
+__global__ __launch_bounds__(256) void csc_spmm_kernel(T* __restrict__ c, int64_t c_rs, int64_t c_cs,
+                                                       const int32_t* __restrict__ colptr,
+                                                       const int32_t* __restrict__ rowval,
+                                                       const T* __restrict__ val,
+                                                       const T* __restrict__ b, int64_t b_rs,
+                                                       int64_t b_cs, int64_t n, int64_t k) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n * k) return;
+  const int64_t col = t / k, j = t - col * k;
+  const T bc = b[col * b_rs + j * b_cs];
+  for (int32_t i = colptr[col] - 1; i < colptr[col + 1] - 1; ++i) {
+    const T v = jmul(bc, val[i]);
+    T* dst = &c[(int64_t)(rowval[i] - 1) * c_rs + j * c_cs];
+    if constexpr (std::is_same<T, int32_t>::value) atomicAdd((int*)dst, (int)v);
+    else atomicAdd(dst, v);
+  }
+}
+
 }  // namespace gg
 
 using namespace gg;
@@ -605,13 +628,41 @@ extern "C" int gg_spmm(gg_ctx* ctx, void* C, int c_dtype, int64_t c_rows, int64_
   if (!ctx) return set_error(GG_ERR_ARGUMENT, "ctx is NULL");
   clear_error();
   GG_TRY(check_matrix(A));
-  if (A->format == GG_CSC) return set_error(GG_ERR_UNSUPPORTED, "CSC SpMM is not provided (src/spmm/CSC.jl:1)");
   // src/spmm/CSR.jl:131-133
   if (A->n != b_rows || c_rows != A->m || c_cols != b_cols)
     return set_error(GG_ERR_ASSERTION, "SpMM dimensions must agree");
   if ((c_layout != GG_COL_MAJOR && c_layout != GG_ROW_MAJOR) ||
       (b_layout != GG_COL_MAJOR && b_layout != GG_ROW_MAJOR))
     return set_error(GG_ERR_ARGUMENT, "unknown layout");
+  if (A->format == GG_CSC) {
+    // as the CSC SpMV: plus-times by atomic adds into C, add and accum not
+    // applied, one element type, neither mask nor range
+    if (mask || range_len >= 0)
+      return set_error(GG_ERR_ARGUMENT, "CSC SpMM takes neither mask nor range");
+    if (A->val_dtype != b_dtype || c_dtype != b_dtype)
+      return set_error(GG_ERR_ARGUMENT, "CSC SpMM needs one element type for A, B and C");
+    if (A->n == 0 || b_cols == 0) return GG_OK;
+    const int64_t b_rs = b_layout == GG_ROW_MAJOR ? b_cols : 1;
+    const int64_t b_cs = b_layout == GG_ROW_MAJOR ? 1 : b_rows;
+    const int64_t c_rs = c_layout == GG_ROW_MAJOR ? c_cols : 1;
+    const int64_t c_cs = c_layout == GG_ROW_MAJOR ? 1 : c_rows;
+    timer_begin(ctx);
+    int st = with_type(c_dtype, [&](auto t) -> int {
+      typedef typename decltype(t)::type T;
+      if constexpr (std::is_same<T, float>::value || std::is_same<T, double>::value ||
+                    std::is_same<T, int32_t>::value) {
+        hipLaunchKernelGGL(csc_spmm_kernel<T>, dim3(blocks_for(A->n * b_cols, 256)), dim3(256), 0,
+                           ctx->stream, (T*)C, c_rs, c_cs, A->ptr, A->idx, (const T*)A->val,
+                           (const T*)B, b_rs, b_cs, A->n, b_cols);
+        GG_HIP(hipGetLastError());
+        return GG_OK;
+      } else {
+        return set_error(GG_ERR_UNSUPPORTED, "CSC SpMM supports Float32, Float64, Int32");
+      }
+    });
+    timer_end(ctx);
+    return st;
+  }
   if (A->format == GG_SELL && (mask || range_len >= 0))
     return set_error(GG_ERR_ARGUMENT, "SELL SpMM takes neither mask nor range (src/spmm/SELL.jl:84-98)");
   int64_t r0 = 0, nrows = A->m;

@@ -58,14 +58,16 @@ __global__ __launch_bounds__(kBlock) void csr_spmv_kernel(
     typename Types<S, TA, TX>::TC* __restrict__ c, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ colval, const TA* __restrict__ val, const TX* __restrict__ x,
     int accum, const void* __restrict__ mask, int mdt, int64_t r0, int64_t nrows, int logL,
-    int any_fast) {
+    int any_fast, const int32_t* __restrict__ rows, int64_t m) {
   typedef Types<S, TA, TX> T;
   typedef typename T::A A;
   const int L = 1 << logL;
   const int lane = threadIdx.x & (L - 1);
   const int64_t G = ((int64_t)gridDim.x * kBlock) >> logL;
   for (int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> logL; q < nrows; q += G) {
-    const int64_t row = r0 + q;
+    // a row list (1-based; entries outside 1..m are skipped), or r0 + q
+    const int64_t row = rows ? (int64_t)rows[q] - 1 : r0 + q;
+    if (rows && (row < 0 || row >= m)) continue;
     if (!mask_on(mask, mdt, row)) continue;
     const int32_t s = rowptr[row] - 1, e = rowptr[row + 1] - 1;
     if (any_fast) {
@@ -282,7 +284,7 @@ static int ilog2_exact(int64_t v) {
 
 int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const void* B,
                 int b_dtype, int add, int mul, int accum, const void* mask, int mask_dtype,
-                int64_t r0, int64_t nrows, bool has_range) {
+                int64_t r0, int64_t nrows, bool has_range, const int32_t* rows) {
   if (nrows <= 0) return GG_OK;
   return dispatch(add, mul, A->val_dtype, b_dtype, [&](auto s, auto ta, auto tx) -> int {
     typedef decltype(s) S;
@@ -299,7 +301,7 @@ int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const voi
       if (S::ADD == GG_OP_ADD && is_int<typename T::CT>()) lg = 0;  // exact absorb position
       // mask without range only (_dense_mask_gpu_spmv!, src/spmv/CSR.jl:191-205);
       // range + mask runs the generic kernel, src/spmv/CSR.jl:338-353
-      const bool any_fast = S::ADD == GG_OP_ANY && mask != nullptr && !has_range;
+      const bool any_fast = S::ADD == GG_OP_ANY && mask != nullptr && !has_range && !rows;
       const int64_t groups = nrows;
       const int64_t threads = groups << lg;
       unsigned nb = blocks_for(threads, kBlock);
@@ -309,7 +311,7 @@ int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const voi
         HotTimer ht(ctx);
         hipLaunchKernelGGL((csr_spmv_kernel<S, TA, TX>), dim3(nb), dim3(kBlock), 0, ctx->stream,
                            (TC*)C, A->ptr, A->idx, (const TA*)A->val, (const TX*)B, accum, mask,
-                           mask_dtype, r0, nrows, lg, (int)any_fast);
+                           mask_dtype, r0, nrows, lg, (int)any_fast, rows, A->m);
       }
       GG_HIP(hipGetLastError());
       return GG_OK;
@@ -421,6 +423,28 @@ extern "C" int gg_spmv(gg_ctx* ctx, void* C, int c_dtype, int64_t c_len, const g
   timer_begin(ctx);
   int st = launch_spmv(ctx, C, c_dtype, A, B, b_dtype, add, mul, accum, mask, mask_dtype, r0, nrows,
                        range_len >= 0);
+  timer_end(ctx);
+  if (st == GG_ERR_UNSUPPORTED && gg_last_error()[0] == 0)
+    set_error(st, "no kernel for add=%d mul=%d with value types %d, %d", add, mul, A->val_dtype, b_dtype);
+  return st;
+}
+
+// The sparse-mask SpMV of sparse_masked_csr_spmv_kernel! (src/spmv/CSR.jl:90-115):
+// C[r] = accum(C[r], fold of row r) for each row r of the list.
+extern "C" int gg_spmv_rows(gg_ctx* ctx, void* C, int c_dtype, int64_t c_len, const gg_matrix* A,
+                            const void* B, int b_dtype, int64_t b_len, int add, int mul,
+                            int accum, const int32_t* rows, int64_t n_rows) {
+  if (!ctx) return set_error(GG_ERR_ARGUMENT, "ctx is NULL");
+  clear_error();
+  GG_TRY(check_matrix(A));
+  if (A->n != b_len) return set_error(GG_ERR_DIMENSION, "Matrix dimensions must agree");
+  if (c_len != A->m) return set_error(GG_ERR_DIMENSION, "Matrix dimensions must agree");
+  if (A->format != GG_CSR)
+    return set_error(GG_ERR_UNSUPPORTED, "a row-list mask is a CSR kernel (src/spmv/CSR.jl:90-115)");
+  if (n_rows < 0 || (n_rows > 0 && !rows)) return set_error(GG_ERR_ARGUMENT, "bad row list");
+  timer_begin(ctx);
+  int st = launch_spmv(ctx, C, c_dtype, A, B, b_dtype, add, mul, accum, nullptr, 0, 0, n_rows,
+                       false, rows);
   timer_end(ctx);
   if (st == GG_ERR_UNSUPPORTED && gg_last_error()[0] == 0)
     set_error(st, "no kernel for add=%d mul=%d with value types %d, %d", add, mul, A->val_dtype, b_dtype);

@@ -160,6 +160,8 @@ SIGNATURES = {
                              C.POINTER(C.POINTER(GGCsrHost))], _i32),
     "gg_spmv": ([_vp, _vp, _i32, _i64, _PM, _vp, _i32, _i64, _i32, _i32, _i32, _vp, _i32,
                  _i64, _i64, _i64], _i32),
+    "gg_spmv_rows": ([_vp, _vp, _i32, _i64, _PM, _vp, _i32, _i64, _i32, _i32, _i32, _vp, _i64],
+                     _i32),
     "gg_spmm": ([_vp, _vp, _i32, _i64, _i64, _i32, _PM, _vp, _i32, _i64, _i64, _i32, _i32,
                  _i32, _i32, _vp, _i32, _i64, _i64, _i64], _i32),
     "gg_spmspv": ([_vp, _vp, _vp, C.POINTER(_i64), _i32, _i64, _PM, _vp, _vp, _i64, _i32, _i64,

@@ -42,7 +42,22 @@ def _range_args(r):
 
 def gpu_spmv_(C_, A, B, mul=GPUGraphs_mul, add=GPUGraphs_add, accum=GPUGraphs_second,
               range_=None, mask=None):
-    """C = accum(C, A add.mul B) on the rows of mask / range_ (src/spmv/CSR.jl:2-29)."""
+    """C = accum(C, A add.mul B) on the rows of mask / range_ (src/spmv/CSR.jl:2-29).
+
+    A SparseGPUVector mask of length m lists the rows to compute (its stored
+    indices, distinct): the sparse-mask CSR kernel of src/spmv/CSR.jl:90-115."""
+    if isinstance(mask, SparseGPUVector):
+        if range_ is not None:
+            raise ArgumentError("a sparse mask takes no range")
+        if mask.n != A.m:
+            raise DimensionMismatch("Mask length must be equal to the number of rows in A")
+        b = _same_backend(A, B, C_)
+        d = A.descriptor()
+        check(lib().gg_spmv_rows(context(b.device), ptr(C_), dtype_code(np_dtype_of(C_)),
+                                 C_.numel(), C.byref(d), ptr(B), dtype_code(np_dtype_of(B)),
+                                 B.numel(), op_code(add), op_code(mul), op_code(accum),
+                                 ptr(mask.nzind), mask.nnz()), "gpu_spmv!")
+        return None
     b = _same_backend(A, B, C_, mask)
     d = A.descriptor()
     r0, rl = _range_args(range_)

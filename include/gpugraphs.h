@@ -341,10 +341,22 @@ int gg_spmv(gg_ctx* ctx, void* C, int c_dtype, int64_t c_len, const gg_matrix* A
             const void* mask, int mask_dtype, int64_t mask_len, int64_t range_start,
             int64_t range_len);
 
+/* Sparse-mask SpMV: C[r] = accum(C[r], A[r,:] (add.mul) B) for each row r of
+ * rows[0..n_rows) (1-based, distinct; entries outside 1..m are skipped).
+ * Replaces sparse_masked_csr_spmv_kernel!, src/spmv/CSR.jl:90-115, which no
+ * dispatch of the reference reaches (gpu_spmv! takes a dense mask there).  CSR
+ * only; SELL and CSC return GG_ERR_UNSUPPORTED. */
+int gg_spmv_rows(gg_ctx* ctx, void* C, int c_dtype, int64_t c_len, const gg_matrix* A,
+                 const void* B, int b_dtype, int64_t b_len, int add, int mul, int accum,
+                 const int32_t* rows, int64_t n_rows);
+
 /* Dense-operand SpMM, C[m x k] = accum(C, A (add.mul) B[n x k]).  Replaces
  * gpu_spmm!, src/spmm/CSR.jl:111-204 and src/spmm/SELL.jl:84-124.  The
  * reference passes Julia (column-major) matrices; GG_ROW_MAJOR is the layout
- * of the batched multi-source kernels. */
+ * of the batched multi-source kernels.  A CSC A (src/spmm/CSC.jl is a
+ * placeholder) runs the k-column form of the CSC SpMV: C += A * B by atomic
+ * adds, add and accum not applied, one element type (Float32, Float64,
+ * Int32), no mask or range. */
 int gg_spmm(gg_ctx* ctx, void* C, int c_dtype, int64_t c_rows, int64_t c_cols,
             int c_layout, const gg_matrix* A, const void* B, int b_dtype, int64_t b_rows,
             int64_t b_cols, int b_layout, int add, int mul, int accum, const void* mask,

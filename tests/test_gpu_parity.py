@@ -178,6 +178,58 @@ def test_spmv_semirings_exact(env, fmt, sr, variant):
     np.testing.assert_array_equal(got, co)
 
 
+@pytest.mark.parametrize("sr", SEMIRINGS, ids=lambda s: "-".join(str(x) for x in s[:3]) + f"-{np.dtype(s[3]).name}")
+def test_spmv_sparse_mask_rows(env, sr):
+    """A SparseGPUVector mask lists the rows to compute (the CSR kernel of
+    src/spmv/CSR.jl:90-115): listed rows equal the unmasked product's rows
+    (the generic fold, so any = max as in that kernel), the others keep C."""
+    gg, torch, orc = env
+    add, mul, acc, ta, tx, tc = sr
+    print_case("spmv sparse mask", sr)
+    rng = np.random.default_rng(hash(sr[:3]) % 2**32)
+    m = n = 700
+    S = random_csr(rng, m, n, 0.02)
+    S.data = _values(rng, ta, S.nnz)
+    if ta == np.bool_:
+        S.data[:] = True
+    S = S.astype(ta)
+    x = _values(rng, tx, n)
+    if add == "any":
+        x = (rng.random(n) < 0.3).astype(tx)
+    c0 = _values(rng, tc, m)
+    rows = np.sort(rng.choice(m, 250, replace=False)).astype(np.int32) + 1
+    rows = rng.permutation(rows)
+    kw = dict(mul=_ops(gg, mul), add=_ops(gg, add), accum=_ops(gg, acc))
+    A = make(gg, S, "csr")
+    mk = gg.SparseGPUVector(m, rows, np.ones(len(rows), np.int32), gg.ROCBackend(0))
+    c = dev(torch, c0.copy())
+    gg.gpu_spmv_(c, A, dev(torch, x), mask=mk, **kw)
+    torch.cuda.synchronize()
+    full = c0.copy()
+    if full.dtype == np.bool_:
+        full = full.view(np.uint8)
+    orc.spmv(full, orc_mat(orc, S, "csr"), x, add=getattr(orc, "OP_" + add.upper()),
+             mul=getattr(orc, "OP_" + mul.upper()), accum=getattr(orc, "OP_" + acc.upper()))
+    want = c0.copy()
+    if want.dtype == np.bool_:
+        want = want.view(np.uint8)
+    want[rows - 1] = full[rows - 1]
+    got = host(c)
+    if got.dtype == np.bool_:
+        got = got.view(np.uint8)
+    np.testing.assert_array_equal(got, want)
+    # an empty list leaves C alone; SELL has no row-list kernel; lengths are checked
+    empty = gg.SparseGPUVector(m, np.zeros(0, np.int32), np.zeros(0, np.int32), gg.ROCBackend(0))
+    gg.gpu_spmv_(c, A, dev(torch, x), mask=empty, **kw)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(got, host(c).view(np.uint8) if host(c).dtype == np.bool_ else host(c))
+    with pytest.raises(gg.MethodError):
+        gg.gpu_spmv_(c, make(gg, S, "sell_ref"), dev(torch, x), mask=mk, **kw)
+    with pytest.raises(gg.DimensionMismatch):
+        gg.gpu_spmv_(c, A, dev(torch, x), mask=gg.SparseGPUVector(m + 1, rows, np.ones(len(rows), np.int32),
+                                                                    gg.ROCBackend(0)), **kw)
+
+
 def test_spmv_errors(env):
     gg, torch, orc = env
     A = gg.SparseGPUMatrixCSR(sp.random(30, 40, density=0.1, format="csr", dtype=np.float32),
@@ -279,6 +331,40 @@ def test_csc_spmv(env):
         gg.gpu_spmv_(c, A, dev(torch, x))
         torch.cuda.synchronize()
         np.testing.assert_allclose(host(c), S.astype(np.float64) @ x, rtol=RTOL, atol=1e-6)
+
+
+@pytest.mark.parametrize("layout", ["col", "row"])
+def test_csc_spmm(env, layout):
+    """CSC SpMM, the k-column form of the CSC SpMV (atomic adds into C): fp32
+    within the tolerance of an FP64 product, Int32 exact; Julia's
+    column-major and row-major operands; C accumulates."""
+    gg, torch, orc = env
+    rng = np.random.default_rng(3)
+    conv = (lambda a: _cm(torch, a)) if layout == "col" else (lambda a: dev(torch, a))
+    for m, n, k, p in ((10, 12, 3, 0.5), (1000, 900, 20, 0.05)):
+        S = random_csr(rng, m, n, p)
+        print_case("csc spmm", layout, m, n, k)
+        B = rng.random((n, k), dtype=np.float32)
+        A = gg.SparseGPUMatrixCSC(S, gg.ROCBackend(0))
+        C0 = rng.random((m, k), dtype=np.float32)
+        c = conv(C0.copy())
+        gg.gpu_spmm_(c, A, conv(B))
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(host(c), C0 + S.astype(np.float64) @ B.astype(np.float64),
+                                   rtol=RTOL, atol=1e-5)
+        Si = S.copy()
+        Si.data = rng.integers(1, 50, S.nnz).astype(np.float32)
+        Si = Si.astype(np.int32)
+        Bi = rng.integers(0, 50, (n, k)).astype(np.int32)
+        Ai = gg.SparseGPUMatrixCSC(Si, gg.ROCBackend(0))
+        ci = conv(np.zeros((m, k), np.int32))
+        gg.gpu_spmm_(ci, Ai, conv(Bi))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(host(ci), (Si.astype(np.int64) @ Bi).astype(np.int32))
+    with pytest.raises(gg.ArgumentError):
+        gg.gpu_spmm_(conv(np.zeros((m, k), np.float32)), Ai, conv(Bi))
+    with pytest.raises(AssertionError):
+        gg.gpu_spmm_(conv(np.zeros((m + 1, k), np.float32)), A, conv(B))
 
 
 @pytest.mark.parametrize("semiring", ["plus_times", "min_plus", "any_secondi", "or_and"])
