@@ -156,9 +156,11 @@ inline BfSparse bf_sparse_layout(void* base, const gg_matrix* A, double alpha) {
 }
 
 // bf_multi_step with sparse iterations when sp is set (cus sizes their grids).
+// Columns 0..k-1 of rows ld elements apart (ld < 0: ld = k); the other
+// columns are left alone.
 int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k, void* d,
                      const uint32_t* chp, uint32_t* chn, unsigned int* changed,
                      unsigned long long* changed_mask, const int32_t* done, const BfSparse* sp,
-                     int cus);
+                     int cus, int64_t ld = -1);
 
 }  // namespace gg

@@ -13,7 +13,8 @@
 //  - SSSP: 32 columns of a row-major min-plus working set (the in-place step
 //    of gg_shortest_path_multi).  A query resets its column between two steps
 //    and marks its source as changed; it is finished after a step that
-//    lowered nothing in its column.
+//    lowered nothing in its column.  A step relaxes the columns up to the
+//    highest busy slot only (slots are taken lowest first).
 // Every query is exactly the single-source BFS / Bellman-Ford from its source:
 // slots never interact (bit-parallel or column-parallel), and each slot's
 // traversal is the level-synchronous one.
@@ -75,9 +76,15 @@ __global__ void serve_bfs_seed_kernel(SeedList sl, const int32_t* vinv, uint64_t
   for (int q = 0; q < sl.n; ++q) {
     const int64_t v = vinv ? (int64_t)vinv[sl.src[q]] - 1 : sl.src[q];
     const uint64_t b = 1ull << sl.slot[q];
+    const uint32_t fb = 1u << (v & 31);
+    // a row outside the frontier may hold a stale word (a top-down step writes
+    // only the rows it touched, so an untouched row keeps what it held two
+    // steps before, bits of finished or reused slots included): start it
+    // from the seed alone
+    if (!(fany[v >> 5] & fb)) front[v] = 0ull;
     front[v] |= b;
     seen[v] |= b;
-    fany[v >> 5] |= 1u << (v & 31);
+    fany[v >> 5] |= fb;
     start[sl.slot[q]] = level;
   }
 }
@@ -468,8 +475,12 @@ static int launch_step(gg_server* s, Engine& e) {
     const int64_t words = (ns + 31) / 32;
     hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(words, 256)), dim3(256), 0,
                        e.stream, e.chn, words, 0u);
-    GG_TRY(bf_multi_step_sp(e.stream, &e.A, e.dist_dtype, 32, e.buf, e.chp, e.chn, e.cnt_d,
-                            e.mask_d, nullptr, e.sparse ? &e.sp : nullptr, s->num_cus));
+    // only the columns up to the highest active slot (slots are taken lowest
+    // first), in groups of 4 so that the 16-byte column kernels apply
+    const int hi = 64 - __builtin_clzll(e.active | 1ull);
+    const int64_t kc = std::min<int64_t>(32, (hi + 3) & ~3);
+    GG_TRY(bf_multi_step_sp(e.stream, &e.A, e.dist_dtype, kc, e.buf, e.chp, e.chn, e.cnt_d,
+                            e.mask_d, nullptr, e.sparse ? &e.sp : nullptr, s->num_cus, 32));
     std::swap(e.chp, e.chn);
   }
   GG_HIP(hipGetLastError());

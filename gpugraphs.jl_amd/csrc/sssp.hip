@@ -152,7 +152,7 @@ template <class TA, class TD>
 __device__ __forceinline__ TD multi_row_min(const SellView& g, const TA* __restrict__ val,
                                             int64_t base, int64_t len, bool has_len,
                                             const uint32_t* __restrict__ chp, const TD* d,
-                                            int64_t k, int64_t j) {
+                                            int64_t k, int64_t ld, int64_t j) {
   TD acc = Lim<TD>::max();
   const bool jin = j < k;
   int64_t kk = 0;
@@ -170,7 +170,7 @@ __device__ __forceinline__ TD multi_row_min(const SellView& g, const TA* __restr
       for (int u = 0; u < 4; ++u)
         if (on[u]) {
           w[u] = val[base + (kk + u) * g.C];
-          dn[u] = d[(int64_t)c[u] * k + j];
+          dn[u] = d[(int64_t)c[u] * ld + j];
         }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
@@ -181,7 +181,7 @@ __device__ __forceinline__ TD multi_row_min(const SellView& g, const TA* __restr
     const int64_t i = base + kk * g.C;
     const int32_t col = g.colval[i];
     if (!has_len && col == -1) break;
-    if (jin && bit_get(chp, col - 1)) acc = jmin(acc, sadd(TD(val[i]), d[(int64_t)(col - 1) * k + j]));
+    if (jin && bit_get(chp, col - 1)) acc = jmin(acc, sadd(TD(val[i]), d[(int64_t)(col - 1) * ld + j]));
   }
   return acc;
 }
@@ -195,7 +195,7 @@ __device__ __forceinline__ TD multi_row_min(const SellView& g, const TA* __restr
 template <class TA, class TD>
 __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t* __restrict__ rowptr,
                                                       SellView g, int64_t n_light_groups,
-                                                      int64_t parts, int64_t k, int logLk, TD* d,
+                                                      int64_t parts, int64_t k, int64_t ld, int logLk, TD* d,
                                                       const uint32_t* __restrict__ chp,
                                                       uint32_t* __restrict__ chn,
                                                       unsigned int* changed,
@@ -224,21 +224,21 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
         rid = unit;
         for (int32_t i = rowptr[rid] - 1; i < rowptr[rid + 1] - 1; ++i) {
           const int32_t c = g.colval[i] - 1;
-          if (bit_get(chp, c) && j < k) acc = jmin(acc, sadd(TD(val[i]), d[(int64_t)c * k + j]));
+          if (bit_get(chp, c) && j < k) acc = jmin(acc, sadd(TD(val[i]), d[(int64_t)c * ld + j]));
         }
       } else {
         int64_t row, base, len;
         bool has_len;
         if (sell_slot(g, unit, row, base, len, has_len)) {
           rid = g.internal ? unit : row;
-          acc = multi_row_min<TA, TD>(g, val, base, len, has_len, chp, d, k, j);
+          acc = multi_row_min<TA, TD>(g, val, base, len, has_len, chp, d, k, ld, j);
         }
       }
       if (rid >= 0 && j < k) {
-        const TD dv = d[rid * k + j];
+        const TD dv = d[rid * ld + j];
         const TD nv = jmin(dv, acc);
         if (nv < dv) {
-          d[rid * k + j] = nv;
+          d[rid * ld + j] = nv;
           ch = true;
         }
       }
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
   if (j < k)
     for (int64_t i = s + grp; i < e; i += gpb) {
       const int32_t c = g.heavy_col[i] - 1;
-      if (bit_get(chp, c)) acc = jmin(acc, sadd(TD(hv[i]), d[(int64_t)c * k + j]));
+      if (bit_get(chp, c)) acc = jmin(acc, sadd(TD(hv[i]), d[(int64_t)c * ld + j]));
     }
   red[grp * Lk + lane] = acc;
   __syncthreads();
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
     for (int64_t o = 1; o < gpb; ++o) acc = jmin(acc, red[o * Lk + lane]);
     if (j < k) {
       const int64_t rid = heavy_rid(g, h);
-      if (atomic_min_lowered(&d[rid * k + j], acc)) {
+      if (atomic_min_lowered(&d[rid * ld + j], acc)) {
         set_flag(changed);
         if (changed_mask) or_new_bits(changed_mask, 1ull << j);
         s_any = 1;
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
 // neighbour's V distances as one 16-byte load, so a wavefront covers 64 / Lv
 // rows.  Same relaxation as bf_multi_kernel's light part.
 template <class TA, class TD, int V>
-__global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_light, int64_t k,
+__global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_light, int64_t k, int64_t ld,
                                                           int logLv, TD* d,
                                                           const uint32_t* __restrict__ chp,
                                                           uint32_t* __restrict__ chn,
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_
             for (int u = 0; u < 4; ++u)
               if (on[u]) {
                 w[u] = val[base + (kk + u) * g.C];
-                dn[u] = *(const VT*)&d[(int64_t)c[u] * k + j0];
+                dn[u] = *(const VT*)&d[(int64_t)c[u] * ld + j0];
               }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
@@ -363,18 +363,18 @@ __global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_
           if (!has_len && col == -1) break;
           if (!bit_get(chp, col - 1)) continue;
           const TD w = TD(val[i]);
-          const VT dn = *(const VT*)&d[(int64_t)(col - 1) * k + j0];
+          const VT dn = *(const VT*)&d[(int64_t)(col - 1) * ld + j0];
 #pragma unroll
           for (int t = 0; t < V; ++t) acc[t] = jmin(acc[t], sadd(w, dn[t]));
         }
-        VT own = *(const VT*)&d[rid * k + j0];
+        VT own = *(const VT*)&d[rid * ld + j0];
 #pragma unroll
         for (int t = 0; t < V; ++t)
           if (acc[t] < own[t]) {
             own[t] = acc[t];
             chbits |= 1u << t;
           }
-        if (chbits) *(VT*)&d[rid * k + j0] = own;
+        if (chbits) *(VT*)&d[rid * ld + j0] = own;
       }
     }
   }
@@ -400,7 +400,7 @@ constexpr int64_t kHeavyVecPart = 8192;
 
 template <class TA, class TD, int V>
 __global__ __launch_bounds__(kT) void bf_multi_heavy_vec_kernel(SellView g, int64_t parts,
-                                                                int64_t k, int logLv, TD* d,
+                                                                int64_t k, int64_t ld, int logLv, TD* d,
                                                                 const uint32_t* __restrict__ chp,
                                                                 uint32_t* __restrict__ chn,
                                                                 unsigned int* changed,
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(kT) void bf_multi_heavy_vec_kernel(SellView g, int6
       for (int u = 0; u < 4; ++u)
         if (on[u]) {
           w[u] = hv[i + u * gpb];
-          dn[u] = *(const VT*)&d[(int64_t)c[u] * k + j0];
+          dn[u] = *(const VT*)&d[(int64_t)c[u] * ld + j0];
         }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
@@ -457,7 +457,7 @@ __global__ __launch_bounds__(kT) void bf_multi_heavy_vec_kernel(SellView g, int6
       const int32_t c = g.heavy_col[i] - 1;
       if (!bit_get(chp, c)) continue;
       const TD w = TD(hv[i]);
-      const VT dn = *(const VT*)&d[(int64_t)c * k + j0];
+      const VT dn = *(const VT*)&d[(int64_t)c * ld + j0];
 #pragma unroll
       for (int t = 0; t < V; ++t) acc[t] = jmin(acc[t], sadd(w, dn[t]));
     }
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(kT) void bf_multi_heavy_vec_kernel(SellView g, int6
       unsigned long long cm = 0;
 #pragma unroll
       for (int t = 0; t < V; ++t)
-        if (atomic_min_lowered(&d[rid * k + j0 + t], acc[t])) cm |= 1ull << t;
+        if (atomic_min_lowered(&d[rid * ld + j0 + t], acc[t])) cm |= 1ull << t;
       if (cm) {
         set_flag(changed);
         if (changed_mask) or_new_bits(changed_mask, cm << j0);
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(256) void bf_sp_mark_big_kernel(BfSparse s, SellVie
 // The marked light rows of a k-column working set: a group of Lk lanes per
 // row, one lane per column (as bf_multi_kernel's light part).
 template <class TA, class TD>
-__global__ __launch_bounds__(kT) void bf_sp_rows_multi_kernel(BfSparse s, SellView g, int64_t k,
+__global__ __launch_bounds__(kT) void bf_sp_rows_multi_kernel(BfSparse s, SellView g, int64_t k, int64_t ld,
                                                               int logLk, TD* d,
                                                               const uint32_t* __restrict__ chp,
                                                               uint32_t* __restrict__ chn,
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_multi_kernel(BfSparse s, SellVi
           for (int u = 0; u < 4; ++u)
             if (on[u]) {
               w[u] = val[r.base + (kk + u) * r.stride];
-              dn[u] = d[(int64_t)c[u] * k + j];
+              dn[u] = d[(int64_t)c[u] * ld + j];
             }
 #pragma unroll
           for (int u = 0; u < 4; ++u)
@@ -620,12 +620,12 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_multi_kernel(BfSparse s, SellVi
         for (; kk < r.len; ++kk) {
           const int64_t e = r.base + kk * r.stride;
           const int32_t c = r.col[e] - 1;
-          if (bit_get(chp, c)) acc = jmin(acc, sadd(TD(val[e]), d[(int64_t)c * k + j]));
+          if (bit_get(chp, c)) acc = jmin(acc, sadd(TD(val[e]), d[(int64_t)c * ld + j]));
         }
-        const TD dv = d[v * k + j];
+        const TD dv = d[v * ld + j];
         const TD nv = jmin(dv, acc);
         if (nv < dv) {
-          d[v * k + j] = nv;
+          d[v * ld + j] = nv;
           ch = true;
         }
       }
@@ -644,7 +644,7 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_multi_kernel(BfSparse s, SellVi
 
 // The marked light rows with V columns per lane (as bf_multi_vec_kernel).
 template <class TA, class TD, int V>
-__global__ __launch_bounds__(kT) void bf_sp_rows_vec_kernel(BfSparse s, SellView g, int64_t k,
+__global__ __launch_bounds__(kT) void bf_sp_rows_vec_kernel(BfSparse s, SellView g, int64_t k, int64_t ld,
                                                             int logLv, TD* d,
                                                             const uint32_t* __restrict__ chp,
                                                             uint32_t* __restrict__ chn,
@@ -692,7 +692,7 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_vec_kernel(BfSparse s, SellView
           for (int u = 0; u < 4; ++u)
             if (on[u]) {
               w[u] = val[r.base + (kk + u) * r.stride];
-              dn[u] = *(const VT*)&d[(int64_t)c[u] * k + j0];
+              dn[u] = *(const VT*)&d[(int64_t)c[u] * ld + j0];
             }
 #pragma unroll
           for (int u = 0; u < 4; ++u)
@@ -705,18 +705,18 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_vec_kernel(BfSparse s, SellView
           const int32_t c = r.col[e] - 1;
           if (!bit_get(chp, c)) continue;
           const TD w = TD(val[e]);
-          const VT dn = *(const VT*)&d[(int64_t)c * k + j0];
+          const VT dn = *(const VT*)&d[(int64_t)c * ld + j0];
 #pragma unroll
           for (int t = 0; t < V; ++t) acc[t] = jmin(acc[t], sadd(w, dn[t]));
         }
-        VT own = *(const VT*)&d[v * k + j0];
+        VT own = *(const VT*)&d[v * ld + j0];
 #pragma unroll
         for (int t = 0; t < V; ++t)
           if (acc[t] < own[t]) {
             own[t] = acc[t];
             chbits |= 1u << t;
           }
-        if (chbits) *(VT*)&d[v * k + j0] = own;
+        if (chbits) *(VT*)&d[v * ld + j0] = own;
       }
     }
     const bool ch = chbits != 0;
@@ -967,7 +967,9 @@ int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_
 int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k, void* d,
                      const uint32_t* chp, uint32_t* chn, unsigned int* changed,
                      unsigned long long* changed_mask, const int32_t* done, const BfSparse* sp,
-                     int cus) {
+                     int cus, int64_t ld) {
+  if (ld < 0) ld = k;
+  if (ld < k) return set_error(GG_ERR_ARGUMENT, "row stride below the columns");
   if (changed_mask && k > 64) return set_error(GG_ERR_ARGUMENT, "column masks need k <= 64");
   return with_ad(A->val_dtype, dist_dtype, [&](auto a, auto dt) -> int {
     typedef typename decltype(a)::type TA;
@@ -991,7 +993,7 @@ int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int
     if (nb > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
     if (sp) bf_sparse_prologue(stream, *sp, g, chp, done, cus);
     constexpr int V = 16 / (int)sizeof(TD);
-    const bool vec = A->format == GG_SELL && k % V == 0 && ((uintptr_t)d & 15) == 0;
+    const bool vec = A->format == GG_SELL && k % V == 0 && ld % V == 0 && ((uintptr_t)d & 15) == 0;
     int lv = 0;
     while (lv < 6 && (1LL << lv) < k / V) ++lv;
     if (vec) {
@@ -1002,26 +1004,26 @@ int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int
       if (lbv > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
       if (lbv > 0)
         hipLaunchKernelGGL((bf_multi_vec_kernel<TA, TD, V>), dim3((unsigned)lbv), dim3(kT), 0,
-                           stream, g, n_light, k, lv, (TD*)d, chp, chn, changed, changed_mask,
+                           stream, g, n_light, k, ld, lv, (TD*)d, chp, chn, changed, changed_mask,
                            done, sp ? &sp->cnt[4] : nullptr);
       const int64_t pv = g.n_seg > 0 ? (g.seg_len + kHeavyVecPart - 1) / kHeavyVecPart : 1;
       const int64_t nh = g.n_seg * pv * jbv;
       if (nh > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
       if (nh > 0)
         hipLaunchKernelGGL((bf_multi_heavy_vec_kernel<TA, TD, V>), dim3((unsigned)nh), dim3(kT), 0,
-                           stream, g, pv, k, lv, (TD*)d, chp, chn, changed, changed_mask, done);
+                           stream, g, pv, k, ld, lv, (TD*)d, chp, chn, changed, changed_mask, done);
     } else if (nb > 0) {
       hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
-                         stream, A->format, A->ptr, g, n_light, parts, k, lg, (TD*)d, chp, chn,
+                         stream, A->format, A->ptr, g, n_light, parts, k, ld, lg, (TD*)d, chp, chn,
                          changed, changed_mask, done, sp ? &sp->cnt[4] : nullptr);
     }
     if (sp && vec) {
       hipLaunchKernelGGL((bf_sp_rows_vec_kernel<TA, TD, V>), dim3((unsigned)std::max(cus, 1) * 8),
-                         dim3(kT), 0, stream, *sp, g, k, lv, (TD*)d, chp, chn, changed,
+                         dim3(kT), 0, stream, *sp, g, k, ld, lv, (TD*)d, chp, chn, changed,
                          changed_mask, done);
     } else if (sp) {
       hipLaunchKernelGGL((bf_sp_rows_multi_kernel<TA, TD>), dim3((unsigned)std::max(cus, 1) * 8),
-                         dim3(kT), 0, stream, *sp, g, k, lg, (TD*)d, chp, chn, changed,
+                         dim3(kT), 0, stream, *sp, g, k, ld, lg, (TD*)d, chp, chn, changed,
                          changed_mask, done);
     }
     if (sp) {

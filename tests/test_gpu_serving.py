@@ -63,7 +63,8 @@ def test_server_results_equal_alone_and_in_mix(graphs):
         rids = [srv.submit(gids[g], k, s) for g, k, s in qs]
         for i, rid in enumerate(rids):
             g, k, s = qs[i]
-            np.testing.assert_array_equal(srv.fetch(rid, gids[g], k), exp[i])
+            np.testing.assert_array_equal(srv.fetch(rid, gids[g], k), exp[i],
+                                          err_msg=f"query {i}: graph {g} kind {k} source {s}")
         st = srv.stats()
         assert st["completed"] == len(qs) + 6 and st["failed"] == 0
         assert st["arena_used"] == 0
