@@ -48,6 +48,7 @@ enum WsSlot {
   WS_MSDIST,        // row-major distances of a multi-source traversal
   WS_MSHEAVY,       // per-heavy-row segment accumulators (kept zero between levels)
   WS_MSPUSH,        // top-down step buffers of multi-source BFS (accumulators kept zero)
+  WS_BROWS,         // a column-major dense operand made row-major
   WS_NSLOTS
 };
 

@@ -486,12 +486,25 @@ int launch_spmm(gg_ctx* ctx, void* C, int c_dtype, int c_layout, const gg_matrix
     if (!accum_ok(accum, std::is_floating_point<TC>::value))
       return set_error(GG_ERR_UNSUPPORTED, "accum operator %d not supported for this type", accum);
     if (A->format == GG_SELL && (A->flags & GG_FLAG_INTERNAL_IDS)) {
-      // columns are internal ids: B rows gathered into internal order, row-major
+      // columns are internal ids: B rows gathered into internal order, row-major;
+      // a column-major B is made row-major first (LDS-tiled), so that the
+      // gather moves whole rows instead of one element per line
       const int64_t n_int = A->nslices * A->slice_size + A->n_heavy;
+      const TB* bsrc = (const TB*)B;
+      if (!g.b_rowmajor && A->n > 0 && k > 1) {
+        void* tp;
+        GG_TRY(ws_get(ctx, WS_BROWS, sizeof(TB) * (size_t)(A->n * k), &tp));
+        const int64_t tiles = ((A->n + 63) / 64) * ((k + 63) / 64);
+        if (tiles > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "SpMM launch too large");
+        hipLaunchKernelGGL(to_rowmajor_kernel<TB>, dim3((unsigned)tiles), dim3(kBlk), 0,
+                           ctx->stream, (const TB*)B, A->n, k, (TB*)tp);
+        bsrc = (const TB*)tp;
+        g.b_rowmajor = 1;
+      }
       void* bp;
       GG_TRY(ws_get(ctx, WS_XPERM, sizeof(TB) * (size_t)(n_int * k), &bp));
       hipLaunchKernelGGL(gather_internal_kernel<TB>, dim3(blocks_for(n_int * k, kBlk)), dim3(kBlk),
-                         0, ctx->stream, (const TB*)B, A->perm, A->nslices * A->slice_size,
+                         0, ctx->stream, bsrc, A->perm, A->nslices * A->slice_size,
                          A->heavy_row, A->n_heavy, A->n, k, g.b_rowmajor, (TB*)bp);
       B = bp;
       g.b_rowmajor = 1;
