@@ -102,6 +102,10 @@ template <int W> __device__ __forceinline__ void alive_wave(const MsArgs& a, con
   }
 }
 
+// Entries per lane per batch of msbfs_sell_kernel's light rows: more loads in
+// flight per lane; with more mask words per vertex fewer, for the registers.
+template <int W> constexpr int ms_batch() { return W == 1 ? 16 : 8; }
+
 template <class TA, bool UNIT, int W>
 __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_sell_blocks, MsArgs a) {
   if (a.ctl && a.ctl[1]) return;  // a captured step after the last level
@@ -124,19 +128,20 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
       if (!saturated<W>(acc, sn, a)) {
         const TA* val = (const TA*)g.val;
         int64_t kk = 0;
-        if (has_len) {  // batches of 4: index loads, bitmap tests, then mask loads
+        if (has_len) {  // batches of kMsBatch: index loads, bitmap tests, then mask loads
+          constexpr int kMsBatch = ms_batch<W>();
           bool full = false;
-          for (; kk + 4 <= len && !full; kk += 4) {
-            int32_t c[4];
-            bool on[4];
+          for (; kk + kMsBatch <= len && !full; kk += kMsBatch) {
+            int32_t c[kMsBatch];
+            bool on[kMsBatch];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) c[u] = __builtin_nontemporal_load(&g.colval[base + (kk + u) * g.C]);
+            for (int u = 0; u < kMsBatch; ++u) c[u] = __builtin_nontemporal_load(&g.colval[base + (kk + u) * g.C]);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < kMsBatch; ++u)
               on[u] = (!test_any || bit_get(a.front_any, c[u] - 1)) &&
                       (UNIT || edge_on(val, base + (kk + u) * g.C));
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < kMsBatch; ++u)
               if (on[u])
 #pragma unroll
                 for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(c[u] - 1) * W + w];
