@@ -26,6 +26,7 @@
 namespace gg {
 
 constexpr int kB = 256;
+constexpr int kPullBatch = 16;  // entries per lane per batch of the SELL pull level
 
 __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -98,13 +99,13 @@ __global__ __launch_bounds__(kB) void bfs_pull_sell_kernel(
         const TA* val = (const TA*)g.val;
         int32_t hit = 0;
         int64_t k = 0;
-        if (!PARENTS && has_len) {  // batches of 4 independent loads, first hit wins
-          for (; k + 4 <= len && !hit; k += 4) {
-            int32_t c[4];
+        if (!PARENTS && has_len) {  // batches of kPullBatch independent loads, first hit wins
+          for (; k + kPullBatch <= len && !hit; k += kPullBatch) {
+            int32_t c[kPullBatch];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) c[u] = g.colval[base + (k + u) * g.C];
+            for (int u = 0; u < kPullBatch; ++u) c[u] = g.colval[base + (k + u) * g.C];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < kPullBatch; ++u)
               if (!hit && bit_get(frontier, c[u] - 1) &&
                   (UNIT || edge_on(val, base + (k + u) * g.C)))
                 hit = c[u];
