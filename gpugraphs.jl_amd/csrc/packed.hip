@@ -254,6 +254,11 @@ int gg_bfs_packed(gg_ctx* ctx, const gg_matrix* A_T, const int64_t* offsets, int
     return set_error(GG_ERR_ARGUMENT, "a packed graph is a square block-diagonal CSR");
   if (offsets[0] != 0 || offsets[n_graphs] != A_T->m)
     return set_error(GG_ERR_DIMENSION, "offsets do not cover the %lld rows", (long long)A_T->m);
+  // the chunk queue walks whole 256-row chunks: a last block that is not
+  // padded to a multiple of 256 rows would leave its trailing rows unvisited
+  if (A_T->m % kP)
+    return set_error(GG_ERR_ARGUMENT, "packed rows (%lld) must be a multiple of %d",
+                     (long long)A_T->m, kP);
   for (int g = 0; g < n_graphs; ++g) {
     if (offsets[g + 1] < offsets[g] || (offsets[g] % kP))
       return set_error(GG_ERR_ARGUMENT, "graph offsets must rise in multiples of %d", kP);

@@ -180,6 +180,7 @@ struct Query {
   bool has_block = false;
   hipEvent_t ready = nullptr;
   bool polled = false;
+  std::string msg;  // why a failed query failed
 };
 
 // Graphs with at least this many entries get streams of their own.
@@ -198,6 +199,7 @@ struct Engine {
   bool inflight = false;
   uint64_t active = 0, stepped = 0;
   std::vector<int64_t> slot_q;          // query id per slot
+  std::vector<int64_t> slot_steps;      // steps each SSSP slot has run (Bellman-Ford cap)
   std::vector<std::pair<int, int64_t>> seeds;  // (slot, 0-based source) waiting for the next step
   std::deque<int64_t> waiting;          // query ids
   unsigned long long* mask_h = nullptr;  // pinned: alive (BFS) / changed (SSSP) mask
@@ -266,6 +268,7 @@ static int make_engine(gg_server* s, const gg_matrix* A, int kind, std::unique_p
   e->vinv = (A->format == GG_SELL && (A->flags & GG_FLAG_INTERNAL_IDS)) ? A->vinv : nullptr;
   e->slots = kind == GG_QUERY_BFS ? 64 : 32;
   e->slot_q.assign(e->slots, -1);
+  e->slot_steps.assign(e->slots, 0);
   const int64_t m = A->m, ns = e->ns;
   if (A->nnz >= kOwnStreamNnz) {
     GG_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
@@ -360,6 +363,24 @@ static hipEvent_t take_event(gg_server* s) {
   return ev;
 }
 
+// Fail the query of a slot: its result block goes back to the arena and the
+// slot is free again (called with s->mu held; no step of the engine is in
+// flight, so nothing writes the block any more).
+static void fail_slot(gg_server* s, Engine& e, int sl, int status, const char* msg) {
+  Query& q = s->queries[e.slot_q[sl]];
+  if (q.has_block) {
+    s->arena.release(q.off, q.bytes);
+    q.has_block = false;
+  }
+  q.state = Q_FAILED;
+  q.status = status;
+  q.msg = msg;
+  q.t_done = now_s();
+  ++s->failed;
+  e.active &= ~(1ull << sl);
+  e.slot_q[sl] = -1;
+}
+
 // Retire the slots whose traversal ended in the step that just completed
 // (called with s->mu held).
 static int retire(gg_server* s, Engine& e) {
@@ -394,6 +415,21 @@ static int retire(gg_server* s, Engine& e) {
   return GG_OK;
 }
 
+// A Bellman-Ford column still falling after m + 16 steps has a negative
+// cycle in reach (a shortest path has at most m - 1 edges): its query fails
+// with GG_ERR_NOT_CONVERGED, as gg_shortest_path does (s->mu held).
+static void cap_sssp_slots(gg_server* s, Engine& e) {
+  if (e.kind != GG_QUERY_SSSP) return;
+  bool any = false;
+  for (int sl = 0; sl < e.slots; ++sl) {
+    if (!((e.active >> sl) & 1ull) || e.slot_steps[sl] <= e.m + 16) continue;
+    fail_slot(s, e, sl, GG_ERR_NOT_CONVERGED,
+              "Bellman-Ford did not converge (a negative cycle is reachable from the source)");
+    any = true;
+  }
+  if (any) s->cv_done.notify_all();
+}
+
 // Move waiting queries into free slots while the arena has room (s->mu held).
 static void admit(gg_server* s, Engine& e) {
   const double t = now_s();
@@ -412,6 +448,7 @@ static void admit(gg_server* s, Engine& e) {
     q.state = Q_RUNNING;
     e.active |= 1ull << free_slot;
     e.slot_q[free_slot] = q.id;
+    e.slot_steps[free_slot] = 0;
     e.seeds.emplace_back(free_slot, q.source - 1);
   }
 }
@@ -489,6 +526,8 @@ static int launch_step(gg_server* s, Engine& e) {
   e.stepped = e.active;
   e.inflight = true;
   e.steps += 1;
+  for (int i = 0; i < e.slots; ++i)
+    if ((e.active >> i) & 1ull) e.slot_steps[i] += 1;
   return GG_OK;
 }
 
@@ -523,14 +562,21 @@ static void worker_main(gg_server* s) {
         e.inflight = false;
         int st = retire(s, e);
         if (st != GG_OK) s->worker_error = gg_last_error();
+        cap_sssp_slots(s, e);
         progress = true;
       }
       admit(s, e);
       if (e.active) {
         int st = launch_step(s, e);
         if (st != GG_OK) {
+          // the step did not run: every query of the engine's slots fails
+          // with the launch error and gives its block back
           s->worker_error = gg_last_error();
-          e.active = 0;
+          (void)hipStreamSynchronize(e.stream);
+          for (int sl = 0; sl < e.slots; ++sl)
+            if ((e.active >> sl) & 1ull) fail_slot(s, e, sl, st, s->worker_error.c_str());
+          e.seeds.clear();
+          s->cv_done.notify_all();
         } else {
           busy = true;
           progress = true;
@@ -684,8 +730,11 @@ static int fetch_impl(gg_server* s, int64_t request_id, void* dst, size_t bytes,
   Query q = it->second;
   if (q.state == Q_FAILED) {
     s->queries.erase(it);
-    return set_error(q.status, "request %lld failed: result of %zu bytes exceeds the %zu-byte budget",
-                     (long long)request_id, q.bytes, s->arena.total());
+    s->cv_in.notify_one();
+    if (q.status == GG_ERR_BUDGET)
+      return set_error(q.status, "request %lld failed: result of %zu bytes exceeds the %zu-byte budget",
+                       (long long)request_id, q.bytes, s->arena.total());
+    return set_error(q.status, "request %lld failed: %s", (long long)request_id, q.msg.c_str());
   }
   lk.unlock();
   int st = GG_OK;

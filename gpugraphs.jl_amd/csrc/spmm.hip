@@ -5,7 +5,7 @@
 // storage order, so each (row, column) result is accumulated exactly as the
 // reference's work-item does, including its absorbing exit:
 //   CSR  csr_spmm_kernel!  src/spmm/CSR.jl:16-34 (break after the update),
-//   SELL sell_spmm_kernel! src/spmm/SELL.jl:221-242 (break on pad -1 or on
+//   SELL sell_spmm_kernel! src/spmm/SELL.jl:2-39 (break on pad -1 or on
 //        acc == terminal before the entry).
 // The lanes of a group read the same colval / nzval entry (one broadcast
 // load) and, with a row-major operand, k consecutive B values of the same
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kBlk) void spmm_kernel(typename MmTypes<S, TA, TB>:
 // read as one 16-byte load per entry, entries in batches of 4 (index / value
 // loads, then the B loads, then the folds in storage order).  Each column
 // folds its terms in the reference's order and stops at the absorbing value
-// (src/spmm/SELL.jl:221-242), so light rows give the one-lane-per-column
+// (src/spmm/SELL.jl:22-27), so light rows give the one-lane-per-column
 // result exactly.  Heavy rows: one block per (segment, column block), its
 // groups splitting the segment's entries; a row of one segment is written
 // directly, the others fold their segment partials in segment order

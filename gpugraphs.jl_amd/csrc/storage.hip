@@ -1,10 +1,10 @@
 // storage.hip -- host-side validation and the SELL-C-sigma builder.
 //
-// gg_sell_build lays out slices as src/storage/SELL.jl:288-370 does (slice of
+// gg_sell_build lays out slices as src/storage/SELL.jl:99-181 does (slice of
 // C rows padded to its longest row, entry k of every row contiguous, pad col
 // -1 / val 0).  With sigma > 1 rows are first sorted by length (longest first,
 // ties in row order) inside windows of sigma rows, the ordering the
-// reference leaves commented out at src/storage/SELL.jl:302-305; with a heavy
+// reference leaves commented out at src/storage/SELL.jl:113-116; with a heavy
 // threshold the rows longer than it are kept in a CSR partition processed in
 // fixed-size segments, so one hub row does not widen a whole slice.
 #include <stdlib.h>
@@ -137,7 +137,7 @@ int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* co
   std::vector<int32_t> heavy;
   int64_t C = slice_size;
   if (reference_layout) {
-    C = m > 0 ? std::min<int64_t>(slice_size, m) : slice_size;  // src/storage/SELL.jl:297
+    C = m > 0 ? std::min<int64_t>(slice_size, m) : slice_size;  // src/storage/SELL.jl:108
     slot_rows.resize(m);
     std::iota(slot_rows.begin(), slot_rows.end(), 1);
   } else {
@@ -203,7 +203,7 @@ int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* co
     if (sort_entries) std::stable_sort(ent.begin(), ent.end());
   };
 
-  // slice widths and pointers (src/storage/SELL.jl:309-321)
+  // slice widths and pointers (src/storage/SELL.jl:117-131)
   std::vector<int64_t> width(ns, 0);
   parallel_for(ns, 1024, [&](int64_t s0, int64_t s1) {
     for (int64_t s = s0; s < s1; ++s) {

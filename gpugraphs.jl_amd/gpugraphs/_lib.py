@@ -6,9 +6,12 @@ reference throws for the same conditions (see ``errors``).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
 import threading
+import weakref
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libgpugraphs_hip.so")
@@ -220,6 +223,43 @@ def check(status: int, what: str = ""):
 
 # ------------------------------------------------------------------ contexts
 _ctxs: dict = {}
+_servers = weakref.WeakSet()   # live GraphServers, closed before the contexts at exit
+_teardown_registered = False
+
+
+def shutdown():
+    """Explicit teardown, run from an atexit hook registered after torch
+    initialised the GPU (so it runs before torch's own hooks and before the
+    HIP runtime unloads): stop and destroy every live server first (their
+    worker threads step engines on the device), then every cached context
+    with its captured graphs, events, pinned buffer and workspace.  No HIP
+    object of this library outlives the runtime."""
+    for srv in list(_servers):
+        try:
+            srv.close()
+        except Exception:
+            pass
+    while _ctxs:
+        _, ctx = _ctxs.popitem()
+        lib().gg_ctx_destroy(ctx)
+
+
+def _register_teardown():
+    global _teardown_registered
+    if not _teardown_registered:
+        atexit.register(shutdown)
+        _teardown_registered = True
+
+
+def register_server(srv):
+    """Track a server so that shutdown() closes it before the contexts go."""
+    _register_teardown()
+    _servers.add(srv)
+
+
+def finalizing() -> bool:
+    """True once the interpreter is shutting down (no native calls then)."""
+    return sys.is_finalizing()
 
 
 def context(device: int = 0, stream=None):
@@ -232,6 +272,7 @@ def context(device: int = 0, stream=None):
     key = (device, int(stream.cuda_stream))
     ctx = _ctxs.get(key)
     if ctx is None:
+        _register_teardown()
         out = C.c_void_p()
         check(lib().gg_ctx_create(device, C.c_void_p(key[1]), C.byref(out)), "gg_ctx_create")
         ctx = out

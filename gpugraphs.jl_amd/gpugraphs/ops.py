@@ -61,8 +61,7 @@ def gpu_spmv_(C_, A, B, mul=GPUGraphs_mul, add=GPUGraphs_add, accum=GPUGraphs_se
     b = _same_backend(A, B, C_, mask)
     d = A.descriptor()
     r0, rl = _range_args(range_)
-    if isinstance(A, SparseGPUMatrixCSC) and (add is not GPUGraphs_add or accum is not GPUGraphs_second):
-        pass  # the CSC kernel ignores add and accum, as src/spmv/CSC.jl:14-20 does
+    # a CSC matrix ignores add and accum, as src/spmv/CSC.jl:14-20 does
     ctx = context(b.device)
     check(lib().gg_spmv(ctx, ptr(C_), dtype_code(np_dtype_of(C_)), C_.numel(), C.byref(d), ptr(B),
                         dtype_code(np_dtype_of(B)), B.numel(), op_code(add), op_code(mul),

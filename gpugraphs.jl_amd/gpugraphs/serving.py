@@ -51,6 +51,7 @@ class GraphServer:
         self._graphs = []       # keep matrices (and their descriptors) alive
         self._meta = []         # (n, result dtype per kind)
         self._started = False
+        _lib.register_server(self)
 
     def add_graph(self, A_T) -> int:
         d = A_T.descriptor()
@@ -106,6 +107,10 @@ class GraphServer:
         self.close()
 
     def __del__(self):
+        # at interpreter shutdown the atexit teardown has closed every live
+        # server; no native call is made once finalisation has begun
+        if _lib.finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -364,7 +364,7 @@ class SparseGPUMatrixCSC(AbstractSparseGPUMatrix):
 
 
 class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
-    """SparseGPUMatrixSELL, src/storage/SELL.jl:3-370.
+    """SparseGPUMatrixSELL, src/storage/SELL.jl:3-247.
 
       SparseGPUMatrixSELL(S, backend, slice_size=32, sigma=1, heavy_threshold=0)
       SparseGPUMatrixSELL(dense, backend, slice_size=32, ...)
@@ -373,7 +373,7 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
                           slice_ptr, colval, nzval, backend)   raw arrays
 
     sigma = 1 with no heavy threshold is the reference layout (identity perm,
-    slice_size clipped to the row count, src/storage/SELL.jl:297,304).
+    slice_size clipped to the row count, src/storage/SELL.jl:108,115).
     sigma > 1 sorts rows by length inside windows of sigma rows; rows longer
     than heavy_threshold are held in a CSR partition processed in segments of
     heavy_seg_len entries.  relabel=True (square matrices) sorts all rows by
@@ -442,7 +442,7 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
                 self.perm = arr(h.perm, h.n_slots, np.int32)
                 self.slot_len = to_backend(arr(h.slot_len, h.n_slots, np.int32), np.int32, backend)
             else:
-                self.perm = np.arange(1, m + 1, dtype=np.int64)   # identity, src/storage/SELL.jl:304
+                self.perm = np.arange(1, m + 1, dtype=np.int64)   # identity, src/storage/SELL.jl:115
                 self.slot_len = None
             self._perm_dev = (to_backend(self.perm, np.int32, backend) if h.perm else None)
             self.n_heavy, self.heavy_seg_len, self.n_seg = int(h.n_heavy), int(h.heavy_seg_len), int(h.n_seg)
@@ -564,7 +564,7 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
         self.unit = unit_pattern(h[to_host(self.colval) != -1])
 
     def nnz(self):
-        return self._nnz   # true nonzeros, src/storage/SELL.jl:431
+        return self._nnz   # true nonzeros, src/storage/SELL.jl:242
 
     def descriptor(self, host_ok: bool = False) -> GGMatrix:
         def P(a):

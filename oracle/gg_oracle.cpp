@@ -351,7 +351,7 @@ int spmv_any(void* c, int cdt, const orc_matrix& A, const void* b, int bdt, int 
 
 // ---------------------------------------------------------------- SpMM
 // csr_spmm_kernel! src/spmm/CSR.jl:16-34 (+ range :51-70, row mask :89-108)
-// and sell_spmm_kernel! src/spmm/SELL.jl:221-242 (break on -1 or on
+// and sell_spmm_kernel! src/spmm/SELL.jl:2-39 (break on -1 or on
 // acc == terminal before the entry).  B and C are column-major.
 template <class T, class TA>
 int spmm_typed(T* Cm, int64_t k, const orc_matrix& A, const void* Bv, int bdt, int add, int mul,
@@ -427,16 +427,16 @@ int orc_csr_check(int64_t m, int64_t n, const int64_t* rowptr, int64_t rowptr_le
   return S_OK;
 }
 
-// SELL builder from a CSR, src/storage/SELL.jl:288-370 (reference layout:
+// SELL builder from a CSR, src/storage/SELL.jl:99-181 (reference layout:
 // identity perm, slices padded to their longest row, column-major inside a
 // slice, pad col -1 / val 0).  Phase 1 returns sizes.
 int orc_sell_size(int64_t m, const int32_t* rowptr, int64_t slice_size, int64_t* eff_slice,
                   int64_t* nslices, int64_t* n_stored) {
   if (m == 0) { *eff_slice = slice_size; *nslices = 0; *n_stored = 0; return S_OK; }
-  int64_t C = std::min(slice_size, m);                           // :297
-  int64_t ns = (m + C - 1) / C;                                  // :298
+  int64_t C = std::min(slice_size, m);                           // :108
+  int64_t ns = (m + C - 1) / C;                                  // :109
   int64_t st = 0;
-  for (int64_t s = 1; s <= ns; ++s) {                            // :309-314
+  for (int64_t s = 1; s <= ns; ++s) {                            // :118-124
     int64_t r0 = (s - 1) * C + 1, r1 = std::min(s * C, m), mx = 0;
     for (int64_t r = r0; r <= r1; ++r) mx = std::max<int64_t>(mx, rowptr[r] - rowptr[r - 1]);
     st += mx * C;
@@ -449,18 +449,18 @@ int orc_sell_fill(int64_t m, const int32_t* rowptr, const int32_t* colval, const
                   int val_dtype, int64_t C, int64_t nslices, int32_t* slice_ptr,
                   int32_t* colval_out, void* nzval_out) {
   int esz = val_dtype == D_BOOL ? 1 : (val_dtype == D_I32 || val_dtype == D_F32) ? 4 : 8;
-  slice_ptr[0] = 1;                                              // :318
-  for (int64_t s = 1; s <= nslices; ++s) {                       // :319-321
+  slice_ptr[0] = 1;                                              // :127
+  for (int64_t s = 1; s <= nslices; ++s) {                       // :128-130
     int64_t r0 = (s - 1) * C + 1, r1 = std::min(s * C, m), mx = 0;
     for (int64_t r = r0; r <= r1; ++r) mx = std::max<int64_t>(mx, rowptr[r] - rowptr[r - 1]);
     slice_ptr[s] = (int32_t)(slice_ptr[s - 1] + mx * C);
   }
-  for (int64_t s = 1; s <= nslices; ++s) {                       // :323-354
+  for (int64_t s = 1; s <= nslices; ++s) {                       // :133-165
     int64_t base = slice_ptr[s - 1] - 1;
     int64_t width = (slice_ptr[s] - slice_ptr[s - 1]) / C;
     for (int64_t k = 0; k < width; ++k)
       for (int64_t r = 0; r < C; ++r) {
-        int64_t pos = base + k * C + r;                          // column-major flatten :347-350
+        int64_t pos = base + k * C + r;                          // column-major flatten :158-161
         int64_t row = (s - 1) * C + r + 1;
         int64_t len = row <= m ? rowptr[row] - rowptr[row - 1] : 0;
         if (k < len) {
@@ -468,7 +468,7 @@ int orc_sell_fill(int64_t m, const int32_t* rowptr, const int32_t* colval, const
           colval_out[pos] = colval[src];
           memcpy((char*)nzval_out + pos * esz, (const char*)nzval + src * esz, esz);
         } else {
-          colval_out[pos] = -1;                                  // pad :330-331
+          colval_out[pos] = -1;                                  // pad :139-141
           memset((char*)nzval_out + pos * esz, 0, esz);
         }
       }
@@ -628,23 +628,23 @@ template <class Td>
 int sssp_typed(const orc_matrix& A, int64_t source, Td* dist, int nth, int64_t max_iter,
                int64_t* iters) {
   const int64_t m = A.m;
-  std::vector<Td> next(m, J<Td>::tmax());                                // :210-211
+  std::vector<Td> next(m, J<Td>::tmax());                                // :20,28
   for (int64_t i = 0; i < m; ++i) dist[i] = J<Td>::tmax();
-  dist[source - 1] = Td(0);                                              // :234
-  next[source - 1] = Td(0);                                              // :235
+  dist[source - 1] = Td(0);                                              // :51
+  next[source - 1] = Td(0);                                              // :52
   int64_t iter = 0;
   const int dt = dcode<Td>();
-  while (true) {                                                         // :240
+  while (true) {                                                         // :57
     iter += 1;
     int s = orc_spmv(next.data(), dt, m, &A, dist, dt, m, OP_MIN, OP_SAFE_ADD, OP_MIN,
-                     nullptr, 0, 0, 0, -1, nth);                         // :244-252
+                     nullptr, 0, 0, 0, -1, nth);                         // :61-69
     if (s) return s;
-    if (iter % 8 == 0) {                                                 // :254-259
+    if (iter % 8 == 0) {                                                 // :71-76
       bool progress = false;
       for (int64_t v = 0; v < m && !progress; ++v) progress = next[v] < dist[v];
       if (!progress) break;
     }
-    for (int64_t v = 0; v < m; ++v) dist[v] = next[v];                   // :262
+    for (int64_t v = 0; v < m; ++v) dist[v] = next[v];                   // :79
     if (max_iter > 0 && iter >= max_iter) { if (iters) *iters = iter; return S_NOCONV; }
   }
   if (iters) *iters = iter;
@@ -657,9 +657,9 @@ template <class Td>
 int mssp_typed(const orc_matrix& A, const int32_t* sources, int64_t k, Td* dist, int nth,
                int64_t max_iter, int64_t* iters) {
   const int64_t m = A.m;
-  std::vector<Td> next((size_t)m * k, J<Td>::tmax());                   // :287-288
+  std::vector<Td> next((size_t)m * k, J<Td>::tmax());                   // :97,105
   for (int64_t i = 0; i < m * k; ++i) dist[i] = J<Td>::tmax();
-  for (int64_t i = 0; i < k; ++i) {                                      // :313-318
+  for (int64_t i = 0; i < k; ++i) {                                      // :130-135
     dist[(sources[i] - 1) + i * m] = Td(0);
     next[(sources[i] - 1) + i * m] = Td(0);
   }
@@ -668,12 +668,12 @@ int mssp_typed(const orc_matrix& A, const int32_t* sources, int64_t k, Td* dist,
   while (true) {
     iter += 1;
     int s = orc_spmm(next.data(), dt, m, k, &A, dist, dt, m, k, OP_MIN, OP_SAFE_ADD, OP_MIN,
-                     nullptr, 0, 0, 0, -1, nth);                         // :325-333
+                     nullptr, 0, 0, 0, -1, nth);                         // :142-150
     if (s) return s;
-    bool progress = false;                                               // :335-338
+    bool progress = false;                                               // :152-155
     for (int64_t v = 0; v < m * k && !progress; ++v) progress = next[v] < dist[v];
     if (!progress) break;
-    for (int64_t v = 0; v < m * k; ++v) dist[v] = next[v];               // :340
+    for (int64_t v = 0; v < m * k; ++v) dist[v] = next[v];               // :157
     if (max_iter > 0 && iter >= max_iter) { if (iters) *iters = iter; return S_NOCONV; }
   }
   if (iters) *iters = iter;
@@ -781,6 +781,123 @@ int orc_spmspv(int32_t* c_nzind, void* c_nzval, int64_t* c_nnz, int cdt, int64_t
     case D_F64: return run(double(0));
   }
   return S_UNSUP;
+}
+
+
+// ------------------------------------------------------ full-size checkers
+// Textbook queue BFS, the independent check of BFS levels at any size (it is
+// not the reference's pull loop, bfs.jl:31-73, but computes the same unique
+// levels).  A_T is CSR of the transposed adjacency: row v lists the
+// in-neighbours u of v, so u -> v is an edge when A_T[v, u] is stored and
+// band(A_T[v, u], one) != 0 (the or-and product of bfs.jl:52-59 with the
+// frontier's ones: Bool true, or an odd integer).  One serial BFS per source,
+// sources spread over the threads; dist is m x k column-major Int32 with
+// typemax for unreached vertices (bfs.jl:18).
+int orc_bfs_queue_multi(const orc_matrix* A, const int32_t* sources, int64_t k, int32_t* dist,
+                        int nthreads) {
+  if (A->format != F_CSR || A->m != A->n) return S_ARG;
+  if (A->val_dtype != D_BOOL && A->val_dtype != D_I32 && A->val_dtype != D_I64) return S_UNSUP;
+  const int64_t m = A->m;
+  for (int64_t i = 0; i < k; ++i)
+    if (sources[i] < 1 || sources[i] > m) return S_BOUNDS;
+  auto active = [&](int64_t e) -> bool {
+    switch (A->val_dtype) {
+      case D_BOOL: return ((const uint8_t*)A->val)[e] & 1;
+      case D_I32: return ((const int32_t*)A->val)[e] & 1;
+      default: return ((const int64_t*)A->val)[e] & 1;
+    }
+  };
+  // out-neighbour lists: the transpose of A_T (counting sort by column)
+  std::vector<int64_t> optr(m + 1, 0);
+  const int64_t nnz = (int64_t)A->ptr[m] - 1;
+  for (int64_t e = 0; e < nnz; ++e)
+    if (active(e)) optr[A->idx[e]] += 1;
+  for (int64_t u = 0; u < m; ++u) optr[u + 1] += optr[u];
+  std::vector<int32_t> oidx(optr[m] > 0 ? optr[m] : 1);
+  {
+    std::vector<int64_t> at(optr.begin(), optr.end() - 1);
+    for (int64_t v = 0; v < m; ++v)
+      for (int64_t e = A->ptr[v] - 1; e < A->ptr[v + 1] - 1; ++e)
+        if (active(e)) oidx[at[A->idx[e] - 1]++] = (int32_t)v;
+  }
+  const int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nth)
+  {
+    std::vector<int32_t> queue(m > 0 ? m : 1);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t q = 0; q < k; ++q) {
+      int32_t* d = dist + q * m;
+      for (int64_t v = 0; v < m; ++v) d[v] = std::numeric_limits<int32_t>::max();
+      int64_t head = 0, tail = 0;
+      const int32_t s = sources[q] - 1;
+      d[s] = 0;
+      queue[tail++] = s;
+      while (head < tail) {
+        const int32_t u = queue[head++];
+        for (int64_t e = optr[u]; e < optr[u + 1]; ++e) {
+          const int32_t v = oidx[e];
+          if (d[v] == std::numeric_limits<int32_t>::max()) {
+            d[v] = d[u] + 1;
+            queue[tail++] = v;
+          }
+        }
+      }
+    }
+  }
+  return S_OK;
+}
+
+// Shortest-path certificate for positive edge weights (what the mathematics
+// fixes, independent of the Bellman-Ford loop of shortest_path.jl:57-81):
+// d is the exact distance vector from s iff d[s] = 0, no edge lowers any
+// distance (d[v] <= d[u] + w(u -> v), so d <= the true distance), and every
+// other vertex with a finite d has a tight in-edge (d[v] == d[u] + w for some
+// u with a finite d[u]; following tight edges back strictly decreases d and
+// ends at s, so d is the length of a real path).  Unreached = typemax
+// (Inf for floats).  Sums are formed in double, exact for integer-valued
+// weights and distances below 2^53.  dist is m x k column-major of dist_dtype;
+// bad[q] receives the number of vertices of column q that violate a condition.
+int orc_sssp_certificate(const orc_matrix* A, const int32_t* sources, int64_t k,
+                         const void* dist, int dist_dtype, int nthreads, int64_t* bad) {
+  if (A->format != F_CSR || A->m != A->n) return S_ARG;
+  const int64_t m = A->m;
+  for (int64_t i = 0; i < k; ++i)
+    if (sources[i] < 1 || sources[i] > m) return S_BOUNDS;
+  auto dval = [&](int64_t i) -> double {
+    switch (dist_dtype) {
+      case D_I32: { int32_t x = ((const int32_t*)dist)[i];
+        return x == std::numeric_limits<int32_t>::max() ? INFINITY : (double)x; }
+      case D_I64: { int64_t x = ((const int64_t*)dist)[i];
+        return x == std::numeric_limits<int64_t>::max() ? INFINITY : (double)x; }
+      case D_F32: return (double)((const float*)dist)[i];
+      default: return ((const double*)dist)[i];
+    }
+  };
+  auto wval = [&](int64_t e) -> double { return load_as<double>(A->val, A->val_dtype, e); };
+  if (dist_dtype != D_I32 && dist_dtype != D_I64 && dist_dtype != D_F32 && dist_dtype != D_F64)
+    return S_ARG;
+  const int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
+  for (int64_t q = 0; q < k; ++q) {
+    const int64_t off = q * m;
+    const int64_t s = sources[q] - 1;
+    int64_t nbad = 0;
+#pragma omp parallel for num_threads(nth) reduction(+ : nbad) schedule(static, 4096)
+    for (int64_t v = 0; v < m; ++v) {
+      const double dv = dval(off + v);
+      double best = INFINITY;
+      for (int64_t e = A->ptr[v] - 1; e < A->ptr[v + 1] - 1; ++e) {
+        const double du = dval(off + A->idx[e] - 1);
+        const double c = du + wval(e);
+        if (c < best) best = c;
+      }
+      bool ok = dv <= best;
+      if (v == s) ok = ok && dv == 0.0;
+      else if (std::isfinite(dv)) ok = ok && dv == best;
+      if (!ok) nbad += 1;
+    }
+    bad[q] = nbad;
+  }
+  return S_OK;
 }
 
 }  // extern "C"

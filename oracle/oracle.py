@@ -71,6 +71,8 @@ def lib():
         L.orc_shortest_path_multi.argtypes = [vp, vp, i64, vp, i32, i32, i64, vp]
         L.orc_spmspv.argtypes = [vp, vp, vp, i32, i64, vp, vp, vp, i64, i32, i64, i32, i32, vp,
                                  i32, i64, i32]
+        L.orc_bfs_queue_multi.argtypes = [vp, vp, i64, vp, i32]
+        L.orc_sssp_certificate.argtypes = [vp, vp, i64, vp, i32, i32, vp]
         _lib = L
     return _lib
 
@@ -136,7 +138,7 @@ def csr_check(m, n, rowptr, colval, nzval_len) -> int:
 
 
 def sell_from_csr(A: Matrix, slice_size: int = 32) -> Matrix:
-    """Reference SELL builder, src/storage/SELL.jl:288-370."""
+    """Reference SELL builder, src/storage/SELL.jl:99-181."""
     L = lib()
     eff, ns, st = C.c_int64(), C.c_int64(), C.c_int64()
     _check(L.orc_sell_size(A.m, _p(A.ptr), slice_size, C.byref(eff), C.byref(ns), C.byref(st)))
@@ -245,6 +247,27 @@ def spmspv(A: Matrix, b_nzind, b_nzval, b_len, add, mul, out_dtype, mask=None,
                             _p(mask), 0 if mask is None else _NP2CODE[mask.dtype],
                             0 if mask is None else len(mask), int(mask_complement)))
     return ci[:cnt.value].copy(), cv[:cnt.value].copy()
+
+
+def bfs_levels_queue(A: Matrix, sources, nthreads=0):
+    """Textbook queue BFS levels from each source (m x k, column-major Int32):
+    the independent check of BFS levels at full size (orc_bfs_queue_multi)."""
+    src = np.ascontiguousarray(np.atleast_1d(sources), np.int32)
+    dist = np.empty((A.m, len(src)), np.int32, order="F")
+    _check(lib().orc_bfs_queue_multi(C.byref(A.view()), _p(src), len(src), _p(dist), nthreads))
+    return dist
+
+
+def sssp_certificate(A: Matrix, sources, dist, nthreads=0):
+    """Per source, the number of vertices whose distance fails the
+    shortest-path certificate (0 = exact; positive weights).  dist is m x k
+    (column-major) or a length-m vector for one source."""
+    src = np.ascontiguousarray(np.atleast_1d(sources), np.int32)
+    d = np.asfortranarray(dist.reshape(A.m, len(src), order="F"))
+    bad = np.zeros(len(src), np.int64)
+    _check(lib().orc_sssp_certificate(C.byref(A.view()), _p(src), len(src), _p(d),
+                                      _NP2CODE[d.dtype], nthreads, _p(bad)))
+    return bad
 
 
 def from_scipy_csr(S, dtype=None) -> Matrix:

@@ -162,8 +162,6 @@ def test_spmv_semirings_exact(env, fmt, sr, variant):
         kw["mask"] = dev(torch, mask)
     if rng_ is not None:
         kw["range_"] = rng_
-    if fmt != "csr" and add == "add" and ta in (np.int32, np.int64):
-        pass
     gg.gpu_spmv_(c, A, dev(torch, x), **kw)
     torch.cuda.synchronize()
     co = c0.copy()
@@ -466,23 +464,37 @@ def test_bfs_parents_vs_oracle(env, fmt):
                 np.testing.assert_array_equal(host(p), orc.bfs_parents(Ao, s, batch_size=batch))
 
 
-def test_bfs_direction_optimising_equals_plain(env):
+def test_bfs_direction_optimising_vs_oracle(env):
+    """Direction-optimising BFS levels (push over A's rows, pull over A_T)
+    against the oracle's pull loop (src/algorithms/bfs.jl:31-73) on a
+    symmetric R-MAT pattern in every pull layout, and on a directed graph
+    whose out-edge CSR differs from A_T."""
     gg, torch, orc = env
     from gpugraphs import generators as GEN
     R = GEN.rmat(16, seed=4)
+    Ao = orc.csr(R.m, R.n, R.rowptr, R.colval, np.ones(R.nnz, np.uint8))
     A = R.to_csr(gg.ROCBackend(0))
     H = R.to_sell(gg.ROCBackend(0), 64, 4096, 256, 1024)
     L = R.to_sell(gg.ROCBackend(0), 64, 1, 1024, 1024, relabel=True)
     deg = R.degrees()
-    roots = np.nonzero(deg)[0][:5] + 1
+    roots = list(np.nonzero(deg)[0][:5] + 1) + [int(np.argmax(deg)) + 1]
     for s in roots:
-        print_case("bfs do", s)
-        ref = host(gg.bfs_distances(A, int(s)))
-        np.testing.assert_array_equal(host(gg.bfs_distances(H, int(s))), ref)
-        np.testing.assert_array_equal(host(gg.bfs_distances(L, int(s))), ref)
+        print_case("bfs do rmat16", s)
+        ref = orc.bfs_distances(Ao, int(s))
         np.testing.assert_array_equal(host(gg.bfs_distances_do(A, int(s))), ref)
         np.testing.assert_array_equal(host(gg.bfs_distances_do(H, int(s), A_out=A)), ref)
         np.testing.assert_array_equal(host(gg.bfs_distances_do(L, int(s), A_out=A)), ref)
+    # directed: pushes follow A (row u = out-neighbours of u), pulls A_T
+    rng = np.random.default_rng(13)
+    Adj = G.random_digraph(rng, 5000, 0.002, dtype=np.bool_)
+    AT = Adj.T.tocsr()
+    Ad = orc.from_scipy_csr(AT)
+    A_T = make(gg, AT, "csr")
+    A_out = make(gg, Adj.tocsr(), "csr")
+    for s in (1, 2500, 5000):
+        print_case("bfs do directed", s)
+        np.testing.assert_array_equal(host(gg.bfs_distances_do(A_T, s, A_out=A_out)),
+                                      orc.bfs_distances(Ad, s))
 
 
 @pytest.mark.parametrize("fmt", FORMATS)
@@ -884,3 +896,22 @@ def test_sell_device_builder_degenerate(env, relabel):
                         (b[key] is None or len(b[key]) == 0), key
                 else:
                     np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+
+
+def test_bfs_packed_rejects_unpadded_last_graph(env):
+    """The persistent kernel walks whole 256-row chunks: a pack whose last
+    block is not padded to a multiple of 256 rows is an ArgumentError (its
+    trailing rows would otherwise stay unvisited)."""
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    rng = np.random.default_rng(79)
+    M = G.undirected_adj(300, G.barabasi_albert(300, 3, 5), np.int32).astype(np.bool_)
+    P, off = GEN.pack([_host_csr(sp.csr_matrix(M.T), np.bool_)])
+    assert P.m == 512 and off.tolist() == [0, 512]
+    # drop the padding rows of the only block: 300 rows, offsets [0, 300]
+    Hs = GEN.HostCSR(300, 300, P.rowptr[:301].copy(), P.colval, P.nzval, True)
+    A = Hs.to_csr(gg.ROCBackend(0))
+    print_case("bfs packed unpadded")
+    with pytest.raises(gg.ArgumentError):
+        gg.bfs_packed(A, np.array([0, 300], np.int64), np.array([int(rng.integers(300)) + 1],
+                                                                np.int32))

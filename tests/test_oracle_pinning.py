@@ -338,3 +338,156 @@ def test_spmspv_vs_fp64_and_mask(oracle):
         np.testing.assert_array_equal(ci, exp_idx)
         ref = S.astype(np.float64) @ b
         np.testing.assert_allclose(cv, ref[ci - 1], rtol=1e-5, atol=1e-7)
+
+
+def _tiny_csc(rng, m, n, p, dtype):
+    S = sp.random(m, n, density=p, format="csc", random_state=rng, dtype=np.float64)
+    S.data = rng.integers(1, 9, S.nnz).astype(np.float64)
+    S.sort_indices()
+    return S.astype(dtype)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_spmspv_semirings_brute_force(oracle, seed):
+    """Min-plus, any-secondi and or-and SpMSpV against a dense double loop over
+    rows and the frontier's columns (no sparse restatement): c_i folds
+    mul(A[i,j], b_j, j) over the stored A[i,j] of b's columns, ascending j;
+    rows outside the mask (or inside it, for the complement) take nothing."""
+    rng = np.random.default_rng(seed)
+    m, n = 23, 17
+    for add, mul, vt, ot in [(oracle.OP_MIN, oracle.OP_SAFE_ADD, np.float32, np.float32),
+                             (oracle.OP_MIN, oracle.OP_SAFE_ADD, np.int32, np.int32),
+                             (oracle.OP_ANY, oracle.OP_SECONDI, np.int32, np.int32),
+                             (oracle.OP_BOR, oracle.OP_BAND, np.bool_, np.bool_)]:
+        S = _tiny_csc(rng, m, n, 0.3, vt)
+        D = S.toarray()
+        stored = (S != 0).toarray() if vt != np.bool_ else S.toarray()
+        A = oracle.csc(m, n, S.indptr + 1, S.indices + 1, S.data)
+        nz = np.sort(rng.choice(n, 6, replace=False)) + 1
+        if vt == np.bool_:
+            bv = np.ones(6, np.bool_)
+        elif mul == oracle.OP_SECONDI:
+            bv = np.ones(6, np.int32)
+        else:
+            bv = rng.integers(0, 50, 6).astype(vt)
+        mask = (rng.random(m) < 0.6).astype(np.int32)
+        for use_mask, comp in ((False, False), (True, False), (True, True)):
+            ci, cv = oracle.spmspv(A, nz, bv, n, add, mul, ot,
+                                   mask=mask if use_mask else None, mask_complement=comp)
+            exp_i, exp_v = [], []
+            for i in range(m):
+                if use_mask and (mask[i] != 0) == comp:
+                    continue
+                vals = [(j, D[i, j - 1], b) for j, b in zip(nz, bv) if stored[i, j - 1]]
+                if not vals:
+                    continue
+                if add == oracle.OP_MIN:
+                    acc = min(a + b for _, a, b in vals)
+                elif add == oracle.OP_ANY:
+                    acc = max(j * b for j, _, b in vals)   # any = max, secondi = col * b
+                else:
+                    acc = any(bool(a) and bool(b) for _, a, b in vals)
+                exp_i.append(i + 1)
+                exp_v.append(acc)
+            np.testing.assert_array_equal(ci, np.array(exp_i, np.int32))
+            np.testing.assert_array_equal(cv, np.array(exp_v).astype(ot))
+
+
+@pytest.mark.parametrize("fmt", ["csr", "sell"])
+def test_spmm_mask_and_range_closed_form(oracle, fmt):
+    """Row mask and row range of gpu_spmm! (src/spmm/CSR.jl:36-109):
+    selected rows get A*B, every other row of C keeps its prior value."""
+    rng = np.random.default_rng(17)
+    S = random_csr(rng, 200, 200, 0.1)
+    k = 7
+    B = np.asfortranarray(rng.random((200, k), dtype=np.float32))
+    full = S.astype(np.float64) @ B.astype(np.float64)
+    A = oracle.from_scipy_csr(S)
+    if fmt == "sell":
+        A = oracle.sell_from_csr(A, 32)
+    prior = np.asfortranarray(np.full((200, k), -3.0, np.float32))
+    if fmt == "csr":
+        mask = (rng.random(200) < 0.4).astype(np.int32)
+        Cm = prior.copy(order="F")
+        oracle.spmm(Cm, A, B, mask=mask)
+        want = np.where(mask[:, None] != 0, full, -3.0)
+        np.testing.assert_allclose(Cm, want, rtol=1e-5, atol=1e-6)
+        Cr = prior.copy(order="F")
+        oracle.spmm(Cr, A, B, range_=(41, 100))
+        want = np.full((200, k), -3.0)
+        want[40:140] = full[40:140]
+        np.testing.assert_allclose(Cr, want, rtol=1e-5, atol=1e-6)
+    else:
+        # SELL SpMM has neither mask nor range (src/spmm/SELL.jl:84-102)
+        with pytest.raises(oracle.OracleError):
+            oracle.spmm(prior.copy(order="F"), A, B, mask=np.ones(200, np.int32))
+
+
+@pytest.mark.parametrize("name,n,edges", _graphs())
+def test_bfs_queue_levels_vs_csgraph_and_pull_loop(oracle, name, n, edges):
+    """The full-size BFS checker (textbook queue BFS) against scipy's BFS and
+    against the restated pull loop, from every source of a small graph."""
+    Adj = G.undirected_adj(n, edges, np.bool_)
+    A = oracle.from_scipy_csr(Adj.T.tocsr())
+    D = oracle.bfs_levels_queue(A, np.arange(1, n + 1))
+    ref = csgraph.shortest_path(Adj.astype(np.float64), unweighted=True, directed=True)
+    for s in range(n):
+        r = ref[s]
+        want = np.where(np.isinf(r), INT32_MAX, r).astype(np.int32)
+        np.testing.assert_array_equal(D[:, s], want)
+        np.testing.assert_array_equal(D[:, s], oracle.bfs_distances(A, s + 1))
+
+
+def test_bfs_queue_directed_and_even_weights(oracle):
+    """Directed digraph with Int32 values: an even value is an inactive edge
+    under band with the frontier's ones, as in the pull loop."""
+    rng = np.random.default_rng(3)
+    Adj = G.random_digraph(rng, 300, 0.01, wmax=6, dtype=np.int32)
+    AT = Adj.T.tocsr()
+    A = oracle.from_scipy_csr(AT)
+    srcs = [1, 50, 300]
+    D = oracle.bfs_levels_queue(A, srcs)
+    odd = Adj.copy()
+    odd.data = (odd.data % 2 == 1).astype(np.float64)
+    odd.eliminate_zeros()
+    ref = csgraph.shortest_path(odd.astype(np.float64), unweighted=True, directed=True)
+    for q, s in enumerate(srcs):
+        want = np.where(np.isinf(ref[s - 1]), INT32_MAX, ref[s - 1]).astype(np.int32)
+        np.testing.assert_array_equal(D[:, q], want)
+        np.testing.assert_array_equal(D[:, q], oracle.bfs_distances(A, s))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.int32])
+def test_sssp_certificate_accepts_exact_and_rejects_mistakes(oracle, dt):
+    """The full-size Bellman-Ford checker: Dijkstra's distances pass; a
+    distance one too high or one too low, a source not at zero, a reachable
+    vertex marked unreached and an unreachable one given a value all fail."""
+    rng = np.random.default_rng(11)
+    Adj = G.random_digraph(rng, 400, 0.008, wmax=1000, dtype=np.float32)
+    AT = Adj.T.tocsr().astype(dt)
+    A = oracle.from_scipy_csr(AT)
+    srcs = [1, 200]
+    ref = csgraph.dijkstra(Adj.astype(np.float64), directed=True, indices=[s - 1 for s in srcs]).T
+    top = np.iinfo(dt).max if np.issubdtype(dt, np.integer) else np.inf
+    D = np.asfortranarray(np.where(np.isinf(ref), top, ref).astype(dt))
+    assert oracle.sssp_certificate(A, srcs, D).tolist() == [0, 0]
+    assert oracle.sssp_certificate(A, srcs, oracle.shortest_path_multi(A, srcs)).tolist() == [0, 0]
+    fin = np.nonzero(np.isfinite(ref[:, 0]))[0]
+    fin = fin[fin != 0]
+    unr = np.nonzero(np.isinf(ref[:, 0]))[0]
+    v = int(fin[len(fin) // 2])
+    for change in ("plus", "minus", "source", "drop", "invent"):
+        E = D.copy(order="F")
+        if change == "plus":
+            E[v, 0] += 1
+        elif change == "minus":
+            E[v, 0] -= 1
+        elif change == "source":
+            E[0, 0] = 1
+        elif change == "drop":
+            E[v, 0] = top
+        else:
+            if len(unr) == 0:
+                continue
+            E[unr[0], 0] = 5
+        assert oracle.sssp_certificate(A, srcs, E)[0] > 0, change
