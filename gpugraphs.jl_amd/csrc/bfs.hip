@@ -21,6 +21,8 @@
 // direction-optimising rule of Beamer et al. (SC'12): go bottom-up when the
 // frontier's out-edges exceed the unexplored edges / alpha, return top-down
 // when the frontier shrinks below n / beta.
+#include <mutex>
+
 #include "frontier.hpp"
 
 namespace gg {
@@ -509,9 +511,9 @@ static const int32_t* vinv_of(const gg_matrix* A) {
   return (A->format == GG_SELL && (A->flags & GG_FLAG_INTERNAL_IDS)) ? A->vinv : nullptr;
 }
 
-static int bfs_begin(gg_ctx* ctx, const gg_matrix* A, int64_t source, int32_t* out,
-                     int32_t src_val, BfsWork& w, bool queues) {
-  GG_TRY(bfs_work(ctx, state_size(A), queues, w));
+// Clears the bitmaps (fill kernels, never a memset node) and seeds the source.
+static int bfs_seed(gg_ctx* ctx, const gg_matrix* A, int64_t source, int32_t* out,
+                    int32_t src_val, BfsWork& w, bool queues) {
   hipLaunchKernelGGL(fill_i32v_kernel, dim3(blocks_for(A->m, kB)), dim3(kB), 0, ctx->stream, out,
                      A->m, (int32_t)2147483647);
   fill_words(ctx, w.visited, w.words, 0u);
@@ -522,12 +524,36 @@ static int bfs_begin(gg_ctx* ctx, const gg_matrix* A, int64_t source, int32_t* o
   return GG_OK;
 }
 
+static int bfs_begin(gg_ctx* ctx, const gg_matrix* A, int64_t source, int32_t* out,
+                     int32_t src_val, BfsWork& w, bool queues) {
+  GG_TRY(bfs_work(ctx, state_size(A), queues, w));
+  return bfs_seed(ctx, A, source, out, src_val, w, queues);
+}
+
 static int check_square_source(const gg_matrix* A, int64_t source) {
   GG_TRY(check_matrix(A));
   if (A->format == GG_CSC) return set_error(GG_ERR_ARGUMENT, "BFS runs on CSR or SELL A_T");
   if (A->m != A->n) return set_error(GG_ERR_DIMENSION, "A_T must be square");
   if (source < 1 || source > A->m) return set_error(GG_ERR_BOUNDS, "source %lld outside 1:%lld",
                                                     (long long)source, (long long)A->m);
+  return GG_OK;
+}
+
+// Host-stepped pull levels (one round trip per level) until one finds nothing.
+static int pull_levels_host(gg_ctx* ctx, const gg_matrix* A, int32_t* dist, BfsWork& w,
+                            int64_t* nlevels) {
+  BfsCounters hc{};
+  int32_t level = 0;
+  while (true) {
+    ++level;
+    fill_words(ctx, w.next, w.words, 0u);
+    hipLaunchKernelGGL(level_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt, level, dist);
+    GG_TRY(pull_step(ctx, ctx->stream, A, false, w.front, w.next, w.visited, w.cnt));
+    GG_TRY(read_counters(ctx, w, &hc));
+    if (hc.found == 0) break;
+    std::swap(w.front, w.next);
+  }
+  if (nlevels) *nlevels = level - 1;
   return GG_OK;
 }
 
@@ -561,17 +587,34 @@ int bfs_levels_pull(gg_ctx* ctx, const gg_matrix* A, int64_t source, int32_t* di
     if (nlevels) *nlevels = hc.level - 1;
     return GG_OK;
   }
-  int32_t level = 0;
-  while (true) {
-    ++level;
-    fill_words(ctx, w.next, w.words, 0u);
-    hipLaunchKernelGGL(level_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt, level, dist);
-    GG_TRY(pull_step(ctx, ctx->stream, A, false, w.front, w.next, w.visited, w.cnt));
-    GG_TRY(read_counters(ctx, w, &hc));
-    if (hc.found == 0) break;
-    std::swap(w.front, w.next);
+  return pull_levels_host(ctx, A, dist, w, nlevels);
+}
+
+// ------------------------------------------------- allocation-free variant
+// Traversal state of gg_bfs_distances_noalloc: the visited / frontier / next
+// bitmaps of up to kScratchVertices state rows and the level counters, in one
+// device array that lives as long as the process (each GPU holds its own
+// copy).  Every call on a GPU shares that GPU's copy.
+constexpr int64_t kScratchVertices = int64_t(1) << 26;
+constexpr int64_t kScratchWords = kScratchVertices / 32;
+static __device__ __align__(256) uint32_t scratch[3 * kScratchWords + 64];
+static_assert(sizeof(BfsCounters) <= 64 * sizeof(uint32_t), "counters fit the scratch tail");
+
+// Device address of `scratch` per device, resolved once per device with the
+// device current (the host-side name of the array is not a device address).
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+static int scratch_address(int device, void** out) {
+  if (device < 0 || device >= 64) return set_error(GG_ERR_ARGUMENT, "device %d out of range", device);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    GG_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    GG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = p;
   }
-  if (nlevels) *nlevels = level - 1;
+  *out = g_scratch_ptr[device];
   return GG_OK;
 }
 
@@ -589,6 +632,34 @@ int gg_bfs_distances(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, int32_t*
   GG_TRY(check_square_source(A_T, source));
   timer_begin(ctx);
   int st = bfs_levels_pull(ctx, A_T, source, dist, nullptr);
+  timer_end(ctx);
+  return st;
+}
+
+int gg_bfs_distances_noalloc(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, int32_t* dist,
+                             int use_mask) {
+  (void)use_mask;
+  if (!ctx || !dist) return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  clear_error();
+  GG_TRY(check_square_source(A_T, source));
+  const int64_t ns = state_size(A_T);
+  if (ns > kScratchVertices)
+    return set_error(GG_ERR_ARGUMENT,
+                     "gg_bfs_distances_noalloc handles at most %lld vertices (state rows); "
+                     "this matrix needs %lld: use gg_bfs_distances",
+                     (long long)kScratchVertices, (long long)ns);
+  void* base;
+  GG_TRY(scratch_address(ctx->device, &base));
+  BfsWork w;
+  w.m = ns;
+  w.words = (ns + 31) / 32;
+  w.visited = (uint32_t*)base;
+  w.front = w.visited + kScratchWords;
+  w.next = w.front + kScratchWords;
+  w.cnt = (BfsCounters*)(w.next + kScratchWords);
+  timer_begin(ctx);
+  GG_TRY(bfs_seed(ctx, A_T, source, dist, 0, w, false));
+  const int st = pull_levels_host(ctx, A_T, dist, w, nullptr);
   timer_end(ctx);
   return st;
 }

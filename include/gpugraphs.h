@@ -378,6 +378,19 @@ int gg_spmspv(gg_ctx* ctx, int32_t* c_nzind, void* c_nzval, int64_t* c_nnz, int 
  * unreached = typemax(Int32); source is 1-based.  src/algorithms/bfs.jl:1-111. */
 int gg_bfs_distances(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, int32_t* dist,
                      int use_mask);
+/* gg_bfs_distances without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as gg_bfs_distances; the traversal state (3
+ * bitmaps and the level counters, about 24 MiB) lives in a fixed device array
+ * of the library, one per GPU, held until the process ends, so the state
+ * rows (A_T->m, or the relabelled SELL's internal ids) must not exceed 2^26
+ * (ArgumentError otherwise).  Levels are host-stepped (no captured graphs).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (two host threads, or two streams) race and must be serialised by the
+ * caller. */
+int gg_bfs_distances_noalloc(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, int32_t* dist,
+                             int use_mask);
 /* BFS parents: parents[source] = 0, unreached = typemax(Int32), otherwise the
  * 1-based frontier in-neighbour chosen by GPUGraphs_any (CSR: first in row
  * order; SELL: largest).  src/algorithms/bfs.jl:113-183. */
