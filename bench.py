@@ -391,6 +391,147 @@ def copy_calibration(torch, dev, nbytes=1 << 30, reps=10):
                       "gbs_read_plus_write": round(2 * nbytes / ms / 1e6, 1)}), flush=True)
 
 
+def cpu_leg(np, G, Rg, roots, seed, gpu_levels):
+    """The oracle (the reference's CPU loops restated in C++/OpenMP) on this
+    host, on bounded samples of the benchmark workloads: one BFS (the pull
+    loop of bfs.jl) and one fp32 SpMV on R-MAT 22 with all the job's threads
+    and with one; the CSR SpMV of the n = 65,536 uniform-rows matrix x100
+    (all threads and one); one Bellman-Ford (Jacobi loop of
+    shortest_path.jl) from one source of the 4096^2 grid; one iteration of
+    the k = 32 multi-source Bellman-Ford on weighted R-MAT 22, scaled by the
+    iteration count the GPU run needs.  The GPU's BFS levels from the first
+    root are checked against the oracle's here (full-size parity)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes as C
+    import oracle as orc
+    T, nproc = cpu_threads()
+    n, nnz = Rg.m, Rg.nnz
+    out = dict(threads=T, nproc=nproc)
+    Ao = orc.csr(n, n, Rg.rowptr, Rg.colval, np.ones(nnz, np.uint8))
+    root = int(roots[0])
+    for th, key in ((T, "bfs_s"), (1, "bfs_1t_s")):
+        t0 = time.perf_counter()
+        dref = orc.bfs_distances(Ao, root, nthreads=th)
+        out[key] = time.perf_counter() - t0
+    if gpu_levels is not None and gpu_levels[0] == root:
+        assert np.array_equal(dref, gpu_levels[1]), "GPU BFS levels differ from the oracle at full size"
+    Af = orc.csr(n, n, Rg.rowptr, Rg.colval, Rg.nzval)
+    xh = np.random.default_rng(1).random(n, dtype=np.float32)
+    yh = np.zeros(n, np.float32)
+    B = nnz * 8 + 12 * n
+    for th, key in ((T, "spmv_s"), (1, "spmv_1t_s")):
+        t0 = time.perf_counter()
+        orc.spmv(yh, Af, xh, nthreads=th)
+        out[key] = time.perf_counter() - t0
+    out["spmv_gbs"] = B / out["spmv_s"] / 1e9
+    U = G.uniform_rows(65536, 16, seed=seed, dtype=np.float32)
+    Uo = orc.csr(U.m, U.n, U.rowptr, U.colval, U.nzval)
+    xu = np.random.default_rng(2).random(U.n, dtype=np.float32)
+    yu = np.zeros(U.m, np.float32)
+    for th, key in ((T, "csr65536_x100_s"), (1, "csr65536_x100_1t_s")):
+        t0 = time.perf_counter()
+        for _ in range(100):
+            orc.spmv(yu, Uo, xu, nthreads=th)
+        out[key] = time.perf_counter() - t0
+    Gd = G.grid(4096, 4096, 0.01, 1, 1000, seed=seed, dtype=np.float32)
+    Go = orc.csr(Gd.m, Gd.n, Gd.rowptr, Gd.colval, Gd.nzval)
+    src = int(np.random.default_rng(seed + 1).integers(Gd.m)) + 1
+    it = C.c_int64()
+    dist = np.empty(Gd.m, np.float32)
+    t0 = time.perf_counter()
+    st = orc.lib().orc_shortest_path(C.byref(Go.view()), src, dist.ctypes.data, orc.FLOAT32, T, 0,
+                                     C.byref(it))
+    out["grid_bf_s"] = time.perf_counter() - t0
+    out["grid_bf_iters"] = int(it.value)
+    assert st == 0
+    del Gd, Go, dist
+    w = np.random.default_rng(seed + 1).integers(1, 1001, nnz).astype(np.float32)
+    Wo = orc.csr(n, n, Rg.rowptr, Rg.colval, w)
+    srcs = np.ascontiguousarray(roots[:32], np.int32)
+    D = np.empty((n, 32), np.float32, order="F")
+    t0 = time.perf_counter()
+    st = orc.lib().orc_shortest_path_multi(C.byref(Wo.view()), srcs.ctypes.data, 32, D.ctypes.data,
+                                           orc.FLOAT32, T, 1, C.byref(it))
+    out["rmat_bf32_iter_s"] = time.perf_counter() - t0
+    assert st in (0, 8)   # 8: stopped at the one-iteration sample
+    del D, Wo, w
+    log(f"CPU oracle ({T} threads of {nproc}): BFS {out['bfs_s']:.2f} s (1 thread {out['bfs_1t_s']:.2f} s), "
+        f"SpMV {out['spmv_s'] * 1e3:.0f} ms (1 thread {out['spmv_1t_s'] * 1e3:.0f} ms), "
+        f"CSR n=65536 x100 {out['csr65536_x100_s'] * 1e3:.0f} ms (1 thread "
+        f"{out['csr65536_x100_1t_s'] * 1e3:.0f} ms), grid Bellman-Ford {out['grid_bf_s']:.1f} s "
+        f"({out['grid_bf_iters']} iterations), k=32 Bellman-Ford {out['rmat_bf32_iter_s']:.1f} s per iteration")
+    return out
+
+
+def hbm_ceiling(torch, dev, nbytes=4 << 30, reps=10):
+    """What this device's HBM delivers to a streaming kernel of this project
+    (libgg_calibrate.so: 16-byte non-temporal loads / stores on 8 blocks of
+    256 threads per CU): GB/s for a pure read, a pure write and a copy
+    (read + write bytes) of nbytes, each the mean of reps launches timed with
+    HIP events on the current stream."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "gpugraphs.jl_amd", "lib", "libgg_calibrate.so"))
+    lib.ggc_sink_bytes.restype = C.c_size_t
+    for f in (lib.ggc_stream_read, lib.ggc_stream_write, lib.ggc_stream_copy):
+        f.restype = C.c_int
+    buf = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(buf)
+    sink = torch.empty(lib.ggc_sink_bytes() // 4, dtype=torch.float32, device=dev)
+    st = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    ms = C.c_double()
+    out = {"bytes": nbytes}
+    assert lib.ggc_stream_read(st, C.c_void_p(buf.data_ptr()), C.c_size_t(nbytes),
+                               C.c_void_p(sink.data_ptr()), reps, C.byref(ms)) == 0
+    out["read_gbs"] = round(nbytes / ms.value / 1e6, 1)
+    assert lib.ggc_stream_write(st, C.c_void_p(dst.data_ptr()), C.c_size_t(nbytes), reps,
+                                C.byref(ms)) == 0
+    out["write_gbs"] = round(nbytes / ms.value / 1e6, 1)
+    assert lib.ggc_stream_copy(st, C.c_void_p(buf.data_ptr()), C.c_void_p(dst.data_ptr()),
+                               C.c_size_t(nbytes), reps, C.byref(ms)) == 0
+    out["copy_gbs_read_plus_write"] = round(2 * nbytes / ms.value / 1e6, 1)
+    del buf, dst, sink
+    log(f"HBM ceiling ({nbytes >> 20} MiB streams): read {out['read_gbs']:.0f} GB/s, write "
+        f"{out['write_gbs']:.0f} GB/s, copy {out['copy_gbs_read_plus_write']:.0f} GB/s (read + write)")
+    return out
+
+
+def gather_only_ms(torch, dev, A, reps):
+    """The SpMV of a SELL matrix without its arithmetic, values or result: its
+    column streams (slices, then heavy rows) read in storage order with the
+    x gathers they name (libgg_calibrate.so ggc_gather), in ms."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "gpugraphs.jl_amd", "lib", "libgg_calibrate.so"))
+    lib.ggc_sink_bytes.restype = C.c_size_t
+    lib.ggc_gather.restype = C.c_int
+    lib.ggc_gather.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int,
+                               C.POINTER(C.c_double)]
+    n_int = A.nslices * A.slice_size + A.n_heavy
+    xg = torch.rand(max(n_int, A.n), dtype=torch.float32, device=dev)
+    sink = torch.empty(lib.ggc_sink_bytes() // 4, dtype=torch.float32, device=dev)
+    st = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    total = 0.0
+    for cols in (A.colval, A.heavy_col if A.n_heavy else None):
+        if cols is None or cols.numel() == 0:
+            continue
+        ms = C.c_double()
+        assert lib.ggc_gather(st, cols.data_ptr(), cols.numel(), xg.data_ptr(), sink.data_ptr(),
+                              reps, C.byref(ms)) == 0
+        total += ms.value
+    return total
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: the job's CPU share (OMP_NUM_THREADS when the
+    environment sets it), else every core nproc reports."""
+    n = os.cpu_count() or 1
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return (int(env) if env.isdigit() and int(env) > 0 else n), n
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -434,8 +575,9 @@ def main():
     ap.add_argument("--serve-budget-gib", type=float, default=8.0,
                     help="HBM result-arena budget of the server (GiB)")
     ap.add_argument("--only", default="",
-                    help="one part: spmv | spmm | msbfs | bfs | sssp | packed | tuned | serve, or copy (a 1 GiB device copy that "
-                         "calibrates the HBM byte counters)")
+                    help="one part: spmv | spmm | msbfs | bfs | sssp | packed | tuned | serve | cpu, "
+                         "copy (a 1 GiB device copy that calibrates the HBM byte counters) or "
+                         "stream (the project's streaming read / write / copy kernels)")
     args = ap.parse_args()
     relabel = not args.no_relabel
     sigma = 1 if relabel else 4096
@@ -459,6 +601,10 @@ def main():
     if args.only == "copy":
         copy_calibration(torch, dev)
         return
+    if args.only == "stream":
+        print(json.dumps(hbm_ceiling(torch, dev, reps=args.spmv_reps)), flush=True)
+        return
+    ceiling = hbm_ceiling(torch, dev) if args.only in ("", "spmv") else None
     R0 = gg.ROCBackend(0)
     seed = 20261203
 
@@ -504,33 +650,54 @@ def main():
     if args.only in ("", "spmv"):
         x = torch.from_numpy(rng.random(n, dtype=np.float32)).to(dev)
         y = torch.zeros(n, dtype=torch.float32, device=dev)
-        for _ in range(3):
-            gg.gpu_spmv_(y, A_f, x)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(args.spmv_reps):
-            gg.gpu_spmv_(y, A_f, x)
-        e1.record()
-        torch.cuda.synchronize()
-        spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
-        def spmv_reps():
-            for _ in range(args.spmv_reps):
-                gg.gpu_spmv_(y, A_f, x)
-        kms, kn = kernel_pass(_lib, ctx, spmv_reps, args.graph_steps)
-        spmv_kernel_ms = kms / max(kn, 1)
-        B_spmv = nnz * 8 + 4 * (n + 1) + 4 * n + 4 * n  # algorithmic bytes per SpMV
         S64 = Rg.to_scipy().astype(np.float64)
         ref = S64 @ x.cpu().numpy().astype(np.float64)
-        got = y.cpu().numpy()
-        relerr = float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)))
-        assert relerr <= 1e-5, f"SpMV parity failed: max rel err {relerr}"
         del S64
+        B_spmv = nnz * 8 + 4 * (n + 1) + 4 * n + 4 * n  # algorithmic bytes per SpMV
+
+        def spmv_leg(A):
+            """(ms per call, ms per hot-kernel launch, max rel err vs FP64)."""
+            for _ in range(3):
+                gg.gpu_spmv_(y, A, x)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(args.spmv_reps):
+                gg.gpu_spmv_(y, A, x)
+            e1.record()
+            torch.cuda.synchronize()
+            call_ms = e0.elapsed_time(e1) / args.spmv_reps
+
+            def reps():
+                for _ in range(args.spmv_reps):
+                    gg.gpu_spmv_(y, A, x)
+            kms, kn = kernel_pass(_lib, ctx, reps, args.graph_steps)
+            err = float(np.max(np.abs(y.cpu().numpy() - ref) / np.maximum(np.abs(ref), 1e-30)))
+            assert err <= 1e-5, f"SpMV parity failed: max rel err {err}"
+            return call_ms, kms / max(kn, 1), err
+
+        spmv_ms, spmv_kernel_ms, relerr = spmv_leg(A_f)
         res["spmv"] = dict(ms=spmv_ms, kernel_ms=spmv_kernel_ms, bytes=B_spmv,
                            gbs=B_spmv / spmv_ms / 1e6, kernel_gbs=B_spmv / spmv_kernel_ms / 1e6,
                            max_rel_err_vs_fp64=relerr)
+        res["spmv"]["gather_only_ms"] = gather_only_ms(torch, dev, A_f, args.spmv_reps)
+        log(f"the same column streams with their x gathers alone (no values, no y): "
+            f"{res['spmv']['gather_only_ms'] * 1e3:.1f} us")
         log(f"SELL SpMV: {spmv_ms * 1e3:.1f} us/SpMV ({B_spmv / spmv_ms / 1e6:.0f} GB/s), kernel "
             f"{spmv_kernel_ms * 1e3:.1f} us ({B_spmv / spmv_kernel_ms / 1e6:.0f} GB/s), rel err {relerr:.1e}")
+        if relabel:
+            # the configuration's named layout: C = 64, sigma = 4096 windows (no
+            # relabelling), rows longer than 256 entries in the heavy partition
+            A_s = Rg.to_sell(R0, 64, 4096, 256, args.seg)
+            s_ms, s_kms, s_err = spmv_leg(A_s)
+            stored = A_s.n_stored + (int(A_s.heavy_ptr[-1].item()) if A_s.n_heavy else 0)
+            s_gather = gather_only_ms(torch, dev, A_s, args.spmv_reps)
+            del A_s
+            res["spmv_sigma4096"] = dict(ms=s_ms, kernel_ms=s_kms, gbs=B_spmv / s_ms / 1e6,
+                                         kernel_gbs=B_spmv / s_kms / 1e6, max_rel_err_vs_fp64=s_err,
+                                         stored_per_nnz=stored / nnz, gather_only_ms=s_gather)
+            log(f"SELL-64 sigma 4096 SpMV: {s_ms * 1e3:.1f} us/SpMV, kernel {s_kms * 1e3:.1f} us "
+                f"({B_spmv / s_kms / 1e6:.0f} GB/s), stored/nnz {stored / nnz:.3f}, rel err {s_err:.1e}")
 
     # ---------------------------------------------------------------- SpMM
     if args.only in ("", "spmm") and args.spmm_k > 0:
@@ -653,26 +820,11 @@ def main():
         log(f"plain BFS: {len(pr) / sum(lat_plain):.0f} q/s p99 {pct(lat_plain, 99) * 1e3:.2f} ms; "
             f"DO BFS: {len(pr) / sum(lat_do):.0f} q/s p99 {pct(lat_do, 99) * 1e3:.2f} ms")
 
-        # parity at full size: plain levels against the oracle from the first root
-        if not args.no_cpu:
-            sys.path.insert(0, os.path.join(ROOT, "oracle"))
-            import oracle as orc
-            cores = min(16, os.cpu_count() or 1)
-            Ao = orc.csr(n, n, Rg.rowptr, Rg.colval, np.ones(nnz, np.uint8))
-            t0 = time.perf_counter()
-            dref = orc.bfs_distances(Ao, int(pr[0]), nthreads=cores)
-            cpu_bfs_s = time.perf_counter() - t0
-            assert np.array_equal(dref, d0h), "GPU BFS levels differ from the oracle at full size"
-            Af = orc.csr(n, n, Rg.rowptr, Rg.colval, Rg.nzval)
-            xh = np.random.default_rng(1).random(n, dtype=np.float32)
-            yh = np.zeros(n, np.float32)
-            t0 = time.perf_counter()
-            orc.spmv(yh, Af, xh, nthreads=cores)
-            cpu_spmv_s = time.perf_counter() - t0
-            res["cpu"] = dict(bfs_s=cpu_bfs_s, spmv_s=cpu_spmv_s, cores=cores,
-                              qps=1.0 / cpu_bfs_s,
-                              spmv_gbs=(nnz * 8 + 12 * n) / cpu_spmv_s / 1e9)
-            log(f"CPU oracle ({cores} threads): BFS {cpu_bfs_s:.2f} s, SpMV {cpu_spmv_s * 1e3:.0f} ms")
+        res["bfs_root_levels"] = (int(pr[0]), d0h)
+
+    # ------------------------------------------------------------ CPU baseline
+    if args.only in ("", "cpu") and not args.no_cpu:
+        res["cpu"] = cpu_leg(np, G, Rg, roots, seed, res.pop("bfs_root_levels", None))
 
     # ------------------------------------------------------- packed graphs
     if args.only in ("", "packed") and args.packed_graphs > 0:
@@ -716,7 +868,51 @@ def main():
                          traffic=measured_traffic(args, relabel, "sell_spmv"),
                          avg_launch_us=round(spmv["kernel_ms"] * 1e3, 2),
                          spmv_call_gbs=round(spmv["gbs"], 1))
+        if "gather_only_ms" in spmv:
+            spmv_roof["gather_only_us"] = round(spmv["gather_only_ms"] * 1e3, 1)
+        if ceiling:
+            # against what the project's own streaming-read kernel reaches on this box
+            spmv_roof["measured_read_gbs"] = ceiling["read_gbs"]
+            spmv_roof["frac_of_measured_read"] = round(spmv["kernel_gbs"] / ceiling["read_gbs"], 4)
+        if "spmv_sigma4096" in res:
+            s4 = res["spmv_sigma4096"]
+            spmv_roof["sigma4096"] = dict(achieved=round(s4["kernel_gbs"], 1),
+                                          frac=round(s4["kernel_gbs"] / HBM_PEAK_GBS, 4),
+                                          avg_launch_us=round(s4["kernel_ms"] * 1e3, 2),
+                                          call_us=round(s4["ms"] * 1e3, 1),
+                                          stored_per_nnz=round(s4["stored_per_nnz"], 4),
+                                          gather_only_us=round(s4["gather_only_ms"] * 1e3, 1),
+                                          max_rel_err_vs_fp64=s4["max_rel_err_vs_fp64"])
+    if roofline and ceiling:
+        roofline["measured_read_gbs"] = ceiling["read_gbs"]
+        roofline["frac_of_measured_read"] = round(roofline["achieved"] / ceiling["read_gbs"], 4)
     cpu = res.get("cpu")
+    cpu_base = None
+    if cpu:
+        B_spmv = nnz * 8 + 12 * n
+        B_u = 65536 * 16 * 8 + 12 * 65536
+        bf_iters = (res.get("sssp") or {}).get("rmat", {}).get("batched_iters")
+        cpu_base = {
+            "value": round(1.0 / cpu["bfs_s"], 3), "unit": "queries/s", "cores": cpu["threads"],
+            "kind": "port",
+            "sample": (f"one BFS (the pull loop of bfs.jl, levels) from the first root of R-MAT 22 on "
+                       f"{cpu['threads']} threads (nproc {cpu['nproc']}); beside it one fp32 SpMV on "
+                       "R-MAT 22, 100 CSR SpMVs of the n=65,536 uniform-rows matrix, one grid "
+                       "Bellman-Ford source, and one iteration of the k=32 multi-source Bellman-Ford "
+                       "scaled by the GPU run's iteration count"),
+            "nproc": cpu["nproc"],
+            "bfs_1thread_qps": round(1.0 / cpu["bfs_1t_s"], 3),
+            "spmv_gbs": round(B_spmv / cpu["spmv_s"] / 1e9, 2),
+            "spmv_1thread_gbs": round(B_spmv / cpu["spmv_1t_s"] / 1e9, 2),
+            "csr65536_spmv_us": round(cpu["csr65536_x100_s"] * 1e4, 1),
+            "csr65536_gbs": round(B_u * 100 / cpu["csr65536_x100_s"] / 1e9, 2),
+            "csr65536_1thread_gbs": round(B_u * 100 / cpu["csr65536_x100_1t_s"] / 1e9, 2),
+            "grid4096_bf_qps": round(1.0 / cpu["grid_bf_s"], 4),
+            "grid4096_bf_iters": cpu["grid_bf_iters"],
+            "rmat22_bf32_iter_s": round(cpu["rmat_bf32_iter_s"], 3),
+            "rmat22_bf32_qps_est": (round(32 / (cpu["rmat_bf32_iter_s"] * bf_iters), 4)
+                                    if bf_iters else None),
+        }
     out = {
         "metric": "BFS/SSSP queries/s + p99 latency (batched vs plain); SpMV GB/s vs HBM peak",
         "value": None if value is None else round(value, 1),
@@ -764,12 +960,8 @@ def main():
                    for k, v in res["packed"].items()} if "packed" in res else None,
         "tuned": res.get("tuned"),
         "serve": res.get("serve"),
-        "cpu_baseline": None if not cpu else {
-            "value": round(cpu["qps"], 3), "unit": "queries/s", "cores": cpu["cores"],
-            "kind": "port",
-            "sample": "one BFS (reference pull loop, levels) from the first root on the full graph; "
-                      f"one fp32 SpMV took {cpu['spmv_s'] * 1e3:.0f} ms "
-                      f"({cpu['spmv_gbs']:.1f} GB/s)"},
+        "cpu_baseline": cpu_base,
+        "hbm_ceiling": ceiling,
     }
     print(json.dumps(out), flush=True)
 

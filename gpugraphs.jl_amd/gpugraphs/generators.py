@@ -162,4 +162,7 @@ def pack(graphs):
     offsets = np.zeros(len(hs) + 1, np.int64)
     out = C.POINTER(GGCsrHost)()
     check(lib().gg_pack_graphs(len(hs), arr, C.byref(out), offsets.ctypes.data), "pack")
-    return _take(out), offsets
+    P = _take(out)
+    # a block-diagonal pack of symmetric graphs is symmetric (top-down steps)
+    P.symmetric = all(getattr(g, "symmetric", False) for g in graphs)
+    return P, offsets

@@ -915,3 +915,44 @@ def test_bfs_packed_rejects_unpadded_last_graph(env):
     with pytest.raises(gg.ArgumentError):
         gg.bfs_packed(A, np.array([0, 300], np.int64), np.array([int(rng.integers(300)) + 1],
                                                                 np.int32))
+
+
+def test_bfs_packed_symmetric_vs_oracle(env):
+    """Symmetric unit-pattern packs (Barabasi-Albert, Dorogovtsev-Mendes,
+    R-MAT and grid blocks, one source a hub): every graph's levels equal the
+    oracle's pull loop (src/algorithms/bfs.jl:31-73) from its source."""
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    rng = np.random.default_rng(91)
+    hosts, mats = [], []
+    for i in range(48):
+        kind = i % 4
+        if kind == 0:
+            n = int(rng.integers(10, 900))
+            M = G.undirected_adj(n, G.barabasi_albert(n, 3, int(rng.integers(1 << 30))), np.bool_)
+        elif kind == 1:
+            n = int(rng.integers(2, 400))
+            M = G.undirected_adj(n, G.dorogovtsev_mendes(n, int(rng.integers(1 << 30))), np.bool_)
+        elif kind == 2:
+            M = GEN.rmat(int(rng.integers(4, 13)), seed=int(rng.integers(1 << 30))).to_scipy()
+        else:
+            M = GEN.grid(int(rng.integers(2, 40)), int(rng.integers(2, 40)), 0.0, 1, 1,
+                         seed=int(rng.integers(1 << 30)), dtype=np.int32).to_scipy()
+        M = sp.csr_matrix(M != 0).astype(np.bool_)
+        H = _host_csr(M, np.bool_)
+        H.symmetric = True
+        hosts.append(H)
+        mats.append(M)
+    P, off = GEN.pack(hosts)
+    assert P.symmetric
+    A = P.to_csr(gg.ROCBackend(0))
+    srcs = np.array([int(rng.integers(M.shape[0])) + 1 for M in mats], np.int32)
+    srcs[0] = int(np.argmax(np.diff(mats[0].indptr))) + 1       # a hub
+    print_case("bfs packed symmetric", P.m, P.nnz)
+    D, levels = gg.bfs_packed(A, off, srcs)
+    D = host(D)
+    for g, M in enumerate(mats):
+        ref = orc.bfs_distances(orc.from_scipy_csr(sp.csr_matrix(M)), int(srcs[g]))
+        np.testing.assert_array_equal(D[off[g]:off[g] + M.shape[0]], ref, err_msg=f"graph {g}")
+        assert np.all(D[off[g] + M.shape[0]:off[g + 1]] == INT32_MAX)
+        assert levels[g] == ref[ref != INT32_MAX].max()
