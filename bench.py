@@ -314,6 +314,7 @@ def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
     runs = []
     for rate in [float(r) for r in str(args.serve_rate).split(",") if r]:
         srv = S.GraphServer(budget_bytes=budget)
+        srv.set_pacing(args.serve_pacing)
         table, route = [], []
         for name, Mb, Ms in specs:
             gb = srv.add_graph(Mb)
@@ -358,7 +359,8 @@ def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
                                                    round(pct(kl, 99) * 1e3, 1),
                                                    round(float(np.mean([c.complete_s - c.admit_s
                                                                         for c in cs])) * 1e3, 1)]
-        run = dict(rate_qps=rate, arrival_s=args.serve_seconds, submitted=len(submitted),
+        run = dict(rate_qps=rate, pacing=args.serve_pacing, arrival_s=args.serve_seconds,
+                   submitted=len(submitted),
                    completed=len(ok), failed=len(done) - len(ok), wall_s=round(wall, 3),
                    qps=round(len(ok) / wall, 1),
                    p50_ms=round(pct(lat, 50) * 1e3, 2) if lat else None,
@@ -569,10 +571,12 @@ def main():
                     help="graphs of the pack also run one BFS call at a time (plain baseline)")
     ap.add_argument("--tuned-scale", type=int, default=20,
                     help="R-MAT scale of the autotuned batched leg (0: skip)")
-    ap.add_argument("--serve-rate", type=str, default="2000,1000,250",
+    ap.add_argument("--serve-rate", type=str, default="2000,5000,10000",
                     help="Poisson arrivals per second, one serving run per comma-separated rate")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the serving leg")
-    ap.add_argument("--serve-budget-gib", type=float, default=8.0,
+    ap.add_argument("--serve-pacing", type=float, default=24.0,
+                    help="server step pacing factor (gg_server_set_pacing; 0: step whenever a slot is busy)")
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0,
                     help="HBM result-arena budget of the server (GiB)")
     ap.add_argument("--only", default="",
                     help="one part: spmv | spmm | msbfs | bfs | sssp | packed | tuned | serve | cpu, "

@@ -10,7 +10,7 @@
 //    source bit starts at the current level and its distances are counted from
 //    there, so queries admitted at different times share every level kernel.
 //    A slot is finished when its bit is absent from the whole next frontier.
-//  - SSSP: 32 columns of a row-major min-plus working set (the in-place step
+//  - SSSP: 64 columns of a row-major min-plus working set (the in-place step
 //    of gg_shortest_path_multi).  A query resets its column between two steps
 //    and marks its source as changed; it is finished after a step that
 //    lowered nothing in its column.  A step relaxes the columns up to the
@@ -94,23 +94,39 @@ __global__ void serve_zero_kernel(unsigned int* found, unsigned long long* mask)
   *mask = 0ull;
 }
 
-// Working set rows are state indices (ns of them).
+// Working set rows are state indices (ns of them).  One thread per row
+// resets every seeded column of its row (a row's columns share its line, so
+// the line is written once whatever the number of seeds) and marks the
+// seeds' rows changed.
 template <class TD>
 __global__ void serve_sssp_seed_kernel(TD* d, int64_t ns, int k, SeedList sl,
                                        const int32_t* vinv, uint32_t* chp) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ns * sl.n) return;
-  const int64_t q = i / ns, r = i - q * ns;
-  const int64_t rs = vinv ? (int64_t)vinv[sl.src[q]] - 1 : sl.src[q];
-  d[r * k + sl.slot[q]] = r == rs ? TD(0) : Lim<TD>::max();
-  if (r == rs) atomicOr(&chp[r >> 5], 1u << (r & 31));
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ns) return;
+  TD* row = d + r * k;
+  for (int q = 0; q < sl.n; ++q) {
+    const int64_t rs = vinv ? (int64_t)vinv[sl.src[q]] - 1 : sl.src[q];
+    row[sl.slot[q]] = r == rs ? TD(0) : Lim<TD>::max();
+    if (r == rs) atomicOr(&chp[r >> 5], 1u << (r & 31));
+  }
 }
 
+// Columns of the working set that retire together, with their result blocks.
+struct ExtractList {
+  int n;
+  int32_t slot[64];
+  void* dst[64];
+};
+
+// One thread per original vertex copies its row's retiring columns into
+// their result blocks (the row's line is read once for all of them).
 template <class TD>
-__global__ void serve_extract_kernel(const TD* d, int64_t m, int k, int slot,
-                                     const int32_t* vinv, TD* out) {
+__global__ void serve_extract_kernel(const TD* d, int64_t m, int k, ExtractList xl,
+                                     const int32_t* vinv) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < m) out[v] = d[(vinv ? (int64_t)vinv[v] - 1 : v) * k + slot];
+  if (v >= m) return;
+  const TD* row = d + (vinv ? (int64_t)vinv[v] - 1 : v) * k;
+  for (int j = 0; j < xl.n; ++j) ((TD*)xl.dst[j])[v] = row[xl.slot[j]];
 }
 
 template <class TD> __global__ void serve_fill_kernel(TD* p, int64_t n, TD v) {
@@ -172,6 +188,7 @@ enum QState { Q_PENDING = 0, Q_RUNNING = 1, Q_DONE = 2, Q_FAILED = 3 };
 struct Query {
   int64_t id;
   int graph, kind;
+  int engine = 0;  // index into gg_server::engines
   int64_t source;
   double t_arrive = 0, t_admit = 0, t_done = 0;
   int state = Q_PENDING;
@@ -219,12 +236,14 @@ struct Engine {
   // SSSP
   int dist_dtype = GG_INT32;
   size_t esz = 4;
-  void* buf = nullptr;                      // ns x 32 row-major distances
+  void* buf = nullptr;                      // ns x slots row-major distances
   uint32_t *chp = nullptr, *chn = nullptr;  // rows that fell in the last / this step
   bool sparse = false;  // sparse steps (symmetric relabelled SELL): only rows next to fallen rows
   BfSparse sp{};
   std::vector<void*> allocs;
   int64_t steps = 0;
+  // pacing: when the step was launched and a running mean of launch-to-done
+  double t_launch = 0.0, step_s = 0.0;
 };
 
 }  // namespace gg
@@ -247,6 +266,7 @@ struct gg_server {
   hipStream_t small_stream = nullptr;  // shared by the engines of small graphs
   int num_cus = 256;
   std::string worker_error;
+  double pacing = 24.0;  // gg_server_set_pacing
 };
 
 namespace gg {
@@ -266,7 +286,8 @@ static int make_engine(gg_server* s, const gg_matrix* A, int kind, std::unique_p
   e->m = A->m;
   e->ns = state_size(A);
   e->vinv = (A->format == GG_SELL && (A->flags & GG_FLAG_INTERNAL_IDS)) ? A->vinv : nullptr;
-  e->slots = kind == GG_QUERY_BFS ? 64 : 32;
+  // 64 bitmask slots per BFS engine, 64 columns per Bellman-Ford engine
+  e->slots = 64;
   e->slot_q.assign(e->slots, -1);
   e->slot_steps.assign(e->slots, 0);
   const int64_t m = A->m, ns = e->ns;
@@ -315,12 +336,12 @@ static int make_engine(gg_server* s, const gg_matrix* A, int kind, std::unique_p
     if (e->dist_dtype != GG_INT32 && e->dist_dtype != GG_FLOAT32 && e->dist_dtype != GG_INT64 &&
         e->dist_dtype != GG_FLOAT64)
       return set_error(GG_ERR_ARGUMENT, "unsupported SSSP value type");
-    GG_TRY(engine_alloc(s, *e, e->esz * ns * 32, &p));
+    GG_TRY(engine_alloc(s, *e, e->esz * ns * e->slots, &p));
     e->buf = p;
     GG_TRY(with_type(e->dist_dtype, [&](auto t) -> int {
       typedef typename decltype(t)::type TD;
-      hipLaunchKernelGGL(serve_fill_kernel<TD>, dim3(blocks_for(ns * 32, 256)), dim3(256), 0,
-                         e->stream, (TD*)p, ns * 32, Lim<TD>::max());
+      hipLaunchKernelGGL(serve_fill_kernel<TD>, dim3(blocks_for(ns * e->slots, 256)), dim3(256), 0,
+                         e->stream, (TD*)p, ns * e->slots, Lim<TD>::max());
       return GG_OK;
     }));
     const int64_t words = (ns + 31) / 32;
@@ -388,6 +409,7 @@ static int retire(gg_server* s, Engine& e) {
   const uint64_t done = e.stepped & ~mask;
   if (!done) return GG_OK;
   const double t = now_s();
+  ExtractList xl{};
   for (int sl = 0; sl < e.slots; ++sl) {
     if (!((done >> sl) & 1ull)) continue;
     Query& q = s->queries[e.slot_q[sl]];
@@ -396,13 +418,23 @@ static int retire(gg_server* s, Engine& e) {
       GG_HIP(hipMemcpyAsync(dst, e.dist + (int64_t)sl * e.m, 4 * e.m, hipMemcpyDeviceToDevice,
                             e.stream));
     } else {
-      GG_TRY(with_type(e.dist_dtype, [&](auto tt) -> int {
-        typedef typename decltype(tt)::type TD;
-        hipLaunchKernelGGL(serve_extract_kernel<TD>, dim3(blocks_for(e.m, 256)), dim3(256), 0,
-                           e.stream, (const TD*)e.buf, e.m, 32, sl, e.vinv, (TD*)dst);
-        return GG_OK;
-      }));
+      xl.slot[xl.n] = sl;
+      xl.dst[xl.n] = dst;
+      ++xl.n;
     }
+  }
+  if (xl.n) {
+    GG_TRY(with_type(e.dist_dtype, [&](auto tt) -> int {
+      typedef typename decltype(tt)::type TD;
+      hipLaunchKernelGGL(serve_extract_kernel<TD>, dim3(blocks_for(e.m, 256)), dim3(256), 0,
+                         e.stream, (const TD*)e.buf, e.m, e.slots, xl, e.vinv);
+      return GG_OK;
+    }));
+  }
+  // an event per query, recorded after the copies, marks its result ready
+  for (int sl = 0; sl < e.slots; ++sl) {
+    if (!((done >> sl) & 1ull)) continue;
+    Query& q = s->queries[e.slot_q[sl]];
     q.ready = take_event(s);
     GG_HIP(hipEventRecord(q.ready, e.stream));
     q.t_done = t;
@@ -504,8 +536,8 @@ static int launch_step(gg_server* s, Engine& e) {
     if (sl.n) {
       GG_TRY(with_type(e.dist_dtype, [&](auto tt) -> int {
         typedef typename decltype(tt)::type TD;
-        hipLaunchKernelGGL(serve_sssp_seed_kernel<TD>, dim3(blocks_for(ns * sl.n, 256)), dim3(256),
-                           0, e.stream, (TD*)e.buf, ns, 32, sl, e.vinv, e.chp);
+        hipLaunchKernelGGL(serve_sssp_seed_kernel<TD>, dim3(blocks_for(ns, 256)), dim3(256),
+                           0, e.stream, (TD*)e.buf, ns, e.slots, sl, e.vinv, e.chp);
         return GG_OK;
       }));
     }
@@ -515,9 +547,9 @@ static int launch_step(gg_server* s, Engine& e) {
     // only the columns up to the highest active slot (slots are taken lowest
     // first), in groups of 4 so that the 16-byte column kernels apply
     const int hi = 64 - __builtin_clzll(e.active | 1ull);
-    const int64_t kc = std::min<int64_t>(32, (hi + 3) & ~3);
+    const int64_t kc = std::min<int64_t>(e.slots, (hi + 3) & ~3);
     GG_TRY(bf_multi_step_sp(e.stream, &e.A, e.dist_dtype, kc, e.buf, e.chp, e.chn, e.cnt_d,
-                            e.mask_d, nullptr, e.sparse ? &e.sp : nullptr, s->num_cus, 32));
+                            e.mask_d, nullptr, e.sparse ? &e.sp : nullptr, s->num_cus, e.slots));
     std::swap(e.chp, e.chn);
   }
   GG_HIP(hipGetLastError());
@@ -525,10 +557,26 @@ static int launch_step(gg_server* s, Engine& e) {
   GG_HIP(hipEventRecord(e.step_done, e.stream));
   e.stepped = e.active;
   e.inflight = true;
+  e.t_launch = now_s();
   e.steps += 1;
   for (int i = 0; i < e.slots; ++i)
     if ((e.active >> i) & 1ull) e.slot_steps[i] += 1;
   return GG_OK;
+}
+
+// Step pacing.  A step costs about the same whatever the number of its busy
+// slots (it reads the whole graph, or the whole frontier of every slot), so
+// an engine that steps whenever one slot is busy spends a full step on one
+// query.  While other engines have steps in flight, an engine whose slots are
+// not full waits pacing x (its mean step time) x (its free fraction) after its
+// last launch, so that arrivals join its next step instead of starting their
+// own; a full engine, or one alone on the device, steps at once.
+static bool paced(gg_server* s, const Engine& e, int inflight) {
+  if (s->pacing <= 0.0 || inflight == 0) return false;
+  const int busy = __builtin_popcountll(e.active);
+  if (busy >= e.slots) return false;
+  const double wait = s->pacing * e.step_s * (1.0 - (double)busy / e.slots);
+  return now_s() - e.t_launch < wait;
 }
 
 static void worker_main(gg_server* s) {
@@ -540,7 +588,7 @@ static void worker_main(gg_server* s) {
       const int64_t id = s->inbox.front();
       s->inbox.pop_front();
       Query& q = s->queries[id];
-      Engine& e = *s->engines[2 * q.graph + (q.kind == GG_QUERY_BFS ? 0 : 1)];
+      Engine& e = *s->engines[q.engine];
       if (q.bytes > s->arena.total()) {
         q.state = Q_FAILED;
         q.status = GG_ERR_BUDGET;
@@ -552,6 +600,8 @@ static void worker_main(gg_server* s) {
       e.waiting.push_back(id);
     }
     bool busy = false, progress = false;
+    int inflight = 0;
+    for (auto& ep : s->engines) inflight += ep->inflight ? 1 : 0;
     for (auto& ep : s->engines) {
       Engine& e = *ep;
       if (e.inflight) {
@@ -560,13 +610,17 @@ static void worker_main(gg_server* s) {
           continue;
         }
         e.inflight = false;
+        {
+          const double d = now_s() - e.t_launch;
+          e.step_s = e.step_s > 0.0 ? 0.75 * e.step_s + 0.25 * d : d;
+        }
         int st = retire(s, e);
         if (st != GG_OK) s->worker_error = gg_last_error();
         cap_sssp_slots(s, e);
         progress = true;
       }
       admit(s, e);
-      if (e.active) {
+      if (e.active && !paced(s, e, inflight)) {
         int st = launch_step(s, e);
         if (st != GG_OK) {
           // the step did not run: every query of the engine's slots fails
@@ -580,7 +634,10 @@ static void worker_main(gg_server* s) {
         } else {
           busy = true;
           progress = true;
+          ++inflight;
         }
+      } else if (e.active) {
+        busy = true;  // paced: the engine steps again shortly
       }
     }
     // on stop: finish the steps in flight; queries still waiting stay pending
@@ -646,6 +703,14 @@ int gg_server_add_graph(gg_server* s, const gg_matrix* A_T, int* graph_id) {
   return GG_OK;
 }
 
+int gg_server_set_pacing(gg_server* s, double factor) {
+  if (!s) return set_error(GG_ERR_ARGUMENT, "NULL server");
+  if (!(factor >= 0.0)) return set_error(GG_ERR_ARGUMENT, "pacing factor must be >= 0");
+  std::lock_guard<std::mutex> g(s->mu);
+  s->pacing = factor;
+  return GG_OK;
+}
+
 int gg_server_start(gg_server* s) {
   if (!s) return set_error(GG_ERR_ARGUMENT, "NULL server");
   if (s->running) return GG_OK;
@@ -661,7 +726,15 @@ int gg_server_submit(gg_server* s, int graph_id, int kind, int64_t source, int64
     return set_error(GG_ERR_ARGUMENT, "unknown graph %d", graph_id);
   if (kind != GG_QUERY_BFS && kind != GG_QUERY_SSSP)
     return set_error(GG_ERR_ARGUMENT, "unknown query kind %d", kind);
-  Engine& e = *s->engines[2 * graph_id + (kind == GG_QUERY_BFS ? 0 : 1)];
+  // On a Bool unit pattern every edge weighs one (safe_add(true, d) = d + 1),
+  // so the Bellman-Ford distances are the BFS levels (Int32, typemax when
+  // unreached, src/algorithms/shortest_path.jl:18): such an SSSP query runs in
+  // the graph's BFS engine, where it shares the 64-slot level steps.
+  Engine& es = *s->engines[2 * graph_id + 1];
+  const bool hops = kind == GG_QUERY_SSSP && es.A.val_dtype == GG_BOOL &&
+                    (es.A.flags & GG_FLAG_UNIT_PATTERN) != 0;
+  const int engine = 2 * graph_id + (kind == GG_QUERY_BFS || hops ? 0 : 1);
+  Engine& e = *s->engines[engine];
   if (kind == GG_QUERY_BFS && (e.A.val_dtype == GG_FLOAT32 || e.A.val_dtype == GG_FLOAT64) &&
       !(e.A.flags & GG_FLAG_UNIT_PATTERN))
     return set_error(GG_ERR_ARGUMENT, "or-and BFS needs Bool or integer matrix values");
@@ -673,9 +746,10 @@ int gg_server_submit(gg_server* s, int graph_id, int kind, int64_t source, int64
     q.id = s->next_id++;
     q.graph = graph_id;
     q.kind = kind;
+    q.engine = engine;
     q.source = source;
     q.t_arrive = now_s();
-    q.bytes = (size_t)e.m * (kind == GG_QUERY_BFS ? 4 : e.esz);
+    q.bytes = (size_t)e.m * (e.kind == GG_QUERY_BFS ? 4 : e.esz);
     s->queries.emplace(q.id, q);
     s->inbox.push_back(q.id);
     *request_id = q.id;

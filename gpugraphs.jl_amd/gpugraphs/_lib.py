@@ -180,6 +180,7 @@ SIGNATURES = {
     "gg_server_create": ([_i32, C.c_size_t, C.POINTER(_vp)], _i32),
     "gg_server_add_graph": ([_vp, _PM, C.POINTER(_i32)], _i32),
     "gg_server_start": ([_vp], _i32),
+    "gg_server_set_pacing": ([_vp, _d], _i32),
     "gg_server_submit": ([_vp, _i32, _i32, _i64, C.POINTER(_i64)], _i32),
     "gg_server_poll": ([_vp, C.POINTER(QueryInfo), _i64, C.POINTER(_i64), _d], _i32),
     "gg_server_fetch": ([_vp, _i64, _vp, C.c_size_t, _d], _i32),

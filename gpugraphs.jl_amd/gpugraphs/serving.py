@@ -61,6 +61,11 @@ class GraphServer:
         self._meta.append((A_T.m, {BFS: np.dtype(np.int32), SSSP: np.dtype(dist_dtype(A_T))}))
         return gid.value
 
+    def set_pacing(self, factor: float):
+        """Step pacing factor (gg_server_set_pacing; 0 steps whenever a slot
+        is busy)."""
+        check(self._L.gg_server_set_pacing(self._h, float(factor)), "set_pacing")
+
     def start(self):
         check(self._L.gg_server_start(self._h), "start")
         self._started = True

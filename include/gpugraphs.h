@@ -446,7 +446,7 @@ typedef struct gg_query_info {
   int64_t request_id;
   int32_t graph, kind;
   int64_t source;
-  int32_t status;      /* GG_OK or GG_ERR_BUDGET */
+  int32_t status;      /* GG_OK, GG_ERR_BUDGET, GG_ERR_NOT_CONVERGED or the launch error */
   double arrival_s;    /* host steady clock, seconds */
   double admit_s;
   double complete_s;
@@ -457,6 +457,12 @@ typedef struct gg_server_counters {
 } gg_server_counters;
 int gg_server_create(int device, size_t budget_bytes, gg_server** out);
 int gg_server_add_graph(gg_server* s, const gg_matrix* A_T, int* graph_id);
+/* Step pacing (no counterpart in the reference): while other engines have
+ * steps in flight, an engine whose slots are not all busy waits
+ * factor x (its mean step time) x (its free slot fraction) after its last
+ * step before the next one, so that arriving queries share steps; 0 steps
+ * whenever a slot is busy.  Default 8. */
+int gg_server_set_pacing(gg_server* s, double factor);
 int gg_server_start(gg_server* s);
 int gg_server_submit(gg_server* s, int graph_id, int kind, int64_t source, int64_t* request_id);
 /* Finished (or failed) queries not reported before; waits up to wait_s for one. */

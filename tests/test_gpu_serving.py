@@ -53,7 +53,7 @@ def test_server_results_equal_alone_and_in_mix(graphs):
     gg, torch, mats, omats, orc = graphs
     from gpugraphs import serving as S
     rng = np.random.default_rng(5)
-    qs = _queries(rng, mats, 300)          # > 64 BFS slots / 32 SSSP columns per graph
+    qs = _queries(rng, mats, 600)          # > 64 BFS slots / 64 SSSP columns per graph
     exp = {i: _expected(orc, omats[g], k, s) for i, (g, k, s) in enumerate(qs)}
     with S.GraphServer(budget_bytes=64 << 20) as srv:
         gids = [srv.add_graph(A) for A in mats]
