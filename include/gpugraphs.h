@@ -461,7 +461,7 @@ int gg_server_add_graph(gg_server* s, const gg_matrix* A_T, int* graph_id);
  * steps in flight, an engine whose slots are not all busy waits
  * factor x (its mean step time) x (its free slot fraction) after its last
  * step before the next one, so that arriving queries share steps; 0 steps
- * whenever a slot is busy.  Default 8. */
+ * whenever a slot is busy.  Default 24. */
 int gg_server_set_pacing(gg_server* s, double factor);
 int gg_server_start(gg_server* s);
 int gg_server_submit(gg_server* s, int graph_id, int kind, int64_t source, int64_t* request_id);
