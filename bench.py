@@ -210,12 +210,9 @@ def tuned_leg(args, np, torch, gg, G, R0, ctx, _lib, seed):
     nzr = np.nonzero(H.degrees())[0]
     roots = rng.choice(nzr, 256, replace=False).astype(np.int32) + 1
     tuner = AT.Autotuner(R0, reps=2)
-    cands = AT.default_candidates(Hb) + [dict(layout="sell", slice_size=64, sigma=1, relabel=True,
-                                               heavy=1024, seg=65536, sort=True)]
-    lay = tuner.tune_layout(Hb, "msbfs", roots=roots[:64], candidates=cands)
+    lay = tuner.tune_layout(Hb, "msbfs", roots=roots[:64])
     A = AT.build(Hb, R0, lay.best)
-    pa = tuner.tune_push_threshold(lambda: gg.multi_source_bfs(A, roots[:64]),
-                                   alphas=(0.0, 4.0, 16.0, 64.0))
+    pa = tuner.tune_push_threshold(A, alphas=(0.0, 4.0, 16.0, 64.0), roots=roots[:64])
     tune_s = time.time() - t
     out = dict(scale=sc, n=int(H.m), nnz=int(H.nnz), tune_s=round(tune_s, 1),
                layout=lay.best, layout_ms=round(lay.best_ms, 3), push_alpha=pa.best["alpha"],
@@ -497,18 +494,22 @@ def hbm_ceiling(torch, dev, nbytes=4 << 30, reps=10):
     return out
 
 
-def gather_only_ms(torch, dev, A, reps):
+def gather_only_ms(torch, dev, A, reps, wide=False):
     """The SpMV of a SELL matrix without its arithmetic, values or result: its
     column streams (slices, then heavy rows) read in storage order with the
-    x gathers they name (libgg_calibrate.so ggc_gather), in ms."""
+    x gathers they name (libgg_calibrate.so ggc_gather), in ms; wide: 8-byte
+    gathers, one dense pass of a multi-source BFS level over 64-source masks
+    (ggc_gather8)."""
     import ctypes as C
     lib = C.CDLL(os.path.join(ROOT, "gpugraphs.jl_amd", "lib", "libgg_calibrate.so"))
     lib.ggc_sink_bytes.restype = C.c_size_t
-    lib.ggc_gather.restype = C.c_int
-    lib.ggc_gather.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int,
-                               C.POINTER(C.c_double)]
+    fn = lib.ggc_gather8 if wide else lib.ggc_gather
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int,
+                   C.POINTER(C.c_double)]
     n_int = A.nslices * A.slice_size + A.n_heavy
-    xg = torch.rand(max(n_int, A.n), dtype=torch.float32, device=dev)
+    xg = (torch.randint(0, 1 << 62, (max(n_int, A.n),), dtype=torch.int64, device=dev) if wide
+          else torch.rand(max(n_int, A.n), dtype=torch.float32, device=dev))
     sink = torch.empty(lib.ggc_sink_bytes() // 4, dtype=torch.float32, device=dev)
     st = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
     total = 0.0
@@ -516,8 +517,8 @@ def gather_only_ms(torch, dev, A, reps):
         if cols is None or cols.numel() == 0:
             continue
         ms = C.c_double()
-        assert lib.ggc_gather(st, cols.data_ptr(), cols.numel(), xg.data_ptr(), sink.data_ptr(),
-                              reps, C.byref(ms)) == 0
+        assert fn(st, cols.data_ptr(), cols.numel(), xg.data_ptr(), sink.data_ptr(), reps,
+                  C.byref(ms)) == 0
         total += ms.value
     return total
 
@@ -763,7 +764,10 @@ def main():
         B_ms = 4 * nnz + 2 * n * (k // 8)  # per level: pattern once + source masks in and out
         res["msbfs"] = dict(step_ms=step_s * 1e3, qps=k / step_s, p50_ms=pct(lat, 50) * 1e3,
                             p99_ms=pct(lat, 99) * 1e3, levels=kn, kernel_ms=kms / max(kn, 1),
-                            bytes_per_launch=B_ms)
+                            bytes_per_launch=B_ms,
+                            gather8_only_ms=gather_only_ms(torch, dev, A_b, 10, wide=True))
+        log(f"one dense pass of the BFS layout's column streams with 8-byte gathers alone: "
+            f"{res['msbfs']['gather8_only_ms'] * 1e3:.1f} us")
         log(f"batched BFS k={k}: {step_s * 1e3:.2f} ms/step, {k / step_s:.0f} q/s, "
             f"{kn} levels, kernel avg {kms / max(kn, 1) * 1e3:.1f} us")
         if args.wide_roots > 0:
@@ -861,7 +865,8 @@ def main():
                         kernel="multi-source BFS level (msbfs_sell_kernel + direction switch + "
                                "top-down step + its share of the byte-level flush)",
                         bytes_per_launch=ms["bytes_per_launch"],
-                        avg_launch_us=round(kernel_ms * 1e3, 2))
+                        avg_launch_us=round(kernel_ms * 1e3, 2),
+                        dense_pass_gather8_only_us=round(ms["gather8_only_ms"] * 1e3, 1))
     else:
         value, roofline = None, None
     spmv_roof = None

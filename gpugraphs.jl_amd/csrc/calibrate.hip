@@ -8,8 +8,9 @@
 //    word per block so the loads are not dead;
 //  - ggc_stream_write: 16-byte non-temporal stores of a constant;
 //  - ggc_stream_copy: both;
-//  - ggc_gather: the gathers of a sparse product alone (an index stream and
-//    the x values it names), the part of an SpMV that does not stream.
+//  - ggc_gather / ggc_gather8: the gathers of a sparse product alone (an
+//    index stream and the 4- or 8-byte values it names), the part of an SpMV
+//    or of a multi-source BFS level that does not stream.
 // Each runs `reps` launches on the given stream between two events and
 // returns the mean milliseconds per launch.  The grid is 8 blocks of 256
 // threads per CU, enough waves to cover the memory latency of every CU.
@@ -19,6 +20,9 @@
 namespace {
 
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float fold(float a, float b) { return a + b; }
+__device__ __forceinline__ uint64_t fold(uint64_t a, uint64_t b) { return a | b; }
 
 __global__ __launch_bounds__(256) void read_kernel(const u4* __restrict__ src, int64_t n16,
                                                    u4* __restrict__ sink) {
@@ -66,33 +70,35 @@ __global__ __launch_bounds__(256) void copy_kernel(const u4* __restrict__ src, u
 // Gathers x[idx[i] - 1] for every positive 1-based index of idx, read in
 // storage order (lane i of a wavefront takes entry i of its run of 64, as the
 // SELL SpMV's lanes take entry k of 64 rows stored contiguously), 4 entries
-// in flight per lane; the sum of each block goes to sink so nothing is dead.
+// in flight per lane; the sum (or OR, for 8-byte masks) of each block goes to
+// sink so nothing is dead.
+template <class T>
 __global__ __launch_bounds__(256) void gather_kernel(const int32_t* __restrict__ idx, int64_t n,
-                                                     const float* __restrict__ x,
-                                                     float* __restrict__ sink) {
+                                                     const T* __restrict__ x,
+                                                     T* __restrict__ sink) {
   const int64_t stride = (int64_t)gridDim.x * 256;
   int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  float acc = 0.f;
+  T acc = T(0);
   for (; i + 3 * stride < n; i += 4 * stride) {
     int32_t c[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) c[u] = __builtin_nontemporal_load(&idx[i + u * stride]);
-    float v[4];
+    T v[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = c[u] > 0 ? x[c[u] - 1] : 0.f;
+    for (int u = 0; u < 4; ++u) v[u] = c[u] > 0 ? x[c[u] - 1] : T(0);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc += v[u];
+    for (int u = 0; u < 4; ++u) acc = fold(acc, v[u]);
   }
   for (; i < n; i += stride) {
     const int32_t c = idx[i];
-    if (c > 0) acc += x[c - 1];
+    if (c > 0) acc = fold(acc, x[c - 1]);
   }
-  __shared__ float red[256];
+  __shared__ T red[256];
   red[threadIdx.x] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
-    float t = 0.f;
-    for (int j = 0; j < 256; ++j) t += red[j];
+    T t = T(0);
+    for (int j = 0; j < 256; ++j) t = fold(t, red[j]);
     sink[blockIdx.x] = t;
   }
 }
@@ -163,7 +169,19 @@ int ggc_gather(void* stream, const int32_t* idx, int64_t n, const float* x, void
   const unsigned nb = grid_blocks();
   hipStream_t st = (hipStream_t)stream;
   return timed(st, reps, [&] {
-    hipLaunchKernelGGL(gather_kernel, dim3(nb), dim3(256), 0, st, idx, n, x, (float*)sink);
+    hipLaunchKernelGGL(gather_kernel<float>, dim3(nb), dim3(256), 0, st, idx, n, x, (float*)sink);
+  }, ms);
+}
+
+// The same with 8-byte words (the source masks of a multi-source BFS level).
+int ggc_gather8(void* stream, const int32_t* idx, int64_t n, const uint64_t* x, void* sink,
+                int reps, double* ms) {
+  if (!idx || !x || !sink || !ms || n < 0) return 2;
+  const unsigned nb = grid_blocks();
+  hipStream_t st = (hipStream_t)stream;
+  return timed(st, reps, [&] {
+    hipLaunchKernelGGL(gather_kernel<uint64_t>, dim3(nb), dim3(256), 0, st, idx, n, x,
+                       (uint64_t*)sink);
   }, ms);
 }
 

@@ -116,6 +116,12 @@ class ServerCounters(C.Structure):
                 ("arena_used", C.c_size_t)]
 
 
+class TuneCandidate(C.Structure):
+    _fields_ = [("format", C.c_int32), ("csr_lanes", C.c_int32), ("slice_size", C.c_int64),
+                ("sigma", C.c_int64), ("heavy_threshold", C.c_int64), ("heavy_seg_len", C.c_int64),
+                ("sell_flags", C.c_int32), ("reserved", C.c_int32), ("ms", C.c_double)]
+
+
 # every symbol of include/gpugraphs.h with its ctypes signature
 _vp, _i64, _i32, _u64, _d = C.c_void_p, C.c_int64, C.c_int, C.c_uint64, C.c_double
 _PM = C.POINTER(GGMatrix)
@@ -177,6 +183,11 @@ SIGNATURES = {
     "gg_shortest_path_multi": ([_vp, _PM, _vp, _i64, _vp, _i32, C.POINTER(_i64)], _i32),
     "gg_msbfs_distances": ([_vp, _PM, _vp, _i64, _vp, C.POINTER(_i64)], _i32),
     "gg_mssp_distances": ([_vp, _PM, _vp, _i64, _vp, _i32, C.POINTER(_i64)], _i32),
+    "gg_tune_default_candidates": ([_PM, _i64, C.POINTER(TuneCandidate), _i64, C.POINTER(_i64)],
+                                   _i32),
+    "gg_tune_layout": ([_vp, _PM, _i32, _vp, _i64, C.POINTER(TuneCandidate), _i64, _i32,
+                        C.POINTER(TuneCandidate)], _i32),
+    "gg_tune_push_threshold": ([_vp, _PM, _vp, _i64, _vp, _vp, _i64, _i32, C.POINTER(_d)], _i32),
     "gg_server_create": ([_i32, C.c_size_t, C.POINTER(_vp)], _i32),
     "gg_server_add_graph": ([_vp, _PM, C.POINTER(_i32)], _i32),
     "gg_server_start": ([_vp], _i32),

@@ -8,12 +8,16 @@ Times candidates on the device and keeps the fastest:
   - the steps per captured graph launch of the level loops.
 The workload decides what is timed: one plus-times SpMV, BFS levels from the
 given roots, a multi-source BFS of up to 64 roots, Bellman-Ford from the first
-root, or Bellman-Ford from up to 32 roots at once.  Results
-are cached per graph signature (size, entries, degree profile) and workload,
-in memory and optionally in a JSON file.
+root, or Bellman-Ford from up to 32 roots at once.  The layout and push/pull
+searches run in the library (gg_tune_layout, gg_tune_push_threshold: every
+candidate is built on the device from the graph's CSR and timed with HIP
+events), so a caller of the C ABI tunes the same way.  Results are cached per
+graph signature (size, entries, degree profile) and workload, in memory and
+optionally in a JSON file.
 """
 from __future__ import annotations
 
+import ctypes as C
 import json
 import os
 import time
@@ -35,26 +39,43 @@ class TuneResult:
     table: list = field(default_factory=list)   # (candidate, ms) for every candidate timed
 
 
-def default_candidates(H: HostCSR):
-    deg = H.degrees()
-    mx = int(deg.max()) if len(deg) else 0
-    c = [dict(layout="csr", lanes=L) for L in (1, 4, 16, 64)]
-    for heavy in (256, 1024, 4096):
-        if heavy < mx:
-            for seg in (4096, 65536):
-                c.append(dict(layout="sell", slice_size=64, sigma=1, relabel=True, heavy=heavy,
-                              seg=seg))
-    c.append(dict(layout="sell", slice_size=64, sigma=1, relabel=True, heavy=0, seg=4096))
-    if 1024 < mx:  # each row's entries by increasing internal column (light hubs first, heavy rows last)
-        c.append(dict(layout="sell", slice_size=64, sigma=1, relabel=True, heavy=1024, seg=4096,
-                      sort=True))
-    c.append(dict(layout="sell", slice_size=64, sigma=4096, relabel=False,
-                  heavy=256 if 256 < mx else 0, seg=4096))
-    c.append(dict(layout="sell", slice_size=32, sigma=1, relabel=True,
-                  heavy=1024 if 1024 < mx else 0, seg=4096))
-    if H.m != H.n:   # relabelling needs a square matrix
-        c = [x for x in c if not x.get("relabel")]
+_WORKLOAD_CODE = {"spmv": 1, "bfs": 2, "msbfs": 3, "sssp": 4, "msssp": 5}
+
+
+def _to_native(cand: dict) -> _lib.TuneCandidate:
+    c = _lib.TuneCandidate()
+    if cand["layout"] == "csr":
+        c.format, c.csr_lanes = _lib.CSR, int(cand.get("lanes", 0))
+    else:
+        c.format = _lib.SELL
+        c.slice_size, c.sigma = int(cand["slice_size"]), int(cand["sigma"])
+        c.heavy_threshold, c.heavy_seg_len = int(cand["heavy"]), int(cand["seg"])
+        c.sell_flags = ((_lib.SELL_RELABEL if cand.get("relabel") else 0)
+                        | (_lib.SELL_SORT_ENTRIES if cand.get("sort") else 0))
     return c
+
+
+def _from_native(c: _lib.TuneCandidate) -> dict:
+    if c.format == _lib.CSR:
+        return dict(layout="csr", lanes=int(c.csr_lanes))
+    d = dict(layout="sell", slice_size=int(c.slice_size), sigma=int(c.sigma),
+             relabel=bool(c.sell_flags & _lib.SELL_RELABEL), heavy=int(c.heavy_threshold),
+             seg=int(c.heavy_seg_len))
+    if c.sell_flags & _lib.SELL_SORT_ENTRIES:
+        d["sort"] = True
+    return d
+
+
+def default_candidates(H: HostCSR):
+    """The native default candidate set (gg_tune_default_candidates) as dicts."""
+    deg = H.degrees()
+    d = _lib.GGMatrix()
+    d.format, d.m, d.n, d.nnz = _lib.CSR, H.m, H.n, H.nnz
+    buf = (_lib.TuneCandidate * 64)()
+    n = C.c_int64()
+    _lib.check(_lib.lib().gg_tune_default_candidates(C.byref(d), int(deg.max()) if len(deg) else 0,
+                                                     buf, 64, C.byref(n)), "default_candidates")
+    return [_from_native(buf[i]) for i in range(n.value)]
 
 
 def signature(H: HostCSR, workload: str) -> str:
@@ -68,27 +89,6 @@ def build(H: HostCSR, backend, cand: dict):
         return H.to_csr(backend)
     return H.to_sell(backend, cand["slice_size"], cand["sigma"], cand["heavy"], cand["seg"],
                      cand["relabel"], cand.get("sort", False))
-
-
-def _runner(workload, A, H, roots, torch, gg):
-    from .algorithms import multi_source_bfs, shortest_path_
-    dev = torch.device("cuda", A.backend.device)
-    if workload == "spmv":
-        x = torch.rand(H.n, dtype=torch.float32, device=dev)
-        y = torch.zeros(H.m, dtype=torch.float32, device=dev)
-        return lambda: gg.gpu_spmv_(y, A, x)
-    if workload == "bfs":
-        return lambda: [gg.bfs_distances(A, int(r)) for r in roots]
-    if workload == "msbfs":
-        r = np.asarray(roots[:64], np.int32)
-        return lambda: multi_source_bfs(A, r)
-    dt = torch.float32 if H.nzval.dtype == np.float32 else torch.int32
-    if workload == "msssp":
-        r = np.asarray(roots[:32], np.int32)
-        D = torch.empty(len(r) * H.m, dtype=dt, device=dev).view(len(r), H.m).T
-        return lambda: shortest_path_(A, r, D)
-    d = torch.empty(H.m, dtype=dt, device=dev)
-    return lambda: shortest_path_(A, int(roots[0]), d)
 
 
 class Autotuner:
@@ -130,24 +130,24 @@ class Autotuner:
         if key in self.cache and candidates is None:
             c = self.cache[key]
             return TuneResult(workload, c["best"], c["best_ms"], c.get("table", []))
-        Hs = H
-        if workload == "spmv" and H.nzval.dtype != np.float32:
-            Hs = HostCSR(H.m, H.n, H.rowptr, H.colval, np.ones(H.nnz, np.float32))
+        # the native tuner builds every candidate on the device from the CSR
+        # and times the workload (gg_tune_layout); SpMV times fp32 ones when
+        # the values are of another type
+        cands = candidates or default_candidates(H)
+        arr = (_lib.TuneCandidate * len(cands))(*[_to_native(c) for c in cands])
+        A = H.to_csr(self.backend)
+        d = A.descriptor()
+        r = np.ascontiguousarray(roots if roots is not None else [1], np.int32)
+        best = _lib.TuneCandidate()
         ctx = _lib.context(self.backend.device)
-        table = []
-        for cand in candidates or default_candidates(H):
-            A = build(Hs, self.backend, cand)
-            _lib.set_csr_lanes(ctx, cand.get("lanes", 0))
-            try:
-                ms = self._time(_runner(workload, A, Hs, roots, torch, gg), torch)
-            finally:
-                _lib.set_csr_lanes(ctx, 0)
-            table.append((cand, ms))
-            del A
-        best, best_ms = min(table, key=lambda t: t[1])
-        self.cache[key] = dict(best=best, best_ms=best_ms, table=table)
+        _lib.check(_lib.lib().gg_tune_layout(ctx, C.byref(d), _WORKLOAD_CODE[workload],
+                                             r.ctypes.data, len(r), arr, len(cands), self.reps,
+                                             C.byref(best)), "tune_layout")
+        table = [(cands[i], float(arr[i].ms)) for i in range(len(cands)) if arr[i].ms >= 0]
+        best_d, best_ms = min(table, key=lambda t: t[1])
+        self.cache[key] = dict(best=best_d, best_ms=best_ms, table=table)
         self._save()
-        return TuneResult(workload, best, best_ms, table)
+        return TuneResult(workload, best_d, best_ms, table)
 
     def tune_direction(self, A_T, A_out, roots, alphas=(4.0, 15.0, 64.0), betas=(6.0, 18.0, 64.0)):
         """(alpha, beta) of the direction-optimising BFS with the least total
@@ -163,11 +163,25 @@ class Autotuner:
         (a, b), ms = min(table, key=lambda t: t[1])
         return TuneResult("bfs_do", dict(alpha=a, beta=b), ms, table)
 
-    def tune_push_threshold(self, fn, alphas=(0.0, 4.0, 16.0, 64.0)):
+    def tune_push_threshold(self, fn, alphas=(0.0, 4.0, 16.0, 64.0), roots=None):
         """Direction switch of the multi-source BFS (gg_ctx_set_push_threshold)
-        with the least time for fn(); 0 keeps every level bottom-up."""
+        with the least time; 0 keeps every level bottom-up.  fn is a matrix
+        (A_T, timed natively with gg_tune_push_threshold on up to 64 roots)
+        or a callable timed as it is."""
         import torch
         ctx = _lib.context(self.backend.device)
+        if not callable(fn):
+            r = np.ascontiguousarray(roots[:64], np.int32)
+            al = np.ascontiguousarray(alphas, np.float64)
+            ms = np.zeros(len(al), np.float64)
+            best = C.c_double()
+            d = fn.descriptor()
+            _lib.check(_lib.lib().gg_tune_push_threshold(ctx, C.byref(d), r.ctypes.data, len(r),
+                                                         al.ctypes.data, ms.ctypes.data, len(al),
+                                                         self.reps, C.byref(best)),
+                       "tune_push_threshold")
+            return TuneResult("push_threshold", dict(alpha=best.value), float(ms.min()),
+                              list(zip(al.tolist(), ms.tolist())))
         table = []
         for a in alphas:
             _lib.set_push_threshold(ctx, a)

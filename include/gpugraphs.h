@@ -430,6 +430,46 @@ int gg_mssp_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources,
 int gg_bfs_packed(gg_ctx* ctx, const gg_matrix* A_T, const int64_t* offsets, int n_graphs,
                   const int32_t* sources, int32_t* dist, int32_t* levels);
 
+/* ------------------------------------------------------------------ tuning */
+/* The autotuner (no counterpart in the reference): candidate layouts of one
+ * graph are built on the device from its CSR and timed on a workload; the
+ * fastest is returned.  A gg_tune_candidate is a CSR with a lane count, or a SELL of
+ * the gg_sell_plan_device parameters; ms receives the measured time (best of
+ * reps, -1 when the candidate does not apply, e.g. relabelling a non-square
+ * matrix).  Workloads: one plus-times fp32 SpMV (other value types are timed
+ * on fp32 ones); BFS levels from every root; one multi-source BFS of up to 64
+ * roots; Bellman-Ford from the first root; Bellman-Ford from up to 32 roots
+ * at once.  roots: host, 1-based. */
+typedef enum gg_tune_workload {
+  GG_TUNE_SPMV = 1, GG_TUNE_BFS = 2, GG_TUNE_MSBFS = 3, GG_TUNE_SSSP = 4, GG_TUNE_MSSSP = 5
+} gg_tune_workload;
+typedef struct gg_tune_candidate {
+  int32_t format;      /* GG_CSR or GG_SELL */
+  int32_t csr_lanes;   /* CSR: lanes per row (0 = by mean row length) */
+  int64_t slice_size, sigma, heavy_threshold, heavy_seg_len;  /* SELL */
+  int32_t sell_flags;  /* GG_SELL_RELABEL | GG_SELL_SORT_ENTRIES */
+  int32_t reserved;
+  double ms;
+} gg_tune_candidate;
+/* The default candidate set for a CSR A (max_degree: its longest row):
+ * CSR with 1, 4, 16 and 64 lanes; relabelled SELL-64 with heavy thresholds
+ * 256 / 1024 / 4096 below the longest row and segments of 4096 / 65536, with
+ * and without sorted row entries; SELL-32; SELL-64 with sigma 4096 windows.
+ * *n_out receives the count (out must hold capacity entries). */
+int gg_tune_default_candidates(const gg_matrix* A, int64_t max_degree, gg_tune_candidate* out,
+                               int64_t capacity, int64_t* n_out);
+/* Times every candidate (their ms is written) and copies the fastest to *best;
+ * the context's CSR lane count is left as it was. */
+int gg_tune_layout(gg_ctx* ctx, const gg_matrix* A_csr, int workload, const int32_t* roots,
+                   int64_t n_roots, gg_tune_candidate* candidates, int64_t n_candidates, int reps,
+                   gg_tune_candidate* best);
+/* Times gg_msbfs_distances (k <= 64 roots) under each direction switch alpha
+ * (gg_ctx_set_push_threshold; 0 = always bottom-up), writes ms[i] when ms is
+ * given, leaves the fastest set on the context and in *best_alpha. */
+int gg_tune_push_threshold(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* roots, int64_t k,
+                           const double* alphas, double* ms, int64_t n_alphas, int reps,
+                           double* best_alpha);
+
 /* ---------------------------------------------------------------- serving */
 /* Continuous-batching query server over HBM-resident graphs (no counterpart
  * in the reference, which answers one call at a time).  Each graph gets a BFS

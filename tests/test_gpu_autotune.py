@@ -1,6 +1,7 @@
-"""The autotuner times every candidate layout / switch on the device, keeps
-the fastest, caches it per graph signature, and the tuned choice computes
-the same results as any other."""
+"""The autotuner (the library's gg_tune_layout / gg_tune_push_threshold
+behind gpugraphs.autotune) times every candidate layout / switch on the
+device, keeps the fastest, caches it per graph signature, and the tuned
+choice computes the same results as the oracle."""
 import numpy as np
 import pytest
 
@@ -69,6 +70,10 @@ def test_tune_direction_and_graph_steps(env):
         srcs = (np.nonzero(R.degrees())[0][:40] + 1).astype(np.int32)
         pa = tuner.tune_push_threshold(lambda: gg.multi_source_bfs(S, srcs), alphas=(0.0, 16.0))
         assert pa.best["alpha"] in (0.0, 16.0)
+        # the same search in the library (gg_tune_push_threshold)
+        pn = tuner.tune_push_threshold(S, alphas=(0.0, 4.0, 16.0), roots=srcs)
+        assert pn.best["alpha"] in (0.0, 4.0, 16.0) and len(pn.table) == 3
+        assert pn.best_ms == min(ms for _, ms in pn.table) > 0
         D = gg.multi_source_bfs(S, srcs).cpu().numpy()
         np.testing.assert_array_equal(D[:, 0], gg.bfs_distances(A, int(srcs[0])).cpu().numpy())
         gs = tuner.tune_graph_steps(lambda: gg.bfs_distances(A, int(roots[1])), steps=(0, 4))

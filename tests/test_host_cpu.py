@@ -479,3 +479,22 @@ def test_read_edge_list_builds_transpose(tmp_path, directed):
     r.write_text("1 2 5\n2 3\n")
     with pytest.raises(gg.ArgumentError):
         G.read_edge_list(str(r), dtype=np.int32)
+
+
+def test_tune_default_candidates_native():
+    """The native candidate set: CSR lane counts, relabelled SELL only for a
+    square matrix, heavy thresholds only below the longest row, and the
+    sigma-4096 window layout."""
+    from gpugraphs import autotune as AT, generators as GEN
+    R = GEN.rmat(10, seed=3)
+    cands = AT.default_candidates(R)
+    mx = int(R.degrees().max())
+    assert [c["lanes"] for c in cands if c["layout"] == "csr"] == [1, 4, 16, 64]
+    sells = [c for c in cands if c["layout"] == "sell"]
+    assert all(c["heavy"] < mx for c in sells)
+    assert any(c["sigma"] == 4096 and not c["relabel"] for c in sells)
+    assert any(c["relabel"] for c in sells)
+    rect = GEN.HostCSR(R.m, R.n + 7, R.rowptr, R.colval, R.nzval)
+    assert not any(c.get("relabel") for c in AT.default_candidates(rect))
+    for c in cands:   # the dict <-> C struct mapping round-trips
+        assert AT._from_native(AT._to_native(c)) == c
