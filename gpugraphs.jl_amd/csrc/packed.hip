@@ -47,6 +47,7 @@ struct PackedArgs {
   int32_t n_graphs;
   unsigned int* chunk_ctr;    // [2] work queue of the level parity
   int32_t* active;            // graphs still active after the last level
+  int32_t first_level;        // levels before it ran top-down (separate launches)
   int32_t max_level;
 };
 
@@ -70,7 +71,19 @@ __device__ void packed_chunk(const PackedArgs& a, int64_t chunk, int32_t level,
     s = a.rowptr[row] - 1;
     e = a.rowptr[row + 1] - 1;
     if (e - s <= kShort) {
-      for (int32_t i = s; i < e && !hit; ++i) hit = packed_edge<TA, UNIT>(a, front, i);
+      // 4 entries per step: their column loads, then their frontier tests,
+      // are independent, so a lane keeps 4 of each in flight
+      for (int32_t i = s; i < e && !hit; i += 4) {
+        int32_t c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = i + u < e ? a.colval[i + u] - 1 : -1;
+        bool h[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          h[u] = c[u] >= 0 && bit_get(front, c[u]) &&
+                 (UNIT || edge_on((const TA*)a.val, (int64_t)i + u));
+        hit = h[0] || h[1] || h[2] || h[3];
+      }
     } else {
       longrow = true;
     }
@@ -118,7 +131,7 @@ __global__ __launch_bounds__(kP) void packed_bfs_kernel(PackedArgs a) {
   __shared__ long long s_chunk;
   __shared__ int s_any;
   cg::grid_group grid = cg::this_grid();
-  for (int32_t level = 1; level <= a.max_level; ++level) {
+  for (int32_t level = a.first_level; level <= a.max_level; ++level) {
     const uint32_t* front = a.bm[(level - 1) & 1];
     uint32_t* next = a.bm[level & 1];
     unsigned int* ctr = &a.chunk_ctr[level & 1];
@@ -166,7 +179,9 @@ __global__ __launch_bounds__(kP) void packed_bfs_kernel(PackedArgs a) {
 
 __global__ void packed_seed_kernel(const int64_t* __restrict__ offsets, const int32_t* __restrict__ src,
                                    int32_t n_graphs, uint32_t* front, uint32_t* visited,
-                                   int32_t* dist, int32_t* found, int32_t* done) {
+                                   int32_t* dist, int32_t* found, int32_t* done,
+                                   const int32_t* __restrict__ rowptr, int32_t* queue0,
+                                   unsigned long long* cnt) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_graphs) return;
   const int64_t v = offsets[g] + src[g] - 1;
@@ -175,6 +190,129 @@ __global__ void packed_seed_kernel(const int64_t* __restrict__ offsets, const in
   atomicOr(&visited[v >> 5], 1u << (v & 31));
   found[g] = 0;
   done[g] = 0;
+  if (queue0) {  // the sources are the first queue (top-down levels)
+    queue0[g] = (int32_t)v;
+    atomicAdd(&cnt[1], (unsigned long long)(rowptr[v + 1] - rowptr[v]));
+  }
+}
+
+// ----------------------------------------------------- top-down first levels
+// A symmetric unit pattern (row u of A_T lists u's out-neighbours) starts with
+// top-down levels while the frontier is small: each is an ordinary launch
+// that expands the queue of the rows the previous level claimed, and the host
+// reads back the new queue's length and degree sum to pick the next level's
+// direction (Beamer's rule).  The persistent pull kernel then takes over from
+// the first bottom-up level.
+struct PackedTD {
+  const int32_t* rowptr;
+  const int32_t* colval;
+  const int32_t* chunk_graph;
+  const int32_t* cur;          // queue of the previous level
+  int64_t qn;
+  int32_t* nxt;                // queue this level fills
+  uint32_t* next;              // next-frontier bitmap (cleared before the launch)
+  uint32_t* visited;
+  int32_t* dist;
+  int32_t* found;
+  unsigned long long* cnt;     // [0] = queue length, [1] = degree sum, [3] = pieces
+  int2* pieces;                // (row, first entry) of the kTdPiece-entry pieces of long rows
+};
+
+constexpr int32_t kTdPiece = 1024;  // rows longer than this are cut into pieces
+
+// Entries [b0, b1) of row u (0-based entries) expanded by one wavefront, 64
+// at a time: unvisited neighbours are claimed with an atomic OR on the
+// visited bitmap (a plain read first skips the ones already set), get their
+// level and next bit, and are queued with their degree (one atomic per
+// counter per wavefront step).
+__device__ void td_expand(const PackedTD& t, int32_t u, int32_t b0, int32_t b1, int32_t level) {
+  const int lane = threadIdx.x & 63;
+  bool any = false;
+  for (int32_t b = b0; b < b1; b += 64) {
+    const int32_t i = b + lane;
+    bool claim = false;
+    int32_t v = 0;
+    unsigned long long deg = 0;
+    if (i < b1) {
+      v = t.colval[i] - 1;
+      const uint32_t bit = 1u << (v & 31);
+      if (!(t.visited[v >> 5] & bit) && !(atomicOr(&t.visited[v >> 5], bit) & bit)) {
+        claim = true;
+        t.dist[v] = level;
+        atomicOr(&t.next[v >> 5], bit);
+        deg = (unsigned long long)(t.rowptr[v + 1] - t.rowptr[v]);
+      }
+    }
+    const unsigned long long m = __ballot(claim);
+    if (m) {
+      any = true;
+      unsigned long long dsum = deg;
+      for (int o = 32; o > 0; o >>= 1) dsum += __shfl_xor(dsum, o, 64);
+      unsigned long long base = 0;
+      if (lane == 0) {
+        base = atomicAdd(&t.cnt[0], (unsigned long long)__popcll(m));
+        atomicAdd(&t.cnt[1], dsum);
+      }
+      base = __shfl(base, 0, 64);
+      if (claim) t.nxt[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+    }
+  }
+  // every neighbour is in u's graph (the blocks are disjoint)
+  if (any && lane == 0) set_flag((unsigned int*)&t.found[t.chunk_graph[u / kP]]);
+}
+
+// One wavefront per queued row; a row longer than kTdPiece entries queues its
+// pieces for packed_td_big_kernel instead.
+__global__ __launch_bounds__(kP) void packed_td_kernel(PackedTD t, int32_t level) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (kP / 64);
+  for (int64_t q = (int64_t)blockIdx.x * (kP / 64) + (threadIdx.x >> 6); q < t.qn; q += waves) {
+    const int32_t u = t.cur[q];
+    const int32_t rs = t.rowptr[u] - 1, re = t.rowptr[u + 1] - 1;
+    if (re - rs > kTdPiece) {
+      if (lane == 0) {
+        const int32_t np = (re - rs + kTdPiece - 1) / kTdPiece;
+        const unsigned long long p0 = atomicAdd(&t.cnt[3], (unsigned long long)np);
+        for (int32_t j = 0; j < np; ++j) t.pieces[p0 + j] = make_int2(u, rs + j * kTdPiece);
+      }
+      continue;
+    }
+    td_expand(t, u, rs, re, level);
+  }
+}
+
+// One wavefront per piece of a long row.
+__global__ __launch_bounds__(kP) void packed_td_big_kernel(PackedTD t, int32_t level) {
+  const int64_t np = (int64_t)t.cnt[3];
+  const int64_t waves = (int64_t)gridDim.x * (kP / 64);
+  for (int64_t q = (int64_t)blockIdx.x * (kP / 64) + (threadIdx.x >> 6); q < np; q += waves) {
+    const int2 pc = t.pieces[q];
+    const int32_t re = t.rowptr[pc.x + 1] - 1;
+    td_expand(t, pc.x, pc.y, min(pc.y + kTdPiece, re), level);
+  }
+}
+
+// Closes a top-down level per graph (block 0's job in the persistent
+// kernel): a graph whose level claimed nothing is done; cnt[2] counts the
+// graphs still active.
+__global__ void packed_close_kernel(int32_t* found, int32_t* done, int32_t* levels, int n_graphs,
+                                    int32_t level, unsigned long long* cnt) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_graphs || done[g]) return;
+  if (found[g]) {
+    found[g] = 0;
+    atomicAdd(&cnt[2], 1ull);
+  } else {
+    done[g] = 1;
+    levels[g] = level - 1;
+  }
+}
+
+__global__ void packed_td_reset_kernel(unsigned long long* cnt) {
+  cnt[0] = 0ull;
+  cnt[1] = 0ull;
+  cnt[2] = 0ull;
+  cnt[3] = 0ull;
 }
 
 __global__ void chunk_graph_kernel(const int64_t* __restrict__ offsets, int32_t n_graphs,
@@ -276,8 +414,21 @@ int gg_bfs_packed(gg_ctx* ctx, const gg_matrix* A_T, const int64_t* offsets, int
   p_src = (int64_t*)p_off + n_graphs + 1;
   GG_TRY(ws_get(ctx, WS_SPMSPV1, sizeof(int32_t) * (n_chunks + 1), &p_cg));
   GG_TRY(ws_get(ctx, WS_BFS0, sizeof(uint32_t) * 3 * (words + 1), &p_bm));
-  // per graph: found, done, levels; then the two queue counters and the active flag
-  GG_TRY(ws_get(ctx, WS_SPMSPV2, sizeof(int32_t) * (3 * n_graphs + 8), &p_st));
+  // per graph: found, done, levels; then the two queue counters and the active
+  // flag; then, 8-byte aligned, the top-down counters (queue length, degree
+  // sum, active graphs)
+  const size_t st_i32 = (sizeof(int32_t) * (3 * (size_t)n_graphs + 8) + 7) & ~(size_t)7;
+  GG_TRY(ws_get(ctx, WS_SPMSPV2, st_i32 + 4 * sizeof(uint64_t), &p_st));  // td_cnt[0..3]
+  unsigned long long* td_cnt = (unsigned long long*)((char*)p_st + st_i32);
+  // top-down first levels need row u of A_T to list u's out-neighbours
+  const bool topdown = unit && (A_T->flags & GG_FLAG_SYMMETRIC) && ctx->push_alpha > 0.0;
+  void *p_q0 = nullptr, *p_q1 = nullptr, *p_pc = nullptr;
+  if (topdown) {
+    GG_TRY(ws_get(ctx, WS_BFS3, sizeof(int32_t) * m, &p_q0));
+    GG_TRY(ws_get(ctx, WS_BFS4, sizeof(int32_t) * m, &p_q1));
+    // pieces of rows longer than kTdPiece: fewer than nnz / kTdPiece + one per such row
+    GG_TRY(ws_get(ctx, WS_BFS5, sizeof(int2) * (size_t)(2 * (A_T->nnz / kTdPiece) + 16), &p_pc));
+  }
   GG_HIP(hipMemcpyAsync(p_off, offsets, sizeof(int64_t) * (n_graphs + 1), hipMemcpyHostToDevice,
                         ctx->stream));
   GG_HIP(hipMemcpyAsync(p_src, sources, sizeof(int32_t) * n_graphs, hipMemcpyHostToDevice,
@@ -298,6 +449,7 @@ int gg_bfs_packed(gg_ctx* ctx, const gg_matrix* A_T, const int64_t* offsets, int
   a.n_graphs = n_graphs;
   a.chunk_ctr = (unsigned int*)(a.levels + n_graphs);
   a.active = (int32_t*)(a.chunk_ctr + 2);
+  a.first_level = 1;
   a.max_level = (int32_t)std::min<int64_t>(m + 1, 2147483646LL);
   hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(3 * (words + 1), kP)), dim3(kP), 0,
                      ctx->stream, a.bm[0], 3 * (words + 1), 0u);
@@ -308,10 +460,63 @@ int gg_bfs_packed(gg_ctx* ctx, const gg_matrix* A_T, const int64_t* offsets, int
                      a.levels, n_graphs, 0);
   hipLaunchKernelGGL(chunk_graph_kernel, dim3(blocks_for(n_chunks, kP)), dim3(kP), 0, ctx->stream,
                      (const int64_t*)p_off, n_graphs, n_chunks, (int32_t*)p_cg);
+  hipLaunchKernelGGL(packed_td_reset_kernel, dim3(1), dim3(1), 0, ctx->stream, td_cnt);
   hipLaunchKernelGGL(packed_seed_kernel, dim3(blocks_for(n_graphs, kP)), dim3(kP), 0, ctx->stream,
                      (const int64_t*)p_off, (const int32_t*)p_src, n_graphs, a.bm[0], a.visited,
-                     dist, a.found, a.done);
+                     dist, a.found, a.done, A_T->ptr, (int32_t*)p_q0, td_cnt);
   GG_HIP(hipGetLastError());
+  bool all_done = false;
+  if (topdown) {
+    // top-down levels while the queue's degree sum * alpha is below the
+    // entries of the rows not reached yet; one host round trip per level
+    GG_HIP(hipMemcpyAsync(ctx->host_pinned, td_cnt, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    GG_HIP(hipStreamSynchronize(ctx->stream));
+    const unsigned long long* hc = (const unsigned long long*)ctx->host_pinned;
+    int64_t qn = n_graphs;
+    double qdeg = (double)hc[1];
+    double left = (double)A_T->nnz - qdeg;
+    int32_t level = 1;
+    int32_t* q[2] = {(int32_t*)p_q0, (int32_t*)p_q1};
+    while (qn > 0 && qdeg * ctx->push_alpha < left) {
+      PackedTD t{};
+      t.rowptr = A_T->ptr;
+      t.colval = A_T->idx;
+      t.chunk_graph = a.chunk_graph;
+      t.cur = q[(level - 1) & 1];
+      t.qn = qn;
+      t.nxt = q[level & 1];
+      t.next = a.bm[level & 1];
+      t.visited = a.visited;
+      t.dist = dist;
+      t.found = a.found;
+      t.cnt = td_cnt;
+      t.pieces = (int2*)p_pc;
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words + 1, kP)), dim3(kP), 0,
+                         ctx->stream, t.next, words + 1, 0u);
+      hipLaunchKernelGGL(packed_td_reset_kernel, dim3(1), dim3(1), 0, ctx->stream, td_cnt);
+      const int64_t nb = std::min<int64_t>((qn + 3) / 4, (int64_t)ctx->num_cus * 16);
+      hipLaunchKernelGGL(packed_td_kernel, dim3((unsigned)std::max<int64_t>(nb, 1)), dim3(kP), 0,
+                         ctx->stream, t, level);
+      hipLaunchKernelGGL(packed_td_big_kernel, dim3((unsigned)ctx->num_cus * 16), dim3(kP), 0,
+                         ctx->stream, t, level);
+      hipLaunchKernelGGL(packed_close_kernel, dim3(blocks_for(n_graphs, kP)), dim3(kP), 0,
+                         ctx->stream, a.found, a.done, a.levels, n_graphs, level, td_cnt);
+      GG_HIP(hipGetLastError());
+      GG_HIP(hipMemcpyAsync(ctx->host_pinned, td_cnt, 3 * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, ctx->stream));
+      GG_HIP(hipStreamSynchronize(ctx->stream));
+      qn = (int64_t)hc[0];
+      qdeg = (double)hc[1];
+      left -= qdeg;
+      ++level;
+      if (hc[2] == 0) {  // every graph finished top-down
+        all_done = true;
+        break;
+      }
+    }
+    a.first_level = level;
+  }
   auto launch = [&](auto kern) -> int {
     int per_cu = 0;
     GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kP, 0));
@@ -325,7 +530,9 @@ int gg_bfs_packed(gg_ctx* ctx, const gg_matrix* A_T, const int64_t* offsets, int
     return GG_OK;
   };
   int st;
-  if (unit) {
+  if (all_done) {
+    st = GG_OK;
+  } else if (unit) {
     st = launch(packed_bfs_kernel<uint8_t, true>);
   } else {
     st = with_type(A_T->val_dtype, [&](auto t) -> int {

@@ -917,11 +917,16 @@ def test_bfs_packed_rejects_unpadded_last_graph(env):
                                                                 np.int32))
 
 
-def test_bfs_packed_symmetric_vs_oracle(env):
+@pytest.mark.parametrize("alpha", [1e-9, 16.0, 0.0])
+def test_bfs_packed_symmetric_vs_oracle(env, alpha):
     """Symmetric unit-pattern packs (Barabasi-Albert, Dorogovtsev-Mendes,
-    R-MAT and grid blocks, one source a hub): every graph's levels equal the
+    R-MAT and grid blocks, one source a hub) start with top-down levels while
+    the queue's degree sum * alpha is below the entries left (alpha 1e-9:
+    every level top-down until all graphs finish; 16: top-down, then the
+    persistent pull kernel; 0: pull only); every graph's levels equal the
     oracle's pull loop (src/algorithms/bfs.jl:31-73) from its source."""
     gg, torch, orc = env
+    from gpugraphs import _lib
     from gpugraphs import generators as GEN
     rng = np.random.default_rng(91)
     hosts, mats = [], []
@@ -948,8 +953,13 @@ def test_bfs_packed_symmetric_vs_oracle(env):
     A = P.to_csr(gg.ROCBackend(0))
     srcs = np.array([int(rng.integers(M.shape[0])) + 1 for M in mats], np.int32)
     srcs[0] = int(np.argmax(np.diff(mats[0].indptr))) + 1       # a hub
-    print_case("bfs packed symmetric", P.m, P.nnz)
-    D, levels = gg.bfs_packed(A, off, srcs)
+    ctx = _lib.context(0)
+    try:
+        _lib.set_push_threshold(ctx, alpha)
+        print_case("bfs packed symmetric", alpha, P.m, P.nnz)
+        D, levels = gg.bfs_packed(A, off, srcs)
+    finally:
+        _lib.set_push_threshold(ctx, 16.0)
     D = host(D)
     for g, M in enumerate(mats):
         ref = orc.bfs_distances(orc.from_scipy_csr(sp.csr_matrix(M)), int(srcs[g]))
