@@ -422,11 +422,15 @@ int gg_msbfs_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources
 int gg_mssp_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources, int64_t k,
                       void* dist, int dist_dtype, int64_t* iterations);
 /* One BFS per packed graph (gg_pack_graphs layout; A_T the block-diagonal
- * CSR, offsets and sources host arrays, sources 1-based within each graph) in
- * ONE cooperative launch of a persistent kernel fed by an atomic chunk queue:
- * dist[offsets[g] + v] equals gg_bfs_distances(graph g, sources[g])[v];
- * levels (host, optional) receives each graph's level count.  The
- * reference runs bfs_distances (src/algorithms/bfs.jl:1-111) once per graph. */
+ * CSR, offsets and sources host arrays, sources 1-based within each graph):
+ * the bottom-up levels run in ONE cooperative launch of a persistent kernel
+ * fed by an atomic chunk queue; a symmetric unit pattern (GG_FLAG_SYMMETRIC)
+ * first runs top-down levels over a queue of the claimed rows while their
+ * degree sum * the context's push threshold is below the entries not yet
+ * reached (0 = never).  dist[offsets[g] + v] equals gg_bfs_distances(graph g,
+ * sources[g])[v]; levels (host, optional) receives each graph's level count.
+ * The reference runs bfs_distances (src/algorithms/bfs.jl:1-111) once per
+ * graph. */
 int gg_bfs_packed(gg_ctx* ctx, const gg_matrix* A_T, const int64_t* offsets, int n_graphs,
                   const int32_t* sources, int32_t* dist, int32_t* levels);
 
@@ -473,10 +477,11 @@ int gg_tune_push_threshold(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* roo
 /* ---------------------------------------------------------------- serving */
 /* Continuous-batching query server over HBM-resident graphs (no counterpart
  * in the reference, which answers one call at a time).  Each graph gets a BFS
- * engine (64 bitmask slots) and an SSSP engine (32 min-plus columns) on
- * streams of their own; a query joins a free slot between two steps of its
- * engine, its result is identical to gg_bfs_distances / gg_shortest_path
- * from its source, and it is admitted only when an n-element result block
+ * engine (64 bitmask slots) and an SSSP engine (64 min-plus columns) on
+ * streams of their own; an SSSP query on a Bool graph whose values are all
+ * true (GG_FLAG_UNIT_PATTERN: every edge weighs one) is served by the BFS
+ * engine.  A query joins a free slot between two steps of its engine, its
+ * result is identical to gg_bfs_distances / gg_shortest_path from its source, and it is admitted only when an n-element result block
  * fits in the result arena of budget_bytes (else it waits; a query that can
  * never fit fails with GG_ERR_BUDGET).  Graph arrays stay owned by the
  * caller and must outlive the server. */
