@@ -323,6 +323,17 @@ def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
                                                        seed=seed, keep_every=97)
         st = srv.stats()
         srv.stop()
+        # per graph and engine: steps, mean busy slots per step, queries
+        # finished, device ms per step and per finished query
+        engines = {}
+        for name, gb, gs, Mb, Ms in table:
+            for g, kind, label in ((gb, S.BFS, "bfs"), (gs, S.SSSP, "sssp")):
+                es = srv.engine_stats(g, kind)
+                if es["steps"]:
+                    engines[f"{name}/{label}"] = [
+                        es["steps"], round(es["busy_slot_steps"] / es["steps"], 1), es["finished"],
+                        round(1e3 * es["device_s"] / es["steps"], 3),
+                        round(1e3 * es["device_s"] / max(1, es["finished"]), 3)]
         ok = [c for c in done if c.status == 0]
         lat = [c.latency_s for c in ok]
         by_id = {r: (g, k, s) for r, g, k, s in submitted}
@@ -362,7 +373,10 @@ def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
                    qps=round(len(ok) / wall, 1),
                    p50_ms=round(pct(lat, 50) * 1e3, 2) if lat else None,
                    p99_ms=round(pct(lat, 99) * 1e3, 2) if lat else None, by_kind=per_kind,
-                   by_graph_n_p50_p99_service_ms=per_graph, steps=st["steps"],
+                   by_graph_n_p50_p99_service_ms=per_graph,
+                   engines_steps_busy_finished_ms_per_step_ms_per_query=engines,
+                   device_s_all_engines=round(sum(v[0] * v[3] for v in engines.values()) / 1e3, 3),
+                   steps=st["steps"],
                    checked_alone=checked)
         runs.append(run)
         log(f"serving at {rate:.0f} q/s offered: {len(ok)}/{len(submitted)} queries in {wall:.1f}s "

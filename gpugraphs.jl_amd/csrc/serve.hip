@@ -212,7 +212,7 @@ struct Engine {
   const int32_t* vinv = nullptr;  // original -> internal id + 1 (relabelled SELL)
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  hipEvent_t step_done = nullptr;
+  hipEvent_t step_start = nullptr, step_done = nullptr;  // around each step, on the stream
   bool inflight = false;
   uint64_t active = 0, stepped = 0;
   std::vector<int64_t> slot_q;          // query id per slot
@@ -242,8 +242,15 @@ struct Engine {
   BfSparse sp{};
   std::vector<void*> allocs;
   int64_t steps = 0;
-  // pacing: when the step was launched and a running mean of launch-to-done
+  // pacing: when the step was launched and a running mean of the steps' own
+  // time on the device (start to done event on the engine's stream, so the
+  // steps of other engines queued before it on a shared stream, and the
+  // scheduler's polling, are not counted)
   double t_launch = 0.0, step_s = 0.0;
+  // gg_server_engine_stats: device seconds of all steps, busy slots summed
+  // over the steps, queries finished
+  double device_s = 0.0;
+  int64_t busy_slot_steps = 0, finished = 0;
 };
 
 }  // namespace gg
@@ -297,7 +304,8 @@ static int make_engine(gg_server* s, const gg_matrix* A, int kind, std::unique_p
   } else {
     e->stream = s->small_stream;
   }
-  GG_HIP(hipEventCreateWithFlags(&e->step_done, hipEventDisableTiming));
+  GG_HIP(hipEventCreate(&e->step_start));
+  GG_HIP(hipEventCreate(&e->step_done));
   GG_HIP(hipHostMalloc((void**)&e->mask_h, 64, hipHostMallocDefault));
   void* p;
   GG_TRY(engine_alloc(s, *e, 64, &p)); e->mask_d = (unsigned long long*)p;
@@ -369,6 +377,7 @@ static void free_engine(Engine& e) {
   for (void* p : e.allocs) (void)hipFree(p);
   e.allocs.clear();
   if (e.mask_h) (void)hipHostFree(e.mask_h);
+  if (e.step_start) (void)hipEventDestroy(e.step_start);
   if (e.step_done) (void)hipEventDestroy(e.step_done);
   if (e.stream && e.own_stream) (void)hipStreamDestroy(e.stream);
 }
@@ -440,6 +449,7 @@ static int retire(gg_server* s, Engine& e) {
     q.t_done = t;
     q.state = Q_DONE;
     ++s->completed;
+    ++e.finished;
     e.active &= ~(1ull << sl);
     e.slot_q[sl] = -1;
   }
@@ -496,6 +506,7 @@ static int launch_step(gg_server* s, Engine& e) {
     seedmask |= 1ull << sl.slot[i];
   }
   e.seeds.clear();
+  GG_HIP(hipEventRecord(e.step_start, e.stream));
   hipLaunchKernelGGL(serve_zero_kernel, dim3(1), dim3(1), 0, e.stream, e.cnt_d, e.mask_d);
   if (e.kind == GG_QUERY_BFS) {
     const int64_t words = (ns + 31) / 32;
@@ -559,6 +570,7 @@ static int launch_step(gg_server* s, Engine& e) {
   e.inflight = true;
   e.t_launch = now_s();
   e.steps += 1;
+  e.busy_slot_steps += __builtin_popcountll(e.active);
   for (int i = 0; i < e.slots; ++i)
     if ((e.active >> i) & 1ull) e.slot_steps[i] += 1;
   return GG_OK;
@@ -568,8 +580,8 @@ static int launch_step(gg_server* s, Engine& e) {
 // slots (it reads the whole graph, or the whole frontier of every slot), so
 // an engine that steps whenever one slot is busy spends a full step on one
 // query.  While other engines have steps in flight, an engine whose slots are
-// not full waits pacing x (its mean step time) x (its free fraction) after its
-// last launch, so that arrivals join its next step instead of starting their
+// not full waits pacing x (its mean step time on the device) x (its free
+// fraction) after its last launch, so that arrivals join its next step instead of starting their
 // own; a full engine, or one alone on the device, steps at once.
 static bool paced(gg_server* s, const Engine& e, int inflight) {
   if (s->pacing <= 0.0 || inflight == 0) return false;
@@ -611,8 +623,12 @@ static void worker_main(gg_server* s) {
         }
         e.inflight = false;
         {
-          const double d = now_s() - e.t_launch;
-          e.step_s = e.step_s > 0.0 ? 0.75 * e.step_s + 0.25 * d : d;
+          float ms = 0.f;
+          if (hipEventElapsedTime(&ms, e.step_start, e.step_done) == hipSuccess) {
+            const double d = 1e-3 * ms;
+            e.step_s = e.step_s > 0.0 ? 0.75 * e.step_s + 0.25 * d : d;
+            e.device_s += d;
+          }
         }
         int st = retire(s, e);
         if (st != GG_OK) s->worker_error = gg_last_error();
@@ -863,6 +879,22 @@ int gg_server_stats(gg_server* s, gg_server_counters* out) {
   out->running = r;
   out->steps = steps;
   return s->worker_error.empty() ? GG_OK : set_error(GG_ERR_HIP, "%s", s->worker_error.c_str());
+}
+
+int gg_server_engine_stats(gg_server* s, int graph_id, int kind, gg_engine_counters* out) {
+  if (!s || !out) return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  if (kind != GG_QUERY_BFS && kind != GG_QUERY_SSSP)
+    return set_error(GG_ERR_ARGUMENT, "unknown engine kind %d", kind);
+  std::lock_guard<std::mutex> g(s->mu);
+  if (graph_id < 0 || 2 * (size_t)graph_id >= s->engines.size())
+    return set_error(GG_ERR_ARGUMENT, "unknown graph %d", graph_id);
+  const Engine& e = *s->engines[2 * graph_id + (kind == GG_QUERY_BFS ? 0 : 1)];
+  out->steps = e.steps;
+  out->busy_slot_steps = e.busy_slot_steps;
+  out->finished = e.finished;
+  out->slots = e.slots;
+  out->device_s = e.device_s;
+  return GG_OK;
 }
 
 int gg_server_stop(gg_server* s) {

@@ -116,6 +116,11 @@ class ServerCounters(C.Structure):
                 ("arena_used", C.c_size_t)]
 
 
+class EngineCounters(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("busy_slot_steps", C.c_int64), ("finished", C.c_int64),
+                ("slots", C.c_int32), ("reserved", C.c_int32), ("device_s", C.c_double)]
+
+
 class TuneCandidate(C.Structure):
     _fields_ = [("format", C.c_int32), ("csr_lanes", C.c_int32), ("slice_size", C.c_int64),
                 ("sigma", C.c_int64), ("heavy_threshold", C.c_int64), ("heavy_seg_len", C.c_int64),
@@ -198,6 +203,7 @@ SIGNATURES = {
     "gg_server_fetch_device": ([_vp, _i64, _vp, C.c_size_t, _d], _i32),
     "gg_server_release": ([_vp, _i64, _d], _i32),
     "gg_server_stats": ([_vp, C.POINTER(ServerCounters)], _i32),
+    "gg_server_engine_stats": ([_vp, _i32, _i32, C.POINTER(EngineCounters)], _i32),
     "gg_server_stop": ([_vp], _i32),
     "gg_server_destroy": ([_vp], _i32),
 }

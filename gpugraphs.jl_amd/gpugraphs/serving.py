@@ -16,7 +16,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import QueryInfo, ServerCounters, check, lib
+from ._lib import EngineCounters, QueryInfo, ServerCounters, check, lib
 from .algorithms import dist_dtype
 
 BFS, SSSP = 1, 2
@@ -96,6 +96,13 @@ class GraphServer:
         c = ServerCounters()
         check(self._L.gg_server_stats(self._h, C.byref(c)), "stats")
         return {f: getattr(c, f) for f, _ in ServerCounters._fields_}
+
+    def engine_stats(self, graph: int, kind: int) -> dict:
+        """Counters of one engine of a graph (gg_server_engine_stats): steps,
+        busy_slot_steps, finished, slots, device_s."""
+        c = EngineCounters()
+        check(self._L.gg_server_engine_stats(self._h, graph, kind, C.byref(c)), "engine_stats")
+        return {f: getattr(c, f) for f, _ in EngineCounters._fields_ if f != "reserved"}
 
     def stop(self):
         check(self._L.gg_server_stop(self._h), "stop")

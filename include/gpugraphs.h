@@ -504,8 +504,8 @@ int gg_server_create(int device, size_t budget_bytes, gg_server** out);
 int gg_server_add_graph(gg_server* s, const gg_matrix* A_T, int* graph_id);
 /* Step pacing (no counterpart in the reference): while other engines have
  * steps in flight, an engine whose slots are not all busy waits
- * factor x (its mean step time) x (its free slot fraction) after its last
- * step before the next one, so that arriving queries share steps; 0 steps
+ * factor x (its mean step time on the device, start to end of a step on its
+ * stream) x (its free slot fraction) after its last step before the next one, so that arriving queries share steps; 0 steps
  * whenever a slot is busy.  Default 24. */
 int gg_server_set_pacing(gg_server* s, double factor);
 int gg_server_start(gg_server* s);
@@ -519,6 +519,19 @@ int gg_server_fetch_device(gg_server* s, int64_t request_id, void* dev_dst, size
                            double wait_s);
 int gg_server_release(gg_server* s, int64_t request_id, double wait_s);
 int gg_server_stats(gg_server* s, gg_server_counters* out);
+/* Per engine of a graph (kind GG_QUERY_BFS: its BFS engine, which also serves
+ * the unit-weight SSSP queries; GG_QUERY_SSSP: its Bellman-Ford engine): the
+ * steps run, the busy slots summed over those steps, the queries finished
+ * and the device seconds of all steps (each timed from its first to its last
+ * kernel on the engine's stream; engines on other streams run meanwhile, so
+ * the sum over engines can exceed the wall time).  No counterpart in the
+ * reference. */
+typedef struct gg_engine_counters {
+  int64_t steps, busy_slot_steps, finished;
+  int32_t slots, reserved;
+  double device_s;
+} gg_engine_counters;
+int gg_server_engine_stats(gg_server* s, int graph_id, int kind, gg_engine_counters* out);
 int gg_server_stop(gg_server* s);   /* finish the steps in flight and join */
 int gg_server_destroy(gg_server* s);
 

@@ -75,6 +75,17 @@ def test_server_results_equal_alone_and_in_mix(graphs):
         assert st["completed"] == len(qs) + 6 and st["failed"] == 0
         assert st["arena_used"] == 0
         srv.stop()
+        # every finished query is counted by exactly one engine, and a query
+        # needs at least one step of its engine
+        es = [srv.engine_stats(g, k) for g in gids for k in (S.BFS, S.SSSP)]
+        assert sum(e["finished"] for e in es) == len(qs) + 6
+        assert sum(e["steps"] for e in es) == st["steps"]
+        for e in es:
+            assert e["slots"] == 64 and e["busy_slot_steps"] <= 64 * e["steps"]
+            assert (e["finished"] == 0) or (e["steps"] > 0 and e["device_s"] > 0.0)
+        from gpugraphs._lib import ArgumentError
+        with pytest.raises(ArgumentError, match="unknown graph"):
+            srv.engine_stats(len(gids), S.BFS)
 
 
 def test_server_budget_admission_and_rejection(graphs):
