@@ -95,14 +95,17 @@ def _time_queries(torch, fn, items):
 def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma, relabel):
     """Bellman-Ford one source at a time vs k sources in one multi-source run,
     on the 4-neighbour grid (+1% long-range edges, integer weights 1..1000 as
-    fp32) and on the R-MAT graph with the same kind of weights."""
+    fp32) and on the R-MAT graph with the same kind of weights.  The
+    k-source run's sparse-iteration switch is tuned per graph first
+    (gg_tune_sssp_threshold); both forms are timed warm."""
+    from gpugraphs import autotune as AT
     from gpugraphs.algorithms import shortest_path_
     out = {}
     rng = np.random.default_rng(seed + 1)
     t = time.time()
     Gd = G.grid(args.grid, args.grid, 0.01, 1, 1000, seed=seed, dtype=np.float32)
     w_r = rng.integers(1, 1001, Rg.nnz).astype(np.float32)
-    Rw = G.HostCSR(Rg.m, Rg.n, Rg.rowptr, Rg.colval, w_r)
+    Rw = G.HostCSR(Rg.m, Rg.n, Rg.rowptr, Rg.colval, w_r, Rg.symmetric)   # same symmetric pattern
     se = args.sort_entries
     cases = [("grid", Gd, Gd.to_sell(R0, 64, sigma, 0, args.seg, relabel, se),
               (rng.choice(Gd.m, args.sssp_sources, replace=False) + 1).astype(np.int32)),
@@ -110,8 +113,10 @@ def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma,
               roots[:args.sssp_k].astype(np.int32))]
     torch.cuda.synchronize()
     log(f"SSSP graphs: grid{args.grid} n={Gd.m} nnz={Gd.nnz}, weighted rmat; {time.time() - t:.1f}s")
+    tuner = AT.Autotuner(R0, reps=2)
     for name, H, A, srcs in cases:
         m, k = H.m, len(srcs)
+        sa = tuner.tune_sssp_threshold(A, srcs)
         dist = torch.empty(m, dtype=torch.float32, device=dev)
         shortest_path_(A, int(srcs[0]), dist)
         torch.cuda.synchronize()
@@ -124,6 +129,8 @@ def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma,
         kms, kn = kernel_pass(_lib, ctx, lambda: shortest_path_(A, int(srcs[0]), dist),
                               args.graph_steps)
         Dm = torch.empty(k * m, dtype=torch.float32, device=dev).view(k, m).T
+        shortest_path_(A, srcs, Dm)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         it_b = shortest_path_(A, srcs, Dm)
         torch.cuda.synchronize()
@@ -137,11 +144,15 @@ def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma,
                          plain_iters_mean=float(np.mean(iters)), iter_kernel_us=kms / max(kn, 1) * 1e3,
                          full_iter_bytes=B_it, batched_qps=k / t_b, batched_ms=t_b * 1e3,
                          batched_iters=it_b, batched_vs_plain=(k / t_b) / (k / sum(lat)),
+                         sparse_alpha=sa.best["alpha"],
+                         sparse_alpha_ms=[[a, round(t, 3)] for a, t in sa.table],
                          exact_certificate=exact)
         log(f"SSSP {name}: plain {k / sum(lat):.1f} q/s (p99 {pct(lat, 99) * 1e3:.1f} ms, "
             f"{np.mean(iters):.0f} iters, {kms / max(kn, 1) * 1e3:.0f} us/iter); batched k={k} "
-            f"{k / t_b:.1f} q/s ({it_b} iters) = {(k / t_b) / (k / sum(lat)):.1f}x")
+            f"{k / t_b:.1f} q/s ({it_b} iters) = {(k / t_b) / (k / sum(lat)):.1f}x, "
+            f"sparse alpha {sa.best['alpha']}")
         del plain, Dm
+    _lib.set_sssp_threshold(ctx, 16.0)
     return out
 
 
@@ -198,8 +209,9 @@ def tuned_leg(args, np, torch, gg, G, R0, ctx, _lib, seed):
     and the push/pull threshold picked by the autotuner: its multi-source BFS
     candidates timed on 64 roots, then the threshold; then batched BFS
     (k = 64 and 256) against the plain BFS, and k = 32 Bellman-Ford
-    (integer weights 1..1000 as fp32) against one source at a time, on the
-    tuned layout."""
+    (integer weights 1..1000 as fp32) against one source at a time, each on
+    its tuned layout, the k-source run with its tuned sparse-iteration
+    switch."""
     from gpugraphs import autotune as AT
     from gpugraphs.algorithms import shortest_path_
     sc = args.tuned_scale
@@ -249,6 +261,8 @@ def tuned_leg(args, np, torch, gg, G, R0, ctx, _lib, seed):
     out["sssp_layout"], out["msssp_layout"] = lw.best, lm.best
     Aw = AT.build(Hw, R0, lw.best)
     Am = Aw if lm.best == lw.best else AT.build(Hw, R0, lm.best)
+    sa = tuner.tune_sssp_threshold(Am, srcs)                # sparse-iteration switch
+    out["sssp_alpha"] = sa.best["alpha"]
     dist = torch.empty(H.m, dtype=torch.float32, device=f"cuda:{R0.device}")
     shortest_path_(Aw, int(srcs[0]), dist)
     torch.cuda.synchronize()
@@ -277,6 +291,7 @@ def tuned_leg(args, np, torch, gg, G, R0, ctx, _lib, seed):
         f"k=256 {out['msbfs_k256_qps']:.0f} q/s, plain {out['plain_qps']:.0f} q/s; SSSP k={k} "
         f"{out['sssp_batched_qps']:.0f} q/s vs {out['sssp_plain_qps']:.0f} q/s")
     _lib.set_push_threshold(ctx, args.push_alpha)
+    _lib.set_sssp_threshold(ctx, 16.0)
     return out
 
 

@@ -124,8 +124,8 @@ struct BfSparse {
   uint32_t* mark;           // [words] bitmap of dq, zero outside a sparse iteration
   int2* big;                // pieces of long fallen rows
   unsigned int* cnt;        // [0] fallen, [1] marked, [2] pieces, [3] sparse, [4] = [3]
-  unsigned long long* scout;  // number of fallen rows
-  int64_t words, ns, big_cap;
+  unsigned long long* scout;  // out-edges of the fallen rows
+  int64_t words, ns, big_cap, nnz;
   double alpha;
 };
 constexpr int kSparseBig = 4096;
@@ -144,6 +144,7 @@ inline BfSparse bf_sparse_layout(void* base, const gg_matrix* A, double alpha) {
   s.words = (ns + 31) / 32;
   s.ns = ns;
   s.big_cap = 2 * (A->nnz / kSparseBig) + 16;
+  s.nnz = A->nnz;
   s.alpha = alpha;
   char* c = (char*)base;
   s.big = (int2*)c; c += 8 * (size_t)s.big_cap;

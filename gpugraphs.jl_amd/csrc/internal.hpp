@@ -72,6 +72,8 @@ struct gg_ctx {
   int graph_steps = 0;
   int csr_log_lanes = -1;  // lanes per CSR row as log2 (gg_ctx_set_csr_lanes), -1 = by row length
   double push_alpha = 16.0;  // top-down when frontier out-edges * alpha < entries
+  double sssp_alpha = 16.0;  // sparse k-source Bellman-Ford iteration: fallen rows'
+                             // out-edges * alpha < entries
   hipStream_t cap_stream = nullptr;
   std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
 };

@@ -295,6 +295,13 @@ int gg_ctx_set_push_threshold(gg_ctx* ctx, double alpha) {
   return GG_OK;
 }
 
+int gg_ctx_set_sssp_threshold(gg_ctx* ctx, double alpha) {
+  if (!ctx) return set_error(GG_ERR_ARGUMENT, "ctx is NULL");
+  if (!(alpha >= 0.0)) return set_error(GG_ERR_ARGUMENT, "alpha must be >= 0");
+  ctx->sssp_alpha = alpha;
+  return GG_OK;
+}
+
 int gg_ctx_last_kernel_ms(gg_ctx* ctx, double* ms) {
   if (!ctx || !ms) return set_error(GG_ERR_ARGUMENT, "NULL argument");
   *ms = ctx->last_ms;

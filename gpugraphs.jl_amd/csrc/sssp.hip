@@ -489,8 +489,8 @@ __global__ __launch_bounds__(kT) void bf_multi_heavy_vec_kernel(SellView g, int6
 
 // ----------------------------------------------------- sparse iterations
 // On the symmetric pattern of a relabelled SELL, row u of A_T also lists u's
-// out-neighbours.  A k-source iteration with few fallen rows (fallen * alpha <
-// rows) relaxes only the light rows their edges reach -- exactly the rows with
+// out-neighbours.  A k-source iteration whose fallen rows have few out-edges
+// (edges * alpha < entries) relaxes only the light rows their edges reach -- exactly the rows with
 // an in-neighbour that fell -- instead of every light slot, a group of lanes
 // per row over its k contiguous distances; the heavy rows keep their segment
 // blocks.  A device-side switch decides, so the iteration is the same launches
@@ -504,8 +504,9 @@ __global__ void bf_sp_start_kernel(BfSparse s, const int32_t* done) {
   *s.scout = 0ull;
 }
 
-// Number of fallen rows (QUEUE = false), or their queue (QUEUE = true,
-// sparse iterations only).
+// Out-edges of the fallen rows (QUEUE = false: their degrees summed; the
+// pattern is symmetric, so a row's in-degree is its out-degree), or their
+// queue (QUEUE = true, sparse iterations only).
 template <bool QUEUE>
 __global__ __launch_bounds__(kT) void bf_sp_fallen_kernel(BfSparse s, const uint32_t* __restrict__ chp,
                                                           SellView g, const int32_t* done) {
@@ -520,7 +521,16 @@ __global__ __launch_bounds__(kT) void bf_sp_fallen_kernel(BfSparse s, const uint
     bits[i] = w[i] < s.words ? chp[w[i]] : 0u;
     c += __popc(bits[i]);
   }
-  if (QUEUE) {
+  if (!QUEUE) {
+    unsigned long long deg = 0;
+#pragma unroll
+    for (int i = 0; i < kScanWords; ++i)
+      for (uint32_t b = bits[i]; b; b &= b - 1)
+        deg += (unsigned long long)state_row(g, false, nullptr, w[i] * 32 + __ffs(b) - 1).len;
+    block_add(deg, s.scout);
+    return;
+  }
+  {
     unsigned pos = block_claim(c, &s.cnt[0]);
 #pragma unroll
     for (int i = 0; i < kScanWords; ++i)
@@ -529,16 +539,15 @@ __global__ __launch_bounds__(kT) void bf_sp_fallen_kernel(BfSparse s, const uint
         bits[i] &= bits[i] - 1;
         s.queue[pos++] = (int32_t)(w[i] * 32 + b);
       }
-  } else {
-    block_add(c, s.scout);
   }
 }
 
-// Sparse when the fallen rows times alpha are below the rows (their out-edges
-// estimated at the average degree).
+// Sparse when the fallen rows' out-edges times alpha are below the matrix's
+// entries (counting their rows alone, at the average degree, took sparse
+// iterations for fallen sets that held hubs, where the sweep is cheaper).
 __global__ void bf_sp_decide_kernel(BfSparse s, const int32_t* done) {
   if (done && *done) return;
-  const unsigned sp = (double)(*s.scout) * s.alpha < (double)s.ns ? 1u : 0u;
+  const unsigned sp = (double)(*s.scout) * s.alpha < (double)s.nnz ? 1u : 0u;
   s.cnt[3] = sp;
   s.cnt[4] = sp;
 }
@@ -1056,11 +1065,11 @@ int sssp_multi(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_host, int
     hipLaunchKernelGGL(bf_multi_seed_kernel<TD>, dim3(blocks_for(k, kT)), dim3(kT), 0,
                        ctx->stream, (TD*)b0, (const int32_t*)src, k, vinv, c.prev);
     BfSparse sp{};
-    const bool sparse = bf_sparse_ok(A) && ctx->push_alpha > 0.0;
+    const bool sparse = bf_sparse_ok(A) && ctx->sssp_alpha > 0.0;
     if (sparse) {
       void* sb;
       GG_TRY(ws_get(ctx, WS_SSSP1, bf_sparse_bytes(A), &sb));
-      sp = bf_sparse_layout(sb, A, ctx->push_alpha);
+      sp = bf_sparse_layout(sb, A, ctx->sssp_alpha);
       hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(sp.words, kT)), dim3(kT), 0, ctx->stream,
                          sp.mark, sp.words, 0u);
     }

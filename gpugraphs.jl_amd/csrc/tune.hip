@@ -183,6 +183,45 @@ gg_tune_candidate sell_layout(int64_t C, int64_t sigma, int64_t heavy, int64_t s
 
 }  // namespace
 
+// Times `workload` (its first `cols` roots at once) under each value of the
+// context threshold *knob, leaves the fastest on the context and in *best.
+static int tune_threshold(gg_ctx* ctx, double* knob, int workload, int64_t cols,
+                          const gg_matrix* A_T, const int32_t* roots, int64_t k,
+                          const double* alphas, double* ms, int64_t n_alphas, int reps,
+                          double* best_alpha) {
+  if (!A_T || !roots || !alphas || !best_alpha || n_alphas < 1 || k < 1)
+    return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  clear_error();
+  for (int64_t i = 0; i < n_alphas; ++i)
+    if (!(alphas[i] >= 0.0)) return set_error(GG_ERR_ARGUMENT, "alpha %g must be >= 0", alphas[i]);
+  Outputs o;
+  Events ev;
+  GG_TRY(ev.init());
+  const size_t esz = workload == GG_TUNE_MSSSP ? 8 : 4;  // room for 64-bit distances
+  if (hipMalloc(&o.D, esz * std::max<int64_t>(A_T->m * cols, 1)) != hipSuccess)
+    return set_error(GG_ERR_NOMEM, "tuner: output buffer");
+  const double saved = *knob;
+  int64_t bi = -1;
+  double bms = 0.0;
+  for (int64_t i = 0; i < n_alphas; ++i) {
+    *knob = alphas[i];
+    double t = 0.0;
+    const int st = time_workload(ctx, A_T, workload, roots, k, o, reps, ev, &t);
+    if (st != GG_OK) {
+      *knob = saved;
+      return st;
+    }
+    if (ms) ms[i] = t;
+    if (bi < 0 || t < bms) {
+      bi = i;
+      bms = t;
+    }
+  }
+  *knob = alphas[bi];  // the context keeps the best switch
+  *best_alpha = alphas[bi];
+  return GG_OK;
+}
+
 extern "C" {
 
 int gg_tune_default_candidates(const gg_matrix* A, int64_t max_degree, gg_tune_candidate* out,
@@ -293,39 +332,17 @@ int gg_tune_layout(gg_ctx* ctx, const gg_matrix* A, int workload, const int32_t*
 int gg_tune_push_threshold(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* roots, int64_t k,
                            const double* alphas, double* ms, int64_t n_alphas, int reps,
                            double* best_alpha) {
-  if (!ctx || !A_T || !roots || !alphas || !best_alpha || n_alphas < 1 || k < 1)
-    return set_error(GG_ERR_ARGUMENT, "NULL argument");
-  clear_error();
-  Outputs o;
-  Events ev;
-  GG_TRY(ev.init());
-  if (hipMalloc(&o.D, sizeof(int32_t) * std::max<int64_t>(A_T->m * std::min<int64_t>(k, 64), 1)) !=
-      hipSuccess)
-    return set_error(GG_ERR_NOMEM, "tuner: output buffer");
-  const double saved = ctx->push_alpha;
-  int64_t bi = -1;
-  double bms = 0.0;
-  for (int64_t i = 0; i < n_alphas; ++i) {
-    if (!(alphas[i] >= 0.0)) {
-      ctx->push_alpha = saved;
-      return set_error(GG_ERR_ARGUMENT, "alpha %g must be >= 0", alphas[i]);
-    }
-    ctx->push_alpha = alphas[i];
-    double t = 0.0;
-    const int st = time_workload(ctx, A_T, GG_TUNE_MSBFS, roots, k, o, reps, ev, &t);
-    if (st != GG_OK) {
-      ctx->push_alpha = saved;
-      return st;
-    }
-    if (ms) ms[i] = t;
-    if (bi < 0 || t < bms) {
-      bi = i;
-      bms = t;
-    }
-  }
-  ctx->push_alpha = alphas[bi];  // the context keeps the best switch
-  *best_alpha = alphas[bi];
-  return GG_OK;
+  if (!ctx) return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  return tune_threshold(ctx, &ctx->push_alpha, GG_TUNE_MSBFS, std::min<int64_t>(k, 64), A_T, roots,
+                        k, alphas, ms, n_alphas, reps, best_alpha);
+}
+
+int gg_tune_sssp_threshold(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* roots, int64_t k,
+                           const double* alphas, double* ms, int64_t n_alphas, int reps,
+                           double* best_alpha) {
+  if (!ctx) return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  return tune_threshold(ctx, &ctx->sssp_alpha, GG_TUNE_MSSSP, std::min<int64_t>(k, 32), A_T, roots,
+                        k, alphas, ms, n_alphas, reps, best_alpha);
 }
 
 }  // extern "C"

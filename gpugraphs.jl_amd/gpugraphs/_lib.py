@@ -146,6 +146,7 @@ SIGNATURES = {
     "gg_ctx_set_graph_steps": ([_vp, _i32], _i32),
     "gg_ctx_set_csr_lanes": ([_vp, _i32], _i32),
     "gg_ctx_set_push_threshold": ([_vp, _d], _i32),
+    "gg_ctx_set_sssp_threshold": ([_vp, _d], _i32),
     "gg_ctx_last_kernel_ms": ([_vp, C.POINTER(_d)], _i32),
     "gg_ctx_hot_kernel_stats": ([_vp, C.POINTER(_d), C.POINTER(_i64), _i32], _i32),
     "gg_matrix_check": ([_PM], _i32),
@@ -193,6 +194,7 @@ SIGNATURES = {
     "gg_tune_layout": ([_vp, _PM, _i32, _vp, _i64, C.POINTER(TuneCandidate), _i64, _i32,
                         C.POINTER(TuneCandidate)], _i32),
     "gg_tune_push_threshold": ([_vp, _PM, _vp, _i64, _vp, _vp, _i64, _i32, C.POINTER(_d)], _i32),
+    "gg_tune_sssp_threshold": ([_vp, _PM, _vp, _i64, _vp, _vp, _i64, _i32, C.POINTER(_d)], _i32),
     "gg_server_create": ([_i32, C.c_size_t, C.POINTER(_vp)], _i32),
     "gg_server_add_graph": ([_vp, _PM, C.POINTER(_i32)], _i32),
     "gg_server_start": ([_vp], _i32),
@@ -324,6 +326,13 @@ def set_push_threshold(ctx, alpha: float):
     """Multi-source BFS levels run top-down when frontier out-edges * alpha <
     entries (0: never)."""
     check(lib().gg_ctx_set_push_threshold(ctx, float(alpha)))
+
+
+def set_sssp_threshold(ctx, alpha: float):
+    """k-source Bellman-Ford iterations on a symmetric relabelled SELL relax
+    only the rows next to the fallen rows when those rows' out-edges * alpha <
+    entries (0: never)."""
+    check(lib().gg_ctx_set_sssp_threshold(ctx, float(alpha)))
 
 
 def hot_kernel_stats(ctx, reset: bool = True):

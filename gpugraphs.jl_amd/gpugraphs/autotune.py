@@ -5,11 +5,14 @@ Times candidates on the device and keeps the fastest:
     sigma windows or degree relabelling, heavy-row threshold and heavy
     segment length (gg_sell_build);
   - the direction-optimising BFS switch (alpha, beta) of bfs_distances_do;
+  - the top-down switch of the multi-source BFS and the sparse-iteration
+    switch of the k-source Bellman-Ford;
   - the steps per captured graph launch of the level loops.
 The workload decides what is timed: one plus-times SpMV, BFS levels from the
 given roots, a multi-source BFS of up to 64 roots, Bellman-Ford from the first
 root, or Bellman-Ford from up to 32 roots at once.  The layout and push/pull
-searches run in the library (gg_tune_layout, gg_tune_push_threshold: every
+searches run in the library (gg_tune_layout, gg_tune_push_threshold,
+gg_tune_sssp_threshold: every
 candidate is built on the device from the graph's CSR and timed with HIP
 events), so a caller of the C ABI tunes the same way.  Results are cached per
 graph signature (size, entries, degree profile) and workload, in memory and
@@ -189,6 +192,24 @@ class Autotuner:
         a, ms = min(table, key=lambda t: t[1])
         _lib.set_push_threshold(ctx, a)
         return TuneResult("push_threshold", dict(alpha=a), ms, table)
+
+    def tune_sssp_threshold(self, A_T, roots, alphas=(0.0, 16.0, 64.0, 256.0, 1024.0)):
+        """Sparse-iteration switch of the k-source Bellman-Ford
+        (gg_ctx_set_sssp_threshold) with the least time for
+        shortest_path(A_T, roots[:32]), timed natively with
+        gg_tune_sssp_threshold; the context keeps the best."""
+        ctx = _lib.context(self.backend.device)
+        r = np.ascontiguousarray(roots[:32], np.int32)
+        al = np.ascontiguousarray(alphas, np.float64)
+        ms = np.zeros(len(al), np.float64)
+        best = C.c_double()
+        d = A_T.descriptor()
+        _lib.check(_lib.lib().gg_tune_sssp_threshold(ctx, C.byref(d), r.ctypes.data, len(r),
+                                                     al.ctypes.data, ms.ctypes.data, len(al),
+                                                     self.reps, C.byref(best)),
+                   "tune_sssp_threshold")
+        return TuneResult("sssp_threshold", dict(alpha=best.value), float(ms.min()),
+                          list(zip(al.tolist(), ms.tolist())))
 
     def tune_graph_steps(self, fn, steps=(0, 2, 8, 32)):
         """Steps per captured graph launch with the least time for fn()."""

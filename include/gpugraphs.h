@@ -234,11 +234,15 @@ int gg_ctx_set_graph_steps(gg_ctx* ctx, int steps);
 int gg_ctx_set_csr_lanes(gg_ctx* ctx, int lanes);
 /* Direction switch on symmetric patterns (GG_FLAG_SYMMETRIC): a level of
  * gg_msbfs_distances (unit pattern) runs top-down when its frontier's
- * out-edges times alpha are below the matrix entries, and an iteration of
- * gg_shortest_path_multi on a relabelled SELL relaxes only the rows next to
- * the rows that fell when those rows times alpha are below the rows
- * (default 16; 0 = never). */
+ * out-edges times alpha are below the matrix entries, and gg_bfs_packed
+ * expands its first levels top-down while that holds (default 16; 0 =
+ * never). */
 int gg_ctx_set_push_threshold(gg_ctx* ctx, double alpha);
+/* Sparse iterations of gg_shortest_path_multi on a symmetric relabelled
+ * SELL: an iteration relaxes only the rows next to the rows that fell in the
+ * previous one when those rows' out-edges times alpha are below the matrix
+ * entries (default 16; 0 = never). */
+int gg_ctx_set_sssp_threshold(gg_ctx* ctx, double alpha);
 
 /* Host-side consistency check of a descriptor (no device access): the checks
  * every entry point runs before launching. */
@@ -471,6 +475,11 @@ int gg_tune_layout(gg_ctx* ctx, const gg_matrix* A_csr, int workload, const int3
  * (gg_ctx_set_push_threshold; 0 = always bottom-up), writes ms[i] when ms is
  * given, leaves the fastest set on the context and in *best_alpha. */
 int gg_tune_push_threshold(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* roots, int64_t k,
+                           const double* alphas, double* ms, int64_t n_alphas, int reps,
+                           double* best_alpha);
+/* The same for gg_shortest_path_multi (k <= 32 roots; A_T with its weights)
+ * under each sparse-iteration alpha (gg_ctx_set_sssp_threshold). */
+int gg_tune_sssp_threshold(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* roots, int64_t k,
                            const double* alphas, double* ms, int64_t n_alphas, int reps,
                            double* best_alpha);
 

@@ -1,5 +1,5 @@
-"""The autotuner (the library's gg_tune_layout / gg_tune_push_threshold
-behind gpugraphs.autotune) times every candidate layout / switch on the
+"""The autotuner (the library's gg_tune_layout / gg_tune_push_threshold /
+gg_tune_sssp_threshold behind gpugraphs.autotune) times every candidate layout / switch on the
 device, keeps the fastest, caches it per graph signature, and the tuned
 choice computes the same results as the oracle."""
 import numpy as np
@@ -82,3 +82,32 @@ def test_tune_direction_and_graph_steps(env):
     finally:
         _lib.set_graph_steps(_lib.context(0), 0)
         _lib.set_push_threshold(_lib.context(0), 16.0)
+
+
+def test_tune_sssp_threshold_and_every_switch_matches_oracle(env):
+    """gg_tune_sssp_threshold times the k-source Bellman-Ford under each
+    sparse-iteration switch and keeps the fastest on the context; whatever
+    the switch (0 = every iteration dense, 1e-9 = sparse whenever the fallen
+    rows have fewer out-edges than 1e9 times the entries, i.e. always), the
+    distances are the oracle's."""
+    gg, torch, orc = env
+    from gpugraphs import _lib, autotune as AT, generators as GEN
+    R = GEN.rmat(12, seed=9)
+    w = np.random.default_rng(4).integers(1, 100, R.nnz).astype(np.int32)
+    H = GEN.HostCSR(R.m, R.n, R.rowptr, R.colval, w, True)   # symmetric pattern
+    S = H.to_sell(gg.ROCBackend(0), 64, 1, 128, 256, relabel=True)
+    srcs = (np.nonzero(R.degrees())[0][:24] + 1).astype(np.int32)
+    ref = orc.shortest_path_multi(orc.from_scipy_csr(H.to_scipy()), srcs)
+    tuner = AT.Autotuner(gg.ROCBackend(0), reps=1)
+    try:
+        print_case("tune sssp threshold", R.m, len(srcs))
+        res = tuner.tune_sssp_threshold(S, srcs, alphas=(0.0, 16.0, 1e-9))
+        assert res.best["alpha"] in (0.0, 16.0, 1e-9) and len(res.table) == 3
+        assert res.best_ms == min(ms for _, ms in res.table) > 0
+        np.testing.assert_array_equal(gg.shortest_path(S, srcs).cpu().numpy(), ref)
+        for a in (0.0, 1e-9):
+            _lib.set_sssp_threshold(_lib.context(0), a)
+            np.testing.assert_array_equal(gg.shortest_path(S, srcs).cpu().numpy(), ref,
+                                          err_msg=f"sparse-iteration alpha {a}")
+    finally:
+        _lib.set_sssp_threshold(_lib.context(0), 16.0)
