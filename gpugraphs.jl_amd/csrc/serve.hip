@@ -581,13 +581,18 @@ static int launch_step(gg_server* s, Engine& e) {
 // an engine that steps whenever one slot is busy spends a full step on one
 // query.  While other engines have steps in flight, an engine whose slots are
 // not full waits pacing x (its mean step time on the device) x (its free
-// fraction) after its last launch, so that arrivals join its next step instead of starting their
-// own; a full engine, or one alone on the device, steps at once.
-static bool paced(gg_server* s, const Engine& e, int inflight) {
-  if (s->pacing <= 0.0 || inflight == 0) return false;
+// fraction) x (its step time / the longest engine step time) after its last
+// launch, so that arrivals join its next step instead of starting their own.
+// The last factor keeps the wait for the engines whose steps are expensive
+// (what pacing saves is their device time) and drops it for cheap ones, whose
+// steps cost little whoever shares them; a full engine, or one alone on the
+// device, steps at once.
+static bool paced(gg_server* s, const Engine& e, int inflight, double max_step_s) {
+  if (s->pacing <= 0.0 || inflight == 0 || max_step_s <= 0.0) return false;
   const int busy = __builtin_popcountll(e.active);
   if (busy >= e.slots) return false;
-  const double wait = s->pacing * e.step_s * (1.0 - (double)busy / e.slots);
+  const double wait =
+      s->pacing * e.step_s * (1.0 - (double)busy / e.slots) * (e.step_s / max_step_s);
   return now_s() - e.t_launch < wait;
 }
 
@@ -613,7 +618,11 @@ static void worker_main(gg_server* s) {
     }
     bool busy = false, progress = false;
     int inflight = 0;
-    for (auto& ep : s->engines) inflight += ep->inflight ? 1 : 0;
+    double max_step_s = 0.0;
+    for (auto& ep : s->engines) {
+      inflight += ep->inflight ? 1 : 0;
+      max_step_s = std::max(max_step_s, ep->step_s);
+    }
     for (auto& ep : s->engines) {
       Engine& e = *ep;
       if (e.inflight) {
@@ -636,7 +645,7 @@ static void worker_main(gg_server* s) {
         progress = true;
       }
       admit(s, e);
-      if (e.active && !paced(s, e, inflight)) {
+      if (e.active && !paced(s, e, inflight, max_step_s)) {
         int st = launch_step(s, e);
         if (st != GG_OK) {
           // the step did not run: every query of the engine's slots fails

@@ -514,8 +514,10 @@ int gg_server_add_graph(gg_server* s, const gg_matrix* A_T, int* graph_id);
 /* Step pacing (no counterpart in the reference): while other engines have
  * steps in flight, an engine whose slots are not all busy waits
  * factor x (its mean step time on the device, start to end of a step on its
- * stream) x (its free slot fraction) after its last step before the next one, so that arriving queries share steps; 0 steps
- * whenever a slot is busy.  Default 24. */
+ * stream) x (its free slot fraction) x (its step time / the longest engine
+ * step time) after its last step before the next one, so that arriving
+ * queries share the steps of expensive engines while cheap ones step at
+ * once; 0 steps whenever a slot is busy.  Default 24. */
 int gg_server_set_pacing(gg_server* s, double factor);
 int gg_server_start(gg_server* s);
 int gg_server_submit(gg_server* s, int graph_id, int kind, int64_t source, int64_t* request_id);
