@@ -732,6 +732,32 @@ def main():
                                          stored_per_nnz=stored / nnz, gather_only_ms=s_gather)
             log(f"SELL-64 sigma 4096 SpMV: {s_ms * 1e3:.1f} us/SpMV, kernel {s_kms * 1e3:.1f} us "
                 f"({B_spmv / s_kms / 1e6:.0f} GB/s), stored/nnz {stored / nnz:.3f}, rel err {s_err:.1e}")
+        # the CPU-runnable configuration: plain CSR SpMV, n = 65,536, exactly 16
+        # distinct uniform columns per row, fp32, 100 repetitions, each result
+        # against the FP64 product (the matrix and x of the CPU leg)
+        U = G.uniform_rows(65536, 16, seed=seed, dtype=np.float32)
+        Au = U.to_csr(R0)
+        xu_h = np.random.default_rng(2).random(U.n, dtype=np.float32)
+        xu, yu = torch.from_numpy(xu_h).to(dev), torch.zeros(U.m, dtype=torch.float32, device=dev)
+        ref_u = U.to_scipy().astype(np.float64) @ xu_h.astype(np.float64)
+        for _ in range(3):
+            gg.gpu_spmv_(yu, Au, xu)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(100):
+            gg.gpu_spmv_(yu, Au, xu)
+        e1.record()
+        torch.cuda.synchronize()
+        u_ms = e0.elapsed_time(e1) / 100
+        u_err = float(np.max(np.abs(yu.cpu().numpy() - ref_u) / np.maximum(np.abs(ref_u), 1e-30)))
+        assert u_err <= 1e-5, f"CSR n=65536 SpMV parity failed: max rel err {u_err}"
+        B_u = U.nnz * 8 + 12 * U.m
+        res["csr65536"] = dict(us_per_spmv=round(u_ms * 1e3, 2), x100_ms=round(u_ms * 100, 3),
+                               gbs=round(B_u / u_ms / 1e6, 1), bytes=B_u, max_rel_err_vs_fp64=u_err)
+        log(f"CSR n=65536 x16 SpMV: {u_ms * 1e3:.1f} us/SpMV ({B_u / u_ms / 1e6:.0f} GB/s "
+            f"algorithmic; 8.5 MB, cache-resident), rel err {u_err:.1e}")
+        del Au, xu, yu
 
     # ---------------------------------------------------------------- SpMM
     if args.only in ("", "spmm") and args.spmm_k > 0:
@@ -989,6 +1015,7 @@ def main():
         "spmv_gbs": round(spmv["gbs"], 1) if spmv else None,
         "spmv_us": round(spmv["ms"] * 1e3, 1) if spmv else None,
         "spmv_roofline": spmv_roof,
+        "csr65536": res.get("csr65536"),
         "spmm": {k: (round(v, 4) if isinstance(v, float) else v)
                  for k, v in res["spmm"].items()} if "spmm" in res else None,
         "roofline": roofline,
