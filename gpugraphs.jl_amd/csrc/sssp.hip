@@ -110,12 +110,16 @@ __global__ __launch_bounds__(kT) void bf_sell_kernel(SellView g, int64_t n_sell_
         if (bit_get(chp, col - 1)) acc = jmin(acc, sadd(TD(val[i]), d[col - 1]));
       }
       const int64_t rid = g.internal ? slot : row;
-      const TD dv = d[rid];
-      const TD nv = jmin(dv, acc);
-      if (nv < dv) {
-        d[rid] = nv;
-        if (!g.internal) bit_claim(chn, rid);
-        ch = true;
+      // no candidate below the unreached value: the row cannot fall, and its
+      // own distance is not read
+      if (acc < Lim<TD>::max()) {
+        const TD dv = d[rid];
+        const TD nv = jmin(dv, acc);
+        if (nv < dv) {
+          d[rid] = nv;
+          if (!g.internal) bit_claim(chn, rid);
+          ch = true;
+        }
       }
     }
     if (g.internal) wave_set_bits(chn, slot & ~(int64_t)63, ch);  // the wavefront's two words
@@ -234,7 +238,7 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
           acc = multi_row_min<TA, TD>(g, val, base, len, has_len, chp, d, k, ld, j);
         }
       }
-      if (rid >= 0 && j < k) {
+      if (rid >= 0 && j < k && acc < Lim<TD>::max()) {
         const TD dv = d[rid * ld + j];
         const TD nv = jmin(dv, acc);
         if (nv < dv) {
@@ -367,14 +371,21 @@ __global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_
 #pragma unroll
           for (int t = 0; t < V; ++t) acc[t] = jmin(acc[t], sadd(w, dn[t]));
         }
-        VT own = *(const VT*)&d[rid * ld + j0];
+        // the own distances are read only when some candidate is below the
+        // unreached value (otherwise none of the lane's columns can fall)
+        bool cand = false;
 #pragma unroll
-        for (int t = 0; t < V; ++t)
-          if (acc[t] < own[t]) {
-            own[t] = acc[t];
-            chbits |= 1u << t;
-          }
-        if (chbits) *(VT*)&d[rid * ld + j0] = own;
+        for (int t = 0; t < V; ++t) cand |= acc[t] < Lim<TD>::max();
+        if (cand) {
+          VT own = *(const VT*)&d[rid * ld + j0];
+#pragma unroll
+          for (int t = 0; t < V; ++t)
+            if (acc[t] < own[t]) {
+              own[t] = acc[t];
+              chbits |= 1u << t;
+            }
+          if (chbits) *(VT*)&d[rid * ld + j0] = own;
+        }
       }
     }
   }
@@ -631,11 +642,13 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_multi_kernel(BfSparse s, SellVi
           const int32_t c = r.col[e] - 1;
           if (bit_get(chp, c)) acc = jmin(acc, sadd(TD(val[e]), d[(int64_t)c * ld + j]));
         }
-        const TD dv = d[v * ld + j];
-        const TD nv = jmin(dv, acc);
-        if (nv < dv) {
-          d[v * ld + j] = nv;
-          ch = true;
+        if (acc < Lim<TD>::max()) {
+          const TD dv = d[v * ld + j];
+          const TD nv = jmin(dv, acc);
+          if (nv < dv) {
+            d[v * ld + j] = nv;
+            ch = true;
+          }
         }
       }
     }
@@ -718,14 +731,19 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_vec_kernel(BfSparse s, SellView
 #pragma unroll
           for (int t = 0; t < V; ++t) acc[t] = jmin(acc[t], sadd(w, dn[t]));
         }
-        VT own = *(const VT*)&d[v * ld + j0];
+        bool cand = false;
 #pragma unroll
-        for (int t = 0; t < V; ++t)
-          if (acc[t] < own[t]) {
-            own[t] = acc[t];
-            chbits |= 1u << t;
-          }
-        if (chbits) *(VT*)&d[v * ld + j0] = own;
+        for (int t = 0; t < V; ++t) cand |= acc[t] < Lim<TD>::max();
+        if (cand) {
+          VT own = *(const VT*)&d[v * ld + j0];
+#pragma unroll
+          for (int t = 0; t < V; ++t)
+            if (acc[t] < own[t]) {
+              own[t] = acc[t];
+              chbits |= 1u << t;
+            }
+          if (chbits) *(VT*)&d[v * ld + j0] = own;
+        }
       }
     }
     const bool ch = chbits != 0;
