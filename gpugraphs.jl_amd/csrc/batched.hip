@@ -284,10 +284,19 @@ __global__ __launch_bounds__(kM) void msbfs_csr_kernel(const int32_t* __restrict
 
 // One thread per source (several sources may share a vertex: atomic ORs);
 // distances row-major m x k, bytes when dist8 is given.
+// With ctl (captured levels) thread 0 also starts the level counters: level
+// 1, not done, window base 0, found 0.
 __global__ void msbfs_seed_kernel(const int32_t* sources, int64_t k, int64_t k8, int W,
                                   const int32_t* vinv, uint64_t* front, uint32_t* front_any,
-                                  uint64_t* seen, int32_t* dist, uint8_t* dist8) {
+                                  uint64_t* seen, int32_t* dist, uint8_t* dist8,
+                                  unsigned int* found, int32_t* ctl) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ctl && s == 0) {
+    *found = 0u;
+    ctl[0] = 1;
+    ctl[1] = 0;
+    ctl[2] = 0;
+  }
   if (s >= k) return;
   const int64_t v = sources[s] - 1;
   const int64_t r = vinv ? (int64_t)vinv[v] - 1 : v;  // state index
@@ -369,15 +378,37 @@ __global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
 }
 __global__ void ms_zero_kernel(unsigned int* p) { *p = 0u; }
 
-// Captured levels: ctl = {level, done, window base}.  Begin: level 1, not
-// done.  End of a level: done (1) when it found nothing, else the next level;
-// done = 2 when the next level would reach `cap` (byte levels full; 0 = no
-// cap).
-__global__ void ms_ctl_begin_kernel(unsigned int* found, int32_t* ctl) {
-  *found = 0u;
-  ctl[0] = 1;
-  ctl[1] = 0;
-  ctl[2] = 0;
+// Captured levels: ctl = {level, done, window base}, started by the seed
+// kernel at level 1, not done.  End of a level: done (1) when it found
+// nothing, else the next level; done = 2 when the next level would reach
+// `cap` (byte levels full; 0 = no cap).
+
+// Up to 8 ranges of 32-bit words zeroed by one launch (grid-stride, 16-byte
+// stores where a range allows).
+struct ZeroRanges {
+  uint32_t* p[8];
+  int64_t n[8];
+  int count;
+  void add(void* ptr, int64_t words) {
+    p[count] = (uint32_t*)ptr;
+    n[count] = words;
+    ++count;
+  }
+};
+__global__ __launch_bounds__(256) void zero_ranges_kernel(ZeroRanges z) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x, stride = (int64_t)gridDim.x * 256;
+  for (int r = 0; r < z.count; ++r) {
+    uint32_t* p = z.p[r];
+    const int64_t n = z.n[r];
+    int64_t head = 0;
+    if (((uintptr_t)p & 15) == 0) {
+      typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+      const u4 zero = {0u, 0u, 0u, 0u};
+      head = n & ~(int64_t)3;
+      for (int64_t i = t; i < head / 4; i += stride) ((u4*)p)[i] = zero;
+    }
+    for (int64_t i = head + t; i < n; i += stride) p[i] = 0u;
+  }
 }
 
 constexpr int kMsRingHost = 8;  // ring slots of a host-stepped traversal
@@ -501,55 +532,6 @@ __global__ void ms_level_end_kernel(unsigned int* found, int32_t* ctl, int32_t c
 }
 
 // ------------------------------------------------------- top-down step
-// Frontier words per thread of ms_frontier_kernel: one, since a dense
-// frontier's out-edge count loops over every set bit of a word.
-constexpr int kMsScanWords = 1;
-
-__global__ void ms_push_start_kernel(MsArgs a, MsPush p) {
-  if (a.ctl && a.ctl[1]) return;
-  p.cnt[0] = p.cnt[1] = p.cnt[2] = 0u;
-  *p.scout = 0ull;
-}
-
-// The frontier's out-edges (QUEUE = false), or, on a push level, the queue of
-// its state ids (QUEUE = true): one pass over front_any each.
-template <bool QUEUE>
-__global__ __launch_bounds__(256) void ms_frontier_kernel(MsArgs a, MsPush p, SellView g, int csr,
-                                                          const int32_t* __restrict__ rowptr) {
-  if (a.ctl && a.ctl[1]) return;
-  if (QUEUE && !p.cnt[3]) return;
-  int64_t w[kMsScanWords];
-  uint32_t bits[kMsScanWords];
-#pragma unroll
-  for (int i = 0; i < kMsScanWords; ++i) {
-    w[i] = (int64_t)blockIdx.x * (256 * kMsScanWords) + i * 256 + threadIdx.x;
-    bits[i] = w[i] < p.words ? a.front_any[w[i]] : 0u;
-  }
-  if (QUEUE) {
-    unsigned c = 0;
-#pragma unroll
-    for (int i = 0; i < kMsScanWords; ++i) c += __popc(bits[i]);
-    unsigned pos = block_claim(c, &p.cnt[0]);
-#pragma unroll
-    for (int i = 0; i < kMsScanWords; ++i)
-      while (bits[i]) {
-        const int b = __ffs(bits[i]) - 1;
-        bits[i] &= bits[i] - 1;
-        p.queue[pos++] = (int32_t)(w[i] * 32 + b);
-      }
-  } else {
-    unsigned long long deg = 0;
-#pragma unroll
-    for (int i = 0; i < kMsScanWords; ++i)
-      while (bits[i]) {
-        const int b = __ffs(bits[i]) - 1;
-        bits[i] &= bits[i] - 1;
-        deg += (unsigned long long)state_row(g, csr, rowptr, w[i] * 32 + b).len;
-      }
-    block_add(deg, p.scout);
-  }
-}
-
 // Push when the frontier's out-edges are below 1/alpha of the matrix.
 // A pull level whose front is complete (every word written by the level
 // before) and whose frontier has more than 1/kDenseFrac of the edges skips
@@ -559,29 +541,65 @@ __global__ __launch_bounds__(256) void ms_frontier_kernel(MsArgs a, MsPush p, Se
 constexpr double kDenseFrac = 8.0;
 constexpr double kWidePush = 64.0;
 
-__global__ void ms_decide_kernel(MsArgs a, MsPush p) {
+// The first kernel of a level with a direction switch (one thread per word of
+// front_any): clears the level's next_any word and sums the out-degrees of its
+// frontier rows.  Each block adds (1 << 40) + its sum to p.scout with one
+// atomic, so bits 0..39 sum the degrees (at most the entries) and bits 40..63
+// count the blocks; the block whose add completes the count holds the total
+// (no fence needed), decides the direction and resets the level's counters
+// (queue lengths and p.scout) for the kernels after it.  p.scout is zero when
+// a level begins (zeroed with the push buffers, and reset here).
+constexpr int kMsSumBits = 40;
+
+__global__ __launch_bounds__(256) void ms_begin_kernel(MsArgs a, MsPush p, SellView g, int csr,
+                                                       const int32_t* __restrict__ rowptr) {
   if (a.ctl && a.ctl[1]) return;
-  const double e = (double)(*p.scout);
+  __shared__ unsigned long long ws[4];
+  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  uint32_t bits = 0u;
+  if (w < p.words) {
+    bits = a.front_any[w];
+    a.next_any[w] = 0u;
+  }
+  unsigned long long deg = 0;
+  while (bits) {
+    const int b = __ffs(bits) - 1;
+    bits &= bits - 1;
+    deg += (unsigned long long)state_row(g, csr, rowptr, w * 32 + b).len;
+  }
+  for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = deg;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const unsigned long long t = ws[0] + ws[1] + ws[2] + ws[3];
+  const unsigned long long old = atomicAdd(p.scout, (1ull << kMsSumBits) + t);
+  if ((old >> kMsSumBits) != (unsigned long long)gridDim.x - 1) return;
+  const double e = (double)((old & ((1ull << kMsSumBits) - 1)) + t);
   const unsigned push = p.alpha > 0.0 && e * p.alpha < (double)p.nnz ? 1u : 0u;
   p.cnt[3] = push;
   p.cnt[4] = push;  // read by the pull kernels as a.pull_off
   p.cnt[5] = !push && p.cnt[6] && e * kDenseFrac > (double)p.nnz ? 1u : 0u;
   p.cnt[6] = !push || e * kWidePush > (double)p.nnz;
+  p.cnt[0] = p.cnt[1] = p.cnt[2] = 0u;
+  *p.scout = 0ull;
 }
 
-// A wide push level clears next before the push writes the touched rows.
-__global__ __launch_bounds__(256) void ms_clear_next_kernel(MsArgs a, MsPush p, int64_t words) {
+// On a push level: the queue of the frontier's state ids (one pass over
+// front_any), and for a wide push the clearing of next (the push then writes
+// the rows it touches).
+__global__ __launch_bounds__(256) void ms_queue_kernel(MsArgs a, MsPush p, int64_t next_words) {
   if (a.ctl && a.ctl[1]) return;
-  if (!p.cnt[3] || !p.cnt[6]) return;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < words;
-       i += (int64_t)gridDim.x * 256)
-    a.next[i] = 0ull;
-}
-
-// Before a traversal's first level: the front is not known to be complete.
-__global__ void ms_push_reset_kernel(MsPush p) {
-  p.cnt[5] = 0u;
-  p.cnt[6] = 0u;
+  if (!p.cnt[3]) return;
+  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  uint32_t bits = w < p.words ? a.front_any[w] : 0u;
+  unsigned pos = block_claim((unsigned)__popc(bits), &p.cnt[0]);
+  while (bits) {
+    const int b = __ffs(bits) - 1;
+    bits &= bits - 1;
+    p.queue[pos++] = (int32_t)(w * 32 + b);
+  }
+  if (p.cnt[6])
+    for (int64_t i = w; i < next_words; i += (int64_t)gridDim.x * 256) a.next[i] = 0ull;
 }
 
 // Edge u -> v: OR u's frontier bits that v has not seen into acc[v]; the
@@ -696,15 +714,14 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a_in, int 
     a.dense = (const int32_t*)&push->cnt[5];
     const SellView g = sell_view(A);
     const int csr = A->format == GG_CSR;
-    hipLaunchKernelGGL(ms_push_start_kernel, dim3(1), dim3(1), 0, stream, a, *push);
-    hipLaunchKernelGGL(ms_frontier_kernel<false>, dim3(blocks_for(push->words, 256 * kMsScanWords)), dim3(256), 0,
-                       stream, a, *push, g, csr, A->ptr);
-    hipLaunchKernelGGL(ms_decide_kernel, dim3(1), dim3(1), 0, stream, a, *push);
-    hipLaunchKernelGGL(ms_frontier_kernel<true>, dim3(blocks_for(push->words, 256 * kMsScanWords)), dim3(256), 0,
-                       stream, a, *push, g, csr, A->ptr);
-    const int64_t nw = state_size(A) * Wc;
-    hipLaunchKernelGGL(ms_clear_next_kernel, dim3((unsigned)std::min<int64_t>(blocks_for(nw, 256), 4096)),
-                       dim3(256), 0, stream, a, *push, nw);
+    const unsigned nbw = (unsigned)blocks_for(push->words, 256);
+    hipLaunchKernelGGL(ms_begin_kernel, dim3(nbw), dim3(256), 0, stream, a, *push, g, csr, A->ptr);
+    hipLaunchKernelGGL(ms_queue_kernel, dim3(nbw), dim3(256), 0, stream, a, *push,
+                       state_size(A) * Wc);
+  } else {
+    const int64_t words = (state_size(A) + 31) / 32;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks_for(words, 256)), dim3(256), 0, stream,
+                       a.next_any, words, 0u);
   }
   int st = with_type(A->val_dtype, [&](auto ta) -> int {
     typedef typename decltype(ta)::type TA;
@@ -819,8 +836,10 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
   // other data at those places).
   const size_t hb = ms_heavy_bytes(A, Wc);
   GG_TRY(ws_get(ctx, WS_MSHEAVY, hb, &hv));
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(hb / 4, kM)), dim3(kM), 0, ctx->stream,
-                     (uint32_t*)hv, (int64_t)(hb / 4), 0u);
+  // one zeroing launch: the heavy and push accumulators, the push counters,
+  // the first front (ring slot 0), seen and the first front_any
+  ZeroRanges z{};
+  z.add(hv, hb / 4);
   MsPush push{};
   const MsPush* pp = nullptr;
   if (ms_push_ok(A)) {
@@ -828,19 +847,17 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
     GG_TRY(ws_get(ctx, WS_MSPUSH, ms_push_bytes(A, Wc), &pb));
     push = ms_push_layout(pb, A, Wc);
     push.alpha = ctx->push_alpha;
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(ns * Wc, kM)), dim3(kM), 0, ctx->stream,
-                       (uint64_t*)push.acc, ns * Wc, 0ull);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(push.words, kM)), dim3(kM), 0, ctx->stream,
-                       push.touched, push.words, 0u);
-    hipLaunchKernelGGL(ms_push_reset_kernel, dim3(1), dim3(1), 0, ctx->stream, push);
+    z.add(push.acc, 2 * ns * Wc);
+    z.add(push.touched, push.words);
+    z.add(push.scout, 2);
+    z.add(push.cnt, 8);
     pp = &push;
   }
-  hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(ns * Wc, kM)), dim3(kM), 0, ctx->stream,
-                     (uint64_t*)f, ns * Wc, 0ull);
-  hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(ns * Wc, kM)), dim3(kM), 0, ctx->stream,
-                     (uint64_t*)sn, ns * Wc, 0ull);
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, ctx->stream,
-                     (uint32_t*)fa, words, 0u);
+  z.add(f, 2 * ns * Wc);
+  z.add(sn, 2 * ns * Wc);
+  z.add(fa, words);
+  hipLaunchKernelGGL(zero_ranges_kernel, dim3((unsigned)std::max(ctx->num_cus, 1) * 8), dim3(256),
+                     0, ctx->stream, z);
   if (dist8) {
     // no fill: the last flush writes every byte
   } else {
@@ -848,9 +865,10 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
                        dist, m * k, (int32_t)2147483647);
   }
   const int32_t* vinv = (A->flags & GG_FLAG_INTERNAL_IDS) ? A->vinv : nullptr;
+  int32_t* ctl = U > 0 ? (int32_t*)cnt + 4 : nullptr;  // captured levels: {level, done, window}
   hipLaunchKernelGGL(msbfs_seed_kernel, dim3(blocks_for(k, kM)), dim3(kM), 0, ctx->stream,
                      sources_dev, k, k8, Wc, vinv, (uint64_t*)f, (uint32_t*)fa, (uint64_t*)sn,
-                     dist, dist8);
+                     dist, dist8, (unsigned*)cnt, ctl);
   GG_HIP(hipGetLastError());
   MsArgs a{};
   a.m = m;
@@ -885,9 +903,8 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
   };
   int32_t level = 0;
   if (U > 0) {
-    // U levels per graph launch; step i runs level (launch) * U + i + 1
-    int32_t* ctl = (int32_t*)cnt + 4;
-    hipLaunchKernelGGL(ms_ctl_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, (unsigned*)cnt, ctl);
+    // U levels per graph launch; step i runs level (launch) * U + i + 1 (the
+    // seed kernel started the counters)
     a.ctl = ctl;
     std::vector<MsArgs> ab(U, a);
     for (int i = 0; i < U; ++i) {
@@ -904,8 +921,6 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
     auto steps = [&](hipStream_t st) -> int {
       for (int i = 0; i < U; ++i) {
         const MsArgs& x = ab[i];
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, st,
-                           x.next_any, words, 0u);
         GG_TRY(msbfs_level(st, A, x, Wc, pp, ctx->num_cus));
         hipLaunchKernelGGL(ms_level_end_kernel, dim3(1), dim3(1), 0, st, (unsigned*)cnt, ctl,
                            dist8 ? 255 : 0);
@@ -939,8 +954,6 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
       a.next = mask_of(level);
       a.next_any = any_of(level);
       a.level = level;
-      hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(words, kM)), dim3(kM), 0, ctx->stream,
-                         a.next_any, words, 0u);
       hipLaunchKernelGGL(ms_zero_kernel, dim3(1), dim3(1), 0, ctx->stream, (unsigned*)cnt);
       {
         HotTimer ht(ctx);

@@ -55,7 +55,7 @@ struct MsPush {
   unsigned int* cnt;        // [0] qlen, [1] tlen, [2] nbig, [3] mode (1 push), [4] = [3],
                             // [5] dense pull (MsArgs::dense), [6] every word of this
                             // level's next will be written (a pull, or a wide push)
-  unsigned long long* scout;  // out-edges of the frontier
+  unsigned long long* scout;  // a level's first kernel: (blocks << 40) + frontier out-edges; else 0
   int64_t words, nnz, big_cap;
   double alpha;             // push when frontier out-edges * alpha < nnz
 };
@@ -100,8 +100,10 @@ inline size_t ms_heavy_bytes(const gg_matrix* A, int W) {
 }
 
 // One or-and level over A_T with W = Wc words per vertex (1, 2, 4 or 8).
-// With push (ms_push_ok(A)), the level first decides on the device between
-// the pull kernel and the top-down step; `ctx_cus` sizes the push grids.
+// The level clears a.next_any first.  With push (ms_push_ok(A)), it decides
+// on the device between the pull kernel and the top-down step; the push
+// buffers' counters (scout, cnt) must be zero before a traversal's first
+// level (each level leaves them so); `ctx_cus` sizes the push grids.
 int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a, int Wc,
                 const MsPush* push = nullptr, int ctx_cus = 256);
 

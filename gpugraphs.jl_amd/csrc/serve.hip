@@ -509,7 +509,6 @@ static int launch_step(gg_server* s, Engine& e) {
   GG_HIP(hipEventRecord(e.step_start, e.stream));
   hipLaunchKernelGGL(serve_zero_kernel, dim3(1), dim3(1), 0, e.stream, e.cnt_d, e.mask_d);
   if (e.kind == GG_QUERY_BFS) {
-    const int64_t words = (ns + 31) / 32;
     if (sl.n) {
       hipLaunchKernelGGL(serve_bfs_clear_kernel, dim3(blocks_for(ns, 256)), dim3(256), 0,
                          e.stream, e.seen, e.front, ns, seedmask);
@@ -518,9 +517,7 @@ static int launch_step(gg_server* s, Engine& e) {
       hipLaunchKernelGGL(serve_bfs_seed_kernel, dim3(1), dim3(1), 0, e.stream, sl, e.vinv,
                          e.front, e.seen, e.fany, e.start, e.level);
     }
-    hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(words, 256)), dim3(256), 0,
-                       e.stream, e.nany, words, 0u);
-    MsArgs a{};
+    MsArgs a{};  // msbfs_level clears next_any
     a.m = m;
     a.k = 64;
     a.layout = GG_COL_MAJOR;
