@@ -13,6 +13,8 @@
 //
 // dist[:, s] equals the single-source BFS levels from sources[s]: the
 // bit-parallel traversal is k independent level-synchronous BFS runs.
+#include <cstring>
+
 #include "batched.hpp"
 
 namespace gg {
@@ -412,6 +414,7 @@ __global__ __launch_bounds__(256) void zero_ranges_kernel(ZeroRanges z) {
 }
 
 constexpr int kMsRingHost = 8;  // ring slots of a host-stepped traversal
+constexpr int64_t kMsPinnedSources = 512;  // sources staged in ctx->host_pinned (2 KiB)
 constexpr int kMsRingMax = 16;  // most ring slots of a captured one
 
 // Byte t of the result is 0xff when bit t of b is set, else 0.
@@ -932,10 +935,8 @@ static int msbfs_once(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_de
       GG_HIP(hipGetLastError());
       return GG_OK;
     };
-    GG_TRY(graph_loop(ctx, key, ctl + 1, m / U + 3, steps, nullptr));
-    GG_HIP(hipMemcpyAsync(ctx->host_pinned, ctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    GG_HIP(hipStreamSynchronize(ctx->stream));
+    // the loop's last check leaves ctl[0..1] (level, done) in host_pinned
+    GG_TRY(graph_loop(ctx, key, ctl + 1, m / U + 3, steps, nullptr, ctl, 2));
     const int32_t* hctl = (const int32_t*)ctx->host_pinned;
     if (hctl[1] == 2) {
       *overflow = true;
@@ -1016,7 +1017,16 @@ extern "C" int gg_msbfs_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32
   timer_begin(ctx);
   void* src;
   GG_TRY(ws_get(ctx, WS_MS3, sizeof(int32_t) * k, &src));
-  GG_HIP(hipMemcpyAsync(src, sources, sizeof(int32_t) * k, hipMemcpyHostToDevice, ctx->stream));
+  if (k <= kMsPinnedSources) {
+    // staged in the pinned page's upper half (its lower half takes the
+    // readbacks): an asynchronous copy; the call synchronises before it
+    // returns, so the page is free again for the next call
+    int32_t* stage = (int32_t*)((char*)ctx->host_pinned + 2048);
+    std::memcpy(stage, sources, sizeof(int32_t) * k);
+    GG_HIP(hipMemcpyAsync(src, stage, sizeof(int32_t) * k, hipMemcpyHostToDevice, ctx->stream));
+  } else {
+    GG_HIP(hipMemcpyAsync(src, sources, sizeof(int32_t) * k, hipMemcpyHostToDevice, ctx->stream));
+  }
   // column-major m x k: batches of <= 512 sources write disjoint column blocks
   int64_t maxlev = 0;
   for (int64_t b0 = 0; b0 < k; b0 += 512) {

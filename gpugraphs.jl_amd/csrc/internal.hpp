@@ -162,9 +162,12 @@ template <class T> inline void sig_add(std::vector<uint64_t>& key, const T& v) {
 // context); the graph is launched on the context stream until the device
 // flag *done becomes nonzero, at most max_launches times.  The steps must be
 // kernels only, and each must do nothing once *done is set.
+// With readback, each check copies readback[0 .. readback_words) (which holds
+// done) into ctx->host_pinned, so the caller finds those counters there after
+// the loop without another round trip.
 int graph_loop(gg_ctx* ctx, const std::vector<uint64_t>& key, const int32_t* done,
                int64_t max_launches, const std::function<int(hipStream_t)>& enqueue,
-               int64_t* launches);
+               int64_t* launches, const int32_t* readback = nullptr, int readback_words = 0);
 
 // Internal launchers (used by the algorithms; no argument checks).
 // rows (CSR only): a 1-based row list of nrows entries instead of r0 + q.

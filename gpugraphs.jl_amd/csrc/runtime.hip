@@ -54,7 +54,16 @@ void timer_end(gg_ctx* ctx) {
 
 int graph_loop(gg_ctx* ctx, const std::vector<uint64_t>& key, const int32_t* done,
                int64_t max_launches, const std::function<int(hipStream_t)>& enqueue,
-               int64_t* launches) {
+               int64_t* launches, const int32_t* readback, int readback_words) {
+  int64_t di = 0;  // done's index in the readback
+  if (readback) {
+    di = done - readback;
+    if (di < 0 || di >= readback_words || readback_words * 4 > 1024)
+      return set_error(GG_ERR_ARGUMENT, "readback range must hold the done flag");
+  } else {
+    readback = done;
+    readback_words = 1;
+  }
   hipGraphExec_t exec = nullptr;
   auto it = ctx->graphs.find(key);
   if (it != ctx->graphs.end()) {
@@ -81,10 +90,10 @@ int graph_loop(gg_ctx* ctx, const std::vector<uint64_t>& key, const int32_t* don
   }
   for (int64_t l = 1;; ++l) {
     GG_HIP(hipGraphLaunch(exec, ctx->stream));
-    GG_HIP(hipMemcpyAsync(ctx->host_pinned, done, sizeof(int32_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
+    GG_HIP(hipMemcpyAsync(ctx->host_pinned, readback, sizeof(int32_t) * readback_words,
+                          hipMemcpyDeviceToHost, ctx->stream));
     GG_HIP(hipStreamSynchronize(ctx->stream));
-    if (*(const int32_t*)ctx->host_pinned) {
+    if (((const int32_t*)ctx->host_pinned)[di]) {
       if (launches) *launches = l;
       return GG_OK;
     }
