@@ -180,6 +180,23 @@ def packed_leg(args, np, torch, gg, G, R0, seed):
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t0)
     t_pack = float(np.mean(lat))
+    # the same pack as ONE direction-optimising traversal seeded with every
+    # graph's source, pulling over a relabelled SELL of the pack (the BFS
+    # layout of the single-graph legs) and pushing over the packed CSR
+    A_s = gg.SparseGPUMatrixSELL.build_on_device(A, 64, 1, args.heavy, args.bfs_seg, True, True)
+    D_do, lv_do = gg.bfs_packed_do(A_s, off, roots, A_out=A)
+    torch.cuda.synchronize()
+    assert torch.equal(D_do, D) and np.array_equal(lv_do, levels), \
+        "direction-optimising packed BFS differs from the persistent launch"
+    del D_do
+    lat_do = []
+    for _ in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        D_do, lv_do = gg.bfs_packed_do(A_s, off, roots, A_out=A)
+        torch.cuda.synchronize()
+        lat_do.append(time.perf_counter() - t0)
+    del D_do, A_s
+    t_do = float(np.mean(lat_do))
     sample = rng.choice(len(graphs), min(args.packed_plain, len(graphs)), replace=False)
     mats = [(i, graphs[i].to_csr(R0)) for i in sample]
     gg.bfs_distances(mats[0][1], int(roots[mats[0][0]]))
@@ -194,14 +211,17 @@ def packed_leg(args, np, torch, gg, G, R0, seed):
     assert same, "packed BFS differs from the per-graph BFS"
     q_pack = len(graphs) / t_pack
     q_plain = len(mats) / sum(plain_lat)
-    log(f"packed BFS: {len(graphs)} graphs in {t_pack * 1e3:.2f} ms = {q_pack:.0f} q/s; "
-        f"one graph per call {q_plain:.0f} q/s (p99 {pct(plain_lat, 99) * 1e3:.2f} ms); "
-        f"{q_pack / q_plain:.1f}x; max levels {int(levels.max())}")
+    q_do = len(graphs) / t_do
+    log(f"packed BFS: {len(graphs)} graphs in {t_pack * 1e3:.2f} ms = {q_pack:.0f} q/s "
+        f"(persistent launch), {t_do * 1e3:.2f} ms = {q_do:.0f} q/s (one direction-optimising "
+        f"traversal); one graph per call {q_plain:.0f} q/s (p99 {pct(plain_lat, 99) * 1e3:.2f} ms); "
+        f"{q_pack / q_plain:.1f}x and {q_do / q_plain:.1f}x; max levels {int(levels.max())}")
     del A, D
     return dict(graphs=len(graphs), rows=int(P.m), entries=int(P.nnz), launch_ms=t_pack * 1e3,
                 qps=q_pack, plain_qps=q_plain, plain_p99_ms=pct(plain_lat, 99) * 1e3,
-                batched_vs_plain=q_pack / q_plain, max_levels=int(levels.max()),
-                plain_sample=len(mats))
+                batched_vs_plain=q_pack / q_plain, do_ms=t_do * 1e3, do_qps=q_do,
+                do_p99_ms=pct(lat_do, 99) * 1e3, do_vs_plain=q_do / q_plain,
+                max_levels=int(levels.max()), plain_sample=len(mats))
 
 
 def tuned_leg(args, np, torch, gg, G, R0, ctx, _lib, seed):

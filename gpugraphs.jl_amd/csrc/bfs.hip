@@ -425,6 +425,72 @@ __global__ void degree_kernel(const int32_t* rowptr, int64_t v, BfsCounters* c) 
   c->scout = (unsigned long long)(rowptr[v + 1] - rowptr[v]);
 }
 
+// Many sources at level 0 (original 0-based ids, distinct): out = 0, the
+// bitmaps at their state index, the first queue and its out-degree sum.
+__global__ __launch_bounds__(kB) void bfs_seeds_kernel(int32_t* out, uint32_t* visited,
+                                                       uint32_t* frontier, int32_t* queue,
+                                                       const int32_t* __restrict__ seeds,
+                                                       int64_t n_seeds, const int32_t* vinv,
+                                                       const int32_t* __restrict__ rowptr_out,
+                                                       BfsCounters* c) {
+  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i >= n_seeds) return;
+  const int64_t s = seeds[i];
+  const int64_t r = vinv ? (int64_t)vinv[s] - 1 : s;
+  out[s] = 0;
+  atomicOr(&visited[r >> 5], 1u << (r & 31));
+  atomicOr(&frontier[r >> 5], 1u << (r & 31));
+  queue[i] = (int32_t)s;
+  atomicAdd(&c->scout, (unsigned long long)(rowptr_out[s + 1] - rowptr_out[s]));
+}
+
+// Level count of each packed graph = the largest finite level among its rows
+// (offsets: n_graphs + 1 rising row offsets).  A wavefront walks kLvRows
+// consecutive rows 64 at a time, keeps a running maximum while its rows stay
+// in one graph and folds it into levels[g] with one atomic max per graph met.
+constexpr int kLvRows = 4096;
+__global__ __launch_bounds__(kB) void graph_levels_kernel(const int32_t* __restrict__ dist, int64_t m,
+                                                          const int64_t* __restrict__ offsets,
+                                                          int n_graphs, int32_t* levels) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6;
+  const int64_t r0 = wave * kLvRows;
+  if (r0 >= m) return;
+  const int64_t r1 = r0 + kLvRows < m ? r0 + kLvRows : m;
+  int g_run = -1;
+  int32_t mx = -1;
+  for (int64_t b = r0; b < r1; b += 64) {
+    const int64_t r = b + lane;
+    int g = -1;
+    int32_t d = -1;
+    if (r < r1) {
+      int lo = 0, hi = n_graphs;  // offsets[lo] <= r < offsets[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (offsets[mid] <= r) lo = mid; else hi = mid;
+      }
+      g = lo;
+      d = dist[r];
+      if (d == 2147483647) d = -1;
+    }
+    const int g0 = __shfl(g, 0, 64);
+    const bool uniform = __ballot(r < r1 && g != g0) == 0ull;
+    if (uniform) {
+      int32_t v = d;
+      for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+      if (g0 != g_run) {
+        if (lane == 0 && g_run >= 0 && mx >= 0) atomicMax(&levels[g_run], mx);
+        g_run = g0;
+        mx = -1;
+      }
+      mx = max(mx, v);
+    } else {  // rows of two or more graphs: each lane folds its own row
+      if (g >= 0 && d >= 0) atomicMax(&levels[g], d);
+    }
+  }
+  if (lane == 0 && g_run >= 0 && mx >= 0) atomicMax(&levels[g_run], mx);
+}
+
 // --------------------------------------------------------------- host side
 struct BfsWork {
   int64_t m = 0, words = 0;
@@ -688,31 +754,31 @@ int gg_bfs_parents(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, int32_t* p
   return GG_OK;
 }
 
-int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_out,
-                        int64_t source, int32_t* dist, double alpha, double beta) {
-  if (!ctx || !dist) return set_error(GG_ERR_ARGUMENT, "NULL argument");
-  clear_error();
-  GG_TRY(check_square_source(A_T, source));
+}  // extern "C"
+
+namespace gg {
+
+static int check_do_pair(const gg_matrix* A_T, const gg_matrix* A_out) {
   GG_TRY(check_matrix(A_out));
   if (A_out->format != GG_CSR || A_out->m != A_T->m || A_out->n != A_T->n)
     return set_error(GG_ERR_ARGUMENT, "A_out must be the CSR of A (out-neighbour rows), same size");
   if (!(A_T->flags & GG_FLAG_UNIT_PATTERN) || !(A_out->flags & GG_FLAG_UNIT_PATTERN))
     return set_error(GG_ERR_ARGUMENT, "direction-optimising BFS needs unit-pattern matrices");
+  return GG_OK;
+}
+
+// Direction-optimising levels from the seeded state: w.q0 holds the qlen
+// level-0 vertices (original ids), `scout` their out-degree sum.
+static int do_levels(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_out, BfsWork& w,
+                     int64_t qlen, double scout, int32_t* dist, double alpha, double beta) {
   if (alpha <= 0) alpha = 15.0;
   if (beta <= 0) beta = 18.0;
-  timer_begin(ctx);
-  BfsWork w;
-  GG_TRY(bfs_begin(ctx, A_T, source, dist, 0, w, true));
   const int64_t n = A_T->m;
   const int32_t* vinv = vinv_of(A_T);
   SellView g{};  // state index -> original vertex for the queue rebuild
   if (A_T->format == GG_SELL) g = sell_view(A_T);
   BfsCounters hc{};
-  hipLaunchKernelGGL(degree_kernel, dim3(1), dim3(1), 0, ctx->stream, A_out->ptr, source - 1, w.cnt);
-  GG_TRY(read_counters(ctx, w, &hc));
   double edges_to_check = (double)A_out->nnz;
-  double scout = (double)hc.scout;
-  int64_t qlen = 1;
   int32_t level = 0;
   // hub pieces: at most one per kBigDeg edges plus one per hub
   void* bp;
@@ -767,6 +833,88 @@ int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_ou
       std::swap(w.q0, w.q1);
       std::swap(w.front, w.next);
     }
+  }
+  return GG_OK;
+}
+
+}  // namespace gg
+
+extern "C" {
+
+int gg_bfs_distances_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_out,
+                        int64_t source, int32_t* dist, double alpha, double beta) {
+  if (!ctx || !dist) return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  clear_error();
+  GG_TRY(check_square_source(A_T, source));
+  GG_TRY(check_do_pair(A_T, A_out));
+  timer_begin(ctx);
+  BfsWork w;
+  GG_TRY(bfs_begin(ctx, A_T, source, dist, 0, w, true));
+  BfsCounters hc{};
+  hipLaunchKernelGGL(degree_kernel, dim3(1), dim3(1), 0, ctx->stream, A_out->ptr, source - 1, w.cnt);
+  GG_TRY(read_counters(ctx, w, &hc));
+  const int st = do_levels(ctx, A_T, A_out, w, 1, (double)hc.scout, dist, alpha, beta);
+  timer_end(ctx);
+  return st;
+}
+
+int gg_bfs_packed_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_out,
+                     const int64_t* offsets, int n_graphs, const int32_t* sources, int32_t* dist,
+                     int32_t* levels, double alpha, double beta) {
+  if (!ctx || !A_T || !offsets || !sources || !dist || n_graphs < 1)
+    return set_error(GG_ERR_ARGUMENT, "NULL argument");
+  clear_error();
+  GG_TRY(check_matrix(A_T));
+  if (A_T->format == GG_CSC) return set_error(GG_ERR_ARGUMENT, "BFS runs on CSR or SELL A_T");
+  if (A_T->m != A_T->n) return set_error(GG_ERR_DIMENSION, "A_T must be square");
+  GG_TRY(check_do_pair(A_T, A_out));
+  if (offsets[0] != 0 || offsets[n_graphs] != A_T->m)
+    return set_error(GG_ERR_DIMENSION, "offsets do not cover the %lld rows", (long long)A_T->m);
+  std::vector<int32_t> seeds(n_graphs);
+  for (int gi = 0; gi < n_graphs; ++gi) {
+    if (offsets[gi + 1] < offsets[gi])
+      return set_error(GG_ERR_ARGUMENT, "graph offsets must not decrease");
+    if (sources[gi] < 1 || sources[gi] > offsets[gi + 1] - offsets[gi])
+      return set_error(GG_ERR_BOUNDS, "source %d of graph %d outside its rows", sources[gi], gi);
+    seeds[gi] = (int32_t)(offsets[gi] + sources[gi] - 1);
+  }
+  const int64_t m = A_T->m;
+  timer_begin(ctx);
+  BfsWork w;
+  GG_TRY(bfs_work(ctx, state_size(A_T), true, w));
+  // offsets (int64, n + 1), then the seeds (int32, n) and the levels (int32, n)
+  void* p;
+  GG_TRY(ws_get(ctx, WS_SPMSPV0, sizeof(int64_t) * (n_graphs + 1) + 2 * sizeof(int32_t) * n_graphs,
+                &p));
+  int64_t* d_off = (int64_t*)p;
+  int32_t* d_seeds = (int32_t*)(d_off + n_graphs + 1);
+  int32_t* d_levels = d_seeds + n_graphs;
+  GG_HIP(hipMemcpyAsync(d_off, offsets, sizeof(int64_t) * (n_graphs + 1), hipMemcpyHostToDevice,
+                        ctx->stream));
+  GG_HIP(hipMemcpyAsync(d_seeds, seeds.data(), sizeof(int32_t) * n_graphs, hipMemcpyHostToDevice,
+                        ctx->stream));
+  hipLaunchKernelGGL(fill_i32v_kernel, dim3(blocks_for(m, kB)), dim3(kB), 0, ctx->stream, dist, m,
+                     (int32_t)2147483647);
+  fill_words(ctx, w.visited, w.words, 0u);
+  fill_words(ctx, w.front, w.words, 0u);
+  hipLaunchKernelGGL(level_begin_kernel, dim3(1), dim3(1), 0, ctx->stream, w.cnt, 0, dist);
+  hipLaunchKernelGGL(bfs_seeds_kernel, dim3(blocks_for(n_graphs, kB)), dim3(kB), 0, ctx->stream,
+                     dist, w.visited, w.front, w.q0, d_seeds, (int64_t)n_graphs, vinv_of(A_T),
+                     A_out->ptr, w.cnt);
+  GG_HIP(hipGetLastError());
+  BfsCounters hc{};
+  GG_TRY(read_counters(ctx, w, &hc));
+  GG_TRY(do_levels(ctx, A_T, A_out, w, n_graphs, (double)hc.scout, dist, alpha, beta));
+  if (levels) {
+    hipLaunchKernelGGL(fill_i32v_kernel, dim3(blocks_for(n_graphs, kB)), dim3(kB), 0, ctx->stream,
+                       d_levels, (int64_t)n_graphs, 0);
+    const int64_t waves = (m + kLvRows - 1) / kLvRows;
+    hipLaunchKernelGGL(graph_levels_kernel, dim3(blocks_for(waves * 64, kB)), dim3(kB), 0,
+                       ctx->stream, dist, m, d_off, n_graphs, d_levels);
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipMemcpyAsync(levels, d_levels, sizeof(int32_t) * n_graphs, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    GG_HIP(hipStreamSynchronize(ctx->stream));
   }
   timer_end(ctx);
   return GG_OK;

@@ -12,7 +12,7 @@ from .algebra import (GPUGraphs_add, GPUGraphs_any, GPUGraphs_band, GPUGraphs_bo
                       GPUGraphs_land, GPUGraphs_lor, GPUGraphs_max, GPUGraphs_min, GPUGraphs_mul,
                       GPUGraphs_pair, GPUGraphs_safe_add, GPUGraphs_second, GPUGraphs_secondi,
                       monoid_absorb, monoid_neutral)
-from .algorithms import (bfs_distances, bfs_distances_, bfs_distances_do, bfs_packed,
+from .algorithms import (bfs_distances, bfs_distances_, bfs_distances_do, bfs_packed, bfs_packed_do,
                          bfs_parents, bfs_parents_, dist_dtype, multi_source_bfs, shortest_path,
                          shortest_path_)
 from .ops import csc_view, gpu_spmm_, gpu_spmspv, gpu_spmspv_, gpu_spmv_
@@ -31,7 +31,8 @@ __all__ = [
     "GPUGraphs_secondi", "GPUGraphs_max", "GPUGraphs_min",
     # beyond the reference's exports
     "GPUGraphs_first", "GPUGraphs_pair", "GPUGraphs_lor", "GPUGraphs_land", "gpu_spmspv",
-    "gpu_spmspv_", "csc_view", "bfs_distances_do", "multi_source_bfs", "bfs_packed", "dist_dtype",
+    "gpu_spmspv_", "csc_view", "bfs_distances_do", "multi_source_bfs", "bfs_packed", "bfs_packed_do",
+    "dist_dtype",
     "ROCBackend", "CPUBackend", "get_backend", "DimensionMismatch", "ArgumentError",
     "BoundsError", "MethodError", "HIPError", "NotConvergedError", "BudgetExceeded", "lib",
 ]

@@ -171,6 +171,7 @@ SIGNATURES = {
                               _i32),
     "gg_read_edge_list": ([C.c_char_p, _i32, _i32, _i32, C.POINTER(C.POINTER(GGCsrHost))], _i32),
     "gg_bfs_packed": ([_vp, _PM, _vp, _i32, _vp, _vp, _vp], _i32),
+    "gg_bfs_packed_do": ([_vp, _PM, _PM, _vp, _i32, _vp, _vp, _vp, _d, _d], _i32),
     "gg_gen_random_graph": ([_i64, _i64, _u64, _i32, _i32, C.c_int32, C.c_int32,
                              C.POINTER(C.POINTER(GGCsrHost))], _i32),
     "gg_spmv": ([_vp, _vp, _i32, _i64, _PM, _vp, _i32, _i64, _i32, _i32, _i32, _vp, _i32,

@@ -68,6 +68,32 @@ def bfs_packed(A_T, offsets, sources):
     return dist, levels
 
 
+def bfs_packed_do(A_T, offsets, sources, A_out=None, alpha=-1.0, beta=-1.0):
+    """The results of bfs_packed from one direction-optimising traversal of
+    the pack seeded with every graph's source: (dist over all packed rows,
+    levels per graph).  A_T may be any CSR or SELL layout of the pack; A_out
+    is the pack's CSR of A (for a symmetric pack, the packed CSR; the default
+    when A_T is that CSR)."""
+    dev = _dev(A_T)
+    if A_out is None:
+        if not isinstance(A_T, SparseGPUMatrixCSR):
+            raise ArgumentError("A_out (the packed CSR of A) is needed with a SELL A_T")
+        A_out = A_T
+    if not isinstance(A_out, SparseGPUMatrixCSR):
+        raise ArgumentError("A_out must be a SparseGPUMatrixCSR")
+    off = np.ascontiguousarray(offsets, np.int64)
+    src = np.ascontiguousarray(sources, np.int32)
+    if len(off) != len(src) + 1:
+        raise ArgumentError("offsets needs one entry more than sources")
+    dist = _empty(A_T.m, np.int32, dev)
+    levels = np.zeros(len(src), np.int32)
+    d, o = A_T.descriptor(), A_out.descriptor()
+    check(lib().gg_bfs_packed_do(context(dev), C.byref(d), C.byref(o), off.ctypes.data, len(src),
+                                 src.ctypes.data, ptr(dist), levels.ctypes.data, float(alpha),
+                                 float(beta)), "bfs_packed_do")
+    return dist, levels
+
+
 def bfs_distances_do(A_T, source, A_out=None, alpha=-1.0, beta=-1.0):
     """Direction-optimising BFS levels (push over A_out's rows, pull over A_T).
     A_out = CSR of A; for a symmetric graph it is A_T itself (the default)."""

@@ -437,6 +437,20 @@ int gg_mssp_distances(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources,
  * graph. */
 int gg_bfs_packed(gg_ctx* ctx, const gg_matrix* A_T, const int64_t* offsets, int n_graphs,
                   const int32_t* sources, int32_t* dist, int32_t* levels);
+/* The same results as gg_bfs_packed from ONE direction-optimising traversal
+ * of the packed graph seeded with every graph's source at level 0 (the blocks
+ * share no edge, so each vertex's level is its level within its own graph):
+ * push levels over A_out's rows while the frontier is small, pull levels over
+ * A_T (any CSR or SELL layout, relabelled included) while it is large, as in
+ * gg_bfs_distances_do (alpha, beta <= 0: the defaults).  A_out is the CSR of
+ * the packed A (for a symmetric pack, the packed CSR itself); offsets
+ * (n_graphs + 1, rising from 0 to m, no padding needed) and sources (1-based
+ * within each graph) are host arrays; levels (host, optional) receives the
+ * largest level reached in each graph.  No counterpart in the reference,
+ * which runs bfs_distances (src/algorithms/bfs.jl:1-111) once per graph. */
+int gg_bfs_packed_do(gg_ctx* ctx, const gg_matrix* A_T, const gg_matrix* A_out,
+                     const int64_t* offsets, int n_graphs, const int32_t* sources, int32_t* dist,
+                     int32_t* levels, double alpha, double beta);
 
 /* ------------------------------------------------------------------ tuning */
 /* The autotuner (no counterpart in the reference): candidate layouts of one

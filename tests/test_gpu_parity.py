@@ -966,3 +966,111 @@ def test_bfs_packed_symmetric_vs_oracle(env, alpha):
         np.testing.assert_array_equal(D[off[g]:off[g] + M.shape[0]], ref, err_msg=f"graph {g}")
         assert np.all(D[off[g] + M.shape[0]:off[g + 1]] == INT32_MAX)
         assert levels[g] == ref[ref != INT32_MAX].max()
+
+
+def _unpadded_pack(hosts):
+    """Block-diagonal pack of A_T blocks with no padding rows: (HostCSR, offsets)."""
+    from gpugraphs import generators as GEN
+    blocks = [sp.csr_matrix((h.nzval, h.colval - 1, h.rowptr.astype(np.int64) - 1),
+                            shape=(h.m, h.n)) for h in hosts]
+    S = sp.block_diag(blocks, format="csr").astype(np.bool_)
+    S.sort_indices()
+    off = np.r_[0, np.cumsum([h.m for h in hosts])].astype(np.int64)
+    return GEN.HostCSR(S.shape[0], S.shape[1], (S.indptr + 1).astype(np.int32),
+                       (S.indices + 1).astype(np.int32), np.ones(S.nnz, np.bool_),
+                       all(h.symmetric for h in hosts)), off
+
+
+@pytest.mark.parametrize("alpha", [1e-9, 15.0, 1e9])
+@pytest.mark.parametrize("layout", ["csr", "sell_relabel", "sell_sigma"])
+def test_bfs_packed_do_vs_oracle(env, layout, alpha):
+    """One direction-optimising traversal of a pack seeded with every graph's
+    source (gg_bfs_packed_do) gives each graph the oracle's levels
+    (src/algorithms/bfs.jl:31-73) from its source: symmetric blocks (R-MAT,
+    Barabasi-Albert, grids, a lone vertex, a hub source) and directed blocks
+    whose out-edge CSR differs from A_T, padded (gg_pack_graphs) and unpadded
+    offsets; alpha 1e-9 pushes every level, 1e9 pulls every level; the pull
+    runs over CSR, a relabelled SELL with heavy rows, and sigma windows."""
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    rng = np.random.default_rng(123)
+    sym_hosts, dir_hosts, sym_mats, dir_mats = [], [], [], []
+    sym_mats.append(sp.csr_matrix((1, 1), dtype=np.bool_))               # a lone vertex
+    for i in range(23):
+        kind = i % 3
+        if kind == 0:
+            M = GEN.rmat(int(rng.integers(4, 13)), seed=int(rng.integers(1 << 30))).to_scipy()
+        elif kind == 1:
+            n = int(rng.integers(10, 900))
+            M = G.undirected_adj(n, G.barabasi_albert(n, 3, int(rng.integers(1 << 30))), np.bool_)
+        else:
+            M = GEN.grid(int(rng.integers(2, 40)), int(rng.integers(2, 40)), 0.0, 1, 1,
+                         seed=int(rng.integers(1 << 30)), dtype=np.int32).to_scipy()
+        sym_mats.append(sp.csr_matrix(M != 0).astype(np.bool_))
+    for M in sym_mats:
+        H = _host_csr(M, np.bool_)
+        H.symmetric = True
+        sym_hosts.append(H)
+    for i in range(12):
+        Adj = G.random_digraph(rng, int(rng.integers(5, 700)), 0.01, dtype=np.bool_)
+        AT = sp.csr_matrix(Adj.T).astype(np.bool_)
+        dir_mats.append(AT)
+        dir_hosts.append(_host_csr(AT, np.bool_))
+    R = gg.ROCBackend(0)
+    for name, hosts, mats in (("symmetric", sym_hosts, sym_mats),
+                              ("directed", dir_hosts, dir_mats)):
+        for padded in (True, False):
+            P, off = GEN.pack(hosts) if padded else _unpadded_pack(hosts)
+            A_T_csr = P.to_csr(R)
+            if name == "symmetric":
+                A_out = A_T_csr
+            else:
+                A_out = gg.SparseGPUMatrixCSR(sp.csr_matrix(P.to_scipy().T), R)
+            if layout == "csr":
+                A_T = A_T_csr
+            elif layout == "sell_relabel":
+                A_T = P.to_sell(R, 64, 1, 8, 16, relabel=True, sort_entries=True)
+            else:
+                A_T = P.to_sell(R, 64, 64, 8, 16)
+            srcs = np.array([int(rng.integers(M.shape[0])) + 1 for M in mats], np.int32)
+            if name == "symmetric":
+                srcs[1] = int(np.argmax(np.diff(mats[1].indptr))) + 1   # a hub
+            print_case("bfs packed do", name, layout, alpha, padded, P.m, P.nnz)
+            D, levels = gg.bfs_packed_do(A_T, off, srcs, A_out=A_out, alpha=alpha)
+            D = host(D)
+            for g, M in enumerate(mats):
+                ref = orc.bfs_distances(orc.from_scipy_csr(sp.csr_matrix(M)), int(srcs[g]))
+                np.testing.assert_array_equal(D[off[g]:off[g] + M.shape[0]], ref,
+                                              err_msg=f"{name} graph {g}")
+                assert np.all(D[off[g] + M.shape[0]:off[g + 1]] == INT32_MAX)
+                assert levels[g] == ref[ref != INT32_MAX].max(), f"{name} graph {g}"
+            if padded and layout == "csr":
+                # the persistent launch gives the same arrays
+                D2, lv2 = gg.bfs_packed(A_T_csr, off, srcs)
+                np.testing.assert_array_equal(host(D2), D)
+                np.testing.assert_array_equal(lv2, levels)
+
+
+def test_bfs_packed_do_errors(env):
+    """Offsets that do not cover the rows, a source outside its graph, a
+    missing or non-CSR A_out are rejected before any launch."""
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    hosts = [GEN.rmat(6, seed=s) for s in (1, 2)]
+    P, off = GEN.pack(hosts)
+    A = P.to_csr(gg.ROCBackend(0))
+    S = P.to_sell(gg.ROCBackend(0), 64, 1, 0, 4096, relabel=True)
+    srcs = np.array([1, 1], np.int32)
+    print_case("bfs packed do errors")
+    with pytest.raises(gg.DimensionMismatch):
+        gg.bfs_packed_do(A, off[:-1].tolist() + [off[-1] - 1], srcs)
+    with pytest.raises(gg.BoundsError):
+        gg.bfs_packed_do(A, off, np.array([1, off[2] - off[1] + 1], np.int32))
+    with pytest.raises(gg.ArgumentError):
+        gg.bfs_packed_do(S, off, srcs)
+    with pytest.raises(gg.ArgumentError):
+        gg.bfs_packed_do(S, off, srcs, A_out=S)
+    D, lv = gg.bfs_packed_do(S, off, srcs, A_out=A)
+    D2, lv2 = gg.bfs_packed(A, off, srcs)
+    np.testing.assert_array_equal(host(D), host(D2))
+    np.testing.assert_array_equal(lv, lv2)

@@ -15,7 +15,8 @@ distances) at full size:
   sources, one at a time and batched, by the exact shortest-path
   certificate, one source against the restated Jacobi loop;
 - batched BFS k = 64 and 256 on R-MAT 20 and 22, k = 32 min-plus on
-  weighted R-MAT 22, and 512 packed R-MAT graphs of scale 10..18;
+  weighted R-MAT 22, and 512 packed R-MAT graphs of scale 10..18 (the
+  persistent launch and the direction-optimising traversal);
 - the serving loop over its 8 resident graphs under Poisson arrivals, every
   completed result checked.
 
@@ -253,6 +254,13 @@ def test_cfg_packed_512_rmat_graphs(env):
         np.testing.assert_array_equal(got, ref, err_msg=f"packed graph {i} (scale {scales[i]})")
         assert np.all(Dh[int(off[i]) + g.m:int(off[i + 1])] == INT32_MAX)   # padding rows
         assert levels[i] == ref[ref != INT32_MAX].max()
+    # the same pack as one direction-optimising traversal over a relabelled
+    # SELL (the benchmark's layout): the same arrays as the checked ones
+    A_s = gg.SparseGPUMatrixSELL.build_on_device(A, 64, 1, 1024, 65536, True, True)
+    print_case("packed bfs 512 graphs, direction-optimising")
+    D2, lv2 = gg.bfs_packed_do(A_s, off, roots, A_out=A)
+    np.testing.assert_array_equal(_host(D2), Dh)
+    np.testing.assert_array_equal(lv2, levels)
 
 
 def test_cfg_serving_8_graphs_poisson(env, rmat22):
