@@ -138,6 +138,25 @@ def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma,
         same = all(torch.equal(Dm[:, i], plain[i]) for i in range(k))
         exact = sssp_certificate(H.rowptr, H.colval, H.nzval, plain[0].cpu().numpy(), int(srcs[0]))
         assert same and exact, f"SSSP {name}: batched equals plain {same}, certificate {exact}"
+        del Dm
+        # delta-stepping one source at a time, push over the CSR of A (the
+        # transpose of A_T: the weights differ per direction), default delta
+        S = H.to_scipy().T.tocsr()
+        S.sort_indices()
+        A_out = gg.SparseGPUMatrixCSR(S, R0)
+        del S
+        gg.shortest_path_delta(A_out, int(srcs[0]))
+        torch.cuda.synchronize()
+        phases = []
+
+        def one_delta(s):
+            d, ph = gg.shortest_path_delta(A_out, int(s), return_phases=True)
+            phases.append(ph)
+            return d
+        lat_d, dres = _time_queries(torch, one_delta, srcs)
+        same_d = all(torch.equal(dres[i], plain[i]) for i in range(k))
+        assert same_d, f"SSSP {name}: delta-stepping differs from Bellman-Ford"
+        del dres, A_out
         B_it = 8 * H.nnz + 4 * (m + 1) + 4 * m + 8 * m  # one full Jacobi iteration
         out[name] = dict(n=m, nnz=H.nnz, sources=k, plain_qps=k / sum(lat),
                          plain_p50_ms=pct(lat, 50) * 1e3, plain_p99_ms=pct(lat, 99) * 1e3,
@@ -146,12 +165,17 @@ def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma,
                          batched_iters=it_b, batched_vs_plain=(k / t_b) / (k / sum(lat)),
                          sparse_alpha=sa.best["alpha"],
                          sparse_alpha_ms=[[a, round(t, 3)] for a, t in sa.table],
+                         delta_qps=k / sum(lat_d), delta_p50_ms=pct(lat_d, 50) * 1e3,
+                         delta_p99_ms=pct(lat_d, 99) * 1e3, delta_phases_mean=float(np.mean(phases)),
+                         delta_vs_plain=(k / sum(lat_d)) / (k / sum(lat)),
                          exact_certificate=exact)
         log(f"SSSP {name}: plain {k / sum(lat):.1f} q/s (p99 {pct(lat, 99) * 1e3:.1f} ms, "
             f"{np.mean(iters):.0f} iters, {kms / max(kn, 1) * 1e3:.0f} us/iter); batched k={k} "
             f"{k / t_b:.1f} q/s ({it_b} iters) = {(k / t_b) / (k / sum(lat)):.1f}x, "
-            f"sparse alpha {sa.best['alpha']}")
-        del plain, Dm
+            f"sparse alpha {sa.best['alpha']}; delta-stepping {k / sum(lat_d):.1f} q/s "
+            f"(p99 {pct(lat_d, 99) * 1e3:.1f} ms, {np.mean(phases):.0f} phases) = "
+            f"{(k / sum(lat_d)) / (k / sum(lat)):.1f}x one-source Bellman-Ford")
+        del plain
     _lib.set_sssp_threshold(ctx, 16.0)
     return out
 

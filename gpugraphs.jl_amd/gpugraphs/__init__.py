@@ -14,7 +14,7 @@ from .algebra import (GPUGraphs_add, GPUGraphs_any, GPUGraphs_band, GPUGraphs_bo
                       monoid_absorb, monoid_neutral)
 from .algorithms import (bfs_distances, bfs_distances_, bfs_distances_do, bfs_packed, bfs_packed_do,
                          bfs_parents, bfs_parents_, dist_dtype, multi_source_bfs, shortest_path,
-                         shortest_path_)
+                         shortest_path_, shortest_path_delta)
 from .ops import csc_view, gpu_spmm_, gpu_spmspv, gpu_spmspv_, gpu_spmv_
 from .storage import (AbstractSparseGPUMatrix, CPUBackend, ROCBackend, SparseGPUMatrixCSC,
                       SparseGPUMatrixCSR, SparseGPUMatrixSELL, SparseGPUVector, get_backend,
@@ -31,7 +31,7 @@ __all__ = [
     "GPUGraphs_secondi", "GPUGraphs_max", "GPUGraphs_min",
     # beyond the reference's exports
     "GPUGraphs_first", "GPUGraphs_pair", "GPUGraphs_lor", "GPUGraphs_land", "gpu_spmspv",
-    "gpu_spmspv_", "csc_view", "bfs_distances_do", "multi_source_bfs", "bfs_packed", "bfs_packed_do",
+    "gpu_spmspv_", "csc_view", "bfs_distances_do", "multi_source_bfs", "bfs_packed", "bfs_packed_do", "shortest_path_delta",
     "dist_dtype",
     "ROCBackend", "CPUBackend", "get_backend", "DimensionMismatch", "ArgumentError",
     "BoundsError", "MethodError", "HIPError", "NotConvergedError", "BudgetExceeded", "lib",

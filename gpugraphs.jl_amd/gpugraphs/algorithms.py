@@ -163,6 +163,24 @@ def shortest_path(A_T, source):
     return dist
 
 
+def shortest_path_delta(A_out, source, delta=0.0, return_phases=False):
+    """The distances of shortest_path by delta-stepping: push over A_out, the
+    CSR of A (row u = out-neighbours of u with the weights of u -> v; for a
+    symmetric matrix, A_T itself).  delta = 0 takes 4 x the mean weight.
+    Weights reached from the source must be non-negative."""
+    if not isinstance(A_out, SparseGPUMatrixCSR):
+        raise ArgumentError("delta-stepping runs on a SparseGPUMatrixCSR of A")
+    dev = _dev(A_out)
+    dt = dist_dtype(A_out)
+    dist = _empty(A_out.m, dt, dev)
+    d = A_out.descriptor()
+    ph = C.c_int64()
+    check(lib().gg_shortest_path_delta(context(dev), C.byref(d), int(source), ptr(dist),
+                                       dtype_code(dt), float(delta), C.byref(ph)),
+          "shortest_path_delta")
+    return (dist, ph.value) if return_phases else dist
+
+
 def multi_source_bfs(A_T, sources):
     """Levels from k sources at once (or-and SpMM over 64-bit source masks);
     an m x k column-major int32 matrix whose column i equals

@@ -414,6 +414,17 @@ int gg_shortest_path(gg_ctx* ctx, const gg_matrix* A_T, int64_t source, void* di
  * src/algorithms/shortest_path.jl:88-161.  sources is a host array. */
 int gg_shortest_path_multi(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sources,
                            int64_t k, void* dist, int dist_dtype, int64_t* iterations);
+/* The distances of gg_shortest_path by delta-stepping, the bucketed order of
+ * the relaxations that src/algorithms/shortest_path.jl:49-53 prepares (its
+ * `updated` mask) and leaves unused: vertices below a threshold are expanded
+ * push-style over A (CSR of A: row u = out-neighbours of u with the weights
+ * of u -> v; for a symmetric matrix, A_T itself) until none is left below it,
+ * then the threshold moves by delta to the next bucket that holds a pending
+ * vertex.  delta = 0 takes 4 x the mean weight.  The weights reached from
+ * the source must be non-negative (a negative or NaN one is GG_ERR_ARGUMENT).
+ * phases (optional) receives the expansion phases run. */
+int gg_shortest_path_delta(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist,
+                           int dist_dtype, double delta, int64_t* phases);
 
 /* ------------------------------------------------------------ batched form */
 /* Multi-source BFS as or-and SpMM over 64-bit source bitmasks: k sources

@@ -1074,3 +1074,93 @@ def test_bfs_packed_do_errors(env):
     D2, lv2 = gg.bfs_packed(A, off, srcs)
     np.testing.assert_array_equal(host(D), host(D2))
     np.testing.assert_array_equal(lv, lv2)
+
+
+@pytest.mark.parametrize("steps", [0, 8])
+def test_shortest_path_delta_vs_oracle(env, steps):
+    """Delta-stepping (gg_shortest_path_delta, push over the CSR of A) gives
+    the distances of the oracle's Jacobi Bellman-Ford
+    (src/algorithms/shortest_path.jl:34-83) from A_T: weighted digraphs
+    (fp32, Int32 with zero weights, Int64, fp64), a grid with long-range edges
+    and independent weights per direction, a weighted R-MAT whose hubs are cut
+    into pieces, unreachable vertices, Bool hops; delta from one (a bucket per
+    distance) through the default to 1e12 (one bucket: push Bellman-Ford);
+    host-stepped and captured phases."""
+    gg, torch, orc = env
+    from gpugraphs import _lib, generators as GEN
+    rng = np.random.default_rng(31)
+    R = GEN.rmat(14, seed=8)
+    w = rng.integers(1, 1000, R.nnz).astype(np.float32)
+    RW = sp.csr_matrix((w, R.colval - 1, R.rowptr.astype(np.int64) - 1), shape=(R.m, R.n))
+    Z = G.random_digraph(rng, 500, 0.01, wmax=4, dtype=np.int32)
+    Z.data -= 1                                                     # weights 0..3
+    cases = [("digraph f32", G.random_digraph(rng, 600, 0.008, wmax=1000, dtype=np.float32)),
+             ("digraph i32 zero weights", Z),
+             ("digraph i64", G.random_digraph(rng, 300, 0.02, wmax=50, dtype=np.int64)),
+             ("digraph f64", G.random_digraph(rng, 300, 0.02, wmax=50, dtype=np.float64)),
+             ("grid", GEN.grid(60, 70, 0.01, 1, 1000, seed=9, dtype=np.float32).to_scipy().T.tocsr()),
+             ("rmat14 weighted", RW.T.tocsr()),
+             ("sparse digraph", G.random_digraph(rng, 400, 0.002, wmax=9, dtype=np.float32))]
+    ctx = _lib.context(0)
+    try:
+        _lib.set_graph_steps(ctx, steps)
+        for name, Adj in cases:
+            Adj = sp.csr_matrix(Adj)
+            Adj.sort_indices()
+            AT = sp.csr_matrix(Adj.T)
+            Ao = orc.from_scipy_csr(AT)
+            A_out = gg.SparseGPUMatrixCSR(Adj, gg.ROCBackend(0))
+            deg = np.diff(Adj.indptr)
+            srcs = [1, int(np.argmax(deg)) + 1, Adj.shape[0]]
+            for s in srcs:
+                ref = orc.shortest_path(Ao, s)
+                for delta in (1.0, 0.0, 1e12):
+                    print_case("sssp delta", name, steps, s, delta)
+                    d = gg.shortest_path_delta(A_out, s, delta)
+                    assert d.dtype == torch.from_numpy(ref[:1]).dtype
+                    np.testing.assert_array_equal(host(d), ref, err_msg=f"{name} {s} {delta}")
+        # Bool: hop counts as Int32
+        Adj = sp.csr_matrix(G.random_digraph(rng, 800, 0.004, dtype=np.bool_))
+        A_out = gg.SparseGPUMatrixCSR(Adj, gg.ROCBackend(0))
+        Ao = orc.from_scipy_csr(sp.csr_matrix(Adj.T))
+        for s in (1, 400):
+            print_case("sssp delta bool", steps, s)
+            d = gg.shortest_path_delta(A_out, s)
+            assert d.dtype == torch.int32
+            np.testing.assert_array_equal(host(d), orc.shortest_path(Ao, s))
+    finally:
+        _lib.set_graph_steps(ctx, 0)
+
+
+def test_shortest_path_delta_errors(env):
+    """A negative weight reached from the source is an ArgumentError (one that
+    is not reached does not matter); a source outside 1:n is a BoundsError; a
+    SELL matrix or a negative delta is an ArgumentError."""
+    gg, torch, orc = env
+    rng = np.random.default_rng(32)
+    Adj = sp.csr_matrix(G.random_digraph(rng, 200, 0.03, wmax=9, dtype=np.float32))
+    neg = Adj.copy()
+    neg.data[0] = -1.0
+    r0 = int(np.searchsorted(neg.indptr, 0, side="right")) - 1     # row of entry 0
+    A_neg = gg.SparseGPUMatrixCSR(neg, gg.ROCBackend(0))
+    print_case("sssp delta errors")
+    with pytest.raises(gg.ArgumentError):
+        gg.shortest_path_delta(A_neg, r0 + 1)
+    # a negative edge out of an unreachable vertex: vertex 201 has no in-edges
+    iso = sp.bmat([[Adj, None], [sp.csr_matrix(np.r_[-5.0, np.zeros(199)][None, :]), None]],
+                  format="csr").astype(np.float32)
+    iso.resize((201, 201))
+    iso.eliminate_zeros()
+    A_iso = gg.SparseGPUMatrixCSR(sp.csr_matrix(iso), gg.ROCBackend(0))
+    Ao = orc.from_scipy_csr(sp.csr_matrix(iso.T))
+    np.testing.assert_array_equal(host(gg.shortest_path_delta(A_iso, 1)), orc.shortest_path(Ao, 1))
+    A = gg.SparseGPUMatrixCSR(Adj, gg.ROCBackend(0))
+    with pytest.raises(gg.BoundsError):
+        gg.shortest_path_delta(A, 201)
+    with pytest.raises(gg.ArgumentError):
+        gg.shortest_path_delta(A, 1, -1.0)
+    with pytest.raises(gg.ArgumentError):
+        gg.shortest_path_delta(gg.SparseGPUMatrixSELL(Adj, gg.ROCBackend(0), 32), 1)
+    # the context still works after the errors
+    np.testing.assert_array_equal(host(gg.shortest_path_delta(A, 1)),
+                                  orc.shortest_path(orc.from_scipy_csr(sp.csr_matrix(Adj.T)), 1))

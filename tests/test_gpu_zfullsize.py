@@ -13,7 +13,8 @@ distances) at full size:
   pull loop, parents checked as BFS trees;
 - Bellman-Ford on the 4096 x 4096 grid with 1% long-range edges, 16
   sources, one at a time and batched, by the exact shortest-path
-  certificate, one source against the restated Jacobi loop;
+  certificate, one source against the restated Jacobi loop, and
+  delta-stepping from the same sources equal to them;
 - batched BFS k = 64 and 256 on R-MAT 20 and 22, k = 32 min-plus on
   weighted R-MAT 22, and 512 packed R-MAT graphs of scale 10..18 (the
   persistent launch and the direction-optimising traversal);
@@ -194,6 +195,15 @@ def test_cfg_grid4096_bellman_ford_16_sources(env):
     print_case("grid4096 batched bf k=16")
     D = gg.shortest_path(A, srcs)
     np.testing.assert_array_equal(_host(D), plain)
+    del D
+    # delta-stepping over the CSR of A (weights differ per direction)
+    S = Gd.to_scipy().T.tocsr()
+    S.sort_indices()
+    A_out = gg.SparseGPUMatrixCSR(S, R0)
+    for i, s in enumerate(srcs):
+        if i % 4 == 0:
+            print_case("grid4096 delta-stepping source", i, s)
+        np.testing.assert_array_equal(_host(gg.shortest_path_delta(A_out, int(s))), plain[:, i])
 
 
 @pytest.mark.parametrize("scale", [20, 22])
