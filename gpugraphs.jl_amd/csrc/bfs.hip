@@ -279,8 +279,19 @@ struct PushArgs {
   int2* big;            // pieces (vertex, first edge) of the hubs in the queue
 };
 
+// Claimed vertices staged per block in LDS with their out-degree sum: one
+// device-wide atomic per counter per flush instead of one per wavefront (a
+// counter that every wavefront of the 8 XCDs bumps serialises the level).
+constexpr int kPushBuf = 2048;
+struct PushOut {
+  int32_t buf[kPushBuf];
+  unsigned cnt;
+  unsigned base;
+  unsigned long long deg;
+};
+
 // Edge u -> v (0-based v) when valid.  All 64 lanes must call it.
-__device__ __forceinline__ void push_edge(const PushArgs& a, bool valid, int32_t v) {
+__device__ __forceinline__ void push_edge(const PushArgs& a, bool valid, int32_t v, PushOut& o) {
   bool newly = false;
   if (valid) {
     const int64_t r = a.vinv ? (int64_t)a.vinv[v] - 1 : v;
@@ -295,14 +306,38 @@ __device__ __forceinline__ void push_edge(const PushArgs& a, bool valid, int32_t
     const int wl = threadIdx.x & 63;
     const int leader = __ffsll(b) - 1;
     unsigned long long deg = newly ? (unsigned long long)(a.rowptr[v + 1] - a.rowptr[v]) : 0ull;
-    for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o, 64);
+    for (int o2 = 32; o2 > 0; o2 >>= 1) deg += __shfl_xor(deg, o2, 64);
     unsigned base = 0;
     if (wl == leader) {
-      base = atomicAdd(&a.cnt->qlen, (unsigned)__popcll(b));
-      atomicAdd(&a.cnt->scout, deg);
+      base = atomicAdd(&o.cnt, (unsigned)__popcll(b));
+      atomicAdd(&o.deg, deg);
     }
     base = __shfl(base, leader, 64);
-    if (newly) a.next_queue[base + __popcll(b & ((1ull << wl) - 1))] = v;
+    if (newly) o.buf[base + __popcll(b & ((1ull << wl) - 1))] = v;
+  }
+}
+
+// Every thread of the block calls it between rounds (a round stages at most
+// kB vertices): flushes when another round might not fit, or always when
+// `last`.
+__device__ void push_flush(const PushArgs& a, PushOut& o, bool last) {
+  __syncthreads();
+  const unsigned n = o.cnt;
+  const bool go = n > 0 && (last || n > (unsigned)(kPushBuf - kB));
+  if (go) {
+    if (threadIdx.x == 0) {
+      o.base = atomicAdd(&a.cnt->qlen, n);
+      atomicAdd(&a.cnt->scout, o.deg);
+    }
+    __syncthreads();
+    const unsigned base = o.base;
+    for (unsigned i = threadIdx.x; i < n; i += kB) a.next_queue[base + i] = o.buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      o.cnt = 0;
+      o.deg = 0;
+    }
+    __syncthreads();
   }
 }
 
@@ -310,7 +345,13 @@ __global__ __launch_bounds__(kB) void push_tile_kernel(PushArgs a) {
   __shared__ int32_t s_beg[kB];
   __shared__ int32_t s_scan[kB + 1];
   __shared__ int32_t s_wsum[kB / 64];
+  __shared__ PushOut o;
   const int wl = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if ((int64_t)blockIdx.x * a.tile >= a.qlen) return;
+  if (threadIdx.x == 0) {
+    o.cnt = 0;
+    o.deg = 0;
+  }
   for (int64_t t0 = (int64_t)blockIdx.x * a.tile; t0 < a.qlen;
        t0 += (int64_t)gridDim.x * a.tile) {
     const int64_t qi = t0 + threadIdx.x;
@@ -353,23 +394,33 @@ __global__ __launch_bounds__(kB) void push_tile_kernel(PushArgs a) {
         }
         v = a.colval[s_beg[lo] + (e - s_scan[lo])] - 1;
       }
-      push_edge(a, valid, v);
+      push_edge(a, valid, v, o);
+      push_flush(a, o, false);
     }
     __syncthreads();  // the next tile reuses the LDS arrays
   }
+  push_flush(a, o, true);
 }
 
 __global__ __launch_bounds__(kB) void push_big_kernel(PushArgs a) {
+  __shared__ PushOut o;
   const int64_t nbig = __atomic_load_n(&a.cnt->nbig, __ATOMIC_RELAXED);  // from push_tile_kernel
+  if (blockIdx.x >= nbig) return;
+  if (threadIdx.x == 0) {
+    o.cnt = 0;
+    o.deg = 0;
+  }
   for (int64_t p = blockIdx.x; p < nbig; p += gridDim.x) {
     const int2 pc = a.big[p];
     const int32_t stop = min(pc.y + kBigDeg, a.rowptr[pc.x + 1] - 1);
     for (int32_t i0 = pc.y; i0 < stop; i0 += kB) {
       const int32_t i = i0 + (int32_t)threadIdx.x;
       const bool valid = i < stop;
-      push_edge(a, valid, valid ? a.colval[i] - 1 : 0);
+      push_edge(a, valid, valid ? a.colval[i] - 1 : 0, o);
+      push_flush(a, o, false);
     }
   }
+  push_flush(a, o, true);
 }
 
 // Bitmap -> queue of the set bits' original vertices (order not required).
