@@ -429,20 +429,9 @@ __global__ __launch_bounds__(kB) void bitmap_to_queue_kernel(const uint32_t* __r
                                                              int32_t* __restrict__ queue,
                                                              BfsCounters* cnt) {
   const int64_t w = (int64_t)blockIdx.x * kB + threadIdx.x;
-  const int wl = threadIdx.x & 63;
   uint32_t bits = w < words ? bm[w] : 0u;
   if (w == words - 1 && (n & 31)) bits &= (1u << (n & 31)) - 1u;
-  int c = __popc(bits);
-  int incl = c;  // inclusive scan across the wavefront
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(incl, o, 64);
-    if (wl >= o) incl += y;
-  }
-  const int total = __shfl(incl, 63, 64);
-  unsigned base = 0;
-  if (wl == 63 && total) base = atomicAdd(&cnt->qlen, (unsigned)total);
-  base = __shfl(base, 63, 64);
-  unsigned pos = base + incl - c;
+  unsigned pos = block_claim((unsigned)__popc(bits), &cnt->qlen);  // one atomic per block
   while (bits) {
     const int b = __ffs(bits) - 1;
     bits &= bits - 1;

@@ -220,12 +220,41 @@ struct PackedTD {
 
 constexpr int32_t kTdPiece = 1024;  // rows longer than this are cut into pieces
 
+// Claims of a wavefront staged in LDS (the count and degree sum in
+// wave-uniform registers): one device-wide atomic per counter per flush
+// instead of one per 64-entry step (a counter that every wavefront of the 8
+// XCDs bumps serialises the level).
+constexpr int kTdBuf = 512;
+struct TdStage {
+  int32_t* buf;             // this wavefront's kTdBuf entries in LDS
+  unsigned cnt;
+  unsigned long long deg;
+};
+
+// Flushes when another step might not fit, or always when `last`.  All 64
+// lanes call it with the same stage.
+__device__ void td_flush(const PackedTD& t, TdStage& st, bool last) {
+  if (st.cnt == 0 || (!last && st.cnt <= (unsigned)(kTdBuf - 64))) return;
+  __builtin_amdgcn_wave_barrier();
+  const int lane = threadIdx.x & 63;
+  unsigned long long base = 0;
+  if (lane == 0) {
+    base = atomicAdd(&t.cnt[0], (unsigned long long)st.cnt);
+    atomicAdd(&t.cnt[1], st.deg);
+  }
+  base = __shfl(base, 0, 64);
+  for (unsigned i = lane; i < st.cnt; i += 64) t.nxt[base + i] = st.buf[i];
+  __builtin_amdgcn_wave_barrier();
+  st.cnt = 0;
+  st.deg = 0;
+}
+
 // Entries [b0, b1) of row u (0-based entries) expanded by one wavefront, 64
 // at a time: unvisited neighbours are claimed with an atomic OR on the
 // visited bitmap (a plain read first skips the ones already set), get their
-// level and next bit, and are queued with their degree (one atomic per
-// counter per wavefront step).
-__device__ void td_expand(const PackedTD& t, int32_t u, int32_t b0, int32_t b1, int32_t level) {
+// level and next bit, and are staged with their degree.
+__device__ void td_expand(const PackedTD& t, int32_t u, int32_t b0, int32_t b1, int32_t level,
+                          TdStage& st) {
   const int lane = threadIdx.x & 63;
   bool any = false;
   for (int32_t b = b0; b < b1; b += 64) {
@@ -248,13 +277,10 @@ __device__ void td_expand(const PackedTD& t, int32_t u, int32_t b0, int32_t b1, 
       any = true;
       unsigned long long dsum = deg;
       for (int o = 32; o > 0; o >>= 1) dsum += __shfl_xor(dsum, o, 64);
-      unsigned long long base = 0;
-      if (lane == 0) {
-        base = atomicAdd(&t.cnt[0], (unsigned long long)__popcll(m));
-        atomicAdd(&t.cnt[1], dsum);
-      }
-      base = __shfl(base, 0, 64);
-      if (claim) t.nxt[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+      if (claim) st.buf[st.cnt + __popcll(m & ((1ull << lane) - 1ull))] = v;
+      st.cnt += (unsigned)__popcll(m);
+      st.deg += dsum;
+      td_flush(t, st, false);
     }
   }
   // every neighbour is in u's graph (the blocks are disjoint)
@@ -264,7 +290,9 @@ __device__ void td_expand(const PackedTD& t, int32_t u, int32_t b0, int32_t b1, 
 // One wavefront per queued row; a row longer than kTdPiece entries queues its
 // pieces for packed_td_big_kernel instead.
 __global__ __launch_bounds__(kP) void packed_td_kernel(PackedTD t, int32_t level) {
+  __shared__ int32_t s_buf[kP / 64][kTdBuf];
   const int lane = threadIdx.x & 63;
+  TdStage st{s_buf[threadIdx.x >> 6], 0u, 0ull};
   const int64_t waves = (int64_t)gridDim.x * (kP / 64);
   for (int64_t q = (int64_t)blockIdx.x * (kP / 64) + (threadIdx.x >> 6); q < t.qn; q += waves) {
     const int32_t u = t.cur[q];
@@ -277,19 +305,23 @@ __global__ __launch_bounds__(kP) void packed_td_kernel(PackedTD t, int32_t level
       }
       continue;
     }
-    td_expand(t, u, rs, re, level);
+    td_expand(t, u, rs, re, level, st);
   }
+  td_flush(t, st, true);
 }
 
 // One wavefront per piece of a long row.
 __global__ __launch_bounds__(kP) void packed_td_big_kernel(PackedTD t, int32_t level) {
+  __shared__ int32_t s_buf[kP / 64][kTdBuf];
+  TdStage st{s_buf[threadIdx.x >> 6], 0u, 0ull};
   const int64_t np = (int64_t)t.cnt[3];
   const int64_t waves = (int64_t)gridDim.x * (kP / 64);
   for (int64_t q = (int64_t)blockIdx.x * (kP / 64) + (threadIdx.x >> 6); q < np; q += waves) {
     const int2 pc = t.pieces[q];
     const int32_t re = t.rowptr[pc.x + 1] - 1;
-    td_expand(t, pc.x, pc.y, min(pc.y + kTdPiece, re), level);
+    td_expand(t, pc.x, pc.y, min(pc.y + kTdPiece, re), level, st);
   }
+  td_flush(t, st, true);
 }
 
 // Closes a top-down level per graph (block 0's job in the persistent
