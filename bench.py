@@ -156,7 +156,17 @@ def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma,
         lat_d, dres = _time_queries(torch, one_delta, srcs)
         same_d = all(torch.equal(dres[i], plain[i]) for i in range(k))
         assert same_d, f"SSSP {name}: delta-stepping differs from Bellman-Ford"
-        del dres, A_out
+        del dres
+        # the k sources as one multi-source delta-stepping run (shared phases)
+        gg.shortest_path_delta(A_out, srcs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Dd, ph_k = gg.shortest_path_delta(A_out, srcs, return_phases=True)
+        torch.cuda.synchronize()
+        t_dk = time.perf_counter() - t0
+        same_dk = all(torch.equal(Dd[:, i], plain[i]) for i in range(k))
+        assert same_dk, f"SSSP {name}: multi-source delta-stepping differs from Bellman-Ford"
+        del Dd, A_out
         B_it = 8 * H.nnz + 4 * (m + 1) + 4 * m + 8 * m  # one full Jacobi iteration
         out[name] = dict(n=m, nnz=H.nnz, sources=k, plain_qps=k / sum(lat),
                          plain_p50_ms=pct(lat, 50) * 1e3, plain_p99_ms=pct(lat, 99) * 1e3,
@@ -168,13 +178,18 @@ def sssp_leg(args, np, torch, gg, G, R0, ctx, _lib, dev, Rg, roots, seed, sigma,
                          delta_qps=k / sum(lat_d), delta_p50_ms=pct(lat_d, 50) * 1e3,
                          delta_p99_ms=pct(lat_d, 99) * 1e3, delta_phases_mean=float(np.mean(phases)),
                          delta_vs_plain=(k / sum(lat_d)) / (k / sum(lat)),
+                         delta_batched_qps=k / t_dk, delta_batched_ms=t_dk * 1e3,
+                         delta_batched_phases=ph_k,
+                         delta_batched_vs_plain=(k / t_dk) / (k / sum(lat)),
                          exact_certificate=exact)
         log(f"SSSP {name}: plain {k / sum(lat):.1f} q/s (p99 {pct(lat, 99) * 1e3:.1f} ms, "
             f"{np.mean(iters):.0f} iters, {kms / max(kn, 1) * 1e3:.0f} us/iter); batched k={k} "
             f"{k / t_b:.1f} q/s ({it_b} iters) = {(k / t_b) / (k / sum(lat)):.1f}x, "
             f"sparse alpha {sa.best['alpha']}; delta-stepping {k / sum(lat_d):.1f} q/s "
             f"(p99 {pct(lat_d, 99) * 1e3:.1f} ms, {np.mean(phases):.0f} phases) = "
-            f"{(k / sum(lat_d)) / (k / sum(lat)):.1f}x one-source Bellman-Ford")
+            f"{(k / sum(lat_d)) / (k / sum(lat)):.1f}x one-source Bellman-Ford; {k} sources in one "
+            f"delta-stepping run {k / t_dk:.1f} q/s ({ph_k} phases) = "
+            f"{(k / t_dk) / (k / sum(lat)):.1f}x")
         del plain
     _lib.set_sssp_threshold(ctx, 16.0)
     return out

@@ -187,6 +187,7 @@ SIGNATURES = {
     "gg_bfs_parents": ([_vp, _PM, _i64, _vp, _i64], _i32),
     "gg_bfs_distances_do": ([_vp, _PM, _PM, _i64, _vp, _d, _d], _i32),
     "gg_shortest_path_delta": ([_vp, _PM, _i64, _vp, _i32, _d, _vp], _i32),
+    "gg_shortest_path_delta_multi": ([_vp, _PM, _vp, _i64, _vp, _i32, _d, _vp], _i32),
     "gg_shortest_path": ([_vp, _PM, _i64, _vp, _i32, C.POINTER(_i64)], _i32),
     "gg_shortest_path_multi": ([_vp, _PM, _vp, _i64, _vp, _i32, C.POINTER(_i64)], _i32),
     "gg_msbfs_distances": ([_vp, _PM, _vp, _i64, _vp, C.POINTER(_i64)], _i32),

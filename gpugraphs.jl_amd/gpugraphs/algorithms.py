@@ -167,17 +167,27 @@ def shortest_path_delta(A_out, source, delta=0.0, return_phases=False):
     """The distances of shortest_path by delta-stepping: push over A_out, the
     CSR of A (row u = out-neighbours of u with the weights of u -> v; for a
     symmetric matrix, A_T itself).  delta = 0 takes 4 x the mean weight.
-    Weights reached from the source must be non-negative."""
+    Weights reached from the source must be non-negative.  One source: a
+    length-m vector; a sequence of k sources: an m x k column-major matrix
+    whose columns run together (shared phases)."""
     if not isinstance(A_out, SparseGPUMatrixCSR):
         raise ArgumentError("delta-stepping runs on a SparseGPUMatrixCSR of A")
     dev = _dev(A_out)
     dt = dist_dtype(A_out)
-    dist = _empty(A_out.m, dt, dev)
     d = A_out.descriptor()
     ph = C.c_int64()
-    check(lib().gg_shortest_path_delta(context(dev), C.byref(d), int(source), ptr(dist),
-                                       dtype_code(dt), float(delta), C.byref(ph)),
-          "shortest_path_delta")
+    if np.ndim(source) == 0:
+        dist = _empty(A_out.m, dt, dev)
+        check(lib().gg_shortest_path_delta(context(dev), C.byref(d), int(source), ptr(dist),
+                                           dtype_code(dt), float(delta), C.byref(ph)),
+              "shortest_path_delta")
+    else:
+        src = np.ascontiguousarray(source, np.int32)
+        k = len(src)
+        dist = _empty(A_out.m * k, dt, dev).view(k, A_out.m).T  # column-major m x k
+        check(lib().gg_shortest_path_delta_multi(context(dev), C.byref(d), src.ctypes.data, k,
+                                                 ptr(dist), dtype_code(dt), float(delta),
+                                                 C.byref(ph)), "shortest_path_delta")
     return (dist, ph.value) if return_phases else dist
 
 

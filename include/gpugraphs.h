@@ -425,6 +425,15 @@ int gg_shortest_path_multi(gg_ctx* ctx, const gg_matrix* A_T, const int32_t* sou
  * phases (optional) receives the expansion phases run. */
 int gg_shortest_path_delta(gg_ctx* ctx, const gg_matrix* A, int64_t source, void* dist,
                            int dist_dtype, double delta, int64_t* phases);
+/* k delta-stepping runs (1 <= k <= 1024, k * m < 2^32) that share their
+ * phases: the expansions of every source run in the same launches, so a
+ * phase's fixed cost is paid once for all k.  sources (host, 1-based) and
+ * the m x k column-major dist as in gg_shortest_path_multi; column j equals
+ * gg_shortest_path_delta from sources[j].  The batched form of the reference's
+ * k-source shortest_path (src/algorithms/shortest_path.jl:88-161). */
+int gg_shortest_path_delta_multi(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources,
+                                 int64_t k, void* dist, int dist_dtype, double delta,
+                                 int64_t* phases);
 
 /* ------------------------------------------------------------ batched form */
 /* Multi-source BFS as or-and SpMM over 64-bit source bitmasks: k sources

@@ -1119,6 +1119,15 @@ def test_shortest_path_delta_vs_oracle(env, steps):
                     d = gg.shortest_path_delta(A_out, s, delta)
                     assert d.dtype == torch.from_numpy(ref[:1]).dtype
                     np.testing.assert_array_equal(host(d), ref, err_msg=f"{name} {s} {delta}")
+            # k sources sharing their phases (a repeated source included)
+            ks = np.array(srcs + [int(x) for x in rng.integers(1, Adj.shape[0] + 1, 6)] + [srcs[0]],
+                          np.int32)
+            ref_k = orc.shortest_path_multi(Ao, ks)
+            for delta in (1.0, 0.0, 1e12):
+                print_case("sssp delta multi", name, steps, len(ks), delta)
+                D = gg.shortest_path_delta(A_out, ks, delta)
+                assert D.shape == (Adj.shape[0], len(ks))
+                np.testing.assert_array_equal(host(D), ref_k, err_msg=f"{name} k {delta}")
         # Bool: hop counts as Int32
         Adj = sp.csr_matrix(G.random_digraph(rng, 800, 0.004, dtype=np.bool_))
         A_out = gg.SparseGPUMatrixCSR(Adj, gg.ROCBackend(0))
@@ -1133,9 +1142,10 @@ def test_shortest_path_delta_vs_oracle(env, steps):
 
 
 def test_shortest_path_delta_errors(env):
-    """A negative weight reached from the source is an ArgumentError (one that
+    """A negative weight reached from a source is an ArgumentError (one that
     is not reached does not matter); a source outside 1:n is a BoundsError; a
-    SELL matrix or a negative delta is an ArgumentError."""
+    SELL matrix, a negative delta or more than 1024 sources is an
+    ArgumentError."""
     gg, torch, orc = env
     rng = np.random.default_rng(32)
     Adj = sp.csr_matrix(G.random_digraph(rng, 200, 0.03, wmax=9, dtype=np.float32))
@@ -1161,6 +1171,12 @@ def test_shortest_path_delta_errors(env):
         gg.shortest_path_delta(A, 1, -1.0)
     with pytest.raises(gg.ArgumentError):
         gg.shortest_path_delta(gg.SparseGPUMatrixSELL(Adj, gg.ROCBackend(0), 32), 1)
+    with pytest.raises(gg.BoundsError):
+        gg.shortest_path_delta(A, [1, 201])
+    with pytest.raises(gg.ArgumentError):
+        gg.shortest_path_delta(A, np.ones(1025, np.int32))
+    with pytest.raises(gg.ArgumentError):
+        gg.shortest_path_delta(A_neg, [1, r0 + 1])
     # the context still works after the errors
     np.testing.assert_array_equal(host(gg.shortest_path_delta(A, 1)),
                                   orc.shortest_path(orc.from_scipy_csr(sp.csr_matrix(Adj.T)), 1))
