@@ -486,8 +486,11 @@ __global__ __launch_bounds__(kB) void bfs_seeds_kernel(int32_t* out, uint32_t* v
 
 // Level count of each packed graph = the largest finite level among its rows
 // (offsets: n_graphs + 1 rising row offsets).  A wavefront walks kLvRows
-// consecutive rows 64 at a time, keeps a running maximum while its rows stay
-// in one graph and folds it into levels[g] with one atomic max per graph met.
+// consecutive rows 64 at a time: lane 0 finds the graph of the first row by
+// binary search, each lane steps from the chunk's first graph across the
+// boundaries below its row (rarely more than one), and the wavefront keeps a
+// running maximum while its rows stay in one graph, folding it into
+// levels[g] with one atomic max per graph it meets.
 constexpr int kLvRows = 4096;
 __global__ __launch_bounds__(kB) void graph_levels_kernel(const int32_t* __restrict__ dist, int64_t m,
                                                           const int64_t* __restrict__ offsets,
@@ -497,36 +500,42 @@ __global__ __launch_bounds__(kB) void graph_levels_kernel(const int32_t* __restr
   const int64_t r0 = wave * kLvRows;
   if (r0 >= m) return;
   const int64_t r1 = r0 + kLvRows < m ? r0 + kLvRows : m;
+  int gb = 0;
+  if (lane == 0) {
+    int lo = 0, hi = n_graphs;  // offsets[lo] <= r0 < offsets[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (offsets[mid] <= r0) lo = mid; else hi = mid;
+    }
+    gb = lo;
+  }
+  gb = __shfl(gb, 0, 64);  // graph of the chunk's first row
   int g_run = -1;
   int32_t mx = -1;
   for (int64_t b = r0; b < r1; b += 64) {
     const int64_t r = b + lane;
-    int g = -1;
+    int g = gb;
     int32_t d = -1;
     if (r < r1) {
-      int lo = 0, hi = n_graphs;  // offsets[lo] <= r < offsets[hi]
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (offsets[mid] <= r) lo = mid; else hi = mid;
-      }
-      g = lo;
+      while (offsets[g + 1] <= r) ++g;  // r < offsets[n_graphs] = m ends it
       d = dist[r];
       if (d == 2147483647) d = -1;
     }
-    const int g0 = __shfl(g, 0, 64);
-    const bool uniform = __ballot(r < r1 && g != g0) == 0ull;
+    const bool uniform = __ballot(r < r1 && g != gb) == 0ull;
     if (uniform) {
       int32_t v = d;
       for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-      if (g0 != g_run) {
+      if (gb != g_run) {
         if (lane == 0 && g_run >= 0 && mx >= 0) atomicMax(&levels[g_run], mx);
-        g_run = g0;
+        g_run = gb;
         mx = -1;
       }
       mx = max(mx, v);
     } else {  // rows of two or more graphs: each lane folds its own row
-      if (g >= 0 && d >= 0) atomicMax(&levels[g], d);
+      if (r < r1 && d >= 0) atomicMax(&levels[g], d);
     }
+    // the next chunk starts in the graph of this chunk's last row
+    gb = __shfl(r < r1 ? g : gb, 63, 64);
   }
   if (lane == 0 && g_run >= 0 && mx >= 0) atomicMax(&levels[g_run], mx);
 }
