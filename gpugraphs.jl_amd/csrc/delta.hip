@@ -592,7 +592,8 @@ static int delta_run(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources, in
       const double mean = need_mean ? wsum / (double)A->nnz : 1.0;
       d = kDeltaScale * (mean > 0.0 ? mean : 1.0);
     }
-    const int64_t max_phases = 8 * n + 1024;
+    // phases stamp Int32s: the cap stays far below 2^31
+    const int64_t max_phases = std::min<int64_t>(8 * n + 1024, int64_t(1) << 30);
     hipLaunchKernelGGL(ds_fill_kernel<TD>, dim3(blocks_for((int64_t)kn, kDs)), dim3(kDs), 0,
                        ctx->stream, (TD*)dist, (int64_t)kn, Lim<TD>::max());
     hipLaunchKernelGGL(ds_fill_kernel<int32_t>, dim3(blocks_for((int64_t)kn, kDs)), dim3(kDs), 0,

@@ -204,6 +204,8 @@ def test_cfg_grid4096_bellman_ford_16_sources(env):
         if i % 4 == 0:
             print_case("grid4096 delta-stepping source", i, s)
         np.testing.assert_array_equal(_host(gg.shortest_path_delta(A_out, int(s))), plain[:, i])
+    print_case("grid4096 delta-stepping, 16 sources in one run")
+    np.testing.assert_array_equal(_host(gg.shortest_path_delta(A_out, srcs)), plain)
 
 
 @pytest.mark.parametrize("scale", [20, 22])
