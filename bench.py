@@ -59,7 +59,7 @@ def measured_traffic(args, relabel, kernel):
     measured (the default R-MAT 22 configuration), else None."""
     default = (args.scale == 22 and args.roots == 64 and relabel and args.heavy == 1024 and
                args.bfs_seg == 65536 and args.seg == 4096 and args.push_alpha == 16.0 and
-               args.sort_entries)
+               args.sort_entries and not args.rcm)
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if not default or not os.path.exists(path):
         return None
@@ -650,6 +650,9 @@ def main():
     ap.add_argument("--no-sort-entries", dest="sort_entries", action="store_false",
                     help="keep each row's entries in input order (default: by increasing stored "
                          "column: light rows by degree, heavy rows last)")
+    ap.add_argument("--rcm", action="store_true",
+                    help="relabelled R-MAT layouts put equal-degree rows in reverse Cuthill-McKee "
+                         "order (GG_SELL_RCM; default: row order)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--grid", type=int, default=4096, help="Bellman-Ford grid side")
     ap.add_argument("--sssp-sources", type=int, default=16, help="grid Bellman-Ford sources")
@@ -708,11 +711,12 @@ def main():
     log(f"rmat{args.scale}: n={n} nnz={nnz} max_deg={deg.max()} gen {time.time() - t:.1f}s")
 
     t = time.time()
+    rcm = relabel and args.rcm   # equal-degree rows in reverse Cuthill-McKee order
     A_f = Rg.to_sell(R0, 64, sigma, args.heavy, args.seg, relabel,   # fp32 values for SpMV
-                     sort_entries=args.sort_entries)
+                     sort_entries=args.sort_entries, rcm=rcm)
     Rb = G.HostCSR(n, n, Rg.rowptr, Rg.colval, np.ones(nnz, np.bool_), Rg.symmetric)
     A_b = Rb.to_sell(R0, 64, sigma, args.heavy, args.bfs_seg, relabel,   # Bool pattern for BFS
-                     sort_entries=args.sort_entries)
+                     sort_entries=args.sort_entries, rcm=rcm)
     A_csr = Rb.to_csr(R0)                                          # out-edges for push (symmetric)
     torch.cuda.synchronize()
     host_build_s = time.time() - t
@@ -726,9 +730,12 @@ def main():
                                                  relabel, args.sort_entries)
     torch.cuda.synchronize()
     dev_build_ms = (time.perf_counter() - t) * 1e3
-    assert torch.equal(A_d.slice_ptr, A_b.slice_ptr) and torch.equal(A_d.colval, A_b.colval), \
+    # the device builder has no RCM ties: compare it with the host layout without them
+    A_h = Rb.to_sell(R0, 64, sigma, args.heavy, args.bfs_seg, relabel,
+                     sort_entries=args.sort_entries) if rcm else A_b
+    assert torch.equal(A_d.slice_ptr, A_h.slice_ptr) and torch.equal(A_d.colval, A_h.colval), \
         "device-built SELL differs from the host-built one"
-    del A_d
+    del A_d, A_h
     log(f"SELL build on the device from the resident CSR: {dev_build_ms:.1f} ms")
 
     rng = np.random.default_rng(seed)
@@ -1053,9 +1060,12 @@ def main():
                                f"batched BFS from {args.roots} random non-isolated roots "
                                "(one step = one k=64 or-and SpMM traversal); SELL-64 "
                                + ("degree-relabelled" if relabel else "sigma4096")
-                               + (", row entries by column" if args.sort_entries else ""),
+                               + (", row entries by column" if args.sort_entries else "")
+                               + (", equal-degree rows in RCM order" if relabel and args.rcm
+                                  else ""),
                    "n": n, "nnz": nnz, "roots": args.roots, "slice_size": 64, "sigma": sigma,
-                   "relabel": relabel, "sorted_entries": args.sort_entries, "heavy_threshold": args.heavy,
+                   "relabel": relabel, "sorted_entries": args.sort_entries,
+                   "rcm_ties": relabel and args.rcm, "heavy_threshold": args.heavy,
                    "graph_steps": args.graph_steps, "push_alpha": args.push_alpha,
                    "parallelism": "single GPU"},
         "p50_ms": None if not ms else round(ms["p50_ms"], 3),

@@ -411,6 +411,8 @@ int gg_sell_plan_device(gg_ctx* ctx, int64_t m, int64_t n, const int32_t* rowptr
   if (slice_size < 1) return set_error(GG_ERR_ARGUMENT, "slice_size must be >= 1");
   const bool relabel = (flags & GG_SELL_RELABEL) != 0;
   if (relabel && m != n) return set_error(GG_ERR_ARGUMENT, "relabelling needs a square matrix");
+  if (flags & GG_SELL_RCM)
+    return set_error(GG_ERR_ARGUMENT, "reverse Cuthill-McKee ties are built on the host (gg_sell_build)");
   if (heavy_threshold > 0 && heavy_seg_len < 1)
     return set_error(GG_ERR_ARGUMENT, "heavy_seg_len must be >= 1");
   if (m > 2147483646LL) return set_error(GG_ERR_ARGUMENT, "rows exceed Int32");

@@ -110,6 +110,10 @@ struct SellArgs {
   const int32_t* heavy_col;
   const void* heavy_val;
   void* partial;  // [n_seg] accumulators of multi-segment rows
+  // Relabelled layouts: each row's value (the operator type, before accum)
+  // goes to stage[internal id], coalesced, and stage_to_rows_kernel moves it
+  // to the row's original id afterwards (NULL: results go straight to c)
+  void* stage;
   int internal;       // columns are internal ids (slot, then slot_base + heavy index)
   int64_t slot_base;  // nslices * C
 };
@@ -194,7 +198,8 @@ __global__ __launch_bounds__(kBlock) void sell_spmv_kernel(
         acc = T::MO::op(acc, term<S, TA, TX>(val[i], x[col - 1], mul_col<S>(g, col)));
       }
     }
-    store_acc<typename T::TC, typename T::CT>(c, row, accum, acc);
+    if (g.stage) ((typename T::CT*)g.stage)[slot] = typename T::CT(acc);
+    else store_acc<typename T::TC, typename T::CT>(c, row, accum, acc);
     return;
   }
   // heavy segment
@@ -231,8 +236,9 @@ __global__ __launch_bounds__(kBlock) void sell_spmv_kernel(
   }
   acc = block_reduce<typename T::MO>(acc, lds);
   if (threadIdx.x == 0) {
-    if (nseg == 1) store_acc<typename T::TC, typename T::CT>(c, row, accum, acc);
-    else ((A*)g.partial)[sg] = acc;
+    if (nseg > 1) ((A*)g.partial)[sg] = acc;
+    else if (g.stage) ((typename T::CT*)g.stage)[g.slot_base + lo] = typename T::CT(acc);
+    else store_acc<typename T::TC, typename T::CT>(c, row, accum, acc);
   }
 }
 
@@ -252,7 +258,24 @@ __global__ __launch_bounds__(kBlock) void heavy_combine_kernel(
   const A* p = (const A*)g.partial;
   A acc = T::MO::neutral();
   for (int64_t j = s0; j < s1; ++j) acc = T::MO::op(acc, p[j]);
-  store_acc<typename T::TC, typename T::CT>(c, row, accum, acc);
+  if (g.stage) ((typename T::CT*)g.stage)[g.slot_base + h] = typename T::CT(acc);
+  else store_acc<typename T::TC, typename T::CT>(c, row, accum, acc);
+}
+
+// The staged values (internal ids) to the rows they belong to: one thread per
+// original row in the range whose mask is on, so c is read and written
+// coalesced and the staged values are gathered (the same rows the SpMV
+// kernels computed; accum is applied here, as store_acc would have).
+template <class TC, class CT>
+__global__ __launch_bounds__(kBlock) void stage_to_rows_kernel(TC* __restrict__ c,
+                                                               const CT* __restrict__ stage,
+                                                               const int32_t* __restrict__ vinv,
+                                                               int64_t r_lo, int64_t r_hi,
+                                                               int accum, const void* __restrict__ mask,
+                                                               int mdt) {
+  const int64_t row = r_lo + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (row >= r_hi || !mask_on(mask, mdt, row)) return;
+  store_acc<TC, CT>(c, row, accum, stage[vinv[row] - 1]);
 }
 
 // ------------------------------------------------------------------ CSC
@@ -354,6 +377,12 @@ int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const voi
       g.heavy_val = A->heavy_val;
       GG_TRY(ws_get(ctx, WS_PARTIAL, sizeof(typename T::A) * (size_t)n_seg, &g.partial));
     }
+    if (g.internal) {
+      void* st;
+      GG_TRY(ws_get(ctx, WS_SPMSPV2, sizeof(typename T::CT) * (size_t)(g.slot_base + A->n_heavy),
+                    &st));
+      g.stage = st;
+    }
     const int64_t nb = g.n_sell_blocks + n_seg;
     if (nb > 0x7fffffff) return set_error(GG_ERR_ARGUMENT, "SELL launch too large");
     {
@@ -365,6 +394,13 @@ int launch_spmv(gg_ctx* ctx, void* C, int c_dtype, const gg_matrix* A, const voi
     if (n_seg > A->n_heavy) {
       hipLaunchKernelGGL((heavy_combine_kernel<S, TA, TX>), dim3(blocks_for(A->n_heavy, kBlock)),
                          dim3(kBlock), 0, ctx->stream, (TC*)C, g, accum, mask, mask_dtype);
+      GG_HIP(hipGetLastError());
+    }
+    if (g.stage && nrows > 0) {
+      hipLaunchKernelGGL((stage_to_rows_kernel<TC, typename T::CT>),
+                         dim3(blocks_for(nrows, kBlock)), dim3(kBlock), 0, ctx->stream, (TC*)C,
+                         (const typename T::CT*)g.stage, A->vinv, g.r_lo, g.r_hi, accum, mask,
+                         mask_dtype);
       GG_HIP(hipGetLastError());
     }
     return GG_OK;

@@ -104,6 +104,72 @@ void gg_sell_host_free(gg_sell_host* h) {
   delete h;
 }
 
+}  // extern "C"
+
+namespace gg {
+
+// Reverse Cuthill-McKee order (George and Liu, "Computer Solution of Large
+// Sparse Positive Definite Systems", 1981): components are taken from their
+// unvisited vertex of lowest degree (ties by id), each is walked breadth
+// first with every vertex's unvisited neighbours appended by increasing
+// degree (ties by id), and the whole sequence is reversed.  Neighbours are
+// the row's columns (for a symmetric pattern, the vertex's neighbours).
+// out[i] = 1-based vertex at position i.
+static void rcm_order(int64_t m, const int32_t* rowptr, const int32_t* colval, int32_t* out) {
+  std::vector<int32_t> deg(m);
+  int32_t maxd = 0;
+  for (int64_t v = 0; v < m; ++v) {
+    deg[v] = rowptr[v + 1] - rowptr[v];
+    maxd = std::max(maxd, deg[v]);
+  }
+  // vertices by increasing degree, ties by id (counting sort)
+  std::vector<int64_t> cnt((size_t)maxd + 2, 0);
+  for (int64_t v = 0; v < m; ++v) ++cnt[(size_t)deg[v] + 1];
+  for (size_t d = 1; d < cnt.size(); ++d) cnt[d] += cnt[d - 1];
+  std::vector<int32_t> by_deg(m);
+  for (int64_t v = 0; v < m; ++v) by_deg[cnt[deg[v]]++] = (int32_t)v;
+  std::vector<uint8_t> seen(m, 0);
+  std::vector<int32_t> nb;
+  int64_t head = 0, tail = 0;  // out[0 .. m) is the breadth-first queue
+  for (int64_t i = 0; i < m; ++i) {
+    const int32_t s0 = by_deg[i];
+    if (seen[s0]) continue;
+    seen[s0] = 1;
+    out[tail++] = s0;
+    while (head < tail) {
+      const int32_t u = out[head++];
+      nb.clear();
+      for (int64_t e = rowptr[u] - 1; e < rowptr[u + 1] - 1; ++e) {
+        const int32_t c = colval[e] - 1;
+        if (c >= 0 && c < m && !seen[c]) {
+          seen[c] = 1;
+          nb.push_back(c);
+        }
+      }
+      std::sort(nb.begin(), nb.end(), [&](int32_t x, int32_t y) {
+        return deg[x] != deg[y] ? deg[x] < deg[y] : x < y;
+      });
+      for (int32_t c : nb) out[tail++] = c;
+    }
+  }
+  std::reverse(out, out + m);
+  for (int64_t i = 0; i < m; ++i) out[i] += 1;
+}
+
+}  // namespace gg
+
+extern "C" {
+
+int gg_rcm_order(int64_t m, const int32_t* rowptr, const int32_t* colval, int32_t* order) {
+  if (m < 0 || (m > 0 && (!rowptr || !order || (!colval && rowptr[m] > 1))))
+    return set_error(GG_ERR_ARGUMENT, "NULL or negative argument");
+  clear_error();
+  if (m > 2147483646LL) return set_error(GG_ERR_ARGUMENT, "rows exceed Int32");
+  if (m > 0 && rowptr[0] != 1) return set_error(GG_ERR_ARGUMENT, "rowptr[1] must be 1");
+  rcm_order(m, rowptr, colval, order);
+  return GG_OK;
+}
+
 int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* colval,
                   const void* nzval, int val_dtype, int64_t slice_size, int64_t sigma,
                   int64_t heavy_threshold, int64_t heavy_seg_len, int flags,
@@ -115,6 +181,8 @@ int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* co
   if (rowptr && rowptr[0] != 1) return set_error(GG_ERR_ARGUMENT, "rowptr[1] must be 1");
   const bool relabel = (flags & GG_SELL_RELABEL) != 0;
   if (relabel && m != n) return set_error(GG_ERR_ARGUMENT, "relabelling needs a square matrix");
+  if ((flags & GG_SELL_RCM) && !relabel)
+    return set_error(GG_ERR_ARGUMENT, "GG_SELL_RCM orders the ties of GG_SELL_RELABEL");
   const bool reference_layout = sigma <= 1 && heavy_threshold <= 0 && !relabel;
   if (heavy_threshold > 0 && heavy_seg_len < 1)
     return set_error(GG_ERR_ARGUMENT, "heavy_seg_len must be >= 1");
@@ -143,8 +211,9 @@ int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* co
   } else {
     slot_rows.reserve(m);
     std::vector<int32_t> order(m);
-    std::iota(order.begin(), order.end(), 1);
-    if (relabel)  // every row by decreasing length, ties in row order
+    if (flags & GG_SELL_RCM) rcm_order(m, rowptr, colval, order.data());
+    else std::iota(order.begin(), order.end(), 1);
+    if (relabel)  // every row by decreasing length, ties in row (or RCM) order
       std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
         return (rowptr[x] - rowptr[x - 1]) > (rowptr[y] - rowptr[y - 1]);
       });

@@ -27,6 +27,7 @@ FLAG_INTERNAL_IDS = 2
 FLAG_SYMMETRIC = 4
 SELL_RELABEL = 1
 SELL_SORT_ENTRIES = 2
+SELL_RCM = 4
 
 
 class DimensionMismatch(ValueError):
@@ -153,6 +154,7 @@ SIGNATURES = {
     "gg_matrix_slots": ([_PM, C.POINTER(_i64)], _i32),
     "gg_csr_check": ([_i64, _i64, _vp, _i64, _vp, _i64, _i64], _i32),
     "gg_sell_check": ([_i64, _i64, _i64, _i64, _vp, _i64, _i64], _i32),
+    "gg_rcm_order": ([_i64, _vp, _vp, _vp], _i32),
     "gg_sell_build": ([_i64, _i64, _vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64, _i32,
                        C.POINTER(C.POINTER(GGSellHost))], _i32),
     "gg_sell_host_free": ([C.POINTER(GGSellHost)], None),

@@ -42,11 +42,12 @@ class HostCSR:
         return M
 
     def to_sell(self, backend, slice_size=64, sigma=4096, heavy_threshold=0, heavy_seg_len=4096,
-                relabel=False, sort_entries=False):
+                relabel=False, sort_entries=False, rcm=False):
         from .storage import SparseGPUMatrixSELL
         M = SparseGPUMatrixSELL.__new__(SparseGPUMatrixSELL)
         M._build(self.m, self.n, self.rowptr, self.colval, np.ascontiguousarray(self.nzval),
-                 backend, slice_size, sigma, heavy_threshold, heavy_seg_len, relabel, sort_entries)
+                 backend, slice_size, sigma, heavy_threshold, heavy_seg_len, relabel, sort_entries,
+                 rcm)
         M.symmetric = self.symmetric
         return M
 
@@ -166,3 +167,15 @@ def pack(graphs):
     # a block-diagonal pack of symmetric graphs is symmetric (top-down steps)
     P.symmetric = all(getattr(g, "symmetric", False) for g in graphs)
     return P, offsets
+
+
+def rcm_order(H):
+    """Reverse Cuthill-McKee order of a square HostCSR pattern (gg_rcm_order):
+    1-based vertices, position by position."""
+    if H.m != H.n:
+        raise ValueError("rcm_order needs a square matrix")
+    out = np.zeros(H.m, np.int32)
+    rp = np.ascontiguousarray(H.rowptr, np.int32)
+    cv = np.ascontiguousarray(H.colval, np.int32)
+    check(lib().gg_rcm_order(H.m, rp.ctypes.data, cv.ctypes.data, out.ctypes.data), "rcm_order")
+    return out

@@ -382,10 +382,12 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
     arrays; every operation still takes and returns vectors in original order.
     sort_entries=True stores each row's entries by increasing stored column
     (when relabelled: light rows by decreasing degree, then the heavy rows): folds visit a row's terms in that order.
+    rcm=True (with relabel) lays out rows of equal length in reverse
+    Cuthill-McKee order instead of row order (host builder only).
     """
 
     def __init__(self, *args, slice_size=None, sigma=1, heavy_threshold=0, heavy_seg_len=4096,
-                 relabel=False, sort_entries=False):
+                 relabel=False, sort_entries=False, rcm=False):
         if len(args) == 11:
             self._init_raw(*args)
             return
@@ -409,10 +411,11 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
             colval = (S.indices + 1).astype(np.int32)
             nzval = np.ascontiguousarray(S.data)
         self._build(m, n, rowptr, colval, nzval, backend, int(slice_size), int(sigma),
-                    int(heavy_threshold), int(heavy_seg_len), bool(relabel), bool(sort_entries))
+                    int(heavy_threshold), int(heavy_seg_len), bool(relabel), bool(sort_entries),
+                    bool(rcm))
 
     def _build(self, m, n, rowptr, colval, nzval, backend, slice_size, sigma, heavy_threshold,
-               heavy_seg_len, relabel=False, sort_entries=False):
+               heavy_seg_len, relabel=False, sort_entries=False, rcm=False):
         vdt = nzval.dtype
         if vdt == np.bool_:
             nzval = nzval.view(np.uint8)
@@ -421,7 +424,8 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
                                   dtype_code(vdt), slice_size, sigma, heavy_threshold,
                                   heavy_seg_len,
                                   (_lib.SELL_RELABEL if relabel else 0)
-                                  | (_lib.SELL_SORT_ENTRIES if sort_entries else 0),
+                                  | (_lib.SELL_SORT_ENTRIES if sort_entries else 0)
+                                  | (_lib.SELL_RCM if rcm else 0),
                                   C.byref(out)), "SparseGPUMatrixSELL")
         try:
             h = out.contents
@@ -466,7 +470,7 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
 
     @classmethod
     def build_on_device(cls, A, slice_size=32, sigma=1, heavy_threshold=0, heavy_seg_len=4096,
-                        relabel=False, sort_entries=False):
+                        relabel=False, sort_entries=False, rcm=False):
         """The same layouts as the constructor, built in device memory from a
         SparseGPUMatrixCSR on a ROCBackend (gg_sell_plan_device /
         gg_sell_fill_device): the rows never visit the host."""
@@ -482,7 +486,8 @@ class SparseGPUMatrixSELL(AbstractSparseGPUMatrix):
                                         ptr(A.nzval), dtype_code(vdt), int(slice_size),
                                         int(sigma), int(heavy_threshold), int(heavy_seg_len),
                                         (_lib.SELL_RELABEL if relabel else 0)
-                                        | (_lib.SELL_SORT_ENTRIES if sort_entries else 0),
+                                        | (_lib.SELL_SORT_ENTRIES if sort_entries else 0)
+                                        | (_lib.SELL_RCM if rcm else 0),
                                         C.byref(plan),
                                         C.byref(z)), "SparseGPUMatrixSELL")
         try:

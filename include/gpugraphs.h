@@ -172,7 +172,12 @@ typedef enum gg_sell_flags {
    * light rows by decreasing degree, heavy rows last) instead of the input
    * order.  Folds then visit a row's terms in that order: same results for
    * min / max / or / and, a reordered sum for plus on floats. */
-  GG_SELL_SORT_ENTRIES = 2
+  GG_SELL_SORT_ENTRIES = 2,
+  /* With GG_SELL_RELABEL: rows of equal length are laid out in reverse
+   * Cuthill-McKee order (gg_rcm_order) instead of row order, so neighbouring
+   * light rows share more of their gathered columns.  Host builder only
+   * (gg_sell_build); gg_sell_plan_device rejects it. */
+  GG_SELL_RCM = 4
 } gg_sell_flags;
 
 typedef struct gg_ctx gg_ctx;
@@ -270,6 +275,11 @@ int gg_sell_check(int64_t n, int64_t nslices, int64_t n_stored, int64_t slice_pt
  * heavy_threshold <= 0 give the reference layout, src/storage/SELL.jl:99-181;
  * sigma > 1 sorts rows by length inside windows of sigma rows;
  * heavy_threshold > 0 moves rows longer than it into the heavy partition. */
+/* Reverse Cuthill-McKee order of a square CSR pattern (host arrays, 1-based):
+ * order[i] = the vertex at position i.  Components from their lowest-degree
+ * vertex, neighbours by increasing degree, the sequence reversed.  A layout
+ * aid with no counterpart in the reference. */
+int gg_rcm_order(int64_t m, const int32_t* rowptr, const int32_t* colval, int32_t* order);
 int gg_sell_build(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* colval,
                   const void* nzval, int val_dtype, int64_t slice_size, int64_t sigma,
                   int64_t heavy_threshold, int64_t heavy_seg_len, int flags,

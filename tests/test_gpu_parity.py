@@ -34,7 +34,9 @@ def host(t):
 
 
 # sell_relabel_sorted: each row's entries by increasing internal column
-FORMATS = ["csr", "sell_ref", "sell_hybrid", "sell_relabel", "sell_relabel_sorted"]
+# sell_relabel_rcm: equal-length rows in reverse Cuthill-McKee order
+FORMATS = ["csr", "sell_ref", "sell_hybrid", "sell_relabel", "sell_relabel_sorted",
+           "sell_relabel_rcm"]
 
 
 def make(gg, S, fmt):
@@ -48,7 +50,8 @@ def make(gg, S, fmt):
         if S.shape[0] != S.shape[1]:
             pytest.skip("relabelling needs a square matrix")
         return gg.SparseGPUMatrixSELL(S, R, 64, heavy_threshold=8, heavy_seg_len=16, relabel=True,
-                                      sort_entries=fmt.endswith("_sorted"))
+                                      sort_entries=fmt.endswith(("_sorted", "_rcm")),
+                                      rcm=fmt.endswith("_rcm"))
     # sigma-sorted C=64 slices with a heavy partition cut into 16-entry segments
     return gg.SparseGPUMatrixSELL(S, R, 64, sigma=64, heavy_threshold=8, heavy_seg_len=16)
 
@@ -871,6 +874,22 @@ def test_spmm_power_law_heavy_rows(env, relabel):
                  add=getattr(orc, "OP_" + add.upper()), mul=getattr(orc, "OP_" + mul.upper()),
                  accum=getattr(orc, "OP_" + acc.upper()))
         np.testing.assert_array_equal(host(Cd), Co)
+
+
+def test_sell_device_builder_refuses_rcm(env):
+    """Reverse Cuthill-McKee ties are a host-builder layout: the device builder
+    rejects the flag before any launch, and the context keeps working."""
+    gg, torch, orc = env
+    from gpugraphs import generators as GEN
+    R = GEN.rmat(8, seed=2)
+    A = R.to_csr(gg.ROCBackend(0))
+    print_case("sell device builder refuses rcm")
+    with pytest.raises(gg.ArgumentError):
+        gg.SparseGPUMatrixSELL.build_on_device(A, 64, 1, 0, 4096, True, True, rcm=True)
+    got = gg.SparseGPUMatrixSELL.build_on_device(A, 64, 1, 0, 4096, True, True)
+    Ao = orc.from_scipy_csr(R.to_scipy())
+    s = int(np.nonzero(R.degrees())[0][0]) + 1
+    np.testing.assert_array_equal(host(gg.bfs_distances(got, s)), orc.bfs_distances(Ao, s))
 
 
 @pytest.mark.parametrize("relabel", [False, True])

@@ -202,6 +202,64 @@ def test_sell_relabel_keeps_every_row(heavy):
     _walk_sell_indexing(A)
 
 
+def _bandwidth(S, order):
+    """Largest |pos(i) - pos(j)| over the entries when vertex order[k] (1-based) sits at k."""
+    pos = np.empty(S.shape[0], np.int64)
+    pos[np.asarray(order) - 1] = np.arange(S.shape[0])
+    c = S.tocoo()
+    return int(np.abs(pos[c.row] - pos[c.col]).max()) if c.nnz else 0
+
+
+def _host(S):
+    S = sp.csr_matrix(S)
+    S.sort_indices()
+    return G.HostCSR(S.shape[0], S.shape[1], (S.indptr + 1).astype(np.int32),
+                     (S.indices + 1).astype(np.int32), S.data, True)
+
+
+def test_rcm_order_bandwidth():
+    """gg_rcm_order is a permutation; it lays a shuffled path out with bandwidth
+    1 and a shuffled 20 x 30 grid with bandwidth at most 2 x 20 (a breadth-first
+    level of a grid holds at most one anti-diagonal); isolated vertices and
+    several components are all placed."""
+    rng = np.random.default_rng(9)
+    n = 200
+    path = sp.diags([np.ones(n - 1), np.ones(n - 1)], [-1, 1], format="csr")
+    g = sp.kron(sp.identity(20), sp.diags([np.ones(29), np.ones(29)], [-1, 1])) + \
+        sp.kron(sp.diags([np.ones(19), np.ones(19)], [-1, 1]), sp.identity(30))
+    two = sp.block_diag([path, sp.csr_matrix((7, 7)), g]).tocsr()
+    for S, bound in ((path, 1), (sp.csr_matrix(g), 40), (two, 40)):
+        p = rng.permutation(S.shape[0])
+        Sp = sp.csr_matrix(S)[p][:, p]
+        o = G.rcm_order(_host(Sp))
+        assert sorted(o.tolist()) == list(range(1, S.shape[0] + 1))
+        assert _bandwidth(Sp, o) <= bound
+        assert _bandwidth(Sp, o) < _bandwidth(Sp, np.arange(1, S.shape[0] + 1))
+
+
+def test_sell_relabel_rcm_ties():
+    """GG_SELL_RCM keeps the degree relabelling (slot lengths fall, vinv is a
+    permutation, every row kept) and lays rows of equal length out in the
+    order of gg_rcm_order; it needs relabelling."""
+    R = G.rmat(12, seed=6, dtype=np.float32, values=G.UNIFORM01)
+    A = R.to_sell(CPU, 64, 1, 40, heavy_seg_len=64, relabel=True, rcm=True)
+    o = G.rcm_order(R)
+    rank = np.empty(R.m, np.int64)
+    rank[o - 1] = np.arange(R.m)
+    rows = [p for p in A.perm if p]
+    lens = [l for l, p in zip(A.slot_len, A.perm) if p]
+    assert lens == sorted(lens, reverse=True)
+    for a, b, la, lb in zip(rows, rows[1:], lens, lens[1:]):
+        if la == lb:
+            assert rank[a - 1] < rank[b - 1]
+    assert len(set(int(v) for v in A.vinv)) == R.m   # distinct internal ids
+    S = R.to_scipy()
+    for i, j in [(1, 1), (5, int(S.indices[S.indptr[4]]) + 1), (R.m, R.m)]:
+        assert A[i, j] == S[i - 1, j - 1]
+    with pytest.raises(gg.ArgumentError):
+        R.to_sell(CPU, 64, 1, 40, heavy_seg_len=64, relabel=False, rcm=True)
+
+
 def test_sell_relabel_needs_square():
     S = sp.random(30, 40, density=0.2, format="csr", random_state=np.random.default_rng(1),
                   dtype=np.float32)
