@@ -283,9 +283,11 @@ def register_server(srv):
     _servers.add(srv)
 
 
-def finalizing() -> bool:
-    """True once the interpreter is shutting down (no native calls then)."""
-    return sys.is_finalizing()
+def finalizing(_is_finalizing=sys.is_finalizing) -> bool:
+    """True once the interpreter is shutting down (no native calls then).
+    sys.is_finalizing is bound at definition time, so this still answers
+    when module globals have been cleared."""
+    return _is_finalizing()
 
 
 def context(device: int = 0, stream=None):

@@ -118,10 +118,12 @@ class GraphServer:
     def __exit__(self, *exc):
         self.close()
 
-    def __del__(self):
+    def __del__(self, _finalizing=_lib.finalizing):
         # at interpreter shutdown the atexit teardown has closed every live
-        # server; no native call is made once finalisation has begun
-        if _lib.finalizing():
+        # server; no native call is made once finalisation has begun.  The
+        # check is bound at definition time: module globals may already be
+        # None when a late collection runs this.
+        if not getattr(self, "_h", None) or _finalizing():
             return
         try:
             self.close()
