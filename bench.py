@@ -451,10 +451,13 @@ def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
                    engines_steps_busy_finished_ms_per_step_ms_per_query=engines,
                    device_s_all_engines=round(sum(v[0] * v[3] for v in engines.values()) / 1e3, 3),
                    steps=st["steps"],
+                   peak_arena_gib=round(srv.last_run_peaks["arena_used"] / (1 << 30), 2),
+                   peak_waiting=srv.last_run_peaks["waiting"],
                    checked_alone=checked)
         runs.append(run)
         log(f"serving at {rate:.0f} q/s offered: {len(ok)}/{len(submitted)} queries in {wall:.1f}s "
             f"= {len(ok) / wall:.0f} q/s, p50 {run['p50_ms']:.1f} ms p99 {run['p99_ms']:.1f} ms, "
+            f"peak arena {run['peak_arena_gib']} GiB, peak waiting {run['peak_waiting']}, "
             f"{checked} checked against alone")
     return dict(graphs=[s[0] for s in specs], budget_bytes=budget, runs=runs)
 

@@ -149,7 +149,27 @@ __global__ __launch_bounds__(kM) void msbfs_sell_kernel(SellView g, int64_t n_se
                 for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(c[u] - 1) * W + w];
             full = saturated<W>(acc, sn, a);
           }
-          if (full) kk = len;
+          // the rest of the row (short rows: all of it) in masked batches of
+          // 8: their loads are issued together where the entry-by-entry loop
+          // below waits for each one (grids and other low-degree graphs)
+          for (; kk < len && !full; kk += 8) {
+            int32_t c[8];
+            bool on[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              c[u] = kk + u < len ? __builtin_nontemporal_load(&g.colval[base + (kk + u) * g.C]) : 0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              on[u] = c[u] > 0 && (!test_any || bit_get(a.front_any, c[u] - 1)) &&
+                      (UNIT || edge_on(val, base + (kk + u) * g.C));
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              if (on[u])
+#pragma unroll
+                for (int w = 0; w < W; ++w) acc[w] |= a.front[(int64_t)(c[u] - 1) * W + w];
+            full = saturated<W>(acc, sn, a);
+          }
+          kk = len;
         }
         for (; kk < len; ++kk) {
           const int64_t i = base + kk * g.C;

@@ -158,12 +158,22 @@ inline BfSparse bf_sparse_layout(void* base, const gg_matrix* A, double alpha) {
   return s;
 }
 
+// Column groups a k-source step relaxes: the step's group i is columns
+// g[i] * V .. g[i] * V + V - 1, V = 16 / sizeof(distance) (the 16-byte
+// column kernels); n = 0: every group of the first k columns.  The serving
+// engine lists the groups that hold a busy slot, so a step costs what its
+// busy columns cost wherever finished queries left their slots free.
+struct ColGroups {
+  int32_t n;
+  uint8_t g[32];
+};
+
 // bf_multi_step with sparse iterations when sp is set (cus sizes their grids).
-// Columns 0..k-1 of rows ld elements apart (ld < 0: ld = k); the other
-// columns are left alone.
+// Columns 0..k-1 of rows ld elements apart (ld < 0: ld = k), or only the
+// groups of cols when given; the other columns are left alone.
 int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k, void* d,
                      const uint32_t* chp, uint32_t* chn, unsigned int* changed,
                      unsigned long long* changed_mask, const int32_t* done, const BfSparse* sp,
-                     int cus, int64_t ld = -1);
+                     int cus, int64_t ld = -1, const ColGroups* cols = nullptr);
 
 }  // namespace gg

@@ -552,12 +552,27 @@ static int launch_step(gg_server* s, Engine& e) {
     const int64_t words = (ns + 31) / 32;
     hipLaunchKernelGGL(serve_fill_kernel<uint32_t>, dim3(blocks_for(words, 256)), dim3(256), 0,
                        e.stream, e.chn, words, 0u);
-    // only the columns up to the highest active slot (slots are taken lowest
-    // first), in groups of 4 so that the 16-byte column kernels apply
-    const int hi = 64 - __builtin_clzll(e.active | 1ull);
-    const int64_t kc = std::min<int64_t>(e.slots, (hi + 3) & ~3);
-    GG_TRY(bf_multi_step_sp(e.stream, &e.A, e.dist_dtype, kc, e.buf, e.chp, e.chn, e.cnt_d,
-                            e.mask_d, nullptr, e.sparse ? &e.sp : nullptr, s->num_cus, e.slots));
+    // a SELL engine relaxes the groups of V = 16 / (distance bytes) columns
+    // that hold a busy slot (queries finish in any order, so the busy slots
+    // scatter over the 64); otherwise the columns up to the highest busy
+    // slot (slots are taken lowest first), in groups of 4 so that the
+    // 16-byte column kernels apply
+    ColGroups cg{};
+    if (e.A.format == GG_SELL) {
+      const int V = 16 / (int)e.esz;
+      for (int gi = 0; gi < e.slots / V; ++gi)
+        if ((e.active >> (gi * V)) & ((1ull << V) - 1ull)) cg.g[cg.n++] = (uint8_t)gi;
+    }
+    if (cg.n > 0) {
+      GG_TRY(bf_multi_step_sp(e.stream, &e.A, e.dist_dtype, e.slots, e.buf, e.chp, e.chn,
+                              e.cnt_d, e.mask_d, nullptr, e.sparse ? &e.sp : nullptr, s->num_cus,
+                              e.slots, &cg));
+    } else {
+      const int hi = 64 - __builtin_clzll(e.active | 1ull);
+      const int64_t kc = std::min<int64_t>(e.slots, (hi + 3) & ~3);
+      GG_TRY(bf_multi_step_sp(e.stream, &e.A, e.dist_dtype, kc, e.buf, e.chp, e.chn, e.cnt_d,
+                              e.mask_d, nullptr, e.sparse ? &e.sp : nullptr, s->num_cus, e.slots));
+    }
     std::swap(e.chp, e.chn);
   }
   GG_HIP(hipGetLastError());

@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_
                                                           unsigned int* changed,
                                                           unsigned long long* changed_mask,
                                                           const int32_t* done,
-                                                          const unsigned int* light_off) {
+                                                          const unsigned int* light_off, ColGroups cg) {
   typedef TD VT __attribute__((ext_vector_type(V)));
   if (done && *done) return;
   if (light_off && *light_off) return;
@@ -317,14 +317,14 @@ __global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_
   const int wl = threadIdx.x & 63;
   const int lane = wl & (Lv - 1);
   const int gbase = wl & ~(Lv - 1);
-  const int64_t kq = k / V;
+  const int64_t kq = cg.n ? cg.n : k / V;
   const int64_t jblocks = (kq + Lv - 1) >> logLv;
   const int64_t gpb = kT >> logLv;
   const int64_t q = (int64_t)blockIdx.x * gpb + (threadIdx.x >> logLv);
   const int64_t unit = q / jblocks;
   const int64_t jq = (q - unit * jblocks) * Lv + lane;
   const bool jin = jq < kq;
-  const int64_t j0 = jq * V;
+  const int64_t j0 = (cg.n && jin ? (int64_t)cg.g[jq] : jq) * V;
   unsigned chbits = 0;
   int64_t rid = -1;
   if (unit < n_light) {
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(kT) void bf_multi_heavy_vec_kernel(SellView g, int6
                                                                 uint32_t* __restrict__ chn,
                                                                 unsigned int* changed,
                                                                 unsigned long long* changed_mask,
-                                                                const int32_t* done) {
+                                                                const int32_t* done, ColGroups cg) {
   typedef TD VT __attribute__((ext_vector_type(V)));
   __shared__ VT red[kT];
   __shared__ int s_any;
@@ -425,12 +425,12 @@ __global__ __launch_bounds__(kT) void bf_multi_heavy_vec_kernel(SellView g, int6
   const int lane = threadIdx.x & (Lv - 1);
   const int64_t grp = threadIdx.x >> logLv;
   const int64_t gpb = kT >> logLv;
-  const int64_t kq = k / V;
+  const int64_t kq = cg.n ? cg.n : k / V;
   const int64_t jblocks = (kq + Lv - 1) >> logLv;
   const int64_t unit = (int64_t)blockIdx.x / jblocks;
   const int64_t jq = ((int64_t)blockIdx.x - unit * jblocks) * Lv + lane;
   const bool jin = jq < kq;
-  const int64_t j0 = jq * V;
+  const int64_t j0 = (cg.n && jin ? (int64_t)cg.g[jq] : jq) * V;
   int64_t h, s, e;
   heavy_segment(g, unit / parts, h, s, e);
   s += (unit % parts) * kHeavyVecPart;
@@ -672,7 +672,7 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_vec_kernel(BfSparse s, SellView
                                                             uint32_t* __restrict__ chn,
                                                             unsigned int* changed,
                                                             unsigned long long* changed_mask,
-                                                            const int32_t* done) {
+                                                            const int32_t* done, ColGroups cg) {
   typedef TD VT __attribute__((ext_vector_type(V)));
   if (done && *done) return;
   if (!s.cnt[3]) return;
@@ -680,7 +680,7 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_vec_kernel(BfSparse s, SellView
   const int wl = threadIdx.x & 63;
   const int lane = wl & (Lv - 1);
   const int gbase = wl & ~(Lv - 1);
-  const int64_t kq = k / V;
+  const int64_t kq = cg.n ? cg.n : k / V;
   const int64_t jblocks = (kq + Lv - 1) >> logLv;
   const int64_t gpb = kT >> logLv;
   const int64_t ngroups = (int64_t)s.cnt[1] * jblocks;
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(kT) void bf_sp_rows_vec_kernel(BfSparse s, SellView
     if (q < ngroups) {
       const int64_t unit = q / jblocks;
       const int64_t jq = (q - unit * jblocks) * Lv + lane;
-      j0 = jq * V;
+      j0 = (cg.n && jq < kq ? (int64_t)cg.g[jq] : jq) * V;
       v = s.dq[unit];
       if (jq < kq) {
         const StateRow r = state_row(g, false, nullptr, v);
@@ -994,7 +994,7 @@ int bf_multi_step(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_
 int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int64_t k, void* d,
                      const uint32_t* chp, uint32_t* chn, unsigned int* changed,
                      unsigned long long* changed_mask, const int32_t* done, const BfSparse* sp,
-                     int cus, int64_t ld) {
+                     int cus, int64_t ld, const ColGroups* cols) {
   if (ld < 0) ld = k;
   if (ld < k) return set_error(GG_ERR_ARGUMENT, "row stride below the columns");
   if (changed_mask && k > 64) return set_error(GG_ERR_ARGUMENT, "column masks need k <= 64");
@@ -1021,24 +1021,35 @@ int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int
     if (sp) bf_sparse_prologue(stream, *sp, g, chp, done, cus);
     constexpr int V = 16 / (int)sizeof(TD);
     const bool vec = A->format == GG_SELL && k % V == 0 && ld % V == 0 && ((uintptr_t)d & 15) == 0;
+    ColGroups cg{};
+    if (cols && cols->n > 0) {
+      if (!vec) return set_error(GG_ERR_ARGUMENT, "column groups need the 16-byte column kernels");
+      if (cols->n > 32 || cols->n > k / V)
+        return set_error(GG_ERR_ARGUMENT, "more column groups than columns");
+      for (int i = 0; i < cols->n; ++i)
+        if ((int64_t)cols->g[i] >= k / V) return set_error(GG_ERR_ARGUMENT, "column group out of range");
+      cg = *cols;
+    }
+    const int64_t nq = cg.n ? cg.n : k / V;  // V-column groups each row relaxes
     int lv = 0;
-    while (lv < 6 && (1LL << lv) < k / V) ++lv;
+    while (lv < 6 && (1LL << lv) < nq) ++lv;
     if (vec) {
       // light slots and heavy segments V columns per lane
-      const int64_t jbv = (k / V + (1LL << lv) - 1) >> lv;
+      const int64_t jbv = (nq + (1LL << lv) - 1) >> lv;
       const int64_t gpbv = kT >> lv;
       const int64_t lbv = (n_light * jbv + gpbv - 1) / gpbv;
       if (lbv > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
       if (lbv > 0)
         hipLaunchKernelGGL((bf_multi_vec_kernel<TA, TD, V>), dim3((unsigned)lbv), dim3(kT), 0,
                            stream, g, n_light, k, ld, lv, (TD*)d, chp, chn, changed, changed_mask,
-                           done, sp ? &sp->cnt[4] : nullptr);
+                           done, sp ? &sp->cnt[4] : nullptr, cg);
       const int64_t pv = g.n_seg > 0 ? (g.seg_len + kHeavyVecPart - 1) / kHeavyVecPart : 1;
       const int64_t nh = g.n_seg * pv * jbv;
       if (nh > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
       if (nh > 0)
         hipLaunchKernelGGL((bf_multi_heavy_vec_kernel<TA, TD, V>), dim3((unsigned)nh), dim3(kT), 0,
-                           stream, g, pv, k, ld, lv, (TD*)d, chp, chn, changed, changed_mask, done);
+                           stream, g, pv, k, ld, lv, (TD*)d, chp, chn, changed, changed_mask, done,
+                           cg);
     } else if (nb > 0) {
       hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
                          stream, A->format, A->ptr, g, n_light, parts, k, ld, lg, (TD*)d, chp, chn,
@@ -1047,7 +1058,7 @@ int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int
     if (sp && vec) {
       hipLaunchKernelGGL((bf_sp_rows_vec_kernel<TA, TD, V>), dim3((unsigned)std::max(cus, 1) * 8),
                          dim3(kT), 0, stream, *sp, g, k, ld, lv, (TD*)d, chp, chn, changed,
-                         changed_mask, done);
+                         changed_mask, done, cg);
     } else if (sp) {
       hipLaunchKernelGGL((bf_sp_rows_multi_kernel<TA, TD>), dim3((unsigned)std::max(cus, 1) * 8),
                          dim3(kT), 0, stream, *sp, g, k, ld, lg, (TD*)d, chp, chn, changed,

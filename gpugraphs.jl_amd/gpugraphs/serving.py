@@ -140,16 +140,31 @@ def run_poisson(server: GraphServer, graphs, rate_qps: float, duration_s: float,
     queries of one graph go to two registrations (pattern and weights).  A
     collector thread polls completions and fetches or releases each result;
     every keep_every-th result is kept.  Returns (completions, wall_s,
-    results by request id, submitted (request id, graph id, kind, source))."""
+    results by request id, submitted (request id, graph id, kind, source)).
+
+    The submitting thread sleeps until each arrival time, so the collector is
+    not kept waiting for the interpreter lock: a busy-wait here held the lock
+    for most of the run, and results the collector had not yet released
+    filled the server's result arena, which then held back admissions."""
     rng = np.random.default_rng(seed)
     done, results = [], {}
     submitted = []
     lock = threading.Lock()
     finished = threading.Event()
 
+    peaks = dict(arena_used=0, waiting=0)
+    server.last_run_peaks = peaks
+    t_sample = [0.0]
+
     def collector():
         while True:
             batch = server.poll(wait_s=0.01)
+            now = time.perf_counter()
+            if now - t_sample[0] > 0.02:  # admission state, sampled every 20 ms
+                t_sample[0] = now
+                st = server.stats()
+                peaks["arena_used"] = max(peaks["arena_used"], st["arena_used"])
+                peaks["waiting"] = max(peaks["waiting"], st["waiting"])
             for c in batch:
                 if c.status == 0:
                     if keep_every and c.request_id % keep_every == 0:
@@ -170,8 +185,9 @@ def run_poisson(server: GraphServer, graphs, rate_qps: float, duration_s: float,
         t_next += rng.exponential(1.0 / rate_qps)
         if t_next - t0 > duration_s:
             break
-        while time.perf_counter() < t_next:
-            pass
+        wait = t_next - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
         spec = graphs[rng.integers(len(graphs))]
         if len(spec) == 4:
             gb, gs, n, pool = spec
