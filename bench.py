@@ -383,7 +383,10 @@ def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
     log(f"serving graphs built in {time.time() - t:.1f}s")
     budget = int(args.serve_budget_gib * (1 << 30))
     runs = []
-    for rate in [float(r) for r in str(args.serve_rate).split(",") if r]:
+    rates = [float(r) for r in str(args.serve_rate).split(",") if r]
+    secs = [float(x) for x in str(args.serve_seconds).split(",") if x]
+    secs = secs + [secs[-1]] * (len(rates) - len(secs))   # one window per rate
+    for rate, window in zip(rates, secs):
         srv = S.GraphServer(budget_bytes=budget)
         srv.set_pacing(args.serve_pacing)
         table, route = [], []
@@ -393,7 +396,7 @@ def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
             route.append((gb, gs, Mb.m, None))
             table.append((name, gb, gs, Mb, Ms))
         srv.start()
-        done, wall, results, submitted = S.run_poisson(srv, route, rate, args.serve_seconds,
+        done, wall, results, submitted = S.run_poisson(srv, route, rate, window,
                                                        seed=seed, keep_every=97)
         st = srv.stats()
         srv.stop()
@@ -441,7 +444,7 @@ def serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel):
                                                    round(pct(kl, 99) * 1e3, 1),
                                                    round(float(np.mean([c.complete_s - c.admit_s
                                                                         for c in cs])) * 1e3, 1)]
-        run = dict(rate_qps=rate, pacing=args.serve_pacing, arrival_s=args.serve_seconds,
+        run = dict(rate_qps=rate, pacing=args.serve_pacing, arrival_s=window,
                    submitted=len(submitted),
                    completed=len(ok), failed=len(done) - len(ok), wall_s=round(wall, 3),
                    qps=round(len(ok) / wall, 1),
@@ -666,9 +669,10 @@ def main():
                     help="graphs of the pack also run one BFS call at a time (plain baseline)")
     ap.add_argument("--tuned-scale", type=int, default=20,
                     help="R-MAT scale of the autotuned batched leg (0: skip)")
-    ap.add_argument("--serve-rate", type=str, default="2000,5000,10000",
+    ap.add_argument("--serve-rate", type=str, default="1500,2000,10000",
                     help="Poisson arrivals per second, one serving run per comma-separated rate")
-    ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the serving leg")
+    ap.add_argument("--serve-seconds", type=str, default="8,8,1.5",
+                    help="arrival window of each serving rate (comma list; the last one repeats)")
     ap.add_argument("--serve-pacing", type=float, default=24.0,
                     help="server step pacing factor (gg_server_set_pacing; 0: step whenever a slot is busy)")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0,
@@ -967,7 +971,7 @@ def main():
         res["tuned"] = tuned_leg(args, np, torch, gg, G, R0, ctx, _lib, seed)
 
     # ------------------------------------------------------------- serving
-    if args.only in ("", "serve") and args.serve_seconds > 0:
+    if args.only in ("", "serve") and max(float(x) for x in str(args.serve_seconds).split(",") if x) > 0:
         res["serve"] = serving_leg(args, np, torch, gg, G, R0, seed, sigma, relabel)
 
     # ----------------------------------------------------------------- SSSP
