@@ -13,8 +13,9 @@
 //  - SSSP: 64 columns of a row-major min-plus working set (the in-place step
 //    of gg_shortest_path_multi).  A query resets its column between two steps
 //    and marks its source as changed; it is finished after a step that
-//    lowered nothing in its column.  A step relaxes the columns up to the
-//    highest busy slot only (slots are taken lowest first).
+//    lowered nothing in its column.  A step relaxes only the groups of 16
+//    bytes of columns that hold a busy slot (a CSR engine: the columns up to
+//    the highest busy slot; slots are taken lowest first).
 // Every query is exactly the single-source BFS / Bellman-Ford from its source:
 // slots never interact (bit-parallel or column-parallel), and each slot's
 // traversal is the level-synchronous one.
