@@ -784,7 +784,11 @@ int msbfs_level(hipStream_t stream, const gg_matrix* A, const MsArgs& a_in, int 
   if (push) {
     const SellView g = sell_view(A);
     const int csr = A->format == GG_CSR;
-    const unsigned nb = (unsigned)std::max(ctx_cus, 1) * 8;
+    // grid-stride over at most the state rows (queue, touched rows, or all
+    // rows of a wide push): a small graph needs no more blocks than its rows
+    // fill, and these kernels run (returning at once) on pull levels too
+    const unsigned nb = (unsigned)std::min<int64_t>((int64_t)std::max(ctx_cus, 1) * 8,
+                                                    std::max<int64_t>(1, blocks_for(state_size(A), 256)));
     auto run = [&](auto wt) -> int {
       constexpr int WW = decltype(wt)::value;
       hipLaunchKernelGGL(ms_push_tile_kernel<WW>, dim3(nb), dim3(256), 0, stream, a, *push, g, csr,

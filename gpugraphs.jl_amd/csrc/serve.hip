@@ -120,14 +120,32 @@ struct ExtractList {
 };
 
 // One thread per original vertex copies its row's retiring columns into
-// their result blocks (the row's line is read once for all of them).
+// their result blocks and resets them to the unreached value for the slots'
+// next queries (the row's line is read and written once for all of them).
+// The state rows no original vertex maps to (empty slots of a relabelled
+// SELL) are never relaxed and keep the unreached value they were filled with.
 template <class TD>
-__global__ void serve_extract_kernel(const TD* d, int64_t m, int k, ExtractList xl,
+__global__ void serve_extract_kernel(TD* d, int64_t m, int k, ExtractList xl,
                                      const int32_t* vinv) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= m) return;
-  const TD* row = d + (vinv ? (int64_t)vinv[v] - 1 : v) * k;
-  for (int j = 0; j < xl.n; ++j) ((TD*)xl.dst[j])[v] = row[xl.slot[j]];
+  TD* row = d + (vinv ? (int64_t)vinv[v] - 1 : v) * k;
+  for (int j = 0; j < xl.n; ++j) {
+    ((TD*)xl.dst[j])[v] = row[xl.slot[j]];
+    row[xl.slot[j]] = Lim<TD>::max();
+  }
+}
+
+// Seeds of clean slots (their columns hold the unreached value everywhere):
+// the source's entry and its changed bit (one thread).
+template <class TD>
+__global__ void serve_sssp_seed_src_kernel(TD* d, int k, SeedList sl, const int32_t* vinv,
+                                           uint32_t* chp) {
+  for (int q = 0; q < sl.n; ++q) {
+    const int64_t rs = vinv ? (int64_t)vinv[sl.src[q]] - 1 : sl.src[q];
+    d[rs * k + sl.slot[q]] = TD(0);
+    chp[rs >> 5] |= 1u << (rs & 31);
+  }
 }
 
 template <class TD> __global__ void serve_fill_kernel(TD* p, int64_t n, TD v) {
@@ -216,6 +234,10 @@ struct Engine {
   hipEvent_t step_start = nullptr, step_done = nullptr;  // around each step, on the stream
   bool inflight = false;
   uint64_t active = 0, stepped = 0;
+  // SSSP: slots whose column may hold values of a failed query (a retiring
+  // column is reset to the unreached value by its extraction, so a clean
+  // slot's column is all unreached and a seed writes its source alone)
+  uint64_t dirty = 0;
   std::vector<int64_t> slot_q;          // query id per slot
   std::vector<int64_t> slot_steps;      // steps each SSSP slot has run (Bellman-Ford cap)
   std::vector<std::pair<int, int64_t>> seeds;  // (slot, 0-based source) waiting for the next step
@@ -408,6 +430,7 @@ static void fail_slot(gg_server* s, Engine& e, int sl, int status, const char* m
   q.msg = msg;
   q.t_done = now_s();
   ++s->failed;
+  e.dirty |= 1ull << sl;
   e.active &= ~(1ull << sl);
   e.slot_q[sl] = -1;
 }
@@ -437,7 +460,7 @@ static int retire(gg_server* s, Engine& e) {
     GG_TRY(with_type(e.dist_dtype, [&](auto tt) -> int {
       typedef typename decltype(tt)::type TD;
       hipLaunchKernelGGL(serve_extract_kernel<TD>, dim3(blocks_for(e.m, 256)), dim3(256), 0,
-                         e.stream, (const TD*)e.buf, e.m, e.slots, xl, e.vinv);
+                         e.stream, (TD*)e.buf, e.m, e.slots, xl, e.vinv);
       return GG_OK;
     }));
   }
@@ -543,10 +566,18 @@ static int launch_step(gg_server* s, Engine& e) {
     std::swap(e.fany, e.nany);
   } else {
     if (sl.n) {
+      // a dirty slot among the seeds: every seeded column is reset in one
+      // pass over the rows; otherwise the sources alone are written
+      const bool reset = (seedmask & e.dirty) != 0;
+      e.dirty &= ~seedmask;
       GG_TRY(with_type(e.dist_dtype, [&](auto tt) -> int {
         typedef typename decltype(tt)::type TD;
-        hipLaunchKernelGGL(serve_sssp_seed_kernel<TD>, dim3(blocks_for(ns, 256)), dim3(256),
-                           0, e.stream, (TD*)e.buf, ns, e.slots, sl, e.vinv, e.chp);
+        if (reset)
+          hipLaunchKernelGGL(serve_sssp_seed_kernel<TD>, dim3(blocks_for(ns, 256)), dim3(256),
+                             0, e.stream, (TD*)e.buf, ns, e.slots, sl, e.vinv, e.chp);
+        else
+          hipLaunchKernelGGL(serve_sssp_seed_src_kernel<TD>, dim3(1), dim3(1), 0, e.stream,
+                             (TD*)e.buf, e.slots, sl, e.vinv, e.chp);
         return GG_OK;
       }));
     }

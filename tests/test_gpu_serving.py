@@ -185,11 +185,15 @@ def test_server_sssp_busy_column_groups(graphs, dt):
         srv.stop()
 
 
-def test_server_negative_cycle_fails_not_converged(graphs):
+@pytest.mark.parametrize("fmt", ["csr", "sell_relabel"])
+def test_server_negative_cycle_fails_not_converged(graphs, fmt):
     """A query whose source reaches a negative cycle fails with
     NotConvergedError after m + 16 steps (gg_shortest_path's cap); a query
     from a source that cannot reach the cycle, on the same engine at the
-    same time, still returns the oracle's distances."""
+    same time, still returns the oracle's distances.  The failed query's
+    column is not extracted (and so not reset by its extraction): 70 later
+    queries reuse every slot, the failed one first, and each returns the
+    oracle's distances."""
     gg, torch, mats, omats, orc = graphs
     from gpugraphs import serving as S
     import scipy.sparse as sp
@@ -204,16 +208,26 @@ def test_server_negative_cycle_fails_not_converged(graphs):
     Adj = sp.csr_matrix((w, (u, v)), shape=(n, n))
     AT = Adj.T.tocsr()
     AT.sort_indices()
-    A = gg.SparseGPUMatrixCSR(AT, gg.ROCBackend(0))
+    if fmt == "csr":
+        A = gg.SparseGPUMatrixCSR(AT, gg.ROCBackend(0))
+    else:
+        A = gg.SparseGPUMatrixSELL(AT, gg.ROCBackend(0), 64, relabel=True)
     Ao = orc.from_scipy_csr(AT)
     with S.GraphServer(budget_bytes=1 << 20) as srv:
         gid = srv.add_graph(A)
         srv.start()
-        print_case("serve negative cycle")
+        print_case("serve negative cycle", fmt)
         bad = srv.submit(gid, S.SSSP, 1)
         good = srv.submit(gid, S.SSSP, 41)
         np.testing.assert_array_equal(srv.fetch(good, gid, S.SSSP), orc.shortest_path(Ao, 41))
         with pytest.raises(gg.NotConvergedError):
             srv.fetch(bad, gid, S.SSSP)
+        assert srv.stats()["failed"] == 1
+        # sources 13..64 cannot reach the cycle
+        srcs = [int(s) for s in np.random.default_rng(9).integers(13, n + 1, 70)]
+        rids = [srv.submit(gid, S.SSSP, s) for s in srcs]
+        for rid, s in zip(rids, srcs):
+            np.testing.assert_array_equal(srv.fetch(rid, gid, S.SSSP), orc.shortest_path(Ao, s),
+                                          err_msg=f"source {s}")
         assert srv.stats()["failed"] == 1
         srv.stop()
