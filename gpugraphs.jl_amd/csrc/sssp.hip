@@ -302,14 +302,10 @@ __global__ __launch_bounds__(kT) void bf_multi_kernel(int format, const int32_t*
 // neighbour's V distances as one 16-byte load, so a wavefront covers 64 / Lv
 // rows.  Same relaxation as bf_multi_kernel's light part.
 template <class TA, class TD, int V>
-__global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_light, int64_t k, int64_t ld,
-                                                          int logLv, TD* d,
-                                                          const uint32_t* __restrict__ chp,
-                                                          uint32_t* __restrict__ chn,
-                                                          unsigned int* changed,
-                                                          unsigned long long* changed_mask,
-                                                          const int32_t* done,
-                                                          const unsigned int* light_off, ColGroups cg) {
+__device__ __forceinline__ void bf_vec_light(SellView g, int64_t n_light, int64_t k, int64_t ld,
+    int logLv, TD* d, const uint32_t* __restrict__ chp, uint32_t* __restrict__ chn,
+    unsigned int* changed, unsigned long long* changed_mask, const int32_t* done,
+    const unsigned int* light_off, ColGroups cg, int64_t bid) {
   typedef TD VT __attribute__((ext_vector_type(V)));
   if (done && *done) return;
   if (light_off && *light_off) return;
@@ -320,7 +316,7 @@ __global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_
   const int64_t kq = cg.n ? cg.n : k / V;
   const int64_t jblocks = (kq + Lv - 1) >> logLv;
   const int64_t gpb = kT >> logLv;
-  const int64_t q = (int64_t)blockIdx.x * gpb + (threadIdx.x >> logLv);
+  const int64_t q = bid * gpb + (threadIdx.x >> logLv);
   const int64_t unit = q / jblocks;
   const int64_t jq = (q - unit * jblocks) * Lv + lane;
   const bool jin = jq < kq;
@@ -401,6 +397,15 @@ __global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_
   }
 }
 
+template <class TA, class TD, int V>
+__global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_light, int64_t k,
+    int64_t ld, int logLv, TD* d, const uint32_t* __restrict__ chp, uint32_t* __restrict__ chn,
+    unsigned int* changed, unsigned long long* changed_mask, const int32_t* done,
+    const unsigned int* light_off, ColGroups cg) {
+  bf_vec_light<TA, TD, V>(g, n_light, k, ld, logLv, d, chp, chn, changed, changed_mask, done,
+                          light_off, cg, (int64_t)blockIdx.x);
+}
+
 // Heavy rows with V columns per lane: a block per (part of kHeavyVecPart
 // entries of a heavy segment, column block); its groups of Lv lanes split the
 // part's entries (batches of 4 per group), fold through LDS and lower the row
@@ -410,13 +415,10 @@ __global__ __launch_bounds__(kT) void bf_multi_vec_kernel(SellView g, int64_t n_
 constexpr int64_t kHeavyVecPart = 8192;
 
 template <class TA, class TD, int V>
-__global__ __launch_bounds__(kT) void bf_multi_heavy_vec_kernel(SellView g, int64_t parts,
-                                                                int64_t k, int64_t ld, int logLv, TD* d,
-                                                                const uint32_t* __restrict__ chp,
-                                                                uint32_t* __restrict__ chn,
-                                                                unsigned int* changed,
-                                                                unsigned long long* changed_mask,
-                                                                const int32_t* done, ColGroups cg) {
+__device__ __forceinline__ void bf_vec_heavy(SellView g, int64_t parts, int64_t k, int64_t ld,
+    int logLv, TD* d, const uint32_t* __restrict__ chp, uint32_t* __restrict__ chn,
+    unsigned int* changed, unsigned long long* changed_mask, const int32_t* done, ColGroups cg,
+    int64_t bid) {
   typedef TD VT __attribute__((ext_vector_type(V)));
   __shared__ VT red[kT];
   __shared__ int s_any;
@@ -427,8 +429,8 @@ __global__ __launch_bounds__(kT) void bf_multi_heavy_vec_kernel(SellView g, int6
   const int64_t gpb = kT >> logLv;
   const int64_t kq = cg.n ? cg.n : k / V;
   const int64_t jblocks = (kq + Lv - 1) >> logLv;
-  const int64_t unit = (int64_t)blockIdx.x / jblocks;
-  const int64_t jq = ((int64_t)blockIdx.x - unit * jblocks) * Lv + lane;
+  const int64_t unit = bid / jblocks;
+  const int64_t jq = (bid - unit * jblocks) * Lv + lane;
   const bool jin = jq < kq;
   const int64_t j0 = (cg.n && jin ? (int64_t)cg.g[jq] : jq) * V;
   int64_t h, s, e;
@@ -496,6 +498,33 @@ __global__ __launch_bounds__(kT) void bf_multi_heavy_vec_kernel(SellView g, int6
   }
   __syncthreads();
   if (threadIdx.x == 0 && s_any) bit_claim(chn, heavy_rid(g, h));
+}
+
+template <class TA, class TD, int V>
+__global__ __launch_bounds__(kT) void bf_multi_heavy_vec_kernel(SellView g, int64_t parts,
+    int64_t k, int64_t ld, int logLv, TD* d, const uint32_t* __restrict__ chp,
+    uint32_t* __restrict__ chn, unsigned int* changed, unsigned long long* changed_mask,
+    const int32_t* done, ColGroups cg) {
+  bf_vec_heavy<TA, TD, V>(g, parts, k, ld, logLv, d, chp, chn, changed, changed_mask, done, cg,
+                          (int64_t)blockIdx.x);
+}
+
+// Light slots and heavy parts in one launch, the heavy parts first: a heavy
+// part is a long chain of dependent batches on few blocks, and in a launch of
+// their own they ran after the light slots with the device mostly idle; here
+// the light slots' blocks fill the device around them.
+template <class TA, class TD, int V>
+__global__ __launch_bounds__(kT) void bf_multi_fused_vec_kernel(
+    SellView g, int64_t n_light, int64_t parts, int64_t nh, int64_t k, int64_t ld, int logLv,
+    TD* d, const uint32_t* __restrict__ chp, uint32_t* __restrict__ chn, unsigned int* changed,
+    unsigned long long* changed_mask, const int32_t* done, const unsigned int* light_off,
+    ColGroups cg) {
+  if ((int64_t)blockIdx.x < nh)
+    bf_vec_heavy<TA, TD, V>(g, parts, k, ld, logLv, d, chp, chn, changed, changed_mask, done, cg,
+                            (int64_t)blockIdx.x);
+  else
+    bf_vec_light<TA, TD, V>(g, n_light, k, ld, logLv, d, chp, chn, changed, changed_mask, done,
+                            light_off, cg, (int64_t)blockIdx.x - nh);
 }
 
 // ----------------------------------------------------- sparse iterations
@@ -1030,39 +1059,49 @@ int bf_multi_step_sp(hipStream_t stream, const gg_matrix* A, int dist_dtype, int
         if ((int64_t)cols->g[i] >= k / V) return set_error(GG_ERR_ARGUMENT, "column group out of range");
       cg = *cols;
     }
-    const int64_t nq = cg.n ? cg.n : k / V;  // V-column groups each row relaxes
-    int lv = 0;
-    while (lv < 6 && (1LL << lv) < nq) ++lv;
-    if (vec) {
-      // light slots and heavy segments V columns per lane
+    // light slots, heavy parts and sparse rows VV = V columns (16 bytes) per
+    // lane (32 bytes per lane, half the lanes per row, ran at 0.6x on R-MAT)
+    auto run_vec = [&](auto vt) -> int {
+      constexpr int VV = decltype(vt)::value;
+      const int64_t nq = cg.n ? cg.n : k / VV;  // VV-column groups each row relaxes
+      int lv = 0;
+      while (lv < 6 && (1LL << lv) < nq) ++lv;
       const int64_t jbv = (nq + (1LL << lv) - 1) >> lv;
       const int64_t gpbv = kT >> lv;
       const int64_t lbv = (n_light * jbv + gpbv - 1) / gpbv;
       if (lbv > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
-      if (lbv > 0)
-        hipLaunchKernelGGL((bf_multi_vec_kernel<TA, TD, V>), dim3((unsigned)lbv), dim3(kT), 0,
-                           stream, g, n_light, k, ld, lv, (TD*)d, chp, chn, changed, changed_mask,
-                           done, sp ? &sp->cnt[4] : nullptr, cg);
       const int64_t pv = g.n_seg > 0 ? (g.seg_len + kHeavyVecPart - 1) / kHeavyVecPart : 1;
       const int64_t nh = g.n_seg * pv * jbv;
-      if (nh > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
-      if (nh > 0)
-        hipLaunchKernelGGL((bf_multi_heavy_vec_kernel<TA, TD, V>), dim3((unsigned)nh), dim3(kT), 0,
+      if (nh + lbv > 0x7fffffffLL) return set_error(GG_ERR_ARGUMENT, "multi-source launch too large");
+      if (lbv > 0 && nh > 0)
+        hipLaunchKernelGGL((bf_multi_fused_vec_kernel<TA, TD, VV>), dim3((unsigned)(nh + lbv)),
+                           dim3(kT), 0, stream, g, n_light, pv, nh, k, ld, lv, (TD*)d, chp, chn,
+                           changed, changed_mask, done, sp ? &sp->cnt[4] : nullptr, cg);
+      else if (lbv > 0)
+        hipLaunchKernelGGL((bf_multi_vec_kernel<TA, TD, VV>), dim3((unsigned)lbv), dim3(kT), 0,
+                           stream, g, n_light, k, ld, lv, (TD*)d, chp, chn, changed, changed_mask,
+                           done, sp ? &sp->cnt[4] : nullptr, cg);
+      else if (nh > 0)
+        hipLaunchKernelGGL((bf_multi_heavy_vec_kernel<TA, TD, VV>), dim3((unsigned)nh), dim3(kT), 0,
                            stream, g, pv, k, ld, lv, (TD*)d, chp, chn, changed, changed_mask, done,
                            cg);
-    } else if (nb > 0) {
-      hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
-                         stream, A->format, A->ptr, g, n_light, parts, k, ld, lg, (TD*)d, chp, chn,
-                         changed, changed_mask, done, sp ? &sp->cnt[4] : nullptr);
-    }
-    if (sp && vec) {
-      hipLaunchKernelGGL((bf_sp_rows_vec_kernel<TA, TD, V>), dim3((unsigned)std::max(cus, 1) * 8),
-                         dim3(kT), 0, stream, *sp, g, k, ld, lv, (TD*)d, chp, chn, changed,
-                         changed_mask, done, cg);
-    } else if (sp) {
-      hipLaunchKernelGGL((bf_sp_rows_multi_kernel<TA, TD>), dim3((unsigned)std::max(cus, 1) * 8),
-                         dim3(kT), 0, stream, *sp, g, k, ld, lg, (TD*)d, chp, chn, changed,
-                         changed_mask, done);
+      if (sp)
+        hipLaunchKernelGGL((bf_sp_rows_vec_kernel<TA, TD, VV>), dim3((unsigned)std::max(cus, 1) * 8),
+                           dim3(kT), 0, stream, *sp, g, k, ld, lv, (TD*)d, chp, chn, changed,
+                           changed_mask, done, cg);
+      return GG_OK;
+    };
+    if (vec) {
+      GG_TRY(run_vec(std::integral_constant<int, V>{}));
+    } else {
+      if (nb > 0)
+        hipLaunchKernelGGL((bf_multi_kernel<TA, TD>), dim3((unsigned)nb), dim3(kT), sizeof(TD) * kT,
+                           stream, A->format, A->ptr, g, n_light, parts, k, ld, lg, (TD*)d, chp,
+                           chn, changed, changed_mask, done, sp ? &sp->cnt[4] : nullptr);
+      if (sp)
+        hipLaunchKernelGGL((bf_sp_rows_multi_kernel<TA, TD>), dim3((unsigned)std::max(cus, 1) * 8),
+                           dim3(kT), 0, stream, *sp, g, k, ld, lg, (TD*)d, chp, chn, changed,
+                           changed_mask, done);
     }
     if (sp) {
       hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(sp->words, kT)), dim3(kT), 0, stream,
