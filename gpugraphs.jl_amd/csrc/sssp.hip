@@ -839,6 +839,34 @@ __global__ void state_to_result_kernel(const TD* __restrict__ src, const int32_t
   dst[rowmajor ? i : v + j * m] = src[r * k + j];
 }
 
+// The same into the column-major result through a 64 x 64 LDS tile: a block
+// reads 64 state rows (their k values contiguous) and writes each of its
+// columns' 64 consecutive vertices contiguously, where one thread per value
+// wrote the k values of a row k columns apart.
+template <class TD>
+__global__ __launch_bounds__(256) void state_to_cols_kernel(const TD* __restrict__ src,
+                                                            const int32_t* __restrict__ vinv,
+                                                            int64_t m, int64_t k,
+                                                            TD* __restrict__ dst) {
+  __shared__ TD t[64][65];
+  const int64_t v0 = (int64_t)blockIdx.x * 64, j0 = (int64_t)blockIdx.y * 64;
+  const int64_t jn = min((int64_t)64, k - j0);
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int rr = idx >> 6, jj = idx & 63;
+    const int64_t v = v0 + rr;
+    if (v < m && jj < jn) {
+      const int64_t r = vinv ? (int64_t)vinv[v] - 1 : v;
+      t[rr][jj] = src[r * k + j0 + jj];
+    }
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int jj = idx >> 6, rr = idx & 63;
+    const int64_t v = v0 + rr;
+    if (v < m && jj < jn) dst[(j0 + jj) * m + v] = t[rr][jj];
+  }
+}
+
 __global__ void zero_u32_kernel(unsigned int* p) { *p = 0u; }
 
 // Captured iterations: ctl = {iterations, done}.  After an iteration: done
@@ -1155,9 +1183,14 @@ int sssp_multi(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_host, int
     if (st == GG_OK) {
       if (rowmajor_out && !vinv) {
         GG_HIP(hipMemcpyAsync(dist, b0, sizeof(TD) * m * k, hipMemcpyDeviceToDevice, ctx->stream));
-      } else {
+      } else if (rowmajor_out) {
         hipLaunchKernelGGL(state_to_result_kernel<TD>, dim3(blocks_for(m * k, kT)), dim3(kT), 0,
-                           ctx->stream, (const TD*)b0, vinv, m, k, rowmajor_out ? 1 : 0, (TD*)dist);
+                           ctx->stream, (const TD*)b0, vinv, m, k, 1, (TD*)dist);
+        GG_HIP(hipGetLastError());
+      } else {
+        hipLaunchKernelGGL(state_to_cols_kernel<TD>,
+                           dim3((unsigned)blocks_for(m, 64), (unsigned)blocks_for(k, 64)), dim3(256),
+                           0, ctx->stream, (const TD*)b0, vinv, m, k, (TD*)dist);
         GG_HIP(hipGetLastError());
       }
     }
