@@ -839,31 +839,34 @@ __global__ void state_to_result_kernel(const TD* __restrict__ src, const int32_t
   dst[rowmajor ? i : v + j * m] = src[r * k + j];
 }
 
-// The same into the column-major result through a 64 x 64 LDS tile: a block
-// reads 64 state rows (their k values contiguous) and writes each of its
-// columns' 64 consecutive vertices contiguously, where one thread per value
-// wrote the k values of a row k columns apart.
+// The same into the column-major result through an LDS tile of 4096 values:
+// TC = k rounded up to a power of two in 4..64 columns by TR = 4096 / TC rows
+// (rows padded to TC + 1 values, so a column is read without bank conflicts).
+// A block reads its state rows' TC values (contiguous in each row) and writes
+// each column's TR consecutive vertices contiguously, where one thread per
+// value wrote the k values of a row k columns apart.
 template <class TD>
 __global__ __launch_bounds__(256) void state_to_cols_kernel(const TD* __restrict__ src,
                                                             const int32_t* __restrict__ vinv,
-                                                            int64_t m, int64_t k,
+                                                            int64_t m, int64_t k, int logTC,
                                                             TD* __restrict__ dst) {
-  __shared__ TD t[64][65];
-  const int64_t v0 = (int64_t)blockIdx.x * 64, j0 = (int64_t)blockIdx.y * 64;
-  const int64_t jn = min((int64_t)64, k - j0);
-  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
-    const int rr = idx >> 6, jj = idx & 63;
+  __shared__ TD t[4096 + 1024];
+  const int TC = 1 << logTC, TR = 4096 >> logTC;
+  const int64_t v0 = (int64_t)blockIdx.x * TR, j0 = (int64_t)blockIdx.y * TC;
+  const int64_t jn = min((int64_t)TC, k - j0);
+  for (int idx = threadIdx.x; idx < 4096; idx += 256) {
+    const int rr = idx >> logTC, jj = idx & (TC - 1);
     const int64_t v = v0 + rr;
     if (v < m && jj < jn) {
       const int64_t r = vinv ? (int64_t)vinv[v] - 1 : v;
-      t[rr][jj] = src[r * k + j0 + jj];
+      t[rr * (TC + 1) + jj] = src[r * k + j0 + jj];
     }
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
-    const int jj = idx >> 6, rr = idx & 63;
+  for (int idx = threadIdx.x; idx < 4096; idx += 256) {
+    const int jj = idx >> (12 - logTC), rr = idx & (TR - 1);
     const int64_t v = v0 + rr;
-    if (v < m && jj < jn) dst[(j0 + jj) * m + v] = t[rr][jj];
+    if (v < m && jj < jn) dst[(j0 + jj) * m + v] = t[rr * (TC + 1) + jj];
   }
 }
 
@@ -1188,9 +1191,11 @@ int sssp_multi(gg_ctx* ctx, const gg_matrix* A, const int32_t* sources_host, int
                            ctx->stream, (const TD*)b0, vinv, m, k, 1, (TD*)dist);
         GG_HIP(hipGetLastError());
       } else {
+        int ltc = 2;  // columns per tile: k rounded up to a power of two in 4..64
+        while (ltc < 6 && (1LL << ltc) < k) ++ltc;
         hipLaunchKernelGGL(state_to_cols_kernel<TD>,
-                           dim3((unsigned)blocks_for(m, 64), (unsigned)blocks_for(k, 64)), dim3(256),
-                           0, ctx->stream, (const TD*)b0, vinv, m, k, (TD*)dist);
+                           dim3((unsigned)blocks_for(m, 4096 >> ltc), (unsigned)blocks_for(k, 1 << ltc)),
+                           dim3(256), 0, ctx->stream, (const TD*)b0, vinv, m, k, ltc, (TD*)dist);
         GG_HIP(hipGetLastError());
       }
     }
